@@ -1,0 +1,236 @@
+/*
+ * lpm_mi355x.h -- C ABI of the MI355X (gfx950) implementation of the LPM-MAC
+ * static / fatigue Newton load-step path.
+ *
+ * The reference (longfish/LPM-MAC) has no FFI: its only seam is the header-only
+ * C++ template API (include/*.h). Every entry point below replaces one method
+ * of that API; the citation names the reference file:line whose behaviour it
+ * provides. The C++ drop-in headers (lpm-mac_amd/host/) call these functions
+ * from the reference's own class names, and tests/bench call them through
+ * ctypes.
+ *
+ * Conventions
+ *  - Plain pointers and sizes; no C++ or torch types cross the ABI.
+ *  - Every function returns 0 on success and a negative LPM_E* code on error;
+ *    lpm_last_error() gives a message. No exception crosses the ABI.
+ *  - Host arrays are borrowed for the duration of the call only.
+ *  - One context = one particle system (one reference Assembly) on one GPU.
+ *    Calls on a context are ordered on its own HIP stream; functions that
+ *    return host values synchronise that stream.
+ *  - Particle ids are 0..n-1 (the reference's DoF index = particle id,
+ *    solver.h:102,117). Positions are xyz[3*i+k] (FP64) also in 2D (z kept).
+ */
+#ifndef LPM_MI355X_H
+#define LPM_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes */
+#define LPM_OK 0
+#define LPM_E_ARG (-1)      /* invalid argument / inconsistent sizes */
+#define LPM_E_TOPO (-2)     /* topology rejected (asymmetric bond, duplicate, out of range) */
+#define LPM_E_HIP (-3)      /* HIP runtime error */
+#define LPM_E_STATE (-4)    /* call out of order (e.g. solve before assemble) */
+#define LPM_E_NOMEM (-5)
+#define LPM_E_LIMIT (-6)    /* a row or list exceeds a compiled limit */
+
+/* ---- enums (same values as the reference's enum classes, include/lpm.h:56-96) */
+enum { LPM_SQUARE2D = 0, LPM_HEXAGON2D = 1, LPM_SC3D = 2, LPM_FCC3D = 3, LPM_BCC3D = 4 };
+enum { LPM_ELASTIC = 0, LPM_ELASTIC_DAMAGE = 1, LPM_J2PLASTICITY = 2, LPM_FATIGUE_HCF = 3 };
+enum { LPM_RELATIVE = 0, LPM_ABSOLUTE = 1 };                 /* LoadMode */
+enum { LPM_TIME_LINEAR = 0, LPM_TIME_EXPONENTIAL = 1 };       /* TimeMapMode */
+enum { LPM_LOAD_UNIAXIAL_DISP = 0, LPM_LOAD_UNIAXIAL_FORCE = 1,
+       LPM_LOAD_DOGBONE_DISP = 2, LPM_LOAD_CT_FORCE = 3 };    /* FatigueLoadType, load_step_fatigue.h:13-19 */
+
+typedef struct lpm_ctx lpm_ctx;
+
+/* ------------------------------------------------------------------ setup (host C++ inside the library)
+ * Lattice points; replaces createPlateSQ2D / createPlateHEX2D / createCuboidSC3D /
+ * createCuboidFCC3D with createRMatrix (include/utilities.h:8-301).
+ * box[6] = {xmin,xmax,ymin,ymax,zmin,zmax}; angles in radians. Writes at most cap points
+ * into xyz (3 doubles each) and returns the number of points in *n. */
+int lpm_generate_lattice(int lattice, const double *box, double radius, int eulerflag, const double *angles,
+                         double *xyz, int64_t cap, int64_t *n);
+
+/* Directed bonds with the reference's ordering; replaces Assembly::createBonds
+ * (include/assembly.h:212-234): for each particle, layer-0 bonds then layer-1 bonds,
+ * each in increasing partner id; dis < 1.01*cutoff[1], layer 0 if dis < 1.01*cutoff[0].
+ * Pass bond_p2 == NULL to get the count in *nbonds first. bond_ptr has n+1 entries,
+ * bond_split (first layer-1 bond of each particle) n entries. */
+int lpm_build_bonds(int lattice, double radius, int64_t n, const double *xyz, int32_t *bond_ptr,
+                    int32_t *bond_split, int32_t *bond_p2, int64_t cap, int64_t *nbonds);
+
+/* Unit-cell constants; replaces UnitCell (include/unit_cell.h:125-225).
+ * out[0]=dim out[1]=nneighbors out[2]=nneighbors1 out[3]=nneighbors2 out[4]=nneighbors_AFEM
+ * out[5]=cutoff0 out[6]=cutoff1 out[7]=particle_volume; map[9] = el_mapping (row r, col c at
+ * map[3r+c]) as the reference reads it when literal != 0 (column-major Eigen::Map of the
+ * row-major source array, unit_cell.h:147) or as written when literal == 0. */
+int lpm_unit_cell(int lattice, double radius, int literal, double *out, double *map);
+
+/* ------------------------------------------------------------------ context
+ * Particle system on the device; replaces Assembly's per-particle / per-bond objects
+ * (particle.h:13-66, bond.h:12-73) with a structure-of-arrays layout in HBM. The bond
+ * list must be symmetric (every bond i->j of layer l has a reverse j->i of layer l; the
+ * reference dereferences a missing one, particle.h:207-212) and duplicate-free. The
+ * 2-hop sparsity pattern (Assembly::updateConnections, assembly.h:236-266) is built on
+ * the device from the bonds. */
+typedef struct {
+    int32_t lattice;
+    double radius;
+    int32_t model;             /* LPM_ELASTIC .. LPM_FATIGUE_HCF (Assembly ptype) */
+    int64_t n;                 /* particles */
+    const double *xyz;         /* [3n] initial positions (also xyz_initial / xyz_last) */
+    const int32_t *bond_ptr;   /* [n+1] */
+    const int32_t *bond_split; /* [n] first layer-1 bond of each particle */
+    const int32_t *bond_p2;    /* [bond_ptr[n]] partner ids */
+    int32_t device;            /* HIP device ordinal */
+} lpm_topology;
+
+int lpm_create(const lpm_topology *topo, lpm_ctx **out);
+int lpm_destroy(lpm_ctx *ctx);
+const char *lpm_last_error(const lpm_ctx *ctx); /* ctx may be NULL: last error of lpm_create */
+/* sizes: "n","dim","nbonds","nconn","nnl","ndof","knnz","maxconn","maxbond","nl_ptr" */
+int64_t lpm_size(const lpm_ctx *ctx, const char *what);
+
+/* Mode switches (each literal reference behaviour has value shown first).
+ *  "diag10_literal"   1: diagonal block K[1][0] left 0 (derivative.h:38) | 0: = K[0][1]
+ *  "cg_transpose"     1: solve K^T d = r like Eigen CG<Lower|Upper> on column-major K
+ *                        (solver.h:195) | 0: solve K d = r
+ *  "dirichlet_static" 0: static step does not eliminate constrained DoFs (solver_static.h:113)
+ *                      | 1: eliminate (as the fatigue step does, solver_fatigue.h:288)
+ *  "rollback_pex"     0: Pex kept on a step restart (assembly.h:150-159) | 1: restored
+ *  "precision"        0: FP32 K, SpMV and CG (FP64 residual, positions, state) | 1: FP64 everywhere
+ *  "cg_tol"           relative CG tolerance (Eigen default 2.22e-16; FP32 runs use ~1e-5)
+ *  "cg_maxit"         CG iteration cap (-1: 2*ndof, Eigen default)
+ *  "precond"          0: Jacobi (Eigen DiagonalPreconditioner) | 1: 3x3 block Jacobi
+ *  "max_NR_iter", "tol_NR"  Newton limits (Solver ctor arguments, solver.h:39-50)
+ *  "verbose"          print per-iteration norms like the reference
+ */
+int lpm_set_mode(lpm_ctx *ctx, const char *name, double value);
+double lpm_get_mode(const lpm_ctx *ctx, const char *name);
+
+/* ------------------------------------------------------------------ state transfer
+ * Field names follow the reference members: particle fields "xyz","xyz_initial","xyz_last",
+ * "Pin","Pex","stress"(6/particle),"damage","damage_last","damage_visual","Ddot_local",
+ * "Ddot_nonlocal","nonlocal_L","state_var","state_var_last"(7/particle),"mat"(10/particle:
+ * model parameters, see lpm_set_material), "dLe_total","TdLe_total"(2/particle),
+ * "cs_sum"(6/particle); int32 particle fields "type","disp_constraint"(3/particle),"nb";
+ * bond fields "dis_initial","dis","dis_last","dLe","dLp","dLp_last","bstrain","Kn","Tv",
+ * "cs"(3/bond),"bforce","bforce_last","bdamage","bdamage_last"; int32 "bond_ptr",
+ * "bond_split","bond_p2","opp","conn_ptr","conn_idx","nl_ptr","nl_idx";
+ * solver vectors "residual","disp"(dim*n),"reaction" (host NULL and count 0 return the reaction
+ * count instead of 0). count = number of elements. */
+int lpm_upload(lpm_ctx *ctx, const char *field, const void *host, int64_t count);
+int lpm_download(lpm_ctx *ctx, const char *field, void *host, int64_t count);
+/* Stiffness matrix actually handed to CG (K, or K^T when cg_transpose) as CSR over scalar
+ * rows in FP64 (values converted from the stored precision). Sizes via lpm_size("knnz"). */
+int lpm_download_matrix(lpm_ctx *ctx, int32_t *rowptr, int32_t *col, double *val, int64_t cap);
+
+/* ------------------------------------------------------------------ material
+ * Per-particle model parameters and per-bond Kn/Tv; replaces the setParticleProperty
+ * overloads (particle_elastic.h:70-143, particle_elastic_damage.h:175-254,
+ * particle_j2plasticity.h:154-201, particle_fatigue_hcf.h:162-209). id < 0: all particles.
+ * params layout (mat, 10 doubles per particle):
+ *   Elastic        [0]=critical_bstrain
+ *   ElasticDamage  [0]=k0 [1]=k1 [2]=damage_threshold [3]=comp_tensile_ratio
+ *   J2Plasticity   [0]=E [1]=mu [2]=sigmay [3]=xi [4]=H [5]=A [6]=damage_threshold [7]=critical_bstrain
+ *   FatigueHCF     [0]=A [1]=B [2]=d [3]=E [4]=mu [5]=damage_threshold [6]=fatigue_limit_ratio
+ * Ce = (C11, C12, C44); KnTv = el_mapping * Ce (x radius in 3D); elmap_literal selects the
+ * reference's column-major read (1) or the row-major intent (0). */
+int lpm_set_material(lpm_ctx *ctx, int64_t id, double nonlocal_L, const double *Ce, int elmap_literal,
+                     const double *params, int32_t nparams);
+/* Ce from (E, mu, plane_stress) exactly as particle_elastic.h:77-88. */
+int lpm_ce_from_e(int plane_stress, double E, double mu, double *Ce);
+
+/* Nonlocal neighbours within cutoff_ratio * nonlocal_L (Assembly::searchNonlocalNeighbors,
+ * assembly.h:117-130), built on the device. */
+int lpm_search_nonlocal(lpm_ctx *ctx, double cutoff_ratio);
+
+/* ------------------------------------------------------------------ boundary conditions
+ * Particle groups live on the device; replaces the std::vector<Particle*> groups of DispBC /
+ * ForceBC (boundary.h:9-44). */
+int lpm_group_create(lpm_ctx *ctx, int64_t n, const int32_t *ids, int32_t *gid);
+
+typedef struct { int32_t group; int32_t mode; int32_t flag; double step; } lpm_disp_bc;   /* flag 'x','y','z' */
+typedef struct { int32_t group; int32_t mode; double fx, fy, fz; } lpm_force_bc;
+typedef struct {
+    int32_t n_disp;
+    const lpm_disp_bc *disp;
+    int32_t n_force;
+    const lpm_force_bc *force;
+} lpm_load_step;
+
+/* Solver::updateForceBC then Solver::updateDisplacementBC (solver.h:130-190) */
+int lpm_apply_bc(lpm_ctx *ctx, const lpm_load_step *step);
+
+/* ------------------------------------------------------------------ per-phase entry points
+ * Assembly::updateGeometry (assembly.h:132-138; particle.h:163-181; bond.h:30-39) */
+int lpm_update_geometry(lpm_ctx *ctx);
+/* Assembly::updateForceState (assembly.h:178-191): bond forces of the model
+ * (particle_elastic.h:38-50 and the damage variants), Pin, stress, damage_visual */
+int lpm_update_force_state(lpm_ctx *ctx);
+/* Solver::updateRR (solver.h:108-128) plus the two norms NewtonIteration uses (:61-63) */
+int lpm_update_rr(lpm_ctx *ctx, double *norm_residual, double *norm_reaction);
+/* Stiffness::reset + calcStiffness (stiffness.h:43-48, 228-263) with localStiffnessANA /
+ * fdu2dxyz* (stiffness.h:67-101, derivative.h:7-239), into BSR on the device pattern */
+int lpm_calc_stiffness(lpm_ctx *ctx);
+/* Stiffness::updateStiffnessDispBC (stiffness.h:265-294) */
+int lpm_update_stiffness_disp_bc(lpm_ctx *ctx);
+/* Solver::solveLinearSystem (solver.h:91-106): LPM_CG (solver.h:192-217) then move every
+ * particle by its DoFs of disp. iters/error as Eigen reports them (may be NULL). */
+int lpm_solve_linear_system(lpm_ctx *ctx, int32_t *iters, double *error);
+/* Solver::NewtonIteration (solver.h:53-89); *n_iter = return value of the reference */
+int lpm_newton_iteration(lpm_ctx *ctx, int32_t *n_iter);
+/* Assembly::updateStateVar (assembly.h:161-166) */
+int lpm_update_state_var(lpm_ctx *ctx);
+/* SolverStatic::updateStaticDamage (solver_static.h:31-62) */
+int lpm_update_static_damage(lpm_ctx *ctx, int32_t undamaged_type, int32_t *any_damaged);
+/* SolverFatigue::updateFatigueDamage (solver_fatigue.h:75-126) */
+int lpm_update_fatigue_damage(lpm_ctx *ctx, double dNdt, int32_t undamaged_type, int32_t *any_damaged);
+/* Assembly::storeStateVar / resetStateVar (assembly.h:140-159) */
+int lpm_store_state(lpm_ctx *ctx);
+int lpm_reset_state(lpm_ctx *ctx, int32_t reset_xyz);
+
+/* ------------------------------------------------------------------ solver drivers
+ * SolverStatic::solveProblemStep (solver_static.h:97-149); *converged = its return */
+int lpm_solve_step_static(lpm_ctx *ctx, const lpm_load_step *step, int32_t undamaged_type, int32_t *converged);
+/* SolverStatic::solveProblem (solver_static.h:64-95) without the dump writer: halving and
+ * re-insertion of a failed step; *n_steps = steps run including inserted halves */
+int lpm_solve_problem_static(lpm_ctx *ctx, const lpm_load_step *steps, int32_t n_steps_in,
+                             int32_t undamaged_type, int32_t *n_steps);
+/* SolverFatigue::solveProblemStep (solver_fatigue.h:277-298) */
+int lpm_solve_step_fatigue(lpm_ctx *ctx, const lpm_load_step *step, int32_t *converged);
+/* SolverFatigue::solveProblemStatic (solver_fatigue.h:243-275) without the dump writer */
+int lpm_solve_problem_fatigue_static(lpm_ctx *ctx, const lpm_load_step *steps, int32_t n_steps_in,
+                                     int32_t *n_steps);
+/* SolverFatigue::solveProblemCyclic + generateLoadCycle + solveProblemOneCycle
+ * (solver_fatigue.h:128-241) with the load templates of load_step_fatigue.h:22-85.
+ * groups: the template's particle groups (3, or 2 for LPM_LOAD_DOGBONE_DISP).
+ * max_cycles_run > 0 stops after that many simulated cycles. *cycles = final N. */
+int lpm_solve_problem_cyclic(lpm_ctx *ctx, int32_t load_type, int32_t n_interval, const double *spectrum,
+                             int64_t n_spectrum, const int32_t *groups, int32_t n_groups, int32_t time_mode,
+                             double tau, int32_t undamaged_type, int32_t max_cycles_run, int32_t *cycles);
+
+/* ------------------------------------------------------------------ statistics / measurement
+ * Counters since the last reset: "newton_iters","cg_iters","cg_solves","assemblies",
+ * "spmv_launches"; "spmv_ms" (sum of SpMV kernel time, when timing is on);
+ * "last_cg_iters","last_cg_error". */
+double lpm_stat(const lpm_ctx *ctx, const char *name);
+int lpm_reset_stats(lpm_ctx *ctx);
+/* 1: time every SpMV launch with HIP events on the context stream */
+int lpm_set_timing(lpm_ctx *ctx, int on);
+/* Run the CG SpMV kernel `reps` times on the stored matrix (x = current p vector) and return
+ * the mean kernel time in milliseconds (HIP events on the context stream). */
+int lpm_spmv_bench(lpm_ctx *ctx, int32_t reps, double *ms);
+/* Stream handle (hipStream_t) of the context, for callers that order their own work. */
+void *lpm_stream(lpm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LPM_MI355X_H */

@@ -1,0 +1,904 @@
+// Context, state transfer, boundary conditions and the solver drivers of the C ABI
+// (include/lpm_mi355x.h). The drivers are host code that sequences device phases exactly
+// as the reference's Solver / SolverStatic / SolverFatigue methods sequence theirs; every
+// floating-point operation on particle or bond data happens on the device.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lpm_internal.h"
+
+static std::string g_create_error;
+
+int lpm_fail(lpm_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+namespace {
+
+// ---------------------------------------------------------------- BC kernels
+// Solver::updateDisplacementBC (solver.h:130-166) for one DispBC
+__global__ void k_disp_bc(int64_t m, const int32_t *__restrict__ ids, int serial, int mode, int axis, double step,
+                          double *__restrict__ xyz, int32_t *__restrict__ cons) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lo = serial ? 0 : t, hi = serial ? (t == 0 ? m : 0) : (t < m ? t + 1 : t);
+    for (int64_t q = lo; q < hi; ++q) {
+        const int64_t i = ids[q];
+        double d[3] = {0, 0, 0};
+        d[axis] = step;
+        if (mode == LPM_RELATIVE) {
+            xyz[3 * i] = xyz[3 * i] + d[0];
+            xyz[3 * i + 1] = xyz[3 * i + 1] + d[1];
+            xyz[3 * i + 2] = xyz[3 * i + 2] + d[2];
+        } else {  // Absolute: moveTo((step,0,0)) etc., solver.h:142-143
+            xyz[3 * i] = d[0];
+            xyz[3 * i + 1] = d[1];
+            xyz[3 * i + 2] = d[2];
+        }
+        cons[3 * i + axis] = 1;
+    }
+}
+
+// Solver::updateForceBC (solver.h:168-190) for one ForceBC
+__global__ void k_force_bc(int64_t m, const int32_t *__restrict__ ids, int serial, int mode, double fx, double fy,
+                           double fz, double *__restrict__ Pex) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lo = serial ? 0 : t, hi = serial ? (t == 0 ? m : 0) : (t < m ? t + 1 : t);
+    const int num = (int)m;
+    for (int64_t q = lo; q < hi; ++q) {
+        const int64_t i = ids[q];
+        if (mode == LPM_RELATIVE) {
+            Pex[3 * i] += fx / num;
+            Pex[3 * i + 1] += fy / num;
+            Pex[3 * i + 2] += fz / num;
+        } else {
+            Pex[3 * i] = fx / num;
+            Pex[3 * i + 1] = fy / num;
+            Pex[3 * i + 2] = fz / num;
+        }
+    }
+}
+
+struct Field {
+    void *p;
+    size_t es;
+    int64_t count;
+    bool isint;
+};
+
+Field field(lpm_ctx *c, const std::string &w) {
+    const int64_t n = c->n, nb = c->nbond, nd = c->n * c->dim;
+    const size_t D = sizeof(double), I = sizeof(int32_t);
+    if (w == "xyz") return {c->xyz, D, 3 * n, false};
+    if (w == "xyz_initial") return {c->xyz0, D, 3 * n, false};
+    if (w == "xyz_last") return {c->xyz_last, D, 3 * n, false};
+    if (w == "Pin") return {c->Pin, D, 3 * n, false};
+    if (w == "Pex") return {c->Pex, D, 3 * n, false};
+    if (w == "stress") return {c->stress, D, 6 * n, false};
+    if (w == "damage") return {c->damage, D, n, false};
+    if (w == "damage_last") return {c->damage_last, D, n, false};
+    if (w == "damage_visual") return {c->damage_visual, D, n, false};
+    if (w == "Ddot_local") return {c->Ddot_local, D, n, false};
+    if (w == "Ddot_nonlocal") return {c->Ddot_nonlocal, D, n, false};
+    if (w == "nonlocal_L") return {c->nonlocal_L, D, n, false};
+    if (w == "state_var") return {c->sv, D, LPM_NSV * n, false};
+    if (w == "state_var_last") return {c->sv_last, D, LPM_NSV * n, false};
+    if (w == "mat") return {c->mat, D, LPM_NMAT * n, false};
+    if (w == "dLe_total") return {c->dLe_total, D, 2 * n, false};
+    if (w == "TdLe_total") return {c->TdLe_total, D, 2 * n, false};
+    if (w == "cs_sum") return {c->cs_sum, D, 6 * n, false};
+    if (w == "type") return {c->type, I, n, true};
+    if (w == "disp_constraint") return {c->cons, I, 3 * n, true};
+    if (w == "nb") return {c->nbarr, I, n, true};
+    if (w == "dis_initial") return {c->dis0, D, nb, false};
+    if (w == "dis") return {c->dis, D, nb, false};
+    if (w == "dis_last") return {c->dis_last, D, nb, false};
+    if (w == "dLe") return {c->dLe, D, nb, false};
+    if (w == "dLp") return {c->dLp, D, nb, false};
+    if (w == "dLp_last") return {c->dLp_last, D, nb, false};
+    if (w == "bstrain") return {c->bstrain, D, nb, false};
+    if (w == "Kn") return {c->Kn, D, nb, false};
+    if (w == "Tv") return {c->Tv, D, nb, false};
+    if (w == "cs") return {c->cs, D, 3 * nb, false};
+    if (w == "bforce") return {c->bforce, D, nb, false};
+    if (w == "bforce_last") return {c->bforce_last, D, nb, false};
+    if (w == "bdamage") return {c->bdamage, D, nb, false};
+    if (w == "bdamage_last") return {c->bdamage_last, D, nb, false};
+    if (w == "bond_ptr") return {c->bptr, I, n + 1, true};
+    if (w == "bond_split") return {c->bsplit, I, n, true};
+    if (w == "bond_p2") return {c->p2, I, nb, true};
+    if (w == "opp") return {c->opp, I, nb, true};
+    if (w == "conn_ptr") return {c->cptr, I, n + 1, true};
+    if (w == "conn_idx") return {c->cidx, I, c->nconn, true};
+    if (w == "nl_ptr") return {c->nlptr, I, c->nlptr ? n + 1 : 0, true};
+    if (w == "nl_idx") return {c->nlidx, I, c->nnl, true};
+    if (w == "residual") return {c->residual, D, nd, false};
+    if (w == "disp") return {c->disp, D, nd, false};
+    return {nullptr, 0, -1, false};
+}
+
+template <typename T>
+int dalloc(lpm_ctx *c, T **p, int64_t count) {
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (count > 0 ? count : 1));
+    if (e != hipSuccess) return lpm_fail(c, LPM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemsetAsync(*p, 0, sizeof(T) * (count > 0 ? count : 1), c->stream);
+    if (e != hipSuccess) return lpm_fail(c, LPM_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return LPM_OK;
+}
+
+int copy_d2d(lpm_ctx *c, void *dst, const void *src, size_t bytes) {
+    LPM_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return LPM_OK;
+}
+
+#define RC(x)                \
+    do {                     \
+        int rc_ = (x);       \
+        if (rc_) return rc_; \
+    } while (0)
+
+int check_group(lpm_ctx *c, int32_t g) {
+    if (g < 0 || g >= (int32_t)c->groups.size()) return lpm_fail(c, LPM_E_ARG, "unknown particle group " + std::to_string(g));
+    return LPM_OK;
+}
+
+int apply_bc(lpm_ctx *c, const lpm_load_step *s) {
+    if (!s) return lpm_fail(c, LPM_E_ARG, "null load step");
+    if ((s->n_force > 0 && !s->force) || (s->n_disp > 0 && !s->disp) || s->n_force < 0 || s->n_disp < 0)
+        return lpm_fail(c, LPM_E_ARG, "inconsistent load step");
+    for (int k = 0; k < s->n_force; ++k) {  // updateForceBC first (solver_static.h:103)
+        const lpm_force_bc &f = s->force[k];
+        RC(check_group(c, f.group));
+        const Group &g = c->groups[f.group];
+        if (g.n == 0) continue;
+        const int blocks = g.dup ? 1 : blocks_for(g.n, 256);
+        hipLaunchKernelGGL(k_force_bc, dim3(blocks), dim3(256), 0, c->stream, g.n, g.ids, g.dup ? 1 : 0, f.mode, f.fx, f.fy,
+                           f.fz, c->Pex);
+        LPM_HIP(c, hipGetLastError());
+    }
+    for (int k = 0; k < s->n_disp; ++k) {
+        const lpm_disp_bc &d = s->disp[k];
+        RC(check_group(c, d.group));
+        const int axis = d.flag == 'x' ? 0 : (d.flag == 'y' ? 1 : (d.flag == 'z' ? 2 : -1));
+        if (axis < 0) continue;  // the reference ignores other flags
+        const Group &g = c->groups[d.group];
+        if (g.n == 0) continue;
+        const int blocks = g.dup ? 1 : blocks_for(g.n, 256);
+        hipLaunchKernelGGL(k_disp_bc, dim3(blocks), dim3(256), 0, c->stream, g.n, g.ids, g.dup ? 1 : 0, d.mode, axis, d.step,
+                           c->xyz, c->cons);
+        LPM_HIP(c, hipGetLastError());
+    }
+    return LPM_OK;
+}
+
+int geometry_force_state(lpm_ctx *c) {
+    RC(launch_geometry_force(c, true));
+    return launch_particle_force(c);
+}
+
+int calc_stiffness(lpm_ctx *c) {
+    RC(cg_alloc(c));
+    if (c->cg_transpose) RC(launch_tslot(c));
+    RC(launch_assemble(c));
+    c->k_ready = true;
+    c->st_asm++;
+    return LPM_OK;
+}
+
+int update_stiffness_disp_bc(lpm_ctx *c) {
+    if (!c->k_ready) return lpm_fail(c, LPM_E_STATE, "updateStiffnessDispBC before calcStiffness");
+    return launch_dirichlet(c);
+}
+
+int solve_linear_system(lpm_ctx *c, int *iters, double *error) {
+    int it = 0, ok = 0;
+    double err = 0;
+    RC(cg_solve(c, &it, &err, &ok));
+    c->st_cg_iters += it;
+    c->st_cg_solves++;
+    c->last_cg_iters = it;
+    c->last_cg_err = err;
+    if (c->verbose) {
+        if (!ok) printf("Solving failed.\n");
+        else printf("The system has been solved after %d iterations.\nThe estimated error is %g\n", it, err);
+    }
+    RC(launch_move(c));
+    if (iters) *iters = it;
+    if (error) *error = err;
+    return LPM_OK;
+}
+
+// Solver::NewtonIteration (solver.h:53-89)
+int newton_iteration(lpm_ctx *c, int *n_iter) {
+    double r2 = 0, R2 = 0;
+    RC(geometry_force_state(c));
+    RC(launch_residual(c, &r2, &R2));
+    double norm_residual = std::sqrt(r2);
+    const double tol_multiplier = std::max(norm_residual, std::sqrt(R2));
+    if (c->verbose)
+        printf("|  Norm of residual is %.5e, norm of reaction is %.5e, tolerance criterion is based on %s force\n",
+               norm_residual, std::sqrt(R2), norm_residual > std::sqrt(R2) ? "residual" : "reaction");
+    int ni = 0;
+    while (norm_residual > c->tol_NR * tol_multiplier) {
+        if (++ni > c->max_NR_iter) {
+            *n_iter = c->max_NR_iter;
+            return LPM_OK;
+        }
+        if (c->verbose) printf("|  |  Iteration-%d: ", ni);
+        RC(solve_linear_system(c, nullptr, nullptr));
+        RC(geometry_force_state(c));
+        RC(launch_residual(c, &r2, nullptr));
+        norm_residual = std::sqrt(r2);
+        c->st_newton++;
+        if (c->verbose)
+            printf("|  |  Norm of residual is %.3e, residual ratio is %.3e\n", norm_residual, norm_residual / tol_multiplier);
+    }
+    *n_iter = ni;
+    return LPM_OK;
+}
+
+int store_state(lpm_ctx *c) {
+    const size_t n = c->n, nb = c->nbond, D = sizeof(double);
+    RC(copy_d2d(c, c->xyz_last, c->xyz, 3 * n * D));
+    RC(copy_d2d(c, c->damage_last, c->damage, n * D));
+    RC(copy_d2d(c, c->sv_last, c->sv, LPM_NSV * n * D));
+    RC(copy_d2d(c, c->dis_last, c->dis, nb * D));
+    RC(copy_d2d(c, c->bdamage_last, c->bdamage, nb * D));
+    RC(copy_d2d(c, c->bforce_last, c->bforce, nb * D));
+    RC(copy_d2d(c, c->dLp_last, c->dLp, nb * D));
+    return LPM_OK;
+}
+
+int reset_state(lpm_ctx *c, bool reset_xyz) {
+    const size_t n = c->n, nb = c->nbond, D = sizeof(double);
+    if (reset_xyz) RC(copy_d2d(c, c->xyz, c->xyz_last, 3 * n * D));
+    RC(copy_d2d(c, c->damage, c->damage_last, n * D));
+    RC(copy_d2d(c, c->sv, c->sv_last, LPM_NSV * n * D));
+    RC(copy_d2d(c, c->dis, c->dis_last, nb * D));
+    RC(copy_d2d(c, c->bdamage, c->bdamage_last, nb * D));
+    RC(copy_d2d(c, c->bforce, c->bforce_last, nb * D));
+    RC(copy_d2d(c, c->dLp, c->dLp_last, nb * D));
+    return LPM_OK;
+}
+
+// SolverStatic::solveProblemStep (solver_static.h:97-149)
+int solve_step_static(lpm_ctx *c, const lpm_load_step *s, int undamaged, int *converged) {
+    RC(apply_bc(c, s));
+    int n_newton = 0, any = 0;
+    bool new_damaged = false;
+    do {
+        RC(calc_stiffness(c));
+        if (c->dirichlet_static) RC(update_stiffness_disp_bc(c));
+        RC(newton_iteration(c, &n_newton));
+        if (n_newton >= c->max_NR_iter) break;
+        RC(launch_state_var(c));
+        RC(launch_static_damage(c, undamaged, &any));
+        new_damaged = any != 0;
+        RC(store_state(c));
+        if (new_damaged) RC(geometry_force_state(c));
+    } while (new_damaged);
+    *converged = n_newton < c->max_NR_iter;
+    return LPM_OK;
+}
+
+// SolverFatigue::solveProblemStep (solver_fatigue.h:277-298)
+int solve_step_fatigue(lpm_ctx *c, const lpm_load_step *s, int *converged) {
+    RC(apply_bc(c, s));
+    RC(calc_stiffness(c));
+    RC(update_stiffness_disp_bc(c));
+    int n_newton = 0;
+    RC(newton_iteration(c, &n_newton));
+    *converged = n_newton < c->max_NR_iter;
+    return LPM_OK;
+}
+
+// owning copy of a load step, so that halving does not touch the caller's arrays
+struct OwnedStep {
+    std::vector<lpm_disp_bc> disp;
+    std::vector<lpm_force_bc> force;
+    explicit OwnedStep(const lpm_load_step &s) : disp(s.disp, s.disp + s.n_disp), force(s.force, s.force + s.n_force) {}
+    lpm_load_step view() const { return {(int32_t)disp.size(), disp.data(), (int32_t)force.size(), force.data()}; }
+    // LoadStep::loadCutHalf (load_step.h:21-28)
+    void cut_half() {
+        for (auto &d : disp) d.step /= 2.0;
+        for (auto &f : force) { f.fx /= 2.0; f.fy /= 2.0; f.fz /= 2.0; }
+    }
+};
+
+int check_steps(lpm_ctx *c, const lpm_load_step *steps, int32_t n) {
+    if (n < 0 || (n > 0 && !steps)) return lpm_fail(c, LPM_E_ARG, "bad load step list");
+    for (int32_t k = 0; k < n; ++k)
+        if ((steps[k].n_disp > 0 && !steps[k].disp) || (steps[k].n_force > 0 && !steps[k].force))
+            return lpm_fail(c, LPM_E_ARG, "inconsistent load step");
+    return LPM_OK;
+}
+
+// solveProblem / solveProblemStatic loop with halving (solver_static.h:64-95; solver_fatigue.h:243-275)
+int solve_problem(lpm_ctx *c, const lpm_load_step *steps, int32_t n_in, int undamaged, bool fatigue, int32_t *n_out) {
+    RC(check_steps(c, steps, n_in));
+    std::vector<OwnedStep> load;
+    for (int32_t k = 0; k < n_in; ++k) load.emplace_back(steps[k]);
+    std::vector<double> pex;
+    int n_step = (int)load.size();
+    for (int i = 0; i < n_step; i++) {
+        for (;;) {
+            if (c->rollback_pex) {
+                pex.resize(3 * c->n);
+                LPM_HIP(c, hipMemcpyAsync(pex.data(), c->Pex, sizeof(double) * 3 * c->n, hipMemcpyDeviceToHost, c->stream));
+                LPM_HIP(c, hipStreamSynchronize(c->stream));
+            }
+            int ok = 0;
+            lpm_load_step v = load[i].view();
+            if (fatigue) RC(solve_step_fatigue(c, &v, &ok));
+            else RC(solve_step_static(c, &v, undamaged, &ok));
+            if (ok) break;
+            RC(reset_state(c, true));
+            if (c->rollback_pex)
+                LPM_HIP(c, hipMemcpyAsync(c->Pex, pex.data(), sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
+            load[i].cut_half();
+            load.insert(load.begin() + i, load[i]);
+            ++n_step;
+            if (c->verbose) printf("Step-%d not converging\n\n", i);
+        }
+        if (fatigue) {
+            RC(launch_state_var(c));
+            RC(store_state(c));
+        }
+    }
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    if (n_out) *n_out = n_step;
+    return LPM_OK;
+}
+
+// load templates of load_step_fatigue.h:22-85
+OwnedStep fatigue_template(int type, double v, const int32_t *g) {
+    lpm_load_step empty{0, nullptr, 0, nullptr};
+    OwnedStep s(empty);
+    auto D = [&](int32_t grp, int flag, double step) { s.disp.push_back({grp, LPM_RELATIVE, flag, step}); };
+    auto F = [&](int32_t grp, double fy) { s.force.push_back({grp, LPM_RELATIVE, 0.0, fy, 0.0}); };
+    if (type == LPM_LOAD_UNIAXIAL_DISP) { D(g[0], 'x', 0.0); D(g[1], 'y', v); D(g[2], 'y', 0.0); }
+    else if (type == LPM_LOAD_UNIAXIAL_FORCE) { D(g[0], 'x', 0.0); F(g[1], v); D(g[2], 'y', 0.0); }
+    else if (type == LPM_LOAD_DOGBONE_DISP) {
+        D(g[0], 'x', 0); D(g[0], 'y', v); D(g[0], 'z', 0); D(g[1], 'x', 0.0); D(g[1], 'y', 0.0); D(g[1], 'z', 0.0);
+    } else { D(g[0], 'y', 0); D(g[1], 'x', 0.0); D(g[2], 'x', 0.0); F(g[1], v); F(g[2], -v); }
+    return s;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char *lpm_last_error(const lpm_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int lpm_create(const lpm_topology *t, lpm_ctx **out) {
+    if (!t || !out) return lpm_fail(nullptr, LPM_E_ARG, "null argument");
+    *out = nullptr;
+    if (t->n <= 0 || !t->xyz || !t->bond_ptr || !t->bond_split || (!t->bond_p2 && t->bond_ptr[t->n] > 0))
+        return lpm_fail(nullptr, LPM_E_ARG, "empty or incomplete topology");
+    if (t->model < 0 || t->model > 3) return lpm_fail(nullptr, LPM_E_ARG, "unknown particle model");
+    if (t->n > 0x7ffffffeLL) return lpm_fail(nullptr, LPM_E_LIMIT, "too many particles");
+    double uc[8];
+    if (lpm_unit_cell(t->lattice, t->radius, 0, uc, nullptr) != LPM_OK) return lpm_fail(nullptr, LPM_E_ARG, "unknown lattice");
+    const int64_t n = t->n;
+    const int32_t *bp = t->bond_ptr, *bs = t->bond_split, *p2 = t->bond_p2;
+    // ---- host validation of the topology (every index a kernel will dereference)
+    if (bp[0] != 0) return lpm_fail(nullptr, LPM_E_TOPO, "bond_ptr[0] must be 0");
+    int maxb = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (bp[i + 1] < bp[i] || bs[i] < bp[i] || bs[i] > bp[i + 1])
+            return lpm_fail(nullptr, LPM_E_TOPO, "bond_ptr / bond_split not monotone at particle " + std::to_string(i));
+        maxb = std::max(maxb, bp[i + 1] - bp[i]);
+    }
+    if (maxb > LPM_MAXB) return lpm_fail(nullptr, LPM_E_LIMIT, "a particle has more than 32 bonds");
+    const int64_t nbond = bp[n];
+    std::vector<int32_t> opp(nbond, -1);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int b = bp[i]; b < bp[i + 1]; ++b) {
+            const int64_t j = p2[b];
+            if (j < 0 || j >= n || j == i) return lpm_fail(nullptr, LPM_E_TOPO, "bond partner out of range or self bond");
+            const bool l0 = b < bs[i];
+            for (int b2 = (l0 ? bp[i] : bs[i]); b2 < b; ++b2)
+                if (p2[b2] == j) return lpm_fail(nullptr, LPM_E_TOPO, "duplicate bond in one layer");
+            const int lo = l0 ? bp[j] : bs[j], hi = l0 ? bs[j] : bp[j + 1];
+            for (int q = lo; q < hi; ++q)
+                if (p2[q] == i) { opp[b] = q; break; }
+            if (opp[b] < 0)
+                return lpm_fail(nullptr, LPM_E_TOPO, "bond " + std::to_string(i) + "->" + std::to_string(j) + " has no reverse bond");
+        }
+    }
+    lpm_ctx *c = new lpm_ctx();
+    c->device = t->device;
+    c->lattice = t->lattice;
+    c->radius = t->radius;
+    c->model = t->model;
+    c->dim = (int)uc[0];
+    c->nneighbors = (int)uc[1];
+    c->cutoff[0] = uc[5];
+    c->cutoff[1] = uc[6];
+    c->volume = uc[7];
+    c->n = n;
+    c->nbond = nbond;
+    c->maxbond = maxb;
+    c->vpad = c->dim == 3 ? 4 : 2;
+    c->precision = 0;
+    c->cg_tol = 1e-6;
+    auto fail = [&](int rc) {
+        std::string m = c->err;
+        lpm_destroy(c);
+        return lpm_fail(nullptr, rc, m);
+    };
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) { c->err = hipGetErrorString(e); return fail(LPM_E_HIP); }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { c->err = hipGetErrorString(e); return fail(LPM_E_HIP); }
+    int rc = 0;
+#define AL(ptr, cnt) if ((rc = dalloc(c, &c->ptr, (cnt)))) return fail(rc)
+    AL(bptr, n + 1); AL(bsplit, n); AL(p2, nbond); AL(opp, nbond); AL(nbarr, n);
+    AL(xyz, 3 * n); AL(xyz0, 3 * n); AL(xyz_last, 3 * n); AL(Pin, 3 * n); AL(Pex, 3 * n); AL(stress, 6 * n);
+    AL(damage, n); AL(damage_last, n); AL(damage_visual, n); AL(Ddot_local, n); AL(Ddot_nonlocal, n); AL(nonlocal_L, n);
+    AL(sv, LPM_NSV * n); AL(sv_last, LPM_NSV * n); AL(mat, LPM_NMAT * n);
+    AL(dLe_total, 2 * n); AL(TdLe_total, 2 * n); AL(cs_sum, 6 * n); AL(type, n); AL(cons, 3 * n);
+    AL(dis0, nbond); AL(dis, nbond); AL(dis_last, nbond); AL(dLe, nbond); AL(dLp, nbond); AL(dLp_last, nbond);
+    AL(bstrain, nbond); AL(Kn, nbond); AL(Tv, nbond); AL(cs, 3 * nbond); AL(bforce, nbond); AL(bforce_last, nbond);
+    AL(bdamage, nbond); AL(bdamage_last, nbond);
+    AL(residual, c->dim * n); AL(disp, c->dim * n); AL(diag64, c->dim * n);
+    AL(scal, SC_COUNT); AL(counter, 8); AL(iflag, 8);
+#undef AL
+    c->partial_cap = std::max({blocks_for(n * c->dim, 256), blocks_for(n, 4), blocks_for(n, 256), 1});
+    if ((rc = dalloc(c, &c->partial, 3 * (int64_t)c->partial_cap))) return fail(rc);
+    if (hipHostMalloc((void **)&c->h_scal, sizeof(double) * SC_COUNT) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_iflag, sizeof(int32_t) * 8) != hipSuccess) {
+        c->err = "hipHostMalloc failed";
+        return fail(LPM_E_NOMEM);
+    }
+    // ---- initial bond geometry: Bond(p1, p2, layer, dis) (bond.h:61-73)
+    std::vector<double> dis0(nbond), cs(3 * nbond), sv(LPM_NSV * n, 0.0), mat(LPM_NMAT * n, 0.0);
+    std::vector<int32_t> nb(n);
+    const double *x = t->xyz;
+    for (int64_t i = 0; i < n; ++i) {
+        nb[i] = bp[i + 1] - bp[i];
+        for (int b = bp[i]; b < bp[i + 1]; ++b) {
+            const double *a = &x[3 * i], *q = &x[3 * (int64_t)p2[b]];
+            const double d = std::sqrt(std::pow((a[0] - q[0]), 2) + std::pow((a[1] - q[1]), 2) + std::pow((a[2] - q[2]), 2));
+            dis0[b] = d;
+            cs[3 * b] = (a[0] - q[0]) / d;
+            cs[3 * b + 1] = (a[1] - q[1]) / d;
+            cs[3 * b + 2] = (a[2] - q[2]) / d;
+        }
+    }
+    if (c->model == LPM_FATIGUE_HCF)  // particle_fatigue_hcf.h:24-25, 229-236
+        for (int64_t i = 0; i < n; ++i) {
+            sv[LPM_NSV * i + 1] = (double)INT_MAX;
+            mat[LPM_NMAT * i + 5] = 1.0;
+            mat[LPM_NMAT * i + 6] = 1.108;
+        }
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
+    };
+    bool ok = up(c->bptr, bp, sizeof(int32_t) * (n + 1)) && up(c->bsplit, bs, sizeof(int32_t) * n) &&
+              up(c->p2, p2, sizeof(int32_t) * nbond) && up(c->opp, opp.data(), sizeof(int32_t) * nbond) &&
+              up(c->nbarr, nb.data(), sizeof(int32_t) * n) && up(c->xyz, x, sizeof(double) * 3 * n) &&
+              up(c->xyz0, x, sizeof(double) * 3 * n) && up(c->xyz_last, x, sizeof(double) * 3 * n) &&
+              up(c->dis0, dis0.data(), sizeof(double) * nbond) && up(c->dis, dis0.data(), sizeof(double) * nbond) &&
+              up(c->dis_last, dis0.data(), sizeof(double) * nbond) && up(c->cs, cs.data(), sizeof(double) * 3 * nbond) &&
+              up(c->sv, sv.data(), sizeof(double) * LPM_NSV * n) && up(c->sv_last, sv.data(), sizeof(double) * LPM_NSV * n) &&
+              up(c->mat, mat.data(), sizeof(double) * LPM_NMAT * n);
+    if (!ok) { c->err = "topology upload failed"; return fail(LPM_E_HIP); }
+    if ((rc = launch_pattern(c))) return fail(rc);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { c->err = "sync failed"; return fail(LPM_E_HIP); }
+    *out = c;
+    return LPM_OK;
+}
+
+int lpm_destroy(lpm_ctx *c) {
+    if (!c) return LPM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->bptr, c->bsplit, c->p2, c->opp, c->nbarr, c->cptr, c->cidx, c->tslot, c->nlptr, c->nlidx, c->xyz,
+                    c->xyz0, c->xyz_last, c->Pin, c->Pex, c->stress, c->damage, c->damage_last, c->damage_visual,
+                    c->Ddot_local, c->Ddot_nonlocal, c->nonlocal_L, c->sv, c->sv_last, c->mat, c->dLe_total, c->TdLe_total,
+                    c->cs_sum, c->type, c->cons, c->dis0, c->dis, c->dis_last, c->dLe, c->dLp, c->dLp_last, c->bstrain, c->Kn,
+                    c->Tv, c->cs, c->bforce, c->bforce_last, c->bdamage, c->bdamage_last, c->residual, c->disp, c->diag64,
+                    c->aval, c->invdiag, c->v_x, c->v_r, c->v_z, c->v_p, c->v_t, c->scal, c->partial, c->counter, c->iflag};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (auto &g : c->groups)
+        if (g.ids) hipFree(g.ids);
+    if (c->h_scal) hipHostFree(c->h_scal);
+    if (c->h_iflag) hipHostFree(c->h_iflag);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return LPM_OK;
+}
+
+int64_t lpm_size(const lpm_ctx *c, const char *what) {
+    if (!c || !what) return -1;
+    const std::string w(what);
+    if (w == "n") return c->n;
+    if (w == "dim") return c->dim;
+    if (w == "nbonds") return c->nbond;
+    if (w == "nconn") return c->nconn;
+    if (w == "nnl") return c->nnl;
+    if (w == "ndof") return c->n * c->dim;
+    if (w == "knnz") return c->nconn * c->dim * c->dim;
+    if (w == "maxconn") return c->maxconn;
+    if (w == "maxbond") return c->maxbond;
+    if (w == "nl_ptr") return c->nlptr ? c->n + 1 : 0;
+    return -1;
+}
+
+int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
+    if (!c || !name) return LPM_E_ARG;
+    const std::string w(name);
+    if (w == "diag10_literal") c->diag10_literal = (int)v;
+    else if (w == "cg_transpose") c->cg_transpose = (int)v;
+    else if (w == "dirichlet_static") c->dirichlet_static = (int)v;
+    else if (w == "rollback_pex") c->rollback_pex = (int)v;
+    else if (w == "precision") {
+        if ((int)v != c->precision) c->k_ready = false;
+        c->precision = (int)v ? 1 : 0;
+    } else if (w == "cg_tol") c->cg_tol = v;
+    else if (w == "cg_maxit") c->cg_maxit = (int64_t)v;
+    else if (w == "precond") c->precond = (int)v ? 1 : 0;
+    else if (w == "max_NR_iter") c->max_NR_iter = (int)v;
+    else if (w == "tol_NR") c->tol_NR = v;
+    else if (w == "verbose") c->verbose = (int)v;
+    else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
+    else return lpm_fail(c, LPM_E_ARG, "unknown mode " + w);
+    return LPM_OK;
+}
+
+double lpm_get_mode(const lpm_ctx *c, const char *name) {
+    if (!c || !name) return NAN;
+    const std::string w(name);
+    if (w == "diag10_literal") return c->diag10_literal;
+    if (w == "cg_transpose") return c->cg_transpose;
+    if (w == "dirichlet_static") return c->dirichlet_static;
+    if (w == "rollback_pex") return c->rollback_pex;
+    if (w == "precision") return c->precision;
+    if (w == "cg_tol") return c->cg_tol;
+    if (w == "cg_maxit") return (double)c->cg_maxit;
+    if (w == "precond") return c->precond;
+    if (w == "max_NR_iter") return c->max_NR_iter;
+    if (w == "tol_NR") return c->tol_NR;
+    if (w == "verbose") return c->verbose;
+    if (w == "cg_batch") return c->cg_batch;
+    return NAN;
+}
+
+int lpm_upload(lpm_ctx *c, const char *name, const void *host, int64_t count) {
+    if (!c || !name || (!host && count > 0)) return LPM_E_ARG;
+    Field f = field(c, name);
+    if (!f.p && f.count < 0) return lpm_fail(c, LPM_E_ARG, std::string("unknown field ") + name);
+    const std::string w(name);
+    if (w == "bond_ptr" || w == "bond_split" || w == "bond_p2" || w == "opp" || w == "conn_ptr" || w == "conn_idx" ||
+        w == "nl_ptr" || w == "nl_idx" || w == "nb")
+        return lpm_fail(c, LPM_E_ARG, std::string("topology field is read-only: ") + name);
+    if (count != f.count) return lpm_fail(c, LPM_E_ARG, std::string("size mismatch for ") + name);
+    if (count == 0) return LPM_OK;
+    LPM_HIP(c, hipMemcpyAsync(f.p, host, f.es * count, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    if (w == "disp_constraint" || w == "mat") c->k_ready = c->k_ready;  // no derived data to invalidate
+    return LPM_OK;
+}
+
+int lpm_download(lpm_ctx *c, const char *name, void *host, int64_t count) {
+    if (!c || !name || (!host && count > 0)) return LPM_E_ARG;
+    const std::string w(name);
+    if (w == "reaction") {  // Solver::reaction_force (solver.h:119-127): Pin at constrained DoFs in (particle, axis) order
+        std::vector<double> P(3 * c->n);
+        std::vector<int32_t> cons(3 * c->n);
+        LPM_HIP(c, hipMemcpyAsync(P.data(), c->Pin, sizeof(double) * 3 * c->n, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipMemcpyAsync(cons.data(), c->cons, sizeof(int32_t) * 3 * c->n, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+        std::vector<double> R;
+        for (int64_t i = 0; i < c->n; ++i)
+            for (int k = 0; k < c->dim; ++k)
+                if (cons[3 * i + k] == 1) R.push_back(P[3 * i + k]);
+        if (host == nullptr) return (int)R.size();
+        if (count != (int64_t)R.size()) return lpm_fail(c, LPM_E_ARG, "size mismatch for reaction");
+        std::memcpy(host, R.data(), sizeof(double) * R.size());
+        return LPM_OK;
+    }
+    Field f = field(c, name);
+    if (!f.p && f.count < 0) return lpm_fail(c, LPM_E_ARG, std::string("unknown field ") + name);
+    if (count != f.count) return lpm_fail(c, LPM_E_ARG, std::string("size mismatch for ") + name);
+    if (count == 0) return LPM_OK;
+    LPM_HIP(c, hipMemcpyAsync(host, f.p, f.es * count, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_download_matrix(lpm_ctx *c, int32_t *rowptr, int32_t *col, double *val, int64_t cap) {
+    if (!c || !rowptr || !col || !val) return LPM_E_ARG;
+    if (!c->k_ready) return lpm_fail(c, LPM_E_STATE, "no assembled matrix");
+    const int D = c->dim;
+    const int64_t nnz = c->nconn * D * D;
+    if (cap < nnz) return lpm_fail(c, LPM_E_ARG, "matrix buffer too small");
+    std::vector<int32_t> cp(c->n + 1), ci(c->nconn);
+    LPM_HIP(c, hipMemcpyAsync(cp.data(), c->cptr, sizeof(int32_t) * (c->n + 1), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(ci.data(), c->cidx, sizeof(int32_t) * c->nconn, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> a(nnz);
+    if (c->precision == 0) {
+        std::vector<float> af(nnz);
+        LPM_HIP(c, hipMemcpyAsync(af.data(), c->aval, sizeof(float) * nnz, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+        for (int64_t k = 0; k < nnz; ++k) a[k] = af[k];
+    } else {
+        LPM_HIP(c, hipMemcpyAsync(a.data(), c->aval, sizeof(double) * nnz, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    int64_t o = 0;
+    for (int64_t i = 0; i < c->n; ++i) {
+        const int c0 = cp[i], nc = cp[i + 1] - c0;
+        for (int r = 0; r < D; ++r) {
+            rowptr[D * i + r] = (int32_t)o;
+            for (int s = 0; s < nc; ++s)
+                for (int q = 0; q < D; ++q) {
+                    col[o] = ci[c0 + s] * D + q;
+                    val[o] = a[(int64_t)D * D * c0 + (int64_t)(r * D + q) * nc + s];
+                    ++o;
+                }
+        }
+    }
+    rowptr[D * c->n] = (int32_t)o;
+    return LPM_OK;
+}
+
+int lpm_set_material(lpm_ctx *c, int64_t id, double L, const double *Ce, int literal, const double *params, int32_t np) {
+    if (!c || !Ce || np < 0 || np > LPM_NMAT || (np > 0 && !params)) return LPM_E_ARG;
+    if (id >= c->n) return lpm_fail(c, LPM_E_ARG, "particle id out of range");
+    double map[9], uc[8];
+    lpm_unit_cell(c->lattice, c->radius, literal, uc, map);
+    // KnTv = el_mapping * Ce (x radius in 3D), particle_elastic.h:90-93, accumulated like Eigen's GEMV
+    double KnTv[3];
+    for (int r = 0; r < 3; ++r) {
+        double s = map[3 * r] * Ce[0];
+        s = s + map[3 * r + 1] * Ce[1];
+        s = s + map[3 * r + 2] * Ce[2];
+        KnTv[r] = c->dim == 2 ? s : s * c->radius;
+    }
+    const int64_t i0 = id < 0 ? 0 : id, i1 = id < 0 ? c->n : id + 1;
+    std::vector<int32_t> bp(i1 - i0 + 1), bs(i1 - i0);
+    LPM_HIP(c, hipMemcpyAsync(bp.data(), c->bptr + i0, sizeof(int32_t) * (i1 - i0 + 1), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(bs.data(), c->bsplit + i0, sizeof(int32_t) * (i1 - i0), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t b0 = bp[0], b1 = bp[i1 - i0];
+    std::vector<double> kn(b1 - b0), tv(b1 - b0), mat(LPM_NMAT * (i1 - i0), 0.0), Ls(i1 - i0, L);
+    for (int64_t i = i0; i < i1; ++i) {
+        for (int64_t b = bp[i - i0]; b < bp[i - i0 + 1]; ++b) {
+            const int layer = b < bs[i - i0] ? 0 : 1;
+            if (c->lattice == LPM_HEXAGON2D) { kn[b - b0] = KnTv[0]; tv[b - b0] = KnTv[1]; }
+            else { kn[b - b0] = KnTv[layer]; tv[b - b0] = KnTv[2]; }
+        }
+        double *m = &mat[LPM_NMAT * (i - i0)];
+        if (c->model == LPM_FATIGUE_HCF) { m[5] = 1.0; m[6] = 1.108; }
+        for (int k = 0; k < np; ++k) m[k] = params[k];
+    }
+    if (b1 > b0) {
+        LPM_HIP(c, hipMemcpyAsync(c->Kn + b0, kn.data(), sizeof(double) * (b1 - b0), hipMemcpyHostToDevice, c->stream));
+        LPM_HIP(c, hipMemcpyAsync(c->Tv + b0, tv.data(), sizeof(double) * (b1 - b0), hipMemcpyHostToDevice, c->stream));
+    }
+    LPM_HIP(c, hipMemcpyAsync(c->mat + LPM_NMAT * i0, mat.data(), sizeof(double) * mat.size(), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->nonlocal_L + i0, Ls.data(), sizeof(double) * Ls.size(), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_search_nonlocal(lpm_ctx *c, double ratio) {
+    if (!c) return LPM_E_ARG;
+    return launch_nonlocal_search(c, ratio);
+}
+
+int lpm_group_create(lpm_ctx *c, int64_t m, const int32_t *ids, int32_t *gid) {
+    if (!c || !gid || m < 0 || (m > 0 && !ids)) return LPM_E_ARG;
+    std::vector<int32_t> v(ids, ids + m);
+    for (int32_t i : v)
+        if (i < 0 || i >= c->n) return lpm_fail(c, LPM_E_ARG, "group id out of range");
+    std::vector<int32_t> s(v);
+    std::sort(s.begin(), s.end());
+    Group g;
+    g.n = m;
+    g.dup = std::adjacent_find(s.begin(), s.end()) != s.end();
+    LPM_HIP(c, hipMalloc(&g.ids, sizeof(int32_t) * (m > 0 ? m : 1)));
+    if (m > 0) LPM_HIP(c, hipMemcpyAsync(g.ids, v.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    c->groups.push_back(g);
+    *gid = (int32_t)c->groups.size() - 1;
+    return LPM_OK;
+}
+
+int lpm_apply_bc(lpm_ctx *c, const lpm_load_step *s) {
+    if (!c) return LPM_E_ARG;
+    RC(apply_bc(c, s));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_update_geometry(lpm_ctx *c) {
+    if (!c) return LPM_E_ARG;
+    return launch_geometry_force(c, false);
+}
+
+int lpm_update_force_state(lpm_ctx *c) {
+    if (!c) return LPM_E_ARG;
+    RC(launch_bond_force(c));
+    return launch_particle_force(c);
+}
+
+int lpm_update_rr(lpm_ctx *c, double *nr, double *nR) {
+    if (!c) return LPM_E_ARG;
+    double r2 = 0, R2 = 0;
+    RC(launch_residual(c, &r2, &R2));
+    if (nr) *nr = std::sqrt(r2);
+    if (nR) *nR = std::sqrt(R2);
+    return LPM_OK;
+}
+
+int lpm_calc_stiffness(lpm_ctx *c) {
+    if (!c) return LPM_E_ARG;
+    return calc_stiffness(c);
+}
+
+int lpm_update_stiffness_disp_bc(lpm_ctx *c) {
+    if (!c) return LPM_E_ARG;
+    return update_stiffness_disp_bc(c);
+}
+
+int lpm_solve_linear_system(lpm_ctx *c, int32_t *iters, double *error) {
+    if (!c) return LPM_E_ARG;
+    int it = 0;
+    RC(solve_linear_system(c, &it, error));
+    if (iters) *iters = it;
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_newton_iteration(lpm_ctx *c, int32_t *n_iter) {
+    if (!c || !n_iter) return LPM_E_ARG;
+    if (!c->k_ready) return lpm_fail(c, LPM_E_STATE, "NewtonIteration before calcStiffness");
+    int ni = 0;
+    RC(newton_iteration(c, &ni));
+    *n_iter = ni;
+    return LPM_OK;
+}
+
+int lpm_update_state_var(lpm_ctx *c) {
+    if (!c) return LPM_E_ARG;
+    return launch_state_var(c);
+}
+
+int lpm_update_static_damage(lpm_ctx *c, int32_t undamaged, int32_t *any) {
+    if (!c) return LPM_E_ARG;
+    int a = 0;
+    RC(launch_static_damage(c, undamaged, &a));
+    if (any) *any = a;
+    return LPM_OK;
+}
+
+int lpm_update_fatigue_damage(lpm_ctx *c, double dNdt, int32_t undamaged, int32_t *any) {
+    if (!c) return LPM_E_ARG;
+    int a = 0;
+    RC(launch_fatigue_damage(c, dNdt, undamaged, &a));
+    if (any) *any = a;
+    return LPM_OK;
+}
+
+int lpm_store_state(lpm_ctx *c) {
+    if (!c) return LPM_E_ARG;
+    return store_state(c);
+}
+
+int lpm_reset_state(lpm_ctx *c, int32_t reset_xyz) {
+    if (!c) return LPM_E_ARG;
+    return reset_state(c, reset_xyz != 0);
+}
+
+int lpm_solve_step_static(lpm_ctx *c, const lpm_load_step *s, int32_t undamaged, int32_t *converged) {
+    if (!c || !converged) return LPM_E_ARG;
+    RC(check_steps(c, s, 1));
+    int ok = 0;
+    RC(solve_step_static(c, s, undamaged, &ok));
+    *converged = ok;
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_solve_problem_static(lpm_ctx *c, const lpm_load_step *steps, int32_t n_in, int32_t undamaged, int32_t *n_steps) {
+    if (!c) return LPM_E_ARG;
+    return solve_problem(c, steps, n_in, undamaged, false, n_steps);
+}
+
+int lpm_solve_step_fatigue(lpm_ctx *c, const lpm_load_step *s, int32_t *converged) {
+    if (!c || !converged) return LPM_E_ARG;
+    RC(check_steps(c, s, 1));
+    int ok = 0;
+    RC(solve_step_fatigue(c, s, &ok));
+    *converged = ok;
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_solve_problem_fatigue_static(lpm_ctx *c, const lpm_load_step *steps, int32_t n_in, int32_t *n_steps) {
+    if (!c) return LPM_E_ARG;
+    return solve_problem(c, steps, n_in, -1, true, n_steps);
+}
+
+int lpm_solve_problem_cyclic(lpm_ctx *c, int32_t load_type, int32_t n_interval, const double *spec, int64_t nspec,
+                             const int32_t *groups, int32_t ngroups, int32_t tmode, double tau, int32_t undamaged,
+                             int32_t max_cycles_run, int32_t *cycles) {
+    if (!c || !spec || !groups || n_interval <= 0 || tau <= 0) return LPM_E_ARG;
+    const int need = load_type == LPM_LOAD_DOGBONE_DISP ? 2 : 3;
+    if (ngroups < need || load_type < 0 || load_type > 3) return lpm_fail(c, LPM_E_ARG, "load template needs more groups");
+    for (int k = 0; k < need; ++k) RC(check_group(c, groups[k]));
+    const double n_cycles = 0.5 * (nspec - 1);
+    double t = 0, dNdt = 1;
+    int N = 0, ran = 0;
+    do {
+        if (2 * (int64_t)N + 2 >= nspec) return lpm_fail(c, LPM_E_ARG, "load spectrum too short for the cycle count");
+        const double f0 = spec[2 * N], f1 = spec[2 * N + 1], f2 = spec[2 * N + 2];
+        std::vector<OwnedStep> cyc;
+        for (int k = 0; k < n_interval; ++k) cyc.push_back(fatigue_template(load_type, (f1 - f0) / n_interval, groups));
+        for (int k = 0; k < n_interval; ++k) cyc.push_back(fatigue_template(load_type, (f2 - f1) / n_interval, groups));
+        dNdt = (tmode == LPM_TIME_LINEAR) ? (int)(1 / tau) : (int)(1 / tau * std::exp(t++ / tau));
+        // SolverFatigue::solveProblemOneCycle (solver_fatigue.h:220-241)
+        for (auto &s : cyc) {
+            int ok = 0;
+            lpm_load_step v = s.view();
+            RC(solve_step_fatigue(c, &v, &ok));
+            RC(launch_state_var(c));
+        }
+        int any = 0;
+        RC(launch_fatigue_damage(c, dNdt, undamaged, &any));
+        RC(store_state(c));
+        RC(geometry_force_state(c));
+        N += (int)dNdt;
+        ++ran;
+        if (max_cycles_run > 0 && ran >= max_cycles_run) break;
+    } while (N < n_cycles);
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    if (cycles) *cycles = N;
+    return LPM_OK;
+}
+
+double lpm_stat(const lpm_ctx *c, const char *name) {
+    if (!c || !name) return NAN;
+    const std::string w(name);
+    if (w == "newton_iters") return (double)c->st_newton;
+    if (w == "cg_iters") return (double)c->st_cg_iters;
+    if (w == "cg_solves") return (double)c->st_cg_solves;
+    if (w == "assemblies") return (double)c->st_asm;
+    if (w == "spmv_launches") return (double)c->st_spmv;
+    if (w == "spmv_ms") return c->st_spmv_ms;
+    if (w == "last_cg_iters") return c->last_cg_iters;
+    if (w == "last_cg_error") return c->last_cg_err;
+    return NAN;
+}
+
+int lpm_reset_stats(lpm_ctx *c) {
+    if (!c) return LPM_E_ARG;
+    c->st_newton = c->st_cg_iters = c->st_cg_solves = c->st_asm = c->st_spmv = 0;
+    c->st_spmv_ms = 0;
+    return LPM_OK;
+}
+
+int lpm_set_timing(lpm_ctx *c, int on) {
+    if (!c) return LPM_E_ARG;
+    c->timing = on ? 1 : 0;
+    return LPM_OK;
+}
+
+int lpm_spmv_bench(lpm_ctx *c, int32_t reps, double *ms) {
+    if (!c || !ms) return LPM_E_ARG;
+    return spmv_bench(c, reps, ms);
+}
+
+void *lpm_stream(lpm_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+}  // extern "C"

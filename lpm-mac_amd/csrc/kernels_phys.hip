@@ -1,0 +1,923 @@
+// Per-bond / per-particle physics kernels of the Newton load step (FP64) and the
+// block-CSR stiffness assembly. This translation unit is compiled with
+// -ffp-contract=off: every kernel performs the reference's floating-point
+// operations in the reference's order, so FP64 results are reproducible
+// against a CPU restatement.
+//
+// Data layout in HBM (all structure-of-arrays):
+//   particles  xyz[3i+k], Pin[3i+k], Pex[3i+k], stress[6i+k], state_var[7i+k], mat[10i+k],
+//              dLe_total/TdLe_total[2i+layer], cs_sum[6i+3*layer+k]
+//   bonds      CSR by particle: bonds of i are [bptr[i], bptr[i+1]), layer-0 bonds first,
+//              layer 1 from bsplit[i]; per bond p2, opp (reverse bond), dis0, dis, dLe, dLp,
+//              bstrain, Kn, Tv, cs[3b+k], bforce, bdamage
+//   pattern    conns CSR (cptr, cidx) = block rows of the stiffness matrix
+//   matrix     BSR with (dim x dim) blocks; the values of block row i are stored entry-major
+//              ("row-SoA"): value e (= r*dim + s) of slot c lives at
+//              aval[dim*dim*cptr[i] + e*nconn_i + c], so the lanes of a wavefront that each own
+//              one block read consecutive addresses.
+#include <climits>
+
+#include "lpm_internal.h"
+
+namespace {
+
+constexpr int ELASTIC = 0, ELASTIC_DAMAGE = 1, J2 = 2, HCF = 3;
+
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+
+// ---------------------------------------------------------------- reductions
+// Sum NV doubles over the block; the block that finishes last sums the per-block
+// partials in a fixed order (deterministic for a given grid) and returns true with the
+// totals in out[] (valid in every thread of that block).
+template <int NV>
+__device__ bool grid_reduce(double (&v)[NV], double *partial, int cap, unsigned int *counter, double (&out)[NV]) {
+    __shared__ double red[NV][8];
+    __shared__ bool last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < NV; ++k) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) red[k][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; ++k) {
+            double s = 0;
+            for (int w = 0; w < nw; ++w) s += red[k][w];
+            partial[k * cap + blockIdx.x] = s;
+        }
+        __threadfence();
+        unsigned int t = atomicAdd(counter, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    for (int k = 0; k < NV; ++k) {
+        double s = 0;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) s += __builtin_nontemporal_load(&partial[k * cap + b]);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        __syncthreads();
+        if (lane == 0) red[k][wid] = s;
+        __syncthreads();
+        double t = 0;
+        for (int w = 0; w < nw; ++w) t += red[k][w];
+        out[k] = t;
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+    return true;
+}
+
+// ---------------------------------------------------------------- geometry + bond force
+// Particle::updateBondsGeometry + Bond::updatebGeometry (particle.h:163-181, bond.h:30-39),
+// then updateBondsForce of the model (particle_elastic.h:38-50, particle_elastic_damage.h:74-87,
+// particle_j2plasticity.h:63-76, particle_fatigue_hcf.h:65-80). One thread per particle; the
+// per-layer sums run in bond order.
+__global__ __launch_bounds__(256) void k_geometry_force(
+    int64_t n, int model, int do_force, const int32_t *__restrict__ bptr, const int32_t *__restrict__ bsplit,
+    const int32_t *__restrict__ p2, const double *__restrict__ xyz, const double *__restrict__ dis0,
+    const double *__restrict__ dLp, const double *__restrict__ Kn, const double *__restrict__ Tv,
+    const double *__restrict__ damage, double *__restrict__ dis, double *__restrict__ dLe,
+    double *__restrict__ bstrain, double *__restrict__ cs, double *__restrict__ dLe_total,
+    double *__restrict__ TdLe_total, double *__restrict__ cs_sum, double *__restrict__ bforce,
+    double *__restrict__ bdamage) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2];
+    const int b0 = bptr[i], bm = bsplit[i], b1 = bptr[i + 1];
+    double tot[2];
+    for (int l = 0; l < 2; ++l) {
+        const int lo = l == 0 ? b0 : bm, hi = l == 0 ? bm : b1;
+        double dt = 0, tdt = 0, sx = 0, sy = 0, sz = 0;
+        for (int b = lo; b < hi; ++b) {
+            const int64_t j = p2[b];
+            const double dx = x0 - xyz[3 * j], dy = y0 - xyz[3 * j + 1], dz = z0 - xyz[3 * j + 2];
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            const bool broken = fabs(bdamage[b] - 1.0) < LPM_EPS;
+            const double d0 = dis0[b];
+            const double dl = d - d0;
+            const double bs = broken ? 0.0 : (dl / d);
+            const double le = broken ? 0.0 : (d - d0 - dLp[b]);
+            const double cx = broken ? 0.0 : (dx / d), cy = broken ? 0.0 : (dy / d), cz = broken ? 0.0 : (dz / d);
+            dis[b] = d;
+            bstrain[b] = bs;
+            dLe[b] = le;
+            cs[3 * b] = cx;
+            cs[3 * b + 1] = cy;
+            cs[3 * b + 2] = cz;
+            dt += le;
+            tdt += Tv[b] * le;
+            sx += cx;
+            sy += cy;
+            sz += cz;
+        }
+        dLe_total[2 * i + l] = dt;
+        TdLe_total[2 * i + l] = tdt;
+        cs_sum[6 * i + 3 * l] = sx;
+        cs_sum[6 * i + 3 * l + 1] = sy;
+        cs_sum[6 * i + 3 * l + 2] = sz;
+        tot[l] = dt;
+    }
+    if (!do_force) return;
+    const double Di = damage[i];
+    for (int b = b0; b < b1; ++b) {
+        double f = 2. * Kn[b] * dLe[b] + 2. * Tv[b] * tot[b < bm ? 0 : 1];
+        double bd = bdamage[b];
+        if (model != ELASTIC) {
+            bd = dmax(bd, dmax(Di, damage[p2[b]]));
+            bdamage[b] = bd;
+        }
+        f *= (1.0 - bd);
+        bforce[b] = f;
+    }
+}
+
+// Bond forces only (geometry already current); used after a damage update.
+__global__ __launch_bounds__(256) void k_bond_force(int64_t n, int model, const int32_t *__restrict__ bptr,
+                                                     const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2,
+                                                     const double *__restrict__ Kn, const double *__restrict__ Tv,
+                                                     const double *__restrict__ dLe, const double *__restrict__ dLe_total,
+                                                     const double *__restrict__ damage, double *__restrict__ bforce,
+                                                     double *__restrict__ bdamage) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b0 = bptr[i], bm = bsplit[i], b1 = bptr[i + 1];
+    const double Di = damage[i];
+    for (int b = b0; b < b1; ++b) {
+        double f = 2. * Kn[b] * dLe[b] + 2. * Tv[b] * dLe_total[2 * i + (b < bm ? 0 : 1)];
+        double bd = bdamage[b];
+        if (model != ELASTIC) {
+            bd = dmax(bd, dmax(Di, damage[p2[b]]));
+            bdamage[b] = bd;
+        }
+        f *= (1.0 - bd);
+        bforce[b] = f;
+    }
+}
+
+// Particle::updateParticleForce / updateParticleStress / updateParticleDamageVisual
+// (particle.h:150-161, 197-240); the reverse bond comes from the precomputed opp[].
+__global__ __launch_bounds__(256) void k_particle_force(int64_t n, double volume, int nneighbors,
+                                                         const int32_t *__restrict__ bptr, const int32_t *__restrict__ opp,
+                                                         const double *__restrict__ cs, const double *__restrict__ dis,
+                                                         const double *__restrict__ bforce, const double *__restrict__ bdamage,
+                                                         double *__restrict__ Pin, double *__restrict__ stress,
+                                                         double *__restrict__ damage_visual) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b0 = bptr[i], b1 = bptr[i + 1];
+    const int nb = b1 - b0;
+    const double V_m = volume * nb / nneighbors;
+    double P0 = 0., P1 = 0., P2 = 0.;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, dv = 0;
+    for (int b = b0; b < b1; ++b) {
+        const double ff = bforce[b] + bforce[opp[b]];
+        const double cx = cs[3 * b], cy = cs[3 * b + 1], cz = cs[3 * b + 2];
+        P0 += cx * 0.5 * ff;
+        P1 += cy * 0.5 * ff;
+        P2 += cz * 0.5 * ff;
+        const double w = 0.5 / V_m * dis[b] * 0.5 * ff;
+        s0 += w * cx * cx;
+        s1 += w * cy * cy;
+        s2 += w * cz * cz;
+        s3 += w * cy * cz;
+        s4 += w * cx * cz;
+        s5 += w * cx * cy;
+        dv += bdamage[b];
+    }
+    Pin[3 * i] = P0;
+    Pin[3 * i + 1] = P1;
+    Pin[3 * i + 2] = P2;
+    stress[6 * i] = s0;
+    stress[6 * i + 1] = s1;
+    stress[6 * i + 2] = s2;
+    stress[6 * i + 3] = s3;
+    stress[6 * i + 4] = s4;
+    stress[6 * i + 5] = s5;
+    damage_visual[i] = dv / (double)nb;
+}
+
+// Solver::updateRR (solver.h:108-128) with the squared norms of residual and reaction.
+__global__ __launch_bounds__(256) void k_residual(int64_t n, int dim, const int32_t *__restrict__ cons,
+                                                   const double *__restrict__ Pex, const double *__restrict__ Pin,
+                                                   double *__restrict__ residual, double *partial, int cap,
+                                                   unsigned int *counter, double *scal) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (d < n * dim) {
+        const int64_t i = d / dim;
+        const int k = (int)(d - i * dim);
+        const int c = cons[3 * i + k];
+        const double r = (1 - c) * (Pex[3 * i + k] - Pin[3 * i + k]);
+        residual[d] = r;
+        v[0] = r * r;
+        if (c == 1) v[1] = Pin[3 * i + k] * Pin[3 * i + k];
+    }
+    double out[2];
+    if (grid_reduce<2>(v, partial, cap, counter, out) && threadIdx.x == 0) {
+        scal[SC_NORM_R] = out[0];
+        scal[SC_NORM_REAC] = out[1];
+    }
+}
+
+// ---------------------------------------------------------------- stiffness assembly
+// Stiffness::calcStiffness with localStiffnessANA (stiffness.h:67-101, 228-263) and
+// fdu2dxyz / fdu2dxyz1 / fdu2dxyz2 (derivative.h:7-239). One wavefront per particle row i,
+// one lane per block (i, j in conns(i)). Every lane of the wave walks the same bonds of i and
+// of i's neighbours (uniform, broadcast loads) and keeps only the terms that land on its own
+// j, in the reference's loop order, so each FP64 block equals the reference's block bit for
+// bit before it is rounded to the storage precision.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_assemble(
+    int64_t n, int diag10_literal, int transpose, const int32_t *__restrict__ bptr,
+    const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2, const int32_t *__restrict__ cptr,
+    const int32_t *__restrict__ cidx, const int32_t *__restrict__ tslot, const double *__restrict__ cs,
+    const double *__restrict__ cs_sum, const double *__restrict__ Kn, const double *__restrict__ Tv,
+    const double *__restrict__ bdamage, T *__restrict__ aval, double *__restrict__ diag64) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const int c0 = cptr[i], nci = cptr[i + 1] - c0;
+    const int ib0 = bptr[i], ibm = bsplit[i], ib1 = bptr[i + 1];
+    for (int slot = lane; slot < nci; slot += 64) {
+        const int j = cidx[c0 + slot];
+        double K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (j == i) {
+            for (int l = 0; l < 2; ++l) {
+                const int lo = l == 0 ? ib0 : ibm, hi = l == 0 ? ibm : ib1;
+                const double S0 = cs_sum[6 * i + 3 * l], S1 = cs_sum[6 * i + 3 * l + 1], S2 = cs_sum[6 * i + 3 * l + 2];
+                for (int b = lo; b < hi; ++b) {
+                    const double w = (1 - bdamage[b]), kn = Kn[b], tv = Tv[b];
+                    const double c0_ = cs[3 * b], c1 = cs[3 * b + 1], c2 = cs[3 * b + 2];
+                    K[0][0] += w * ((kn * c0_ + tv * S0) * c0_ + (kn + tv) * (c0_ * c0_));
+                    K[0][1] += w * ((kn * c1 + tv * S1) * c0_ + (kn + tv) * c1 * c0_);
+                    K[1][1] += w * ((kn * c1 + tv * S1) * c1 + (kn + tv) * (c1 * c1));
+                    if (D == 3) {
+                        K[0][2] += w * ((kn * c2 + tv * S2) * c0_ + (kn + tv) * c2 * c0_);
+                        K[1][2] += w * ((kn * c2 + tv * S2) * c1 + (kn + tv) * c2 * c1);
+                        K[2][2] += w * ((kn * c2 + tv * S2) * c2 + (kn + tv) * (c2 * c2));
+                    }
+                }
+            }
+            if (!diag10_literal) K[1][0] = K[0][1];
+            K[2][0] = K[0][2];
+            K[2][1] = K[1][2];
+            for (int r = 0; r < D; ++r) diag64[(int64_t)D * i + r] = K[r][r];
+        } else {
+            double Kl[2][3][3];
+            bool flag[2];
+            for (int l = 0; l < 2; ++l) {
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) Kl[l][r][q] = 0.0;
+                const int lo = l == 0 ? ib0 : ibm, hi = l == 0 ? ibm : ib1;
+                int bj = -1;  // first bond i->j of this layer
+                for (int b = lo; b < hi; ++b)
+                    if (bj < 0 && p2[b] == j) bj = b;
+                bool touched = bj >= 0;
+                if (bj >= 0) {  // derivative.h:53-119
+                    const double n1[3] = {cs[3 * bj], cs[3 * bj + 1], cs[3 * bj + 2]};
+                    const double wT1 = (1 - bdamage[bj]) * Tv[bj];
+                    const double S[3] = {cs_sum[6 * j + 3 * l], cs_sum[6 * j + 3 * l + 1], cs_sum[6 * j + 3 * l + 2]};
+                    for (int b2 = lo; b2 < hi; ++b2) {
+                        const int k = p2[b2];
+                        const double n2[3] = {cs[3 * b2], cs[3 * b2 + 1], cs[3 * b2 + 2]};
+                        const double wT2 = (1 - bdamage[b2]) * Tv[b2];
+                        if (k == j) {
+                            const double w = (1 - bdamage[b2]), kn = Kn[b2], kt = Kn[b2] + Tv[b2], tv = Tv[b2];
+                            for (int r = 0; r < D; ++r)
+                                for (int q = 0; q < D; ++q) {
+                                    const double pr = (r == q) ? kt * (n2[r] * n2[r])
+                                                               : kt * n2[r > q ? r : q] * n2[r > q ? q : r];
+                                    Kl[l][r][q] += w * ((-kn * n2[q] + tv * S[q]) * n2[r] - pr);
+                                }
+                        } else {
+                            const double a = 0.5 * (wT1 + wT2);
+                            for (int r = 0; r < D; ++r)
+                                for (int q = 0; q < D; ++q) Kl[l][r][q] -= a * n2[r] * n1[q];
+                            const int klo = l == 0 ? bptr[k] : bsplit[k], khi = l == 0 ? bsplit[k] : bptr[k + 1];
+                            for (int b3 = klo; b3 < khi; ++b3) {
+                                if (p2[b3] == j) {
+                                    const double n3[3] = {cs[3 * b3], cs[3 * b3 + 1], cs[3 * b3 + 2]};
+                                    const double a3 = 0.5 * ((1 - bdamage[b3]) * Tv[b3] + wT2);
+                                    for (int r = 0; r < D; ++r)
+                                        for (int q = 0; q < D; ++q) Kl[l][r][q] -= a3 * n2[r] * n3[q];
+                                }
+                            }
+                        }
+                    }
+                } else {  // derivative.h:122-139
+                    for (int b1 = lo; b1 < hi; ++b1) {
+                        const int k = p2[b1];
+                        const int klo = l == 0 ? bptr[k] : bsplit[k], khi = l == 0 ? bsplit[k] : bptr[k + 1];
+                        for (int b2 = klo; b2 < khi; ++b2) {
+                            if (p2[b2] == j) {
+                                touched = true;
+                                const double n1[3] = {cs[3 * b1], cs[3 * b1 + 1], cs[3 * b1 + 2]};
+                                const double n2[3] = {cs[3 * b2], cs[3 * b2 + 1], cs[3 * b2 + 2]};
+                                const double a = 0.5 * ((1 - bdamage[b2]) * Tv[b2] + (1 - bdamage[b1]) * Tv[b1]);
+                                for (int r = 0; r < D; ++r)
+                                    for (int q = 0; q < D; ++q) Kl[l][r][q] -= a * n1[r] * n2[q];
+                            }
+                        }
+                    }
+                }
+                flag[l] = touched;
+            }
+            for (int r = 0; r < D; ++r)
+                for (int q = 0; q < D; ++q) {
+                    if (flag[0] && flag[1]) K[r][q] = Kl[0][r][q] + Kl[1][r][q];
+                    else if (flag[0]) K[r][q] = Kl[0][r][q];
+                    else if (flag[1]) K[r][q] = Kl[1][r][q];
+                }
+        }
+        if (!transpose) {
+            T *dst = aval + (int64_t)D * D * c0;
+            for (int r = 0; r < D; ++r)
+                for (int q = 0; q < D; ++q) dst[(int64_t)(r * D + q) * nci + slot] = (T)K[r][q];
+        } else {
+            const int cj = cptr[j], ncj = cptr[j + 1] - cj, ts = tslot[c0 + slot];
+            T *dst = aval + (int64_t)D * D * cj;
+            for (int r = 0; r < D; ++r)
+                for (int q = 0; q < D; ++q) dst[(int64_t)(r * D + q) * ncj + ts] = (T)K[q][r];
+        }
+    }
+}
+
+// squared norm of diag(K), Stiffness::updateStiffnessDispBC (stiffness.h:269-270)
+__global__ __launch_bounds__(256) void k_diag_norm(int64_t nd, const double *__restrict__ diag64, double *partial,
+                                                    int cap, unsigned int *counter, double *scal) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {d < nd ? diag64[d] * diag64[d] : 0.0};
+    double out[1];
+    if (grid_reduce<1>(v, partial, cap, counter, out) && threadIdx.x == 0) scal[SC_DIAGNORM] = out[0];
+}
+
+__device__ __forceinline__ bool eliminated(int64_t p, int k, const int32_t *cons, const double *damage_visual) {
+    return cons[3 * p + k] == 1 || fabs(damage_visual[p] - 1.0) < LPM_EPS;
+}
+
+// Stiffness::updateStiffnessDispBC (stiffness.h:272-293): zero the row and the column of every
+// constrained (or fully damaged) DoF and put ||diag K||_2 on its diagonal.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_dirichlet(int64_t n, const int32_t *__restrict__ cptr,
+                                                    const int32_t *__restrict__ cidx, const int32_t *__restrict__ cons,
+                                                    const double *__restrict__ damage_visual, const double *scal,
+                                                    T *__restrict__ aval) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const int c0 = cptr[i], nci = cptr[i + 1] - c0;
+    const T nd = (T)sqrt(scal[SC_DIAGNORM]);
+    bool ei[3];
+    for (int r = 0; r < D; ++r) ei[r] = eliminated(i, r, cons, damage_visual);
+    for (int slot = lane; slot < nci; slot += 64) {
+        const int64_t j = cidx[c0 + slot];
+        T *dst = aval + (int64_t)D * D * c0;
+        for (int q = 0; q < D; ++q) {
+            const bool ej = eliminated(j, q, cons, damage_visual);
+            for (int r = 0; r < D; ++r) {
+                if (ei[r] || ej) {
+                    T v = (T)0;
+                    if (ei[r] && j == i && r == q) v = nd;
+                    dst[(int64_t)(r * D + q) * nci + slot] = v;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- constitutive updates
+// calcStrainEnergyDensity (particle_elastic_damage.h:89-104); equals the energy_total that
+// ParticleFatigueHCF::calcEqEnergy returns (particle_fatigue_hcf.h:82-125).
+__device__ double strain_energy(int64_t i, double volume, int nneighbors, const int32_t *bptr, const int32_t *bsplit,
+                                const double *Kn, const double *dLe, const double *dLe_total,
+                                const double *TdLe_total) {
+    const int nb = bptr[i + 1] - bptr[i];
+    const double V_m = volume * nb / nneighbors;
+    double e = 0;
+    for (int l = 0; l < 2; ++l) {
+        const int lo = l == 0 ? bptr[i] : bsplit[i], hi = l == 0 ? bsplit[i] : bptr[i + 1];
+        for (int b = lo; b < hi; ++b) e += 0.5 * (Kn[b] * dLe[b] * dLe[b]) / V_m;
+        e += 0.5 * TdLe_total[2 * i + l] * dLe_total[2 * i + l] / V_m;
+    }
+    return e;
+}
+
+// updateParticleStateVariables: ElasticDamage (particle_elastic_damage.h:150-161),
+// J2Plasticity (particle_j2plasticity.h:78-140), FatigueHCF (particle_fatigue_hcf.h:127-145)
+__global__ __launch_bounds__(256) void k_state_var(int64_t n, int model, double volume, int nneighbors,
+                                                    const int32_t *__restrict__ bptr, const int32_t *__restrict__ bsplit,
+                                                    const double *__restrict__ Kn, const double *__restrict__ dLe,
+                                                    const double *__restrict__ dLe_total,
+                                                    const double *__restrict__ TdLe_total,
+                                                    const double *__restrict__ stress, const double *__restrict__ cs,
+                                                    const double *__restrict__ dis, const double *__restrict__ mat,
+                                                    const double *__restrict__ damage, const double *__restrict__ sv_last,
+                                                    double *__restrict__ sv, double *__restrict__ dLp,
+                                                    double *__restrict__ Ddot_local) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *m = mat + LPM_NMAT * i;
+    double *s = sv + LPM_NSV * i;
+    if (model == ELASTIC_DAMAGE) {
+        const double k0 = m[0], k1 = m[1];
+        s[0] = strain_energy(i, volume, nneighbors, bptr, bsplit, Kn, dLe, dLe_total, TdLe_total);
+        const double delta = s[0] - sv_last[LPM_NSV * i];
+        const double f = s[0] - k1 * damage[i] - k0;
+        Ddot_local[i] = (f > 0 && delta > 0) ? (1 / k1) * delta : 0;
+    } else if (model == J2) {
+        const double E = m[0], mu = m[1], sigmay = m[2], xi = m[3], H = m[4], A = m[5];
+        double st[6], dpl[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 6; ++k) st[k] = stress[6 * i + k];
+        const double sigma_m = 1.0 / 3.0 * (st[0] + st[1] + st[2]);
+        for (int k = 0; k < 3; ++k) st[k] -= sigma_m;
+        for (int k = 0; k < 6; ++k) st[k] -= s[k + 1];
+        double sigma_eq = 0.0;
+        for (int k = 0; k < 6; ++k) {
+            if (k < 3) sigma_eq += st[k] * st[k];
+            else sigma_eq += 2.0 * st[k] * st[k];
+        }
+        sigma_eq = sqrt(3.0 / 2.0 * sigma_eq);
+        const double yield_func = sigma_eq - (sigmay + (1.0 - xi) * H * s[0]);
+        const double G = E / 2 / (1 + mu);
+        double dlambda = 0.0;
+        if (yield_func > 0.0) dlambda = yield_func / (3 * G + H);
+        s[0] += dlambda;
+        for (int k = 0; k < 6; ++k) {
+            if (fabs(sigma_eq) > LPM_EPS) {
+                dpl[k] = dlambda * 1.5 * st[k] / sigma_eq;
+                s[k + 1] += 2. / 3. * xi * H * dpl[k];
+            }
+        }
+        for (int b = bptr[i]; b < bptr[i + 1]; ++b) {
+            const double cx = cs[3 * b], cy = cs[3 * b + 1], cz = cs[3 * b + 2];
+            const double ddLp = dis[b] * (dpl[0] * cx * cx + dpl[1] * cy * cy + dpl[2] * cz * cz + 2 * dpl[3] * cy * cz +
+                                          2 * dpl[4] * cx * cz + 2 * dpl[5] * cx * cy);
+            dLp[b] += ddLp;
+        }
+        double triax = 0.0;
+        if (sigma_eq > LPM_EPS) triax = sigma_m / sigma_eq;
+        Ddot_local[i] = dlambda * (1.0 + A * triax);
+    } else if (model == HCF) {
+        const double A = m[0], B = m[1], dd = m[2];
+        const double curr = strain_energy(i, volume, nneighbors, bptr, bsplit, Kn, dLe, dLe_total, TdLe_total);
+        s[1] = dmin(curr, s[1]);
+        s[2] = dmax(curr, s[2]);
+        s[0] = dmax(s[2], s[0]);
+        Ddot_local[i] = (s[2] - s[1] > 0) ? A * pow(s[0], B) * pow(s[2] - s[1], dd) : 0;
+    }
+}
+
+// nonlocal damage-rate average (solver_static.h:34-52; solver_fatigue.h:78-114), with
+// func_phi (lpm.h:45-49)
+__global__ __launch_bounds__(256) void k_nonlocal_ddot(int64_t n, int undamaged, double volume, int nneighbors,
+                                                        const int32_t *__restrict__ type, const int32_t *__restrict__ nlptr,
+                                                        const int32_t *__restrict__ nlidx, const int32_t *__restrict__ bptr,
+                                                        const double *__restrict__ xyz, const double *__restrict__ L,
+                                                        const double *__restrict__ Ddot_local,
+                                                        double *__restrict__ Ddot_nonlocal) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || type[i] == undamaged) return;
+    const double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2], Li = L[i];
+    double A = 0, dn = 0;
+    const int lo = nlptr ? nlptr[i] : 0, hi = nlptr ? nlptr[i + 1] : 0;
+    for (int q = lo; q < hi; ++q) {
+        const int64_t j = nlidx[q];
+        if (type[j] == undamaged) continue;
+        const double dx = x0 - xyz[3 * j], dy = y0 - xyz[3 * j + 1], dz = z0 - xyz[3 * j + 2];
+        const double dis = sqrt(dx * dx + dy * dy + dz * dz);
+        const double V_m = volume * (bptr[j + 1] - bptr[j]) / nneighbors;
+        const double u = 1 / (1 + pow(dis / Li, 8.0));
+        const double phi = u * u;
+        dn += Ddot_local[j] * phi * V_m;
+        A += phi * V_m;
+    }
+    Ddot_nonlocal[i] = dn / A;
+}
+
+__device__ __forceinline__ void set_flag(bool f, int32_t *flag) {
+    // one store per wavefront that saw the event
+    const unsigned long long m = __ballot(f);
+    if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicOr(flag, 1);
+}
+
+// updateParticleStaticDamage: Elastic (particle_elastic.h:52-68), ElasticDamage
+// (particle_elastic_damage.h:163-173), J2Plasticity (particle_j2plasticity.h:142-152)
+__global__ __launch_bounds__(256) void k_static_damage(int64_t n, int model, int undamaged,
+                                                        const int32_t *__restrict__ type, const int32_t *__restrict__ bptr,
+                                                        const double *__restrict__ mat, const double *__restrict__ bstrain,
+                                                        double *__restrict__ bdamage, const double *__restrict__ Ddot_nl,
+                                                        const double *__restrict__ damage_last,
+                                                        double *__restrict__ damage, int32_t *flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool any = false;
+    if (i < n && type[i] != undamaged) {
+        const double *m = mat + LPM_NMAT * i;
+        if (model == ELASTIC) {
+            const double crit = m[0];
+            for (int b = bptr[i]; b < bptr[i + 1]; ++b)
+                if (bstrain[b] >= crit && fabs(bdamage[b] - 1.0) > LPM_EPS) {
+                    any = true;
+                    bdamage[b] = 1;
+                }
+        } else if (model == ELASTIC_DAMAGE || model == J2) {
+            const double thr = model == ELASTIC_DAMAGE ? m[2] : m[6];
+            double D = damage[i] + Ddot_nl[i];
+            if (D >= thr) D = 1;
+            damage[i] = D;
+            any = fabs(D - damage_last[i]) > 1e-3;
+        }
+    }
+    set_flag(any, flag);
+}
+
+// ParticleFatigueHCF::updateParticleFatigueDamage (particle_fatigue_hcf.h:147-160)
+__global__ __launch_bounds__(256) void k_fatigue_damage(int64_t n, int model, int undamaged, double dt,
+                                                         const int32_t *__restrict__ type, const double *__restrict__ mat,
+                                                         const double *__restrict__ Ddot_nl,
+                                                         const double *__restrict__ damage_last,
+                                                         double *__restrict__ damage, double *__restrict__ sv,
+                                                         int32_t *flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool any = false;
+    if (i < n && type[i] != undamaged && model == HCF) {
+        double D = damage[i] + Ddot_nl[i] * dt;
+        if (D >= mat[LPM_NMAT * i + 5]) D = 1;
+        damage[i] = D;
+        sv[LPM_NSV * i + 1] = (double)INT_MAX;
+        sv[LPM_NSV * i + 2] = 0;
+        any = fabs(D - damage_last[i]) > 1e-3;
+    }
+    set_flag(any, flag);
+}
+
+// ---------------------------------------------------------------- pattern (2-hop conns)
+// Assembly::updateConnections (assembly.h:236-266): sorted unique {bond partners and their
+// same-layer partners}, self included. One workgroup per particle: candidates are gathered in
+// LDS, sorted with a bitonic network and de-duplicated.
+__global__ __launch_bounds__(256) void k_pattern(int64_t n, const int32_t *__restrict__ bptr,
+                                                  const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2,
+                                                  int32_t *__restrict__ counts, const int32_t *__restrict__ cptr,
+                                                  int32_t *__restrict__ cidx, int32_t *overflow) {
+    __shared__ int32_t buf[LPM_MAXC];
+    __shared__ int32_t off[LPM_MAXB + 1];
+    __shared__ int32_t scan[LPM_MAXC];
+    const int64_t i = blockIdx.x;
+    const int t = threadIdx.x;
+    const int b0 = bptr[i], bm = bsplit[i], nb = bptr[i + 1] - b0;
+    if (t == 0) {
+        int o = 0;
+        for (int q = 0; q < nb; ++q) {
+            off[q] = o;
+            const int k = p2[b0 + q];
+            const bool l0 = (b0 + q) < bm;
+            o += 1 + (l0 ? bsplit[k] - bptr[k] : bptr[k + 1] - bsplit[k]);
+        }
+        off[nb] = o;
+    }
+    for (int q = t; q < LPM_MAXC; q += blockDim.x) buf[q] = INT_MAX;
+    __syncthreads();
+    const int total = off[nb];
+    if (total > LPM_MAXC) {
+        if (t == 0) atomicOr(overflow, 1);
+        return;
+    }
+    if (t < nb) {
+        const int k = p2[b0 + t];
+        const bool l0 = (b0 + t) < bm;
+        int o = off[t];
+        buf[o++] = k;
+        const int lo = l0 ? bptr[k] : bsplit[k], hi = l0 ? bsplit[k] : bptr[k + 1];
+        for (int b = lo; b < hi; ++b) buf[o++] = p2[b];
+    }
+    __syncthreads();
+    // bitonic sort of LPM_MAXC entries, 2 per thread per stage
+    for (int size = 2; size <= LPM_MAXC; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int q = t; q < LPM_MAXC / 2; q += blockDim.x) {
+                const int pos = 2 * q - (q & (stride - 1));
+                const int a = pos, b = pos + stride;
+                const bool up = ((a & size) == 0);
+                const int va = buf[a], vb = buf[b];
+                if ((va > vb) == up) {
+                    buf[a] = vb;
+                    buf[b] = va;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // unique flags and inclusive scan (Hillis-Steele over LPM_MAXC entries)
+    for (int q = t; q < LPM_MAXC; q += blockDim.x)
+        scan[q] = (buf[q] != INT_MAX && (q == 0 || buf[q] != buf[q - 1])) ? 1 : 0;
+    __syncthreads();
+    for (int s = 1; s < LPM_MAXC; s <<= 1) {
+        int v0 = 0, v1 = 0;
+        const int q0 = t, q1 = t + blockDim.x;
+        if (q0 < LPM_MAXC) v0 = scan[q0] + (q0 >= s ? scan[q0 - s] : 0);
+        if (q1 < LPM_MAXC) v1 = scan[q1] + (q1 >= s ? scan[q1 - s] : 0);
+        __syncthreads();
+        if (q0 < LPM_MAXC) scan[q0] = v0;
+        if (q1 < LPM_MAXC) scan[q1] = v1;
+        __syncthreads();
+    }
+    if (cidx == nullptr) {
+        if (t == 0) counts[i] = scan[LPM_MAXC - 1];
+        return;
+    }
+    const int base = cptr[i];
+    for (int q = t; q < LPM_MAXC; q += blockDim.x) {
+        const bool first = buf[q] != INT_MAX && (q == 0 || buf[q] != buf[q - 1]);
+        if (first) cidx[base + scan[q] - 1] = buf[q];
+    }
+}
+
+// position of block (j, i) in block row j, for the transposed store
+__global__ __launch_bounds__(256) void k_tslot(int64_t nconn, int64_t n, const int32_t *__restrict__ cptr,
+                                                const int32_t *__restrict__ cidx, const int32_t *__restrict__ rowof,
+                                                int32_t *__restrict__ tslot, int32_t *bad) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nconn) return;
+    const int i = rowof[e], j = cidx[e];
+    int lo = cptr[j], hi = cptr[j + 1] - 1, pos = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = cidx[mid];
+        if (v == i) { pos = mid; break; }
+        if (v < i) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    if (pos < 0) {
+        atomicOr(bad, 1);
+        tslot[e] = 0;
+    } else {
+        tslot[e] = pos - cptr[j];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rowof(int64_t n, const int32_t *__restrict__ cptr, int32_t *__restrict__ rowof) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int e = cptr[i]; e < cptr[i + 1]; ++e) rowof[e] = (int32_t)i;
+}
+
+// Assembly::searchNonlocalNeighbors (assembly.h:117-130): all j (self included) with
+// |x_i - x_j| < ratio * L_i, in id order. Positions are staged through LDS tiles.
+__global__ __launch_bounds__(256) void k_nonlocal_search(int64_t n, double ratio, const double *__restrict__ xyz,
+                                                          const double *__restrict__ L, int32_t *__restrict__ counts,
+                                                          const int32_t *__restrict__ nlptr, int32_t *__restrict__ nlidx) {
+    __shared__ double tile[3 * 256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < n;
+    double x0 = 0, y0 = 0, z0 = 0, cut = 0;
+    if (act) { x0 = xyz[3 * i]; y0 = xyz[3 * i + 1]; z0 = xyz[3 * i + 2]; cut = ratio * L[i]; }
+    int cnt = 0;
+    int64_t o = (act && nlidx) ? nlptr[i] : 0;
+    for (int64_t base = 0; base < n; base += 256) {
+        const int64_t j = base + threadIdx.x;
+        __syncthreads();
+        if (j < n) { tile[3 * threadIdx.x] = xyz[3 * j]; tile[3 * threadIdx.x + 1] = xyz[3 * j + 1]; tile[3 * threadIdx.x + 2] = xyz[3 * j + 2]; }
+        __syncthreads();
+        const int m = (int)((n - base) < 256 ? (n - base) : 256);
+        if (act) {
+            for (int q = 0; q < m; ++q) {
+                const double dx = x0 - tile[3 * q], dy = y0 - tile[3 * q + 1], dz = z0 - tile[3 * q + 2];
+                if (sqrt(dx * dx + dy * dy + dz * dz) < cut) {
+                    if (nlidx) nlidx[o++] = (int32_t)(base + q);
+                    ++cnt;
+                }
+            }
+        }
+    }
+    if (act && !nlidx) counts[i] = cnt;
+}
+
+}  // namespace
+
+// =================================================================== launchers
+static int check_launch(lpm_ctx *c, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return lpm_fail(c, LPM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return LPM_OK;
+}
+
+int launch_geometry_force(lpm_ctx *c, bool force) {
+    hipLaunchKernelGGL(k_geometry_force, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->model, force ? 1 : 0,
+                       c->bptr, c->bsplit, c->p2, c->xyz, c->dis0, c->dLp, c->Kn, c->Tv, c->damage, c->dis, c->dLe,
+                       c->bstrain, c->cs, c->dLe_total, c->TdLe_total, c->cs_sum, c->bforce, c->bdamage);
+    return check_launch(c, "k_geometry_force");
+}
+
+int launch_bond_force(lpm_ctx *c) {
+    hipLaunchKernelGGL(k_bond_force, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->model, c->bptr, c->bsplit,
+                       c->p2, c->Kn, c->Tv, c->dLe, c->dLe_total, c->damage, c->bforce, c->bdamage);
+    return check_launch(c, "k_bond_force");
+}
+
+int launch_particle_force(lpm_ctx *c) {
+    hipLaunchKernelGGL(k_particle_force, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->volume,
+                       c->nneighbors, c->bptr, c->opp, c->cs, c->dis, c->bforce, c->bdamage, c->Pin, c->stress,
+                       c->damage_visual);
+    return check_launch(c, "k_particle_force");
+}
+
+int launch_residual(lpm_ctx *c, double *nr2, double *nR2) {
+    const int64_t nd = c->n * c->dim;
+    const int g = blocks_for(nd, 256);
+    if (g > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "residual grid exceeds partial buffer");
+    hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->cons, c->Pex, c->Pin, c->residual,
+                       c->partial, c->partial_cap, c->counter, c->scal);
+    int rc = check_launch(c, "k_residual");
+    if (rc) return rc;
+    if (nr2 || nR2) {
+        LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+        if (nr2) *nr2 = c->h_scal[SC_NORM_R];
+        if (nR2) *nR2 = c->h_scal[SC_NORM_REAC];
+    }
+    return LPM_OK;
+}
+
+int launch_assemble(lpm_ctx *c) {
+    const int g = blocks_for(c->n, 4);
+    if (c->cg_transpose && !c->tslot) return lpm_fail(c, LPM_E_STATE, "transpose map missing");
+    if (c->precision == 0) {
+        float *a = (float *)c->aval;
+        if (c->dim == 3)
+            hipLaunchKernelGGL((k_assemble<float, 3>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
+                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->cs, c->cs_sum, c->Kn, c->Tv,
+                               c->bdamage, a, c->diag64);
+        else
+            hipLaunchKernelGGL((k_assemble<float, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
+                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->cs, c->cs_sum, c->Kn, c->Tv,
+                               c->bdamage, a, c->diag64);
+    } else {
+        double *a = (double *)c->aval;
+        if (c->dim == 3)
+            hipLaunchKernelGGL((k_assemble<double, 3>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
+                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->cs, c->cs_sum, c->Kn, c->Tv,
+                               c->bdamage, a, c->diag64);
+        else
+            hipLaunchKernelGGL((k_assemble<double, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
+                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->cs, c->cs_sum, c->Kn, c->Tv,
+                               c->bdamage, a, c->diag64);
+    }
+    return check_launch(c, "k_assemble");
+}
+
+int launch_dirichlet(lpm_ctx *c) {
+    const int64_t nd = c->n * c->dim;
+    const int g = blocks_for(nd, 256);
+    if (g > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "diag grid exceeds partial buffer");
+    hipLaunchKernelGGL(k_diag_norm, dim3(g), dim3(256), 0, c->stream, nd, c->diag64, c->partial, c->partial_cap, c->counter,
+                       c->scal);
+    int rc = check_launch(c, "k_diag_norm");
+    if (rc) return rc;
+    const int g2 = blocks_for(c->n, 4);
+    if (c->precision == 0) {
+        float *a = (float *)c->aval;
+        if (c->dim == 3)
+            hipLaunchKernelGGL((k_dirichlet<float, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->cons,
+                               c->damage_visual, c->scal, a);
+        else
+            hipLaunchKernelGGL((k_dirichlet<float, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->cons,
+                               c->damage_visual, c->scal, a);
+    } else {
+        double *a = (double *)c->aval;
+        if (c->dim == 3)
+            hipLaunchKernelGGL((k_dirichlet<double, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->cons,
+                               c->damage_visual, c->scal, a);
+        else
+            hipLaunchKernelGGL((k_dirichlet<double, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->cons,
+                               c->damage_visual, c->scal, a);
+    }
+    return check_launch(c, "k_dirichlet");
+}
+
+int launch_state_var(lpm_ctx *c) {
+    if (c->model == 0) return LPM_OK;  // ParticleElastic has no state update (particle.h:62)
+    hipLaunchKernelGGL(k_state_var, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->model, c->volume,
+                       c->nneighbors, c->bptr, c->bsplit, c->Kn, c->dLe, c->dLe_total, c->TdLe_total, c->stress, c->cs,
+                       c->dis, c->mat, c->damage, c->sv_last, c->sv, c->dLp, c->Ddot_local);
+    return check_launch(c, "k_state_var");
+}
+
+int launch_nonlocal_ddot(lpm_ctx *c, int undamaged) {
+    hipLaunchKernelGGL(k_nonlocal_ddot, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, undamaged, c->volume,
+                       c->nneighbors, c->type, c->nlptr, c->nlidx, c->bptr, c->xyz, c->nonlocal_L, c->Ddot_local,
+                       c->Ddot_nonlocal);
+    return check_launch(c, "k_nonlocal_ddot");
+}
+
+static int read_flag(lpm_ctx *c, int *any) {
+    LPM_HIP(c, hipMemcpyAsync(c->h_iflag, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    *any = c->h_iflag[0] != 0;
+    return LPM_OK;
+}
+
+int launch_static_damage(lpm_ctx *c, int undamaged, int *any) {
+    int rc = launch_nonlocal_ddot(c, undamaged);
+    if (rc) return rc;
+    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_static_damage, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->model, undamaged,
+                       c->type, c->bptr, c->mat, c->bstrain, c->bdamage, c->Ddot_nonlocal, c->damage_last, c->damage,
+                       c->iflag);
+    rc = check_launch(c, "k_static_damage");
+    if (rc) return rc;
+    return read_flag(c, any);
+}
+
+int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any) {
+    int rc = launch_nonlocal_ddot(c, undamaged);
+    if (rc) return rc;
+    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_fatigue_damage, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->model, undamaged,
+                       dNdt, c->type, c->mat, c->Ddot_nonlocal, c->damage_last, c->damage, c->sv, c->iflag);
+    rc = check_launch(c, "k_fatigue_damage");
+    if (rc) return rc;
+    return read_flag(c, any);
+}
+
+int launch_pattern(lpm_ctx *c) {
+    int32_t *counts = nullptr;
+    LPM_HIP(c, hipMalloc(&counts, sizeof(int32_t) * (c->n + 1)));
+    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+    if (c->n > 0x7fffffff) return lpm_fail(c, LPM_E_LIMIT, "too many particles for the pattern grid");
+    hipLaunchKernelGGL(k_pattern, dim3((unsigned)c->n), dim3(256), 0, c->stream, c->n, c->bptr, c->bsplit, c->p2, counts,
+                       (const int32_t *)nullptr, (int32_t *)nullptr, c->iflag);
+    int rc = check_launch(c, "k_pattern(count)");
+    if (rc) { hipFree(counts); return rc; }
+    std::vector<int32_t> h(c->n);
+    LPM_HIP(c, hipMemcpyAsync(h.data(), counts, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    int any = 0;
+    rc = read_flag(c, &any);
+    if (rc) { hipFree(counts); return rc; }
+    if (any) { hipFree(counts); return lpm_fail(c, LPM_E_LIMIT, "a particle has more than 512 two-hop candidates"); }
+    std::vector<int32_t> cp(c->n + 1, 0);
+    int maxc = 0;
+    for (int64_t i = 0; i < c->n; ++i) {
+        cp[i + 1] = cp[i] + h[i];
+        if (h[i] > maxc) maxc = h[i];
+    }
+    c->nconn = cp[c->n];
+    c->maxconn = maxc;
+    LPM_HIP(c, hipMalloc(&c->cptr, sizeof(int32_t) * (c->n + 1)));
+    LPM_HIP(c, hipMalloc(&c->cidx, sizeof(int32_t) * (c->nconn > 0 ? c->nconn : 1)));
+    LPM_HIP(c, hipMemcpyAsync(c->cptr, cp.data(), sizeof(int32_t) * (c->n + 1), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_pattern, dim3((unsigned)c->n), dim3(256), 0, c->stream, c->n, c->bptr, c->bsplit, c->p2, counts,
+                       (const int32_t *)c->cptr, c->cidx, c->iflag);
+    rc = check_launch(c, "k_pattern(fill)");
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    hipFree(counts);
+    return rc;
+}
+
+int launch_tslot(lpm_ctx *c) {
+    if (c->tslot) return LPM_OK;
+    int32_t *rowof = nullptr;
+    LPM_HIP(c, hipMalloc(&rowof, sizeof(int32_t) * c->nconn));
+    LPM_HIP(c, hipMalloc(&c->tslot, sizeof(int32_t) * c->nconn));
+    hipLaunchKernelGGL(k_rowof, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, rowof);
+    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_tslot, dim3(blocks_for(c->nconn, 256)), dim3(256), 0, c->stream, c->nconn, c->n, c->cptr, c->cidx,
+                       rowof, c->tslot, c->iflag);
+    int rc = check_launch(c, "k_tslot");
+    int bad = 0;
+    if (!rc) rc = read_flag(c, &bad);
+    hipFree(rowof);
+    if (rc) return rc;
+    if (bad) return lpm_fail(c, LPM_E_TOPO, "sparsity pattern is not symmetric; K^T cannot be stored");
+    return LPM_OK;
+}
+
+int launch_nonlocal_search(lpm_ctx *c, double ratio) {
+    int32_t *counts = nullptr;
+    LPM_HIP(c, hipMalloc(&counts, sizeof(int32_t) * (c->n + 1)));
+    hipLaunchKernelGGL(k_nonlocal_search, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, ratio, c->xyz,
+                       c->nonlocal_L, counts, (const int32_t *)nullptr, (int32_t *)nullptr);
+    int rc = check_launch(c, "k_nonlocal_search(count)");
+    if (rc) { hipFree(counts); return rc; }
+    std::vector<int32_t> h(c->n);
+    LPM_HIP(c, hipMemcpyAsync(h.data(), counts, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<int32_t> p(c->n + 1, 0);
+    for (int64_t i = 0; i < c->n; ++i) {
+        if ((int64_t)p[i] + h[i] > 0x7fffffff) { hipFree(counts); return lpm_fail(c, LPM_E_LIMIT, "nonlocal list too long"); }
+        p[i + 1] = p[i] + h[i];
+    }
+    if (c->nlptr) hipFree(c->nlptr);
+    if (c->nlidx) hipFree(c->nlidx);
+    c->nlptr = c->nlidx = nullptr;
+    c->nnl = p[c->n];
+    LPM_HIP(c, hipMalloc(&c->nlptr, sizeof(int32_t) * (c->n + 1)));
+    LPM_HIP(c, hipMalloc(&c->nlidx, sizeof(int32_t) * (c->nnl > 0 ? c->nnl : 1)));
+    LPM_HIP(c, hipMemcpyAsync(c->nlptr, p.data(), sizeof(int32_t) * (c->n + 1), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_nonlocal_search, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, ratio, c->xyz,
+                       c->nonlocal_L, counts, (const int32_t *)c->nlptr, c->nlidx);
+    rc = check_launch(c, "k_nonlocal_search(fill)");
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    hipFree(counts);
+    return rc;
+}

@@ -1,0 +1,129 @@
+// Internal definitions shared by the translation units of liblpm_mi355x.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lpm_mi355x.h"
+
+// Compiled limits, checked on the host before any launch.
+#define LPM_MAXB 32    // bonds per particle (hex 12, SC/FCC 18, BCC 14)
+#define LPM_MAXC 512   // 2-hop candidates per particle before de-duplication
+#define LPM_NSV 7      // state variables per particle (J2 uses 7)
+#define LPM_NMAT 10    // model parameters per particle
+
+#define LPM_EPS 1e-6   // include/lpm.h:37
+
+// Device scalar slots (double) used by the CG and the reductions.
+enum {
+    SC_RHS2 = 0,   // ||b||^2
+    SC_THR,        // max(tol^2 ||b||^2, tiny)
+    SC_ABSNEW,     // r.z
+    SC_ALPHA,
+    SC_BETA,
+    SC_PT,         // p.t
+    SC_RR,         // ||r||^2
+    SC_ITER,       // completed CG iterations
+    SC_DONE,       // 0 running, 1 converged, 2 iteration cap, 3 zero rhs
+    SC_MAXIT,
+    SC_NORM_R,     // residual ||.||^2
+    SC_NORM_REAC,  // reaction ||.||^2
+    SC_DIAGNORM,   // ||diag K||^2
+    SC_FLAG,       // any-flag (bond broken / damage changed) as a double 0/1
+    SC_COUNT = 32
+};
+
+struct Group {
+    int32_t *ids = nullptr;
+    int64_t n = 0;
+    bool dup = false;  // duplicate ids: applied by one thread in list order
+};
+
+struct lpm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // lattice
+    int lattice = 0, dim = 3, model = 0, nneighbors = 1;
+    double radius = 0, volume = 0, cutoff[2] = {0, 0};
+    int64_t n = 0, nbond = 0, nconn = 0, nnl = 0;
+    int maxconn = 0, maxbond = 0;
+    // modes
+    int diag10_literal = 1, cg_transpose = 1, dirichlet_static = 0, rollback_pex = 0;
+    int precision = 0, precond = 0, verbose = 0, max_NR_iter = 30, cg_batch = 16;
+    double cg_tol = 2.220446049250313e-16, tol_NR = 1e-6;
+    int64_t cg_maxit = -1;
+    // topology (device)
+    int32_t *bptr = nullptr, *bsplit = nullptr, *p2 = nullptr, *opp = nullptr, *nbarr = nullptr;
+    int32_t *cptr = nullptr, *cidx = nullptr, *tslot = nullptr;
+    int32_t *nlptr = nullptr, *nlidx = nullptr;
+    // particle state (device, FP64 unless noted)
+    double *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr, *Pin = nullptr, *Pex = nullptr;
+    double *stress = nullptr, *damage = nullptr, *damage_last = nullptr, *damage_visual = nullptr;
+    double *Ddot_local = nullptr, *Ddot_nonlocal = nullptr, *nonlocal_L = nullptr;
+    double *sv = nullptr, *sv_last = nullptr, *mat = nullptr;
+    double *dLe_total = nullptr, *TdLe_total = nullptr, *cs_sum = nullptr;
+    int32_t *type = nullptr, *cons = nullptr;
+    // bond state (device)
+    double *dis0 = nullptr, *dis = nullptr, *dis_last = nullptr, *dLe = nullptr, *dLp = nullptr, *dLp_last = nullptr;
+    double *bstrain = nullptr, *Kn = nullptr, *Tv = nullptr, *cs = nullptr, *bforce = nullptr, *bforce_last = nullptr;
+    double *bdamage = nullptr, *bdamage_last = nullptr;
+    // linear system
+    double *residual = nullptr, *disp = nullptr, *diag64 = nullptr;
+    void *aval = nullptr;        // BSR values of the CG matrix (float or double), row-SoA per particle row
+    int aval_prec = -1;          // precision aval was allocated for
+    void *invdiag = nullptr;     // Jacobi (ndof) or block Jacobi (9 per particle)
+    void *v_x = nullptr, *v_r = nullptr, *v_z = nullptr, *v_p = nullptr, *v_t = nullptr;  // padded vectors
+    int vpad = 4;                // entries per particle in the CG vectors (4 in 3D, 2 in 2D)
+    double *scal = nullptr;      // SC_COUNT device scalars
+    double *partial = nullptr;   // per-block partial sums (3 x partial_cap)
+    int partial_cap = 0;
+    unsigned int *counter = nullptr;  // last-block counters (4)
+    int32_t *iflag = nullptr;    // device int flags (4)
+    double *h_scal = nullptr;    // pinned host mirror of scal
+    int32_t *h_iflag = nullptr;
+    bool k_ready = false;
+    // groups
+    std::vector<Group> groups;
+    // statistics
+    int64_t st_newton = 0, st_cg_iters = 0, st_cg_solves = 0, st_asm = 0, st_spmv = 0;
+    double st_spmv_ms = 0, last_cg_err = 0;
+    int last_cg_iters = 0, timing = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+};
+
+int lpm_fail(lpm_ctx *c, int code, const std::string &msg);
+
+#define LPM_HIP(c, call)                                                                       \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return lpm_fail((c), LPM_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---- launchers (kernels_phys.hip)
+int launch_geometry_force(lpm_ctx *c, bool force);
+int launch_bond_force(lpm_ctx *c);
+int launch_particle_force(lpm_ctx *c);
+int launch_residual(lpm_ctx *c, double *nr2, double *nR2);
+int launch_assemble(lpm_ctx *c);
+int launch_dirichlet(lpm_ctx *c);
+int launch_state_var(lpm_ctx *c);
+int launch_nonlocal_ddot(lpm_ctx *c, int undamaged);
+int launch_static_damage(lpm_ctx *c, int undamaged, int *any);
+int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any);
+int launch_pattern(lpm_ctx *c);
+int launch_tslot(lpm_ctx *c);
+int launch_nonlocal_search(lpm_ctx *c, double ratio);
+// ---- launchers (kernels_cg.hip)
+int cg_alloc(lpm_ctx *c);
+int cg_solve(lpm_ctx *c, int *iters, double *error, int *success);
+int launch_move(lpm_ctx *c);
+int spmv_bench(lpm_ctx *c, int reps, double *ms);
+int launch_precond_setup(lpm_ctx *c);
+
+// helpers
+static inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -1,0 +1,75 @@
+"""CPU checks of the product library: it loads, exports every symbol the C header
+declares, and its host-side setup code (lattice generators, unit cell, bond search)
+reproduces the oracle bit for bit. No GPU is touched."""
+import math
+
+import numpy as np
+import pytest
+
+import lpmgpu as G
+import oracle as O
+
+
+def test_library_exports_every_declared_symbol():
+    L = G.lib()
+    syms = G.header_symbols()
+    assert len(syms) >= 35
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+CASES = [
+    (O.HEXAGON2D, [0, 6, 0, 5, 0, 1], 0.25, 0, (0, 0, 0)),
+    (O.HEXAGON2D, [0, 6, 0, 5, 0, 1], 0.25, 0, (0, 0, 0.3)),
+    (O.SQUARE2D, [0, 4, 0, 4, 0, 1], 0.2, 0, (0, 0, 0)),
+    (O.SC3D, [0, 3, 0, 3, 0, 3], 0.25, 0, (0, 0, 0)),
+    (O.FCC3D, [0, 3, 0, 3, 0, 3], 0.25, 0, (0.4, 1.1, 2.0)),
+    (O.FCC3D, [0, 3, 0, 3, 0, 3], 0.25, 2, (0.4, 1.1, 2.0)),
+    (O.SC3D, [0, 3, 0, 3, 0, 3], 0.25, 1, (0.2, 0.3, 0.5)),
+]
+
+
+@pytest.mark.parametrize("lat,box,r,flag,ang", CASES)
+def test_generators_match_oracle_bitwise(lat, box, r, flag, ang):
+    a = G.generate_lattice(lat, box, r, flag, ang)
+    b = O.generate(lat, box, r, flag, ang)
+    assert a.shape == b.shape and len(a) > 0
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("lat,box,r,flag,ang", CASES)
+def test_bond_search_matches_oracle_bitwise(lat, box, r, flag, ang):
+    xyz = O.generate(lat, box, r, flag, ang)
+    bp, bs, p2 = G.build_bonds(lat, r, xyz)
+    s = O.System(lat, r, O.ELASTIC, xyz, bruteforce=True)
+    np.testing.assert_array_equal(bp, s.geti("bptr"))
+    np.testing.assert_array_equal(bs, s.geti("bsplit"))
+    np.testing.assert_array_equal(p2, s.geti("p2"))
+
+
+def test_unit_cell_and_ce():
+    for lat, r in ((G.HEXAGON2D, 0.2), (G.SC3D, 0.5), (G.FCC3D, 0.3), (G.SQUARE2D, 0.1), (G.BCC3D, 0.4)):
+        out, lit = G.unit_cell(lat, r, literal=1)
+        _, rowm = G.unit_cell(lat, r, literal=0)
+        np.testing.assert_array_equal(lit, rowm.T)
+        assert out[5] == 2 * r
+    out, _ = G.unit_cell(G.SC3D, 0.5)
+    assert out[0] == 3 and out[1] == 18 and out[4] == 60 and out[7] == 1.0
+    out, _ = G.unit_cell(G.HEXAGON2D, 0.5)
+    assert out[0] == 2 and out[1] == 12 and out[6] == pytest.approx(math.sqrt(3))
+    E, mu = 69e3, 0.3
+    np.testing.assert_allclose(G.ce_from_e(False, E, mu), [E * (1 - mu) / (1 + mu) / (1 - 2 * mu),
+                                                           E * mu / (1 + mu) / (1 - 2 * mu), E / 2 / (1 + mu)], rtol=0)
+
+
+def test_bad_topology_is_rejected_without_launching():
+    """An asymmetric bond list is refused on the host (the reference would dereference end(),
+    particle.h:207-212); so is a bad lattice id. No GPU is needed to get these errors."""
+    xyz = np.array([[0.0, 0, 0], [1.0, 0, 0]])
+    bp = np.array([0, 1, 1], dtype=np.int32)
+    bs = np.array([1, 1], dtype=np.int32)
+    p2 = np.array([1], dtype=np.int32)
+    with pytest.raises(G.LPMError, match="reverse bond"):
+        G.Context(G.SC3D, 0.5, G.ELASTIC, xyz, bonds=(bp, bs, p2))
+    with pytest.raises(G.LPMError, match="lattice"):
+        G.Context(17, 0.5, G.ELASTIC, xyz, bonds=(bp, bs, p2))

@@ -1,0 +1,374 @@
+"""GPU parity of every phase of the load-step path against the FP64 oracle, through the
+C ABI (lpmgpu -> liblpm_mi355x.so). Integer/topology outputs and FP64 phases that run the
+reference's operation order are compared bit for bit; mixed-precision solves are compared
+at converged equilibrium with tolerances written in each test."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from scenarios import G, Pair, faces, hex_plate, rel, sc_cube
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _banner():
+    say("[gpu] parity tests start")
+    yield
+
+
+def deformed(xyz, seed, amp=2e-3, dim=3):
+    rng = np.random.default_rng(seed)
+    d = amp * rng.standard_normal(xyz.shape)
+    if dim == 2:
+        d[:, 2] = 0
+    return xyz + d
+
+
+# ----------------------------------------------------------------- topology
+@pytest.mark.parametrize("case", ["sc", "hex", "fcc"])
+def test_topology_bitwise(case):
+    say("[gpu] topology", case)
+    if case == "sc":
+        xyz, box = sc_cube(6)
+        lat, r = O.SC3D, 0.5
+    elif case == "hex":
+        xyz, box = hex_plate(14, 14)
+        lat, r = O.HEXAGON2D, 0.5
+    else:
+        box = [0, 4, 0, 4, 0, 4]
+        lat, r = O.FCC3D, 0.3
+        xyz = O.generate(lat, box, r, 0, (0.3, 0.7, 1.1))
+    p = Pair(lat, r, O.ELASTIC, xyz, box)
+    for f_o, f_g in (("bptr", "bond_ptr"), ("bsplit", "bond_split"), ("p2", "bond_p2"), ("opp", "opp"),
+                     ("cptr", "conn_ptr"), ("cidx", "conn_idx"), ("nb", "nb")):
+        np.testing.assert_array_equal(p.g.get(f_g), p.o.geti(f_o), err_msg=f_o)
+    np.testing.assert_array_equal(p.g.get("dis_initial"), p.o.getd("dis0"))
+    np.testing.assert_array_equal(p.g.get("cs"), p.o.getd("cs"))
+
+
+# ----------------------------------------------------------------- geometry, force, residual (FP64, bitwise)
+@pytest.mark.parametrize("model", [O.ELASTIC, O.ELASTIC_DAMAGE])
+def test_force_state_bitwise(model):
+    say("[gpu] force state model", model)
+    xyz, box = sc_cube(6)
+    p = Pair(O.SC3D, 0.5, model, xyz, box)
+    if model == O.ELASTIC:
+        p.set_elastic(0.0, False, 200e3, 0.3, 1.0)
+    else:
+        p.set_elastic_damage(1.0, False, 200e3, 0.3, 1e-3, 1.0, 10.0, 0.99)
+        dmg = np.linspace(0, 0.5, len(xyz))
+        p.o.setd("damage", dmg)
+        p.g.set("damage", dmg)
+    x = deformed(xyz, 1)
+    p.o.setd("xyz", x)
+    p.g.set("xyz", x)
+    bd = np.zeros(p.o.info("nbonds"))
+    bd[::7] = 1.0
+    bd[3::11] = 0.25
+    p.o.setd("bdamage", bd)
+    p.g.set("bdamage", bd)
+    p.o.update_geometry()
+    p.o.update_force_state()
+    p.g.update_geometry()
+    p.g.update_force_state()
+    for f in ("dis", "dLe", "bstrain", "cs", "bforce", "bdamage"):
+        np.testing.assert_array_equal(p.g.get(f), p.o.getd(f), err_msg=f)
+    for f in ("dLe_total", "TdLe_total", "cs_sum", "Pin", "stress", "damage_visual"):
+        np.testing.assert_array_equal(p.g.get(f), p.o.getd(f), err_msg=f)
+    cons = np.zeros(3 * len(xyz), dtype=np.int32)
+    cons[::5] = 1
+    pex = np.linspace(-1, 1, 3 * len(xyz))
+    p.o.seti("cons", cons)
+    p.g.set("disp_constraint", cons)
+    p.o.setd("Pex", pex)
+    p.g.set("Pex", pex)
+    p.o.update_rr()
+    nr, nR = p.g.update_rr()
+    np.testing.assert_array_equal(p.g.get("residual"), p.o.getd("residual"))
+    np.testing.assert_array_equal(p.g.get("reaction"), p.o.getd("reaction"))
+    assert nr == pytest.approx(p.o.norm("residual"), rel=1e-14)
+    assert nR == pytest.approx(p.o.norm("reaction"), rel=1e-14)
+
+
+# ----------------------------------------------------------------- stiffness assembly
+@pytest.mark.parametrize("lat", [O.SC3D, O.HEXAGON2D])
+@pytest.mark.parametrize("transpose,diag10", [(1, 1), (0, 0)])
+def test_assembly_bitwise(lat, transpose, diag10):
+    say("[gpu] assembly", lat, transpose, diag10)
+    if lat == O.SC3D:
+        xyz, box = sc_cube(5)
+        dim = 3
+    else:
+        xyz, box = hex_plate(10, 10)
+        dim = 2
+    p = Pair(lat, 0.5, O.ELASTIC, xyz, box)
+    p.set_elastic(0.0, False, 200e3, 0.3, 1.0)
+    x = deformed(xyz, 2, dim=dim)
+    bd = np.zeros(p.o.info("nbonds"))
+    bd[::9] = 1.0
+    bd[4::13] = 0.5
+    for s in (p.o,):
+        s.setd("xyz", x)
+        s.setd("bdamage", bd)
+        s.mode(cg_transpose=transpose, diag10_literal=diag10)
+    p.g.set("xyz", x)
+    p.g.set("bdamage", bd)
+    p.g.mode(cg_transpose=transpose, diag10_literal=diag10)
+    p.o.update_geometry()
+    p.g.update_geometry()
+    p.o.calc_stiffness()
+    r, c, v = p.o.K_csr()
+    import scipy.sparse as sp
+    K = sp.csr_matrix((v, c, r), shape=(len(r) - 1, len(r) - 1))
+    A = (K.T if transpose else K).tocsr()
+    A.sort_indices()
+    for prec in (1, 0):
+        p.g.mode(precision=prec)
+        p.g.calc_stiffness()
+        gr, gc, gv = p.g.matrix()
+        np.testing.assert_array_equal(gr, A.indptr)
+        np.testing.assert_array_equal(gc, A.indices)
+        exp = A.data if prec == 1 else A.data.astype(np.float32).astype(np.float64)
+        np.testing.assert_array_equal(gv, exp)
+    # Dirichlet elimination (stiffness.h:265-294)
+    cons = np.zeros(3 * len(xyz), dtype=np.int32)
+    cons[[0, 1, 5, 9, 3 * 7 + 1]] = 1
+    p.o.seti("cons", cons)
+    p.g.set("disp_constraint", cons)
+    p.o.update_force_state()
+    p.g.update_force_state()
+    p.o.update_stiffness_disp_bc()
+    r, c, v = p.o.K_csr()
+    K = sp.csr_matrix((v, c, r), shape=(len(r) - 1, len(r) - 1))
+    A = (K.T if transpose else K).tocsr()
+    A.sort_indices()
+    p.g.mode(precision=1)
+    p.g.calc_stiffness()
+    p.g.update_stiffness_disp_bc()
+    gr, gc, gv = p.g.matrix()
+    np.testing.assert_array_equal(gc, A.indices)
+    d = np.abs(gv - A.data)
+    assert d.max() <= 1e-12 * np.abs(A.data).max(), d.max()
+
+
+# ----------------------------------------------------------------- CG and one Newton step
+def _bar(model=O.ELASTIC, n=5):
+    xyz, box = sc_cube(n)
+    p = Pair(O.SC3D, 0.5, model, xyz, box)
+    p.set_elastic(0.0, False, 200e3, 0.3, 1.0)
+    bottom, top = faces(xyz, 2)
+    return p, xyz, bottom, top
+
+
+def test_cg_fp64_matches_dense_solution():
+    say("[gpu] cg fp64")
+    p, xyz, bottom, top = _bar()
+    ls_o, ls_g = p.load(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", 0.01)])
+    p.o.apply_bc(ls_o)
+    p.g.apply_bc(ls_g)
+    p.o.mode(cg_transpose=0, diag10_literal=0, cg_tol=1e-14)
+    p.g.mode(cg_transpose=0, diag10_literal=0, cg_tol=1e-14, precision=1, cg_maxit=5000)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+        s.update_rr()
+    p.o.calc_stiffness()
+    p.o.update_stiffness_disp_bc()
+    p.g.calc_stiffness()
+    p.g.update_stiffness_disp_bc()
+    K = p.o.K_dense()
+    rhs = p.o.getd("residual")
+    np.testing.assert_array_equal(p.g.get("residual"), rhs)
+    it, err = p.g.solve_linear_system()
+    d = p.g.get("disp")
+    d_ref = np.linalg.solve(K, rhs)
+    assert err <= 1e-14 and it > 0
+    assert rel(d, d_ref) < 1e-11
+    # positions moved by d (solver.h:97-105)
+    x_exp = xyz.copy()
+    x_exp[top, 2] += 0.01
+    np.testing.assert_array_equal(p.g.get("xyz").reshape(-1, 3), x_exp + d.reshape(-1, 3))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_static_step_equilibrium_parity(precision):
+    """One static load step (solver_static.h:97-149) with Dirichlet elimination, both sides
+    converged to a tight residual: displacements and reactions agree to 1e-8 relative."""
+    say("[gpu] static step precision", precision)
+    p, xyz, bottom, top = _bar(n=6)
+    disp = [(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", 0.02)]
+    ls_o, ls_g = p.load(disp=disp)
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-12, max_NR_iter=50)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-6 if precision == 0 else 1e-13, tol_NR=1e-12, max_NR_iter=50,
+             precision=precision, cg_transpose=0, diag10_literal=0, cg_maxit=4000)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+    assert p.o.solve_step_static(ls_o, -1)
+    assert p.g.solve_step_static(ls_g, -1)
+    u_o = p.o.getd("xyz") - xyz.ravel()
+    u_g = p.g.get("xyz") - xyz.ravel()
+    say("   disp rel err", rel(u_g, u_o), "newton", p.g.stat("newton_iters"), "cg", p.g.stat("cg_iters"))
+    assert rel(u_g, u_o) < 1e-8
+    assert rel(p.g.get("reaction"), p.o.getd("reaction")) < 1e-8
+    # constrained particles did not move
+    np.testing.assert_array_equal(p.g.get("xyz").reshape(-1, 3)[bottom], xyz[bottom])
+
+
+def test_literal_modes_reach_the_same_equilibrium():
+    """Oracle in the reference's literal CG semantics (K^T, Q2) and GPU with a symmetric K both
+    converge to the same equilibrium: the fixed point does not depend on K."""
+    say("[gpu] literal modes")
+    xyz, box = hex_plate(12, 10)
+    p = Pair(O.HEXAGON2D, 0.5, O.ELASTIC, xyz, box)
+    p.set_elastic(0.0, False, 69e3, 0.3, 1.0)
+    bottom, top = faces(xyz, 1)
+    ls_o, ls_g = p.load(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (top, "y", 0.01)])
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, cg_maxit=20000, tol_NR=1e-11, max_NR_iter=60)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=60, cg_maxit=3000)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+    assert p.o.solve_step_static(ls_o, -1)
+    assert p.g.solve_step_static(ls_g, -1)
+    u_o = p.o.getd("xyz") - xyz.ravel()
+    u_g = p.g.get("xyz") - xyz.ravel()
+    say("   disp rel err", rel(u_g, u_o))
+    assert rel(u_g, u_o) < 1e-8
+
+
+# ----------------------------------------------------------------- damage and state
+def test_bond_breaking_flags_bitwise():
+    say("[gpu] bond breaking")
+    xyz, box = hex_plate(12, 12)
+    p = Pair(O.HEXAGON2D, 0.5, O.ELASTIC, xyz, box)
+    p.set_elastic(0.0, True, 69e3, 0.3, 3e-3)
+    bottom, top = faces(xyz, 1)
+    ls_o, ls_g = p.load(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (top, "y", 0.05)])
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=60)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=60, cg_maxit=3000)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+    ok_o = p.o.solve_step_static(ls_o, -1)
+    ok_g = p.g.solve_step_static(ls_g, -1)
+    assert ok_o == ok_g
+    bo, bg = p.o.getd("bdamage"), p.g.get("bdamage")
+    say("   broken", int((bo == 1).sum()), int((bg == 1).sum()))
+    assert (bo == 1).sum() > 0
+    np.testing.assert_array_equal(bg == 1.0, bo == 1.0)
+    assert rel(p.g.get("xyz") - xyz.ravel(), p.o.getd("xyz") - xyz.ravel()) < 1e-7
+
+
+def test_elastic_damage_nonlocal_step():
+    say("[gpu] elastic damage")
+    xyz, box = hex_plate(12, 12)
+    p = Pair(O.HEXAGON2D, 0.5, O.ELASTIC_DAMAGE, xyz, box)
+    p.set_elastic_damage(1.2, True, 3.2e3, 0.28, 0.05, 1.0, 10.0, 0.99)
+    bottom, top = faces(xyz, 1)
+    t = np.zeros(len(xyz), dtype=np.int32)
+    t[bottom] = 4
+    p.set_type(t)
+    p.o.search_nonlocal(1.5)
+    p.g.search_nonlocal(1.5)
+    np.testing.assert_array_equal(p.g.get("nl_ptr"), p.o.geti("nlptr"))
+    np.testing.assert_array_equal(p.g.get("nl_idx"), p.o.geti("nlidx"))
+    ls_o, ls_g = p.load(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (top, "y", 0.2)])
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=80)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=80, cg_maxit=3000)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+        s.update_state_var()
+    for k in range(2):
+        ok_o = p.o.solve_step_static(ls_o, 4)
+        ok_g = p.g.solve_step_static(ls_g, 4)
+        assert ok_o == ok_g
+    Do, Dg = p.o.getd("damage"), p.g.get("damage")
+    say("   damage max", Do.max(), "diff", np.abs(Dg - Do).max())
+    assert Do.max() > 0
+    np.testing.assert_allclose(Dg, Do, rtol=0, atol=1e-9)
+    np.testing.assert_array_equal(Dg == 1.0, Do == 1.0)
+    assert rel(p.g.get("xyz") - xyz.ravel(), p.o.getd("xyz") - xyz.ravel()) < 1e-7
+
+
+def test_j2_state_update():
+    say("[gpu] j2")
+    xyz, box = sc_cube(5)
+    p = Pair(O.SC3D, 0.5, O.J2, xyz, box)
+    p.set_j2(0.0, False, 70e3, 0.33, 50.0, 0.3, 2e3, 0.0, 1.0, 1.0)
+    bottom, top = faces(xyz, 2)
+    ls_o, ls_g = p.load(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", 0.01)])
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=80)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=80, cg_maxit=3000)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+    for k in range(3):
+        assert p.o.solve_step_static(ls_o, -1) == p.g.solve_step_static(ls_g, -1)
+    svo, svg = p.o.getd("state_var"), p.g.get("state_var")
+    say("   alpha max", svo.reshape(-1, 7)[:, 0].max(), "diff", np.abs(svg - svo).max())
+    assert svo.reshape(-1, 7)[:, 0].max() > 0
+    np.testing.assert_allclose(svg, svo, rtol=1e-6, atol=1e-9 * np.abs(svo).max())
+    np.testing.assert_allclose(p.g.get("dLp"), p.o.getd("dLp"), rtol=1e-6, atol=1e-9 * np.abs(p.o.getd("dLp")).max())
+
+
+def test_hcf_fatigue_cycles():
+    say("[gpu] hcf fatigue")
+    xyz, box = hex_plate(10, 16)
+    p = Pair(O.HEXAGON2D, 0.5, O.HCF, xyz, box)
+    p.set_hcf(4.0, True, 110e3, 0.34, 5.206e-5, 2.019, 0.5)
+    left = np.nonzero(xyz[:, 0] < 0.5)[0]
+    bottom, top = faces(xyz, 1)
+    t = np.zeros(len(xyz), dtype=np.int32)
+    t[top] = 4
+    t[bottom] = 4
+    p.set_type(t)
+    p.o.search_nonlocal(1.5)
+    p.g.search_nonlocal(1.5)
+    p.o.mode(cg_tol=1e-13, tol_NR=1e-10, max_NR_iter=60)
+    p.g.mode(cg_tol=1e-6, tol_NR=1e-10, max_NR_iter=60, cg_maxit=3000)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+    fmax = 400.0
+    ls_o, ls_g = p.load(disp=[(left, "x", 0.0), (bottom, "y", 0.0)], force=[(top, 0.0, 0.1 * fmax, 0.0)])
+    p.o.solve_problem_fatigue_static([ls_o])
+    p.g.solve_problem_fatigue_static([ls_g])
+    spec = [0.1 * fmax] + [fmax, 0.1 * fmax] * 3
+    No = p.o.solve_problem_cyclic(1, 1, spec, [left, top, bottom], 1.0, 4)
+    Ng = p.g.solve_problem_cyclic(G.LOAD_UNIAXIAL_FORCE, 1, spec, [left, top, bottom], 1.0, 4)
+    assert No == Ng == 3
+    Do, Dg = p.o.getd("damage"), p.g.get("damage")
+    say("   damage max", Do.max(), "diff", np.abs(Dg - Do).max())
+    assert Do.max() > 0
+    np.testing.assert_allclose(Dg, Do, rtol=1e-7, atol=1e-15)
+    svo, svg = p.o.getd("state_var").reshape(-1, 7)[:, :3], p.g.get("state_var").reshape(-1, 7)[:, :3]
+    np.testing.assert_allclose(svg, svo, rtol=1e-7, atol=1e-12)
+
+
+def test_step_halving_path():
+    """max_NR_iter reached -> reset positions/state, halve the step, re-insert it
+    (solver_static.h:78-87); Pex is not rolled back (literal)."""
+    say("[gpu] halving")
+    p, xyz, bottom, top = _bar(n=4)
+    ls_o, ls_g = p.load(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0)],
+                        force=[(top, 0.0, 0.0, 20.0)])
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-9, max_NR_iter=2)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-9, max_NR_iter=2, cg_maxit=2000)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+    ok = p.g.solve_step_static(ls_g, -1)
+    assert not ok
+    p.g.reset(True)
+    np.testing.assert_array_equal(p.g.get("xyz"), xyz.ravel())
+    assert p.g.get("Pex").reshape(-1, 3)[top, 2].sum() == pytest.approx(20.0)
