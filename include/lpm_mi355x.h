@@ -3,7 +3,7 @@
  * static / fatigue Newton load-step path.
  *
  * The reference (longfish/LPM-MAC) has no FFI: its only seam is the header-only
- * C++ template API (include/*.h). Every entry point below replaces one method
+ * C++ template API (the headers under include/). Every entry point below replaces one method
  * of that API; the citation names the reference file:line whose behaviour it
  * provides. The C++ drop-in headers (lpm-mac_amd/host/) call these functions
  * from the reference's own class names, and tests/bench call them through
@@ -56,6 +56,11 @@ typedef struct lpm_ctx lpm_ctx;
  * into xyz (3 doubles each) and returns the number of points in *n. */
 int lpm_generate_lattice(int lattice, const double *box, double radius, int eulerflag, const double *angles,
                          double *xyz, int64_t cap, int64_t *n);
+/* Same with the rotation given as R[3*row+col] (the reference passes createRMatrix's result). */
+int lpm_generate_lattice_r(int lattice, const double *box, double radius, const double *R, double *xyz, int64_t cap,
+                           int64_t *n);
+/* createRMatrix (include/utilities.h:8-82): eulerflag 0 X1-Y2-Z3, 1 Kocks Z1-Y2-Z3, 2 Bunge Z1-X2-Z3. */
+int lpm_rotation_matrix(int eulerflag, const double *angles, double *R);
 
 /* Directed bonds with the reference's ordering; replaces Assembly::createBonds
  * (include/assembly.h:212-234): for each particle, layer-0 bonds then layer-1 bonds,
@@ -218,7 +223,8 @@ int lpm_solve_problem_cyclic(lpm_ctx *ctx, int32_t load_type, int32_t n_interval
 
 /* ------------------------------------------------------------------ statistics / measurement
  * Counters since the last reset: "newton_iters","cg_iters","cg_solves","assemblies",
- * "spmv_launches"; "spmv_ms" (sum of SpMV kernel time, when timing is on);
+ * "spmv_launches" (SpMVs that did work); "spmv_ms" / "spmv_launches_timed" (sum of SpMV kernel
+ * time from HIP events and the number of launches it covers, when timing is on);
  * "last_cg_iters","last_cg_error". */
 double lpm_stat(const lpm_ctx *ctx, const char *name);
 int lpm_reset_stats(lpm_ctx *ctx);

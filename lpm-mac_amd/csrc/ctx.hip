@@ -505,7 +505,7 @@ int lpm_destroy(lpm_ctx *c) {
                     c->Ddot_local, c->Ddot_nonlocal, c->nonlocal_L, c->sv, c->sv_last, c->mat, c->dLe_total, c->TdLe_total,
                     c->cs_sum, c->type, c->cons, c->dis0, c->dis, c->dis_last, c->dLe, c->dLp, c->dLp_last, c->bstrain, c->Kn,
                     c->Tv, c->cs, c->bforce, c->bforce_last, c->bdamage, c->bdamage_last, c->residual, c->disp, c->diag64,
-                    c->aval, c->invdiag, c->v_x, c->v_r, c->v_z, c->v_p, c->v_t, c->scal, c->partial, c->counter, c->iflag};
+                    c->aval, c->invdiag, c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w, c->scal, c->partial, c->counter, c->iflag, c->sp, c->scol};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &g : c->groups)
@@ -530,6 +530,7 @@ int64_t lpm_size(const lpm_ctx *c, const char *what) {
     if (w == "maxconn") return c->maxconn;
     if (w == "maxbond") return c->maxbond;
     if (w == "nl_ptr") return c->nlptr ? c->n + 1 : 0;
+    if (w == "nslots") return c->nslots;
     return -1;
 }
 
@@ -624,25 +625,29 @@ int lpm_download_matrix(lpm_ctx *c, int32_t *rowptr, int32_t *col, double *val, 
     std::vector<int32_t> cp(c->n + 1), ci(c->nconn);
     LPM_HIP(c, hipMemcpyAsync(cp.data(), c->cptr, sizeof(int32_t) * (c->n + 1), hipMemcpyDeviceToHost, c->stream));
     LPM_HIP(c, hipMemcpyAsync(ci.data(), c->cidx, sizeof(int32_t) * c->nconn, hipMemcpyDeviceToHost, c->stream));
-    std::vector<double> a(nnz);
+    const int64_t ebl = (int64_t)D * D;
+    std::vector<int64_t> sp(c->nslice + 1);
+    LPM_HIP(c, hipMemcpyAsync(sp.data(), c->sp, sizeof(int64_t) * (c->nslice + 1), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> a(ebl * c->nslots);
     if (c->precision == 0) {
-        std::vector<float> af(nnz);
-        LPM_HIP(c, hipMemcpyAsync(af.data(), c->aval, sizeof(float) * nnz, hipMemcpyDeviceToHost, c->stream));
+        std::vector<float> af(ebl * c->nslots);
+        LPM_HIP(c, hipMemcpyAsync(af.data(), c->aval, sizeof(float) * af.size(), hipMemcpyDeviceToHost, c->stream));
         LPM_HIP(c, hipStreamSynchronize(c->stream));
-        for (int64_t k = 0; k < nnz; ++k) a[k] = af[k];
+        for (size_t k = 0; k < af.size(); ++k) a[k] = af[k];
     } else {
-        LPM_HIP(c, hipMemcpyAsync(a.data(), c->aval, sizeof(double) * nnz, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipMemcpyAsync(a.data(), c->aval, sizeof(double) * a.size(), hipMemcpyDeviceToHost, c->stream));
         LPM_HIP(c, hipStreamSynchronize(c->stream));
     }
     int64_t o = 0;
     for (int64_t i = 0; i < c->n; ++i) {
         const int c0 = cp[i], nc = cp[i + 1] - c0;
+        const int64_t base = ebl * sp[i >> 6] + (i & 63);
         for (int r = 0; r < D; ++r) {
             rowptr[D * i + r] = (int32_t)o;
             for (int s = 0; s < nc; ++s)
                 for (int q = 0; q < D; ++q) {
                     col[o] = ci[c0 + s] * D + q;
-                    val[o] = a[(int64_t)D * D * c0 + (int64_t)(r * D + q) * nc + s];
+                    val[o] = a[base + ((int64_t)s * ebl + r * D + q) * 64];
                     ++o;
                 }
         }
@@ -876,6 +881,7 @@ double lpm_stat(const lpm_ctx *c, const char *name) {
     if (w == "assemblies") return (double)c->st_asm;
     if (w == "spmv_launches") return (double)c->st_spmv;
     if (w == "spmv_ms") return c->st_spmv_ms;
+    if (w == "spmv_launches_timed") return (double)c->st_spmv_timed;
     if (w == "last_cg_iters") return c->last_cg_iters;
     if (w == "last_cg_error") return c->last_cg_err;
     return NAN;
@@ -883,7 +889,7 @@ double lpm_stat(const lpm_ctx *c, const char *name) {
 
 int lpm_reset_stats(lpm_ctx *c) {
     if (!c) return LPM_E_ARG;
-    c->st_newton = c->st_cg_iters = c->st_cg_solves = c->st_asm = c->st_spmv = 0;
+    c->st_newton = c->st_cg_iters = c->st_cg_solves = c->st_asm = c->st_spmv = c->st_spmv_timed = 0;
     c->st_spmv_ms = 0;
     return LPM_OK;
 }

@@ -1,19 +1,27 @@
-// Block-CSR SpMV and the Jacobi-preconditioned conjugate gradient that replaces
+// Block-sparse SpMV and the preconditioned conjugate gradient that replaces
 // Eigen::ConjugateGradient<SparseMatrix<double>, Lower|Upper> (solver.h:192-217).
 //
-// The CG follows Eigen 3.4's conjugate_gradient(): x0 = 0, r = b, p = M^-1 r,
-// threshold = max(tol^2 ||b||^2, DBL_MIN), stop when ||r||^2 < threshold (the
-// breaking iteration is not counted) or after maxIters iterations. Scalars
-// (dot products, alpha, beta) are FP64 in device memory; vectors and the matrix
-// are FP32 (precision 0) or FP64 (precision 1). No host synchronisation happens
-// inside an iteration: each kernel reads the scalars it needs from device memory
-// and the "done" flag makes the remaining kernels of a batch exit at once.
+// Matrix: SELL-64 (see sell_off in reduce.h). The SpMV runs one wavefront per slice of 64
+// rows with lane l owning row 64k+l, so every value load and every gather of the input
+// vector touches 64 consecutive entries (neighbouring rows of a lattice have neighbouring
+// columns), and no cross-lane reduction is needed.
 //
-// Vectors are stored padded to VP entries per particle (VP = 4 in 3D, 2 in 2D) so
-// that the SpMV gathers one aligned 16-byte (FP32, 3D) load per block.
+// Solver: Jacobi (or 3x3 block-Jacobi) preconditioned CG in the single-reduction form of
+// Chronopoulos and Gear (J. Comput. Appl. Math. 25, 1989): per iteration one SpMV (which
+// also forms delta = (w, u)), one fused vector kernel (p, s, x, r, u updates plus ||r||^2
+// and gamma = (r, u)) and one one-block finalize that turns the partial sums into
+// alpha/beta and the stopping decision. The stopping rule and defaults are Eigen 3.4's:
+// x0 = 0, stop when ||r||^2 < max(tol^2 ||b||^2, DBL_MIN) or after maxIters iterations.
+// It reaches the same solution as Eigen's recurrence (the tests check the converged
+// vectors, not the iterates). Scalars are FP64 in device memory; vectors and matrix are
+// FP32 (precision 0) or FP64 (precision 1). No host synchronisation happens inside an
+// iteration: the "done" flag makes the remaining kernels of a batch return at once.
+//
+// Vectors are stored padded to VP entries per particle (VP = 4 in 3D, 2 in 2D).
 #include <cfloat>
 
 #include "lpm_internal.h"
+#include "reduce.h"
 
 namespace {
 
@@ -22,222 +30,222 @@ struct alignas(sizeof(T) * VP) Vec {
     T v[VP];
 };
 
-__device__ __forceinline__ double agent_load(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Block sum of NV doubles; the last block to finish reduces the per-block partials in a
-// fixed order and returns true with the totals (valid in every thread of that block).
-template <int NV>
-__device__ bool grid_reduce(double (&v)[NV], double *partial, int cap, unsigned int *counter, double (&out)[NV]) {
-    __shared__ double red[NV][8];
-    __shared__ bool last;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    for (int k = 0; k < NV; ++k) {
-        double x = v[k];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if (lane == 0) red[k][wid] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < NV; ++k) {
-            double s = 0;
-            for (int w = 0; w < nw; ++w) s += red[k][w];
-            partial[k * cap + blockIdx.x] = s;
-        }
-        __threadfence();
-        const unsigned int t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    for (int k = 0; k < NV; ++k) {
-        double s = 0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) s += agent_load(&partial[k * cap + b]);
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        __syncthreads();
-        if (lane == 0) red[k][wid] = s;
-        __syncthreads();
-        double t = 0;
-        for (int w = 0; w < nw; ++w) t += red[k][w];
-        out[k] = t;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
-}
-
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD works on
-// a contiguous range of rows (its L2 then holds the matching window of p).
+// a contiguous range of rows (its L2 then holds the matching window of the input vector).
 __device__ __forceinline__ int64_t xcd_block(int remap) {
     const int64_t b = blockIdx.x, g = gridDim.x;
     if (!remap || (g & 7)) return b;
     return (b & 7) * (g >> 3) + (b >> 3);
 }
 
-// ---------------------------------------------------------------- CG init
-// r = b (x0 = 0), p = M^-1 r, absNew = r.p, threshold (Eigen conjugate_gradient prologue)
+template <typename T, int D, int VP>
+__device__ __forceinline__ void precond_apply(const T *__restrict__ invd, int block_pre, int64_t i, const Vec<T, VP> &r,
+                                              Vec<T, VP> &z) {
+    for (int k = 0; k < VP; ++k) z.v[k] = (T)0;
+    if (block_pre) {
+        for (int a = 0; a < D; ++a) {
+            T s = 0;
+            for (int q = 0; q < D; ++q) s += invd[(int64_t)D * D * i + a * D + q] * r.v[q];
+            z.v[a] = s;
+        }
+    } else {
+        for (int k = 0; k < D; ++k) z.v[k] = invd[D * i + k] * r.v[k];
+    }
+}
+
+// ---------------------------------------------------------------- CG prologue
+// x = 0, r = b, u = M^-1 r, p = s = 0; partials ||b||^2 and gamma = (r, u)
 template <typename T, int D, int VP>
 __global__ __launch_bounds__(256) void k_cg_init(int64_t n, const double *__restrict__ b64, const T *__restrict__ invd,
                                                   int block_pre, Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
-                                                  Vec<T, VP> *__restrict__ p, double tol, double maxit, double *partial,
-                                                  int cap, unsigned int *counter, double *scal) {
+                                                  Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
+                                                  Vec<T, VP> *__restrict__ s, double *partial, int cap, double *scal) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the prologue SpMV that follows checks the flag; clear what the previous solve left
+    if (i == 0) scal[SC_DONE] = 0.0;
     double v[2] = {0.0, 0.0};
     if (i < n) {
-        Vec<T, VP> rv, zv, xv;
-        for (int k = 0; k < VP; ++k) { rv.v[k] = (T)0; zv.v[k] = (T)0; xv.v[k] = (T)0; }
+        Vec<T, VP> rv, zv, zero;
+        for (int k = 0; k < VP; ++k) { rv.v[k] = (T)0; zero.v[k] = (T)0; }
         for (int k = 0; k < D; ++k) rv.v[k] = (T)b64[D * i + k];
-        if (block_pre) {
-            for (int a = 0; a < D; ++a) {
-                T s = 0;
-                for (int q = 0; q < D; ++q) s += invd[(int64_t)D * D * i + a * D + q] * rv.v[q];
-                zv.v[a] = s;
-            }
-        } else {
-            for (int k = 0; k < D; ++k) zv.v[k] = invd[D * i + k] * rv.v[k];
-        }
-        x[i] = xv;
+        precond_apply<T, D, VP>(invd, block_pre, i, rv, zv);
+        x[i] = zero;
+        p[i] = zero;
+        s[i] = zero;
         r[i] = rv;
-        p[i] = zv;
+        u[i] = zv;
         for (int k = 0; k < D; ++k) {
             v[0] += (double)rv.v[k] * (double)rv.v[k];
             v[1] += (double)rv.v[k] * (double)zv.v[k];
         }
     }
-    double out[2];
-    if (grid_reduce<2>(v, partial, cap, counter, out) && threadIdx.x == 0) {
-        const double rhs2 = out[0];
-        scal[SC_RHS2] = rhs2;
-        scal[SC_RR] = rhs2;
-        scal[SC_ITER] = 0;
-        scal[SC_MAXIT] = maxit;
-        const double thr = fmax(tol * tol * rhs2, DBL_MIN);
-        scal[SC_THR] = thr;
-        scal[SC_ABSNEW] = out[1];
-        scal[SC_DONE] = (rhs2 == 0.0) ? 3.0 : ((rhs2 < thr) ? 1.0 : (maxit <= 0 ? 2.0 : 0.0));
-    }
+    block_partial<2>(v, partial, cap);
 }
 
-// ---------------------------------------------------------------- SpMV  t = A p,  pt = p.t
-// One wavefront per block row, one lane per block (slot); a row with more than 64 blocks is
-// walked in chunks of 64. The matrix stream uses non-temporal loads so that it does not evict
-// the gathered vector from L2.
-template <typename T, int D, int VP, bool FUSED>
-__global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cidx,
-                                               const T *__restrict__ aval, const Vec<T, VP> *__restrict__ p,
-                                               Vec<T, VP> *__restrict__ t, double *partial, int cap, unsigned int *counter,
-                                               double *scal, int remap) {
+// ---------------------------------------------------------------- SpMV  w = A u  (+ delta = (w, u))
+// One wavefront per 64-row slice; lane l owns row 64k+l and walks its slots. UNR slots are
+// loaded before they are consumed so that each lane has several independent loads in flight.
+template <typename T, int D, int VP, bool FUSED, int UNR>
+__global__ __launch_bounds__(256) void k_spmv(int64_t n, int64_t nslice, const int64_t *__restrict__ sp,
+                                               const int32_t *__restrict__ cptr, const int32_t *__restrict__ scol,
+                                               const T *__restrict__ aval, const Vec<T, VP> *__restrict__ u,
+                                               Vec<T, VP> *__restrict__ w, double *partial, int cap, const double *scal,
+                                               int remap) {
     if (FUSED && scal[SC_DONE] != 0.0) return;
+    constexpr int EB = D * D;
     const int lane = threadIdx.x & 63;
-    const int64_t i = xcd_block(remap) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    double pt = 0.0;
-    if (i < n) {
-        const int c0 = cptr[i], nc = cptr[i + 1] - c0;
-        const T *a = aval + (int64_t)D * D * c0;
+    const int64_t k = xcd_block(remap) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    double delta = 0.0;
+    if (k < nslice) {
+        const int64_t row = k * 64 + lane;
+        const int len = row < n ? cptr[row + 1] - cptr[row] : 0;
+        const int64_t base = sp[k];
+        const int W = (int)((sp[k + 1] - base) >> 6);
+        const int32_t *ci = scol + base + lane;
+        const T *av = aval + EB * base + lane;
         T acc[D];
         for (int r = 0; r < D; ++r) acc[r] = (T)0;
-        for (int slot = lane; slot < nc; slot += 64) {
-            const int64_t j = cidx[c0 + slot];
-            const Vec<T, VP> pj = p[j];
-            T e[D * D];
-            for (int q = 0; q < D * D; ++q) e[q] = __builtin_nontemporal_load(a + (int64_t)q * nc + slot);
-            for (int r = 0; r < D; ++r)
-                for (int q = 0; q < D; ++q) acc[r] += e[r * D + q] * pj.v[q];
+        int s = 0;
+        for (; s + UNR <= W; s += UNR) {
+            int j[UNR];
+            Vec<T, VP> pj[UNR];
+            T e[UNR][EB];
+#pragma unroll
+            for (int q = 0; q < UNR; ++q) j[q] = (s + q < len) ? ci[(int64_t)(s + q) * 64] : -1;
+#pragma unroll
+            for (int q = 0; q < UNR; ++q) {
+                if (j[q] >= 0) {
+                    pj[q] = u[j[q]];
+#pragma unroll
+                    for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = av[((int64_t)(s + q) * EB + e2) * 64];
+                } else {
+                    for (int t = 0; t < VP; ++t) pj[q].v[t] = (T)0;
+                    for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = (T)0;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < UNR; ++q)
+                for (int r = 0; r < D; ++r)
+                    for (int c = 0; c < D; ++c) acc[r] += e[q][r * D + c] * pj[q].v[c];
         }
-        for (int r = 0; r < D; ++r)
-            for (int o = 32; o > 0; o >>= 1) acc[r] += __shfl_xor(acc[r], o, 64);
-        if (lane == 0) {
-            Vec<T, VP> tv;
-            for (int k = 0; k < VP; ++k) tv.v[k] = (T)0;
-            for (int r = 0; r < D; ++r) tv.v[r] = acc[r];
-            t[i] = tv;
+        for (; s < W; ++s) {
+            if (s < len) {
+                const int j = ci[(int64_t)s * 64];
+                const Vec<T, VP> pj = u[j];
+                for (int r = 0; r < D; ++r)
+                    for (int c = 0; c < D; ++c) acc[r] += av[((int64_t)s * EB + r * D + c) * 64] * pj.v[c];
+            }
+        }
+        if (row < n) {
+            Vec<T, VP> wv;
+            for (int t = 0; t < VP; ++t) wv.v[t] = (T)0;
+            for (int r = 0; r < D; ++r) wv.v[r] = acc[r];
+            w[row] = wv;
             if (FUSED) {
-                const Vec<T, VP> pi = p[i];
-                for (int r = 0; r < D; ++r) pt += (double)pi.v[r] * (double)acc[r];
+                const Vec<T, VP> ui = u[row];
+                for (int r = 0; r < D; ++r) delta += (double)ui.v[r] * (double)acc[r];
             }
         }
     }
     if (!FUSED) return;
-    double v[1] = {pt}, out[1];
-    if (grid_reduce<1>(v, partial, cap, counter, out) && threadIdx.x == 0) {
-        scal[SC_PT] = out[0];
-        scal[SC_ALPHA] = scal[SC_ABSNEW] / out[0];
-    }
+    double v[1] = {delta};
+    block_partial<1>(v, partial, cap);
 }
 
-// ---------------------------------------------------------------- x += a p, r -= a t, ||r||^2, r.M^-1 r
+// ---------------------------------------------------------------- fused vector update
+// p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = M^-1 r;
+// partials ||r||^2 and gamma = (r, u)
 template <typename T, int D, int VP>
-__global__ __launch_bounds__(256) void k_cg_update(int64_t n, const T *__restrict__ invd, int block_pre,
-                                                    Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
-                                                    const Vec<T, VP> *__restrict__ p, const Vec<T, VP> *__restrict__ t,
-                                                    double *partial, int cap, unsigned int *counter, double *scal) {
+__global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__ invd, int block_pre,
+                                                 Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
+                                                 Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
+                                                 Vec<T, VP> *__restrict__ s, const Vec<T, VP> *__restrict__ w,
+                                                 double *partial, int cap, const double *scal) {
     if (scal[SC_DONE] != 0.0) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const double alpha = scal[SC_ALPHA];
     double v[2] = {0.0, 0.0};
     if (i < n) {
-        Vec<T, VP> xv = x[i], rv = r[i];
-        const Vec<T, VP> pv = p[i], tv = t[i];
-        const T a = (T)alpha;
+        const T alpha = (T)scal[SC_ALPHA], beta = (T)scal[SC_BETA];
+        Vec<T, VP> pv = p[i], sv = s[i], xv = x[i], rv = r[i], zv;
+        const Vec<T, VP> uv = u[i], wv = w[i];
         for (int k = 0; k < D; ++k) {
-            xv.v[k] += a * pv.v[k];
-            rv.v[k] -= a * tv.v[k];
+            pv.v[k] = uv.v[k] + beta * pv.v[k];
+            sv.v[k] = wv.v[k] + beta * sv.v[k];
+            xv.v[k] += alpha * pv.v[k];
+            rv.v[k] -= alpha * sv.v[k];
         }
+        precond_apply<T, D, VP>(invd, block_pre, i, rv, zv);
+        p[i] = pv;
+        s[i] = sv;
         x[i] = xv;
         r[i] = rv;
-        T z[D];
-        if (block_pre) {
-            for (int q = 0; q < D; ++q) {
-                T s = 0;
-                for (int w = 0; w < D; ++w) s += invd[(int64_t)D * D * i + q * D + w] * rv.v[w];
-                z[q] = s;
-            }
-        } else {
-            for (int k = 0; k < D; ++k) z[k] = invd[D * i + k] * rv.v[k];
-        }
+        u[i] = zv;
         for (int k = 0; k < D; ++k) {
             v[0] += (double)rv.v[k] * (double)rv.v[k];
-            v[1] += (double)rv.v[k] * (double)z[k];
+            v[1] += (double)rv.v[k] * (double)zv.v[k];
         }
     }
-    double out[2];
-    if (grid_reduce<2>(v, partial, cap, counter, out) && threadIdx.x == 0) {
-        scal[SC_RR] = out[0];
-        if (out[0] < scal[SC_THR]) {
-            scal[SC_DONE] = 1.0;
-        } else {
-            const double absOld = scal[SC_ABSNEW];
-            scal[SC_ABSNEW] = out[1];
-            scal[SC_BETA] = out[1] / absOld;
-            const double it = scal[SC_ITER] + 1.0;
-            scal[SC_ITER] = it;
-            if (it >= scal[SC_MAXIT]) scal[SC_DONE] = 2.0;
-        }
-    }
+    block_partial<2>(v, partial, cap);
 }
 
-// ---------------------------------------------------------------- p = M^-1 r + beta p
-template <typename T, int D, int VP>
-__global__ __launch_bounds__(256) void k_cg_direction(int64_t n, const T *__restrict__ invd, int block_pre,
-                                                       const Vec<T, VP> *__restrict__ r, Vec<T, VP> *__restrict__ p,
-                                                       const double *scal) {
-    if (scal[SC_DONE] != 0.0) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const T beta = (T)scal[SC_BETA];
-    const Vec<T, VP> rv = r[i];
-    Vec<T, VP> pv = p[i];
-    if (block_pre) {
-        for (int q = 0; q < D; ++q) {
-            T s = 0;
-            for (int w = 0; w < D; ++w) s += invd[(int64_t)D * D * i + q * D + w] * rv.v[w];
-            pv.v[q] = s + beta * pv.v[q];
-        }
-    } else {
-        for (int k = 0; k < D; ++k) pv.v[k] = invd[D * i + k] * rv.v[k] + beta * pv.v[k];
+// ---------------------------------------------------------------- one-block finalize
+// Sums the per-block partials in a fixed order and advances the scalars.
+//  FIN_INIT : parts (||b||^2, gamma0) + delta0 at offset 2   -> threshold, alpha0, done
+//  FIN_STEP : parts (||r||^2, gamma) + delta at offset 2    -> converged? / beta, alpha, count
+//  FIN_NORMS: (||r||^2, ||R||^2) for the Newton loop; FIN_DIAG: ||diag K||^2
+__global__ __launch_bounds__(1024) void k_finalize(int kind, int na, int nb, const double *__restrict__ partial, int cap,
+                                                   double *__restrict__ scal, double tol, double maxit) {
+    if (kind == FIN_STEP && scal[SC_DONE] != 0.0) return;
+    __shared__ double red[3][16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nv = (kind == FIN_DIAG) ? 1 : ((kind == FIN_NORMS) ? 2 : 3);
+    for (int k = 0; k < nv; ++k) {
+        const int np = (k < 2) ? na : nb;
+        double t = 0;
+        for (int b = threadIdx.x; b < np; b += blockDim.x) t += partial[k * cap + b];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (lane == 0) red[k][wid] = t;
     }
-    p[i] = pv;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double tot[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < nv; ++k)
+        for (int w = 0; w < nw; ++w) tot[k] += red[k][w];
+    if (kind == FIN_INIT) {
+        const double rhs2 = tot[0];
+        const double thr = fmax(tol * tol * rhs2, DBL_MIN);
+        scal[SC_RHS2] = rhs2;
+        scal[SC_RR] = rhs2;
+        scal[SC_ITER] = 0;
+        scal[SC_MAXIT] = maxit;
+        scal[SC_THR] = thr;
+        scal[SC_ABSNEW] = tot[1];
+        scal[SC_BETA] = 0.0;
+        scal[SC_ALPHA] = tot[1] / tot[2];
+        scal[SC_DONE] = (rhs2 == 0.0) ? 3.0 : ((rhs2 < thr) ? 1.0 : (maxit <= 0 ? 2.0 : 0.0));
+    } else if (kind == FIN_STEP) {
+        scal[SC_RR] = tot[0];
+        const double it = scal[SC_ITER] + 1.0;
+        scal[SC_ITER] = it;
+        if (tot[0] < scal[SC_THR]) {
+            scal[SC_DONE] = 1.0;
+        } else if (it >= scal[SC_MAXIT]) {
+            scal[SC_DONE] = 2.0;
+        } else {
+            const double gamma_old = scal[SC_ABSNEW], alpha_old = scal[SC_ALPHA];
+            const double gamma = tot[1], delta = tot[2];
+            const double beta = gamma / gamma_old;
+            scal[SC_BETA] = beta;
+            scal[SC_ALPHA] = gamma / (delta - beta * gamma / alpha_old);
+            scal[SC_ABSNEW] = gamma;
+            scal[SC_PT] = delta;
+        }
+    } else if (kind == FIN_NORMS) {
+        scal[SC_NORM_R] = tot[0];
+        scal[SC_NORM_REAC] = tot[1];
+    } else {
+        scal[SC_DIAGNORM] = tot[0];
+    }
 }
 
 // ---------------------------------------------------------------- preconditioner
@@ -245,7 +253,8 @@ __global__ __launch_bounds__(256) void k_cg_direction(int64_t n, const T *__rest
 // variant: inverse of the (dim x dim) diagonal block (scalar Jacobi where it is singular).
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_precond(int64_t n, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cidx,
-                                                  const T *__restrict__ aval, int block_pre, T *__restrict__ invd) {
+                                                  const int64_t *__restrict__ sp, const T *__restrict__ aval, int block_pre,
+                                                  T *__restrict__ invd) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c0 = cptr[i], nc = cptr[i + 1] - c0;
@@ -259,8 +268,7 @@ __global__ __launch_bounds__(256) void k_precond(int64_t n, const int32_t *__res
     }
     double B[D][D];
     for (int r = 0; r < D; ++r)
-        for (int q = 0; q < D; ++q)
-            B[r][q] = ds >= 0 ? (double)aval[(int64_t)D * D * c0 + (int64_t)(r * D + q) * nc + ds] : 0.0;
+        for (int q = 0; q < D; ++q) B[r][q] = ds >= 0 ? (double)aval[sell_off(sp, i, ds, D * D, r * D + q)] : 0.0;
     if (!block_pre) {
         for (int r = 0; r < D; ++r) invd[D * i + r] = (B[r][r] != 0.0) ? (T)(1.0 / B[r][r]) : (T)1;
         return;
@@ -322,22 +330,35 @@ static int check_launch(lpm_ctx *c, const char *what) {
     return LPM_OK;
 }
 
+int launch_finalize(lpm_ctx *c, int kind, int na, int nb, double tol, double maxit) {
+    if (na > c->partial_cap || nb > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "reduction exceeds partial buffer");
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, kind, na, nb, (const double *)c->partial,
+                       c->partial_cap, c->scal, tol, maxit);
+    return check_launch(c, "k_finalize");
+}
+
 int cg_alloc(lpm_ctx *c) {
     const size_t es = c->precision == 0 ? sizeof(float) : sizeof(double);
     const int64_t ebl = (int64_t)c->dim * c->dim;
     if (c->aval && c->aval_prec != c->precision) {
-        hipFree(c->aval); hipFree(c->invdiag); hipFree(c->v_x); hipFree(c->v_r); hipFree(c->v_p); hipFree(c->v_t);
-        c->aval = c->invdiag = c->v_x = c->v_r = c->v_p = c->v_t = nullptr;
+        hipFree(c->aval);
+        hipFree(c->invdiag);
+        void *vs[] = {c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w};
+        for (void *v : vs) hipFree(v);
+        c->aval = c->invdiag = c->v_x = c->v_r = c->v_u = c->v_p = c->v_s = c->v_w = nullptr;
     }
     if (!c->aval) {
-        LPM_HIP(c, hipMalloc(&c->aval, es * ebl * (c->nconn > 0 ? c->nconn : 1)));
+        const int64_t slots = c->nslots > 0 ? c->nslots : 1;
+        LPM_HIP(c, hipMalloc(&c->aval, es * ebl * slots));
+        // padding slots are never read (lanes past their row length skip them), but start defined
+        LPM_HIP(c, hipMemsetAsync(c->aval, 0, es * ebl * slots, c->stream));
         LPM_HIP(c, hipMalloc(&c->invdiag, es * ebl * c->n));
         const size_t vb = es * c->vpad * c->n;
-        LPM_HIP(c, hipMalloc(&c->v_x, vb));
-        LPM_HIP(c, hipMalloc(&c->v_r, vb));
-        LPM_HIP(c, hipMalloc(&c->v_p, vb));
-        LPM_HIP(c, hipMalloc(&c->v_t, vb));
-        LPM_HIP(c, hipMemsetAsync(c->v_p, 0, vb, c->stream));
+        void **vs[] = {&c->v_x, &c->v_r, &c->v_u, &c->v_p, &c->v_s, &c->v_w};
+        for (void **v : vs) {
+            LPM_HIP(c, hipMalloc(v, vb));
+            LPM_HIP(c, hipMemsetAsync(*v, 0, vb, c->stream));
+        }
         c->aval_prec = c->precision;
     }
     return LPM_OK;
@@ -345,7 +366,7 @@ int cg_alloc(lpm_ctx *c) {
 
 template <typename T, int D, int VP>
 static int precond_t(lpm_ctx *c) {
-    hipLaunchKernelGGL((k_precond<T, D>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx,
+    hipLaunchKernelGGL((k_precond<T, D>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp,
                        (const T *)c->aval, c->precond, (T *)c->invdiag);
     return check_launch(c, "k_precond");
 }
@@ -355,16 +376,21 @@ int launch_precond_setup(lpm_ctx *c) {
     return c->dim == 3 ? precond_t<double, 3, 4>(c) : precond_t<double, 2, 2>(c);
 }
 
+static int spmv_grid(const lpm_ctx *c) { return blocks_for(c->nslice, 4); }
+
+// w = A u (fused: + delta partials at offset 2, skipped once the solve is done)
 template <typename T, int D, int VP>
-static int spmv_launch(lpm_ctx *c, bool fused, int remap) {
+static int spmv_launch(lpm_ctx *c, bool fused, void *in, void *out) {
     typedef Vec<T, VP> V;
-    const int g = blocks_for(c->n, 4);
+    const int g = spmv_grid(c);
     if (fused)
-        hipLaunchKernelGGL((k_spmv<T, D, VP, true>), dim3(g), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, (const T *)c->aval,
-                           (const V *)c->v_p, (V *)c->v_t, c->partial, c->partial_cap, c->counter + 1, c->scal, remap);
+        hipLaunchKernelGGL((k_spmv<T, D, VP, true, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->nslice, c->sp, c->cptr,
+                           c->scol, (const T *)c->aval, (const V *)in, (V *)out, c->partial + 2 * (int64_t)c->partial_cap,
+                           c->partial_cap, (const double *)c->scal, 1);
     else
-        hipLaunchKernelGGL((k_spmv<T, D, VP, false>), dim3(g), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, (const T *)c->aval,
-                           (const V *)c->v_p, (V *)c->v_t, c->partial, c->partial_cap, c->counter + 1, c->scal, remap);
+        hipLaunchKernelGGL((k_spmv<T, D, VP, false, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->nslice, c->sp, c->cptr,
+                           c->scol, (const T *)c->aval, (const V *)in, (V *)out, c->partial, c->partial_cap,
+                           (const double *)c->scal, 1);
     return check_launch(c, "k_spmv");
 }
 
@@ -372,52 +398,62 @@ template <typename T, int D, int VP>
 static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     typedef Vec<T, VP> V;
     const int64_t n = c->n;
-    const int gp = blocks_for(n, 256);
-    if (gp > c->partial_cap || blocks_for(n, 4) > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "CG grid exceeds partial buffer");
+    const int gv = blocks_for(n, 256), gs = spmv_grid(c);
+    if (gv > c->partial_cap || gs > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "CG grid exceeds partial buffer");
     const int64_t maxit = c->cg_maxit < 0 ? 2 * n * c->dim : c->cg_maxit;
-    const int remap = 1;
-    hipLaunchKernelGGL((k_cg_init<T, D, VP>), dim3(gp), dim3(256), 0, c->stream, n, c->residual, (const T *)c->invdiag, c->precond,
-                       (V *)c->v_x, (V *)c->v_r, (V *)c->v_p, c->cg_tol, (double)maxit, c->partial, c->partial_cap,
-                       c->counter, c->scal);
+    hipLaunchKernelGGL((k_cg_init<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, c->residual, (const T *)c->invdiag,
+                       c->precond, (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s, c->partial,
+                       c->partial_cap, c->scal);
     int rc = check_launch(c, "k_cg_init");
     if (rc) return rc;
-    int64_t launched = 0;
+    rc = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
+    if (rc) return rc;
+    rc = launch_finalize(c, FIN_INIT, gv, gs, c->cg_tol, (double)maxit);
+    if (rc) return rc;
     const int batch = c->cg_batch > 0 ? c->cg_batch : 16;
     std::vector<hipEvent_t> evs;
     if (c->timing) {
         evs.resize(2 * batch);
         for (auto &e : evs) LPM_HIP(c, hipEventCreate(&e));
     }
+    int64_t launched = 0;
+    double it_prev = 0.0;
     for (;;) {
         LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
         LPM_HIP(c, hipStreamSynchronize(c->stream));
-        if (c->timing && launched > 0) {
-            for (int k = 0; k < batch; ++k) {
-                float ms = 0;
-                if (hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]) == hipSuccess) c->st_spmv_ms += ms;
+        if (launched > 0) {
+            // SpMVs of the batch that did work: one per completed iteration
+            int real = (int)(c->h_scal[SC_ITER] - it_prev);
+            real = real < 0 ? 0 : (real > batch ? batch : real);
+            c->st_spmv += real;
+            if (c->timing) {
+                for (int k = 0; k < real; ++k) {
+                    float ms = 0;
+                    if (hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]) == hipSuccess) c->st_spmv_ms += ms;
+                }
+                c->st_spmv_timed += real;
             }
         }
+        it_prev = c->h_scal[SC_ITER];
         if (c->h_scal[SC_DONE] != 0.0) break;
         if (launched > maxit + 2 * batch) break;  // the device flag caps the count; this only guards the loop
         for (int k = 0; k < batch; ++k) {
+            hipLaunchKernelGGL((k_cg_vec<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (const T *)c->invdiag, c->precond,
+                               (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s, (const V *)c->v_w, c->partial,
+                               c->partial_cap, (const double *)c->scal);
+            rc = check_launch(c, "k_cg_vec");
+            if (rc) return rc;
             if (c->timing) LPM_HIP(c, hipEventRecord(evs[2 * k], c->stream));
-            rc = spmv_launch<T, D, VP>(c, true, remap);
+            rc = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
             if (rc) return rc;
             if (c->timing) LPM_HIP(c, hipEventRecord(evs[2 * k + 1], c->stream));
-            hipLaunchKernelGGL((k_cg_update<T, D, VP>), dim3(gp), dim3(256), 0, c->stream, n, (const T *)c->invdiag, c->precond,
-                               (V *)c->v_x, (V *)c->v_r, (const V *)c->v_p, (const V *)c->v_t, c->partial, c->partial_cap,
-                               c->counter + 2, c->scal);
-            rc = check_launch(c, "k_cg_update");
-            if (rc) return rc;
-            hipLaunchKernelGGL((k_cg_direction<T, D, VP>), dim3(gp), dim3(256), 0, c->stream, n, (const T *)c->invdiag,
-                               c->precond, (const V *)c->v_r, (V *)c->v_p, c->scal);
-            rc = check_launch(c, "k_cg_direction");
+            rc = launch_finalize(c, FIN_STEP, gv, gs, 0.0, 0.0);
             if (rc) return rc;
         }
         launched += batch;
-        c->st_spmv += batch;
     }
     for (auto &e : evs) hipEventDestroy(e);
+    c->st_spmv += 1;  // the prologue SpMV
     const double rhs2 = c->h_scal[SC_RHS2];
     const double done = c->h_scal[SC_DONE];
     *iters = (int)c->h_scal[SC_ITER];
@@ -430,7 +466,8 @@ int cg_solve(lpm_ctx *c, int *iters, double *error, int *success) {
     if (!c->k_ready) return lpm_fail(c, LPM_E_STATE, "solve before the stiffness matrix was assembled");
     int rc = launch_precond_setup(c);
     if (rc) return rc;
-    if (c->precision == 0) return c->dim == 3 ? cg_t<float, 3, 4>(c, iters, error, success) : cg_t<float, 2, 2>(c, iters, error, success);
+    if (c->precision == 0)
+        return c->dim == 3 ? cg_t<float, 3, 4>(c, iters, error, success) : cg_t<float, 2, 2>(c, iters, error, success);
     return c->dim == 3 ? cg_t<double, 3, 4>(c, iters, error, success) : cg_t<double, 2, 2>(c, iters, error, success);
 }
 
@@ -451,11 +488,11 @@ static int bench_t(lpm_ctx *c, int reps, double *ms) {
     hipEvent_t a, b;
     LPM_HIP(c, hipEventCreate(&a));
     LPM_HIP(c, hipEventCreate(&b));
-    int rc = spmv_launch<T, D, VP>(c, false, 1);  // warm
+    int rc = spmv_launch<T, D, VP>(c, false, c->v_p, c->v_w);  // warm
     if (rc) return rc;
     LPM_HIP(c, hipEventRecord(a, c->stream));
     for (int k = 0; k < reps; ++k) {
-        rc = spmv_launch<T, D, VP>(c, false, 1);
+        rc = spmv_launch<T, D, VP>(c, false, c->v_p, c->v_w);
         if (rc) return rc;
     }
     LPM_HIP(c, hipEventRecord(b, c->stream));

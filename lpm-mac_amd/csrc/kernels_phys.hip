@@ -11,13 +11,13 @@
 //              layer 1 from bsplit[i]; per bond p2, opp (reverse bond), dis0, dis, dLe, dLp,
 //              bstrain, Kn, Tv, cs[3b+k], bforce, bdamage
 //   pattern    conns CSR (cptr, cidx) = block rows of the stiffness matrix
-//   matrix     BSR with (dim x dim) blocks; the values of block row i are stored entry-major
-//              ("row-SoA"): value e (= r*dim + s) of slot c lives at
-//              aval[dim*dim*cptr[i] + e*nconn_i + c], so the lanes of a wavefront that each own
-//              one block read consecutive addresses.
+//   matrix     block-sparse with (dim x dim) blocks on the conns pattern, stored SELL-64
+//              (slices of 64 rows, lane-major; see sell_off in reduce.h)
+#include <algorithm>
 #include <climits>
 
 #include "lpm_internal.h"
+#include "reduce.h"
 
 namespace {
 
@@ -25,49 +25,6 @@ constexpr int ELASTIC = 0, ELASTIC_DAMAGE = 1, J2 = 2, HCF = 3;
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
-
-// ---------------------------------------------------------------- reductions
-// Sum NV doubles over the block; the block that finishes last sums the per-block
-// partials in a fixed order (deterministic for a given grid) and returns true with the
-// totals in out[] (valid in every thread of that block).
-template <int NV>
-__device__ bool grid_reduce(double (&v)[NV], double *partial, int cap, unsigned int *counter, double (&out)[NV]) {
-    __shared__ double red[NV][8];
-    __shared__ bool last;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    for (int k = 0; k < NV; ++k) {
-        double x = v[k];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if (lane == 0) red[k][wid] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < NV; ++k) {
-            double s = 0;
-            for (int w = 0; w < nw; ++w) s += red[k][w];
-            partial[k * cap + blockIdx.x] = s;
-        }
-        __threadfence();
-        unsigned int t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    for (int k = 0; k < NV; ++k) {
-        double s = 0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) s += __builtin_nontemporal_load(&partial[k * cap + b]);
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        __syncthreads();
-        if (lane == 0) red[k][wid] = s;
-        __syncthreads();
-        double t = 0;
-        for (int w = 0; w < nw; ++w) t += red[k][w];
-        out[k] = t;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
-}
 
 // ---------------------------------------------------------------- geometry + bond force
 // Particle::updateBondsGeometry + Bond::updatebGeometry (particle.h:163-181, bond.h:30-39),
@@ -201,8 +158,7 @@ __global__ __launch_bounds__(256) void k_particle_force(int64_t n, double volume
 // Solver::updateRR (solver.h:108-128) with the squared norms of residual and reaction.
 __global__ __launch_bounds__(256) void k_residual(int64_t n, int dim, const int32_t *__restrict__ cons,
                                                    const double *__restrict__ Pex, const double *__restrict__ Pin,
-                                                   double *__restrict__ residual, double *partial, int cap,
-                                                   unsigned int *counter, double *scal) {
+                                                   double *__restrict__ residual, double *partial, int cap) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (d < n * dim) {
@@ -214,11 +170,7 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, int dim, const int3
         v[0] = r * r;
         if (c == 1) v[1] = Pin[3 * i + k] * Pin[3 * i + k];
     }
-    double out[2];
-    if (grid_reduce<2>(v, partial, cap, counter, out) && threadIdx.x == 0) {
-        scal[SC_NORM_R] = out[0];
-        scal[SC_NORM_REAC] = out[1];
-    }
+    block_partial<2>(v, partial, cap);
 }
 
 // ---------------------------------------------------------------- stiffness assembly
@@ -232,9 +184,9 @@ template <typename T, int D>
 __global__ __launch_bounds__(256) void k_assemble(
     int64_t n, int diag10_literal, int transpose, const int32_t *__restrict__ bptr,
     const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2, const int32_t *__restrict__ cptr,
-    const int32_t *__restrict__ cidx, const int32_t *__restrict__ tslot, const double *__restrict__ cs,
-    const double *__restrict__ cs_sum, const double *__restrict__ Kn, const double *__restrict__ Tv,
-    const double *__restrict__ bdamage, T *__restrict__ aval, double *__restrict__ diag64) {
+    const int32_t *__restrict__ cidx, const int32_t *__restrict__ tslot, const int64_t *__restrict__ sp,
+    const double *__restrict__ cs, const double *__restrict__ cs_sum, const double *__restrict__ Kn,
+    const double *__restrict__ Tv, const double *__restrict__ bdamage, T *__restrict__ aval, double *__restrict__ diag64) {
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (i >= n) return;
@@ -332,25 +284,22 @@ __global__ __launch_bounds__(256) void k_assemble(
                 }
         }
         if (!transpose) {
-            T *dst = aval + (int64_t)D * D * c0;
             for (int r = 0; r < D; ++r)
-                for (int q = 0; q < D; ++q) dst[(int64_t)(r * D + q) * nci + slot] = (T)K[r][q];
+                for (int q = 0; q < D; ++q) aval[sell_off(sp, i, slot, D * D, r * D + q)] = (T)K[r][q];
         } else {
-            const int cj = cptr[j], ncj = cptr[j + 1] - cj, ts = tslot[c0 + slot];
-            T *dst = aval + (int64_t)D * D * cj;
+            const int ts = tslot[c0 + slot];
             for (int r = 0; r < D; ++r)
-                for (int q = 0; q < D; ++q) dst[(int64_t)(r * D + q) * ncj + ts] = (T)K[q][r];
+                for (int q = 0; q < D; ++q) aval[sell_off(sp, j, ts, D * D, r * D + q)] = (T)K[q][r];
         }
     }
 }
 
 // squared norm of diag(K), Stiffness::updateStiffnessDispBC (stiffness.h:269-270)
 __global__ __launch_bounds__(256) void k_diag_norm(int64_t nd, const double *__restrict__ diag64, double *partial,
-                                                    int cap, unsigned int *counter, double *scal) {
+                                                    int cap) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {d < nd ? diag64[d] * diag64[d] : 0.0};
-    double out[1];
-    if (grid_reduce<1>(v, partial, cap, counter, out) && threadIdx.x == 0) scal[SC_DIAGNORM] = out[0];
+    block_partial<1>(v, partial, cap);
 }
 
 __device__ __forceinline__ bool eliminated(int64_t p, int k, const int32_t *cons, const double *damage_visual) {
@@ -361,7 +310,8 @@ __device__ __forceinline__ bool eliminated(int64_t p, int k, const int32_t *cons
 // constrained (or fully damaged) DoF and put ||diag K||_2 on its diagonal.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_dirichlet(int64_t n, const int32_t *__restrict__ cptr,
-                                                    const int32_t *__restrict__ cidx, const int32_t *__restrict__ cons,
+                                                    const int32_t *__restrict__ cidx, const int64_t *__restrict__ sp,
+                                                    const int32_t *__restrict__ cons,
                                                     const double *__restrict__ damage_visual, const double *scal,
                                                     T *__restrict__ aval) {
     const int lane = threadIdx.x & 63;
@@ -373,14 +323,13 @@ __global__ __launch_bounds__(256) void k_dirichlet(int64_t n, const int32_t *__r
     for (int r = 0; r < D; ++r) ei[r] = eliminated(i, r, cons, damage_visual);
     for (int slot = lane; slot < nci; slot += 64) {
         const int64_t j = cidx[c0 + slot];
-        T *dst = aval + (int64_t)D * D * c0;
         for (int q = 0; q < D; ++q) {
             const bool ej = eliminated(j, q, cons, damage_visual);
             for (int r = 0; r < D; ++r) {
                 if (ei[r] || ej) {
                     T v = (T)0;
                     if (ei[r] && j == i && r == q) v = nd;
-                    dst[(int64_t)(r * D + q) * nci + slot] = v;
+                    aval[sell_off(sp, i, slot, D * D, r * D + q)] = v;
                 }
             }
         }
@@ -633,6 +582,18 @@ __global__ __launch_bounds__(256) void k_pattern(int64_t n, const int32_t *__res
     }
 }
 
+// SELL-64 column array: block (row i, slot c) at sp[i>>6] + c*64 + (i&63); padding slots keep
+// the row's own index so that every stored column is a valid particle id.
+__global__ __launch_bounds__(256) void k_sell_cols(int64_t n, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cidx,
+                                                    const int64_t *__restrict__ sp, int32_t *__restrict__ scol) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k = i >> 6, l = i & 63;
+    const int W = (int)((sp[k + 1] - sp[k]) >> 6);
+    const int c0 = cptr[i], nc = cptr[i + 1] - c0;
+    for (int c = 0; c < W; ++c) scol[sp[k] + (int64_t)c * 64 + l] = c < nc ? cidx[c0 + c] : (int32_t)i;
+}
+
 // position of block (j, i) in block row j, for the transposed store
 __global__ __launch_bounds__(256) void k_tslot(int64_t nconn, int64_t n, const int32_t *__restrict__ cptr,
                                                 const int32_t *__restrict__ cidx, const int32_t *__restrict__ rowof,
@@ -727,8 +688,10 @@ int launch_residual(lpm_ctx *c, double *nr2, double *nR2) {
     const int g = blocks_for(nd, 256);
     if (g > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "residual grid exceeds partial buffer");
     hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->cons, c->Pex, c->Pin, c->residual,
-                       c->partial, c->partial_cap, c->counter, c->scal);
+                       c->partial, c->partial_cap);
     int rc = check_launch(c, "k_residual");
+    if (rc) return rc;
+    rc = launch_finalize(c, FIN_NORMS, g, 0, 0.0, 0.0);
     if (rc) return rc;
     if (nr2 || nR2) {
         LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
@@ -746,21 +709,21 @@ int launch_assemble(lpm_ctx *c) {
         float *a = (float *)c->aval;
         if (c->dim == 3)
             hipLaunchKernelGGL((k_assemble<float, 3>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
-                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->cs, c->cs_sum, c->Kn, c->Tv,
+                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum, c->Kn, c->Tv,
                                c->bdamage, a, c->diag64);
         else
             hipLaunchKernelGGL((k_assemble<float, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
-                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->cs, c->cs_sum, c->Kn, c->Tv,
+                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum, c->Kn, c->Tv,
                                c->bdamage, a, c->diag64);
     } else {
         double *a = (double *)c->aval;
         if (c->dim == 3)
             hipLaunchKernelGGL((k_assemble<double, 3>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
-                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->cs, c->cs_sum, c->Kn, c->Tv,
+                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum, c->Kn, c->Tv,
                                c->bdamage, a, c->diag64);
         else
             hipLaunchKernelGGL((k_assemble<double, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
-                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->cs, c->cs_sum, c->Kn, c->Tv,
+                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum, c->Kn, c->Tv,
                                c->bdamage, a, c->diag64);
     }
     return check_launch(c, "k_assemble");
@@ -770,26 +733,27 @@ int launch_dirichlet(lpm_ctx *c) {
     const int64_t nd = c->n * c->dim;
     const int g = blocks_for(nd, 256);
     if (g > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "diag grid exceeds partial buffer");
-    hipLaunchKernelGGL(k_diag_norm, dim3(g), dim3(256), 0, c->stream, nd, c->diag64, c->partial, c->partial_cap, c->counter,
-                       c->scal);
+    hipLaunchKernelGGL(k_diag_norm, dim3(g), dim3(256), 0, c->stream, nd, c->diag64, c->partial, c->partial_cap);
     int rc = check_launch(c, "k_diag_norm");
+    if (rc) return rc;
+    rc = launch_finalize(c, FIN_DIAG, g, 0, 0.0, 0.0);
     if (rc) return rc;
     const int g2 = blocks_for(c->n, 4);
     if (c->precision == 0) {
         float *a = (float *)c->aval;
         if (c->dim == 3)
-            hipLaunchKernelGGL((k_dirichlet<float, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->cons,
+            hipLaunchKernelGGL((k_dirichlet<float, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->cons,
                                c->damage_visual, c->scal, a);
         else
-            hipLaunchKernelGGL((k_dirichlet<float, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->cons,
+            hipLaunchKernelGGL((k_dirichlet<float, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->cons,
                                c->damage_visual, c->scal, a);
     } else {
         double *a = (double *)c->aval;
         if (c->dim == 3)
-            hipLaunchKernelGGL((k_dirichlet<double, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->cons,
+            hipLaunchKernelGGL((k_dirichlet<double, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->cons,
                                c->damage_visual, c->scal, a);
         else
-            hipLaunchKernelGGL((k_dirichlet<double, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->cons,
+            hipLaunchKernelGGL((k_dirichlet<double, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->cons,
                                c->damage_visual, c->scal, a);
     }
     return check_launch(c, "k_dirichlet");
@@ -871,6 +835,22 @@ int launch_pattern(lpm_ctx *c) {
     rc = check_launch(c, "k_pattern(fill)");
     LPM_HIP(c, hipStreamSynchronize(c->stream));
     hipFree(counts);
+    if (rc) return rc;
+    // SELL-64 slice offsets: slice k covers rows 64k..64k+63 and is as wide as its longest row
+    c->nslice = (c->n + 63) / 64;
+    std::vector<int64_t> sp(c->nslice + 1, 0);
+    for (int64_t k = 0; k < c->nslice; ++k) {
+        int W = 0;
+        for (int64_t i = 64 * k; i < std::min<int64_t>(c->n, 64 * k + 64); ++i) W = std::max(W, h[i]);
+        sp[k + 1] = sp[k] + 64LL * W;
+    }
+    c->nslots = sp[c->nslice];
+    LPM_HIP(c, hipMalloc(&c->sp, sizeof(int64_t) * (c->nslice + 1)));
+    LPM_HIP(c, hipMalloc(&c->scol, sizeof(int32_t) * (c->nslots > 0 ? c->nslots : 1)));
+    LPM_HIP(c, hipMemcpyAsync(c->sp, sp.data(), sizeof(int64_t) * (c->nslice + 1), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_sell_cols, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->scol);
+    rc = check_launch(c, "k_sell_cols");
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
     return rc;
 }
 

@@ -58,6 +58,10 @@ struct lpm_ctx {
     // topology (device)
     int32_t *bptr = nullptr, *bsplit = nullptr, *p2 = nullptr, *opp = nullptr, *nbarr = nullptr;
     int32_t *cptr = nullptr, *cidx = nullptr, *tslot = nullptr;
+    // SELL-64 copy of the pattern used by the CG matrix (see reduce.h)
+    int64_t *sp = nullptr;
+    int32_t *scol = nullptr;
+    int64_t nslice = 0, nslots = 0;
     int32_t *nlptr = nullptr, *nlidx = nullptr;
     // particle state (device, FP64 unless noted)
     double *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr, *Pin = nullptr, *Pex = nullptr;
@@ -75,7 +79,7 @@ struct lpm_ctx {
     void *aval = nullptr;        // BSR values of the CG matrix (float or double), row-SoA per particle row
     int aval_prec = -1;          // precision aval was allocated for
     void *invdiag = nullptr;     // Jacobi (ndof) or block Jacobi (9 per particle)
-    void *v_x = nullptr, *v_r = nullptr, *v_z = nullptr, *v_p = nullptr, *v_t = nullptr;  // padded vectors
+    void *v_x = nullptr, *v_r = nullptr, *v_u = nullptr, *v_p = nullptr, *v_s = nullptr, *v_w = nullptr;  // padded vectors
     int vpad = 4;                // entries per particle in the CG vectors (4 in 3D, 2 in 2D)
     double *scal = nullptr;      // SC_COUNT device scalars
     double *partial = nullptr;   // per-block partial sums (3 x partial_cap)
@@ -88,7 +92,7 @@ struct lpm_ctx {
     // groups
     std::vector<Group> groups;
     // statistics
-    int64_t st_newton = 0, st_cg_iters = 0, st_cg_solves = 0, st_asm = 0, st_spmv = 0;
+    int64_t st_newton = 0, st_cg_iters = 0, st_cg_solves = 0, st_asm = 0, st_spmv = 0, st_spmv_timed = 0;
     double st_spmv_ms = 0, last_cg_err = 0;
     int last_cg_iters = 0, timing = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -119,6 +123,7 @@ int launch_pattern(lpm_ctx *c);
 int launch_tslot(lpm_ctx *c);
 int launch_nonlocal_search(lpm_ctx *c, double ratio);
 // ---- launchers (kernels_cg.hip)
+int launch_finalize(lpm_ctx *c, int kind, int na, int nb, double tol, double maxit);
 int cg_alloc(lpm_ctx *c);
 int cg_solve(lpm_ctx *c, int *iters, double *error, int *success);
 int launch_move(lpm_ctx *c);

@@ -1,0 +1,38 @@
+// Deterministic block reductions for the multi-block dot products and norms.
+//
+// Each block reduces its values and writes one partial per quantity (no atomics:
+// a counter shared by tens of thousands of workgroups serialises at the memory side).
+// A one-block finalize kernel then sums the partials in a fixed order and updates the
+// device scalars, so results do not depend on workgroup scheduling.
+#pragma once
+
+#include "lpm_internal.h"
+
+template <int NV>
+static __device__ __forceinline__ void block_partial(double (&v)[NV], double *partial, int cap) {
+    __shared__ double red[NV][16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < NV; ++k) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) red[k][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0;
+        for (int w = 0; w < nw; ++w) s += red[threadIdx.x][w];
+        partial[threadIdx.x * cap + blockIdx.x] = s;
+    }
+}
+
+// finalize kinds
+enum { FIN_INIT = 0, FIN_STEP = 1, FIN_NORMS = 3, FIN_DIAG = 4 };
+
+// SELL-64 layout of the CG matrix: block rows are grouped in slices of 64 consecutive rows;
+// slice k starts at block slot sp[k] and is W_k slots wide (W_k = longest row of the slice).
+// Entry e (= r*dim + s) of block (row i, slot c) lives at
+//   aval[EB*sp[i>>6] + ((int64)c*EB + e)*64 + (i & 63)],  EB = dim*dim,
+// so a wavefront whose lane l owns row 64k+l reads 64 consecutive values per load.
+__device__ __forceinline__ int64_t sell_off(const int64_t *sp, int64_t i, int c, int EB, int e) {
+    return (int64_t)EB * sp[i >> 6] + ((int64_t)c * EB + e) * 64 + (i & 63);
+}

@@ -141,12 +141,24 @@ int lpm_unit_cell(int lattice, double radius, int literal, double *out, double *
     return LPM_OK;
 }
 
-// createPlateSQ2D / createPlateHEX2D / createCuboidSC3D / createCuboidFCC3D (utilities.h:84-301)
+int lpm_rotation_matrix(int eulerflag, const double *angles, double *R) {
+    if (!angles || !R || eulerflag < 0 || eulerflag > 2) return LPM_E_ARG;
+    rotation(eulerflag, angles, R);
+    return LPM_OK;
+}
+
 int lpm_generate_lattice(int lattice, const double *box, double radius, int eulerflag, const double *angles, double *xyz,
                          int64_t cap, int64_t *n) {
     if (!box || !angles || !n || radius <= 0) return LPM_E_ARG;
     double R[9];
     rotation(eulerflag, angles, R);
+    return lpm_generate_lattice_r(lattice, box, radius, R, xyz, cap, n);
+}
+
+// createPlateSQ2D / createPlateHEX2D / createCuboidSC3D / createCuboidFCC3D (utilities.h:84-301)
+int lpm_generate_lattice_r(int lattice, const double *box, double radius, const double *R, double *xyz, int64_t cap,
+                           int64_t *n) {
+    if (!box || !R || !n || radius <= 0) return LPM_E_ARG;
     Sink s{xyz, xyz ? cap : 0};
     double p[3] = {0, 0, 0}, q[3];
     if (lattice == LPM_SQUARE2D || lattice == LPM_HEXAGON2D) {

@@ -253,8 +253,10 @@ def test_bond_breaking_flags_bitwise():
     p.set_elastic(0.0, True, 69e3, 0.3, 3e-3)
     bottom, top = faces(xyz, 1)
     ls_o, ls_g = p.load(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (top, "y", 0.05)])
-    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=60)
-    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=60, cg_maxit=3000)
+    # symmetric K on both sides: after bonds break the literal K^T / Q2 CG of the oracle stalls
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=60, cg_transpose=0, diag10_literal=0)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=60, cg_maxit=3000, cg_transpose=0,
+             diag10_literal=0)
     for s in (p.o, p.g):
         s.update_geometry()
         s.update_force_state()
@@ -282,8 +284,9 @@ def test_elastic_damage_nonlocal_step():
     np.testing.assert_array_equal(p.g.get("nl_ptr"), p.o.geti("nlptr"))
     np.testing.assert_array_equal(p.g.get("nl_idx"), p.o.geti("nlidx"))
     ls_o, ls_g = p.load(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (top, "y", 0.2)])
-    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=80)
-    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=80, cg_maxit=3000)
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=80, cg_transpose=0, diag10_literal=0)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=80, cg_maxit=3000, cg_transpose=0,
+             diag10_literal=0)
     for s in (p.o, p.g):
         s.update_geometry()
         s.update_force_state()
@@ -307,8 +310,9 @@ def test_j2_state_update():
     p.set_j2(0.0, False, 70e3, 0.33, 50.0, 0.3, 2e3, 0.0, 1.0, 1.0)
     bottom, top = faces(xyz, 2)
     ls_o, ls_g = p.load(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", 0.01)])
-    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=80)
-    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=80, cg_maxit=3000)
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=80, cg_transpose=0, diag10_literal=0)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-6, tol_NR=1e-11, max_NR_iter=80, cg_maxit=3000, cg_transpose=0,
+             diag10_literal=0)
     for s in (p.o, p.g):
         s.update_geometry()
         s.update_force_state()
@@ -334,8 +338,8 @@ def test_hcf_fatigue_cycles():
     p.set_type(t)
     p.o.search_nonlocal(1.5)
     p.g.search_nonlocal(1.5)
-    p.o.mode(cg_tol=1e-13, tol_NR=1e-10, max_NR_iter=60)
-    p.g.mode(cg_tol=1e-6, tol_NR=1e-10, max_NR_iter=60, cg_maxit=3000)
+    p.o.mode(cg_tol=1e-13, tol_NR=1e-10, max_NR_iter=60, cg_transpose=0, diag10_literal=0)
+    p.g.mode(cg_tol=1e-6, tol_NR=1e-10, max_NR_iter=60, cg_maxit=3000, cg_transpose=0, diag10_literal=0)
     for s in (p.o, p.g):
         s.update_geometry()
         s.update_force_state()
