@@ -46,14 +46,24 @@ def sc_workload(n):
 
 
 def spmv_bytes(ctx):
-    """Algorithmic bytes of one FP32 3x3-BSR SpMV: values + block column indices + row pointers
-    + one read of p and one write of t (both padded to 16 B per particle)."""
+    """Algorithmic bytes of one FP32 3x3-block SpMV and the kernel that ran it.
+
+    Full store: every block's 9 values + its column index, row pointers, one read of the
+    input vector and one write of the output (both padded to 16 B per particle).
+    Half store (symmetric matrix): the upper blocks (j >= i) with their column indices, for
+    each lower block a column index and the position of its transpose, row pointers and
+    diagonal slots, and the same vector traffic. A lower block's values are the transposed
+    upper block, read a second time from cache, so they count once."""
     nblk = ctx.size("nconn")
     n = ctx.n
-    return 36 * nblk + 4 * nblk + 4 * (n + 1) + 16 * n + 16 * n
+    vec = 16 * n + 16 * n
+    if ctx.stat("spmv_sym_active"):
+        nup, nlo = (nblk + n) // 2, (nblk - n) // 2
+        return 40 * nup + 8 * nlo + 4 * (n + 1) + 4 * n + vec, "k_spmv_sym<float,3,4,true,2> (CG SpMV, half store)"
+    return 40 * nblk + 4 * (n + 1) + vec, "k_spmv<float,3,4,true,2> (CG SpMV, full store)"
 
 
-def setup_gpu(n, tol_nr, cg_tol, precond):
+def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym):
     import lpmgpu as G
 
     xyz, box, r, L = sc_workload(n)
@@ -61,7 +71,7 @@ def setup_gpu(n, tol_nr, cg_tol, precond):
     ctx = G.Context(G.SC3D, r, G.ELASTIC, xyz)
     ctx.set_elastic(0.0, False, 200e3, 0.3, 1.0, elmap_literal=0)
     ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, cg_tol=cg_tol, tol_NR=tol_nr,
-             max_NR_iter=30, cg_maxit=20000, precond=precond)
+             max_NR_iter=30, cg_maxit=20000, precond=precond, spmv_sym=spmv_sym)
     z = xyz[:, 2]
     bottom = np.nonzero(z < z.min() + 1e-6)[0]
     top = np.nonzero(z > z.max() - 1e-6)[0]
@@ -139,6 +149,7 @@ def main():
     ap.add_argument("--tol-nr", type=float, default=1e-10)
     ap.add_argument("--cg-tol", type=float, default=1e-4)
     ap.add_argument("--precond", type=int, default=0)
+    ap.add_argument("--spmv-sym", type=int, default=0, help="1: half store when K is symmetric, 0: full store")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
@@ -150,7 +161,7 @@ def main():
     import lpmgpu as G
 
     G.lib()
-    ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond)
+    ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond, args.spmv_sym)
     if args.verbose:
         print(f"setup {t_setup:.2f} s, n={ctx.n} bonds={ctx.size('nbonds')} blocks={ctx.size('nconn')}", flush=True)
     # warmup steps, then back to the initial state so that the timed steps are steps 1..K
@@ -175,7 +186,7 @@ def main():
     K = args.steps
     spmv_n = ctx.stat("spmv_launches_timed")
     spmv_ms = ctx.stat("spmv_ms") / max(spmv_n, 1)
-    B = spmv_bytes(ctx)
+    B, kname = spmv_bytes(ctx)
     achieved = B / (spmv_ms * 1e-3) / 1e9
     counts = (ctx.stat("newton_iters") / K, ctx.stat("cg_iters") / K, ctx.stat("assemblies") / K)
     # equilibrium check of the last step (size-independent): residual and reaction balance
@@ -197,6 +208,8 @@ def main():
             "workload": f"SC {args.n}^3 elastic uniaxial tension, {ctx.n} particles, 0.5% strain in 20 steps",
             "particles": ctx.n, "bonds": ctx.size("nbonds"), "bsr_blocks": ctx.size("nconn"),
             "newton_tol": args.tol_nr, "cg_rtol": args.cg_tol, "precond": "block-jacobi" if args.precond else "jacobi",
+            "spmv_store": "half (symmetric)" if ctx.stat("spmv_sym_active") else "full",
+            "solves_on_half_store": ctx.stat("sym_solves"), "cg_solves": ctx.stat("cg_solves"),
             "newton_iters_per_step": counts[0], "cg_iters_per_step": counts[1],
             "final_residual_over_reaction": nr / nR if nR > 0 else None,
             "parallelism": "single GPU",
@@ -204,7 +217,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved * 1e9 / HBM_PEAK, "traffic": None,
-            "kernel": "k_spmv<float,3,4,true> (CG SpMV)", "bytes_per_launch": B, "mean_kernel_ms": spmv_ms,
+            "kernel": kname, "bytes_per_launch": B, "mean_kernel_ms": spmv_ms,
             "launches_timed": spmv_n, "isolated_spmv_ms": spmv_iso_ms,
         },
     }

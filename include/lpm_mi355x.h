@@ -99,7 +99,8 @@ typedef struct {
 int lpm_create(const lpm_topology *topo, lpm_ctx **out);
 int lpm_destroy(lpm_ctx *ctx);
 const char *lpm_last_error(const lpm_ctx *ctx); /* ctx may be NULL: last error of lpm_create */
-/* sizes: "n","dim","nbonds","nconn","nnl","ndof","knnz","maxconn","maxbond","nl_ptr" */
+/* sizes: "n","dim","nbonds","nconn","nnl","ndof","knnz","maxconn","maxbond","nl_ptr",
+ * "nslots" (SELL-64 slots of the full matrix), "nslots_u","nslots_l" (of the half store) */
 int64_t lpm_size(const lpm_ctx *ctx, const char *what);
 
 /* Mode switches (each literal reference behaviour has value shown first).
@@ -113,6 +114,11 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *  "cg_tol"           relative CG tolerance (Eigen default 2.22e-16; FP32 runs use ~1e-5)
  *  "cg_maxit"         CG iteration cap (-1: 2*ndof, Eigen default)
  *  "precond"          0: Jacobi (Eigen DiagonalPreconditioner) | 1: 3x3 block Jacobi
+ *  "spmv_sym"         0: SpMV on the full matrix | 1: after each assembly check that the
+ *                        stored matrix is symmetric to rounding and, if so, run the SpMV from
+ *                        its upper half (each block read from HBM once, the transposed half
+ *                        gathered again through the caches; slower on 3D lattices whose
+ *                        neighbour planes are further apart than an XCD's L2 holds)
  *  "max_NR_iter", "tol_NR"  Newton limits (Solver ctor arguments, solver.h:39-50)
  *  "verbose"          print per-iteration norms like the reference
  */
@@ -225,7 +231,8 @@ int lpm_solve_problem_cyclic(lpm_ctx *ctx, int32_t load_type, int32_t n_interval
  * Counters since the last reset: "newton_iters","cg_iters","cg_solves","assemblies",
  * "spmv_launches" (SpMVs that did work); "spmv_ms" / "spmv_launches_timed" (sum of SpMV kernel
  * time from HIP events and the number of launches it covers, when timing is on);
- * "last_cg_iters","last_cg_error". */
+ * "last_cg_iters","last_cg_error"; "sym_solves" (solves that ran on the half store),
+ * "sym_rejects" (assemblies whose matrix failed the symmetry check), "spmv_sym_active". */
 double lpm_stat(const lpm_ctx *ctx, const char *name);
 int lpm_reset_stats(lpm_ctx *ctx);
 /* 1: time every SpMV launch with HIP events on the context stream */

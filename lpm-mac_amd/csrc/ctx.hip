@@ -187,12 +187,14 @@ int calc_stiffness(lpm_ctx *c) {
     if (c->cg_transpose) RC(launch_tslot(c));
     RC(launch_assemble(c));
     c->k_ready = true;
+    c->k_version++;
     c->st_asm++;
     return LPM_OK;
 }
 
 int update_stiffness_disp_bc(lpm_ctx *c) {
     if (!c->k_ready) return lpm_fail(c, LPM_E_STATE, "updateStiffnessDispBC before calcStiffness");
+    c->k_version++;
     return launch_dirichlet(c);
 }
 
@@ -505,7 +507,8 @@ int lpm_destroy(lpm_ctx *c) {
                     c->Ddot_local, c->Ddot_nonlocal, c->nonlocal_L, c->sv, c->sv_last, c->mat, c->dLe_total, c->TdLe_total,
                     c->cs_sum, c->type, c->cons, c->dis0, c->dis, c->dis_last, c->dLe, c->dLp, c->dLp_last, c->bstrain, c->Kn,
                     c->Tv, c->cs, c->bforce, c->bforce_last, c->bdamage, c->bdamage_last, c->residual, c->disp, c->diag64,
-                    c->aval, c->invdiag, c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w, c->scal, c->partial, c->counter, c->iflag, c->sp, c->scol};
+                    c->aval, c->invdiag, c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w, c->scal, c->partial, c->counter, c->iflag, c->sp, c->scol,
+                    c->dslot, c->sp_u, c->sp_l, c->ucol, c->lcol, c->lpos, c->uval};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &g : c->groups)
@@ -531,6 +534,8 @@ int64_t lpm_size(const lpm_ctx *c, const char *what) {
     if (w == "maxbond") return c->maxbond;
     if (w == "nl_ptr") return c->nlptr ? c->n + 1 : 0;
     if (w == "nslots") return c->nslots;
+    if (w == "nslots_u") return c->nslots_u;
+    if (w == "nslots_l") return c->nslots_l;
     return -1;
 }
 
@@ -551,7 +556,10 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "tol_NR") c->tol_NR = v;
     else if (w == "verbose") c->verbose = (int)v;
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
-    else return lpm_fail(c, LPM_E_ARG, "unknown mode " + w);
+    else if (w == "spmv_sym") {
+        c->spmv_sym = (int)v ? 1 : 0;
+        c->sym_version = -1;
+    } else return lpm_fail(c, LPM_E_ARG, "unknown mode " + w);
     return LPM_OK;
 }
 
@@ -570,6 +578,7 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "tol_NR") return c->tol_NR;
     if (w == "verbose") return c->verbose;
     if (w == "cg_batch") return c->cg_batch;
+    if (w == "spmv_sym") return c->spmv_sym;
     return NAN;
 }
 
@@ -884,12 +893,16 @@ double lpm_stat(const lpm_ctx *c, const char *name) {
     if (w == "spmv_launches_timed") return (double)c->st_spmv_timed;
     if (w == "last_cg_iters") return c->last_cg_iters;
     if (w == "last_cg_error") return c->last_cg_err;
+    if (w == "sym_solves") return (double)c->st_sym_solves;
+    if (w == "sym_rejects") return (double)c->st_sym_rejects;
+    if (w == "spmv_sym_active") return c->sym_ok ? 1.0 : 0.0;
     return NAN;
 }
 
 int lpm_reset_stats(lpm_ctx *c) {
     if (!c) return LPM_E_ARG;
     c->st_newton = c->st_cg_iters = c->st_cg_solves = c->st_asm = c->st_spmv = c->st_spmv_timed = 0;
+    c->st_sym_solves = c->st_sym_rejects = 0;
     c->st_spmv_ms = 0;
     return LPM_OK;
 }

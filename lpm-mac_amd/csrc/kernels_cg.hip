@@ -18,7 +18,9 @@
 // iteration: the "done" flag makes the remaining kernels of a batch return at once.
 //
 // Vectors are stored padded to VP entries per particle (VP = 4 in 3D, 2 in 2D).
+#include <algorithm>
 #include <cfloat>
+#include <vector>
 
 #include "lpm_internal.h"
 #include "reduce.h"
@@ -134,6 +136,236 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, int64_t nslice, const i
                 const Vec<T, VP> pj = u[j];
                 for (int r = 0; r < D; ++r)
                     for (int c = 0; c < D; ++c) acc[r] += av[((int64_t)s * EB + r * D + c) * 64] * pj.v[c];
+            }
+        }
+        if (row < n) {
+            Vec<T, VP> wv;
+            for (int t = 0; t < VP; ++t) wv.v[t] = (T)0;
+            for (int r = 0; r < D; ++r) wv.v[r] = acc[r];
+            w[row] = wv;
+            if (FUSED) {
+                const Vec<T, VP> ui = u[row];
+                for (int r = 0; r < D; ++r) delta += (double)ui.v[r] * (double)acc[r];
+            }
+        }
+    }
+    if (!FUSED) return;
+    double v[1] = {delta};
+    block_partial<1>(v, partial, cap);
+}
+
+// ---------------------------------------------------------------- symmetric half store
+// When the stored matrix is symmetric (the analytic tangent of an undamaged or uniformly
+// damaged lattice with the diagonal filled symmetrically, Dirichlet rows and columns
+// eliminated), the SpMV reads each off-diagonal block from HBM once: row i keeps the blocks
+// j >= i (upper store), and for j < i it reads block (j, i) of the upper store transposed.
+// Block (j, i) sits in the slice of row j = i - offset, which the same XCD streamed a few MB
+// earlier, so the second read is served from cache in the common case.
+
+// slot of the diagonal block of every row
+__global__ __launch_bounds__(256) void k_dslot(int64_t n, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cidx,
+                                                int32_t *__restrict__ dslot, int32_t *bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = cptr[i], hi = cptr[i + 1] - 1, pos = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = cidx[mid];
+        if (v == i) { pos = mid; break; }
+        if (v < i) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    if (pos < 0) {
+        atomicOr(bad, 1);
+        dslot[i] = 0;
+    } else {
+        dslot[i] = pos - cptr[i];
+    }
+}
+
+// upper columns; lower (column, position of block (j, i) in the upper store)
+__global__ __launch_bounds__(256) void k_sym_cols(int64_t n, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cidx,
+                                                   const int32_t *__restrict__ dslot, const int64_t *__restrict__ sp_u,
+                                                   const int64_t *__restrict__ sp_l, int32_t *__restrict__ ucol,
+                                                   int32_t *__restrict__ lcol, int32_t *__restrict__ lpos, int32_t *bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k = i >> 6, l = i & 63;
+    const int c0 = cptr[i], nc = cptr[i + 1] - c0, ds = dslot[i];
+    const int Wu = (int)((sp_u[k + 1] - sp_u[k]) >> 6), Wl = (int)((sp_l[k + 1] - sp_l[k]) >> 6);
+    for (int u = 0; u < Wu; ++u) ucol[sp_u[k] + (int64_t)u * 64 + l] = (ds + u < nc) ? cidx[c0 + ds + u] : (int32_t)i;
+    for (int t = 0; t < Wl; ++t) {
+        int32_t col = (int32_t)i, pos = 0;
+        if (t < ds) {
+            const int j = cidx[c0 + t];
+            int lo = cptr[j], hi = cptr[j + 1] - 1, f = -1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                const int v = cidx[mid];
+                if (v == i) { f = mid; break; }
+                if (v < i) lo = mid + 1;
+                else hi = mid - 1;
+            }
+            const int uj = f - cptr[j] - dslot[j];  // slot of i among row j's upper blocks
+            if (f < 0 || uj < 1) {
+                atomicOr(bad, 1);
+            } else {
+                col = j;
+                pos = (int32_t)(sp_u[j >> 6] + (int64_t)uj * 64 + (j & 63));
+            }
+        }
+        lcol[sp_l[k] + (int64_t)t * 64 + l] = col;
+        lpos[sp_l[k] + (int64_t)t * 64 + l] = pos;
+    }
+}
+
+// Fill the upper store with (A_ij + A_ji^T)/2 and flag any pair that is not symmetric to
+// rounding: |A_ij - A_ji^T| > 1e-6 max(|A_ij|, |A_ji|) (block max norms) + 1e-10 max|diag A_ii|.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_sym_pack(int64_t n, const int32_t *__restrict__ cptr, const int32_t *__restrict__ dslot,
+                                                   const int32_t *__restrict__ tslot, const int64_t *__restrict__ sp,
+                                                   const T *__restrict__ aval, const int64_t *__restrict__ sp_u,
+                                                   const int32_t *__restrict__ ucol, T *__restrict__ uval, int32_t *asym) {
+    constexpr int EB = D * D;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k = i >> 6, l = i & 63;
+    const int c0 = cptr[i], nc = cptr[i + 1] - c0, ds = dslot[i];
+    const int Wu = (int)((sp_u[k + 1] - sp_u[k]) >> 6);
+    double dmax = 0.0;
+    for (int r = 0; r < D; ++r) dmax = fmax(dmax, fabs((double)aval[sell_off(sp, i, ds, EB, r * D + r)]));
+    bool bad = false;
+    for (int u = 0; u < Wu; ++u) {
+        T *dst = uval + EB * sp_u[k] + (int64_t)u * EB * 64 + l;
+        if (ds + u >= nc) {
+            for (int e = 0; e < EB; ++e) dst[(int64_t)e * 64] = (T)0;
+            continue;
+        }
+        const int c = ds + u;
+        const int64_t j = ucol[sp_u[k] + (int64_t)u * 64 + l];
+        const int ts = tslot[c0 + c];
+        double a[EB], b[EB];
+        double amax = 0.0, bmax = 0.0, dif = 0.0;
+        for (int e = 0; e < EB; ++e) {
+            a[e] = (double)aval[sell_off(sp, i, c, EB, e)];
+            b[e] = (double)aval[sell_off(sp, j, ts, EB, e)];
+            amax = fmax(amax, fabs(a[e]));
+            bmax = fmax(bmax, fabs(b[e]));
+        }
+        for (int r = 0; r < D; ++r)
+            for (int q = 0; q < D; ++q) {
+                const double x = a[r * D + q], y = b[q * D + r];
+                dif = fmax(dif, fabs(x - y));
+                dst[(int64_t)(r * D + q) * 64] = (T)(0.5 * (x + y));
+            }
+        if (dif > 1e-6 * fmax(amax, bmax) + 1e-10 * dmax) bad = true;
+    }
+    if (bad) atomicOr(asym, 1);
+}
+
+// w = A u from the half store (+ delta = (w, u) partials when FUSED)
+template <typename T, int D, int VP, bool FUSED, int UNR>
+__global__ __launch_bounds__(256) void k_spmv_sym(int64_t n, int64_t nslice, const int32_t *__restrict__ cptr,
+                                                   const int32_t *__restrict__ dslot, const int64_t *__restrict__ sp_u,
+                                                   const int32_t *__restrict__ ucol, const T *__restrict__ uval,
+                                                   const int64_t *__restrict__ sp_l, const int32_t *__restrict__ lcol,
+                                                   const int32_t *__restrict__ lpos, const Vec<T, VP> *__restrict__ u,
+                                                   Vec<T, VP> *__restrict__ w, double *partial, int cap, const double *scal,
+                                                   int remap) {
+    if (FUSED && scal[SC_DONE] != 0.0) return;
+    constexpr int EB = D * D;
+    const int lane = threadIdx.x & 63;
+    const int64_t k = xcd_block(remap) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    double delta = 0.0;
+    if (k < nslice) {
+        const int64_t row = k * 64 + lane;
+        const int ds = row < n ? dslot[row] : 0;
+        const int ulen = row < n ? cptr[row + 1] - cptr[row] - ds : 0;
+        const int llen = ds;
+        T acc[D];
+        for (int r = 0; r < D; ++r) acc[r] = (T)0;
+        // upper blocks, row-major as stored
+        {
+            const int64_t base = sp_u[k];
+            const int W = (int)((sp_u[k + 1] - base) >> 6);
+            const int32_t *ci = ucol + base + lane;
+            const T *av = uval + EB * base + lane;
+            int s = 0;
+            for (; s + UNR <= W; s += UNR) {
+                int j[UNR];
+                Vec<T, VP> pj[UNR];
+                T e[UNR][EB];
+#pragma unroll
+                for (int q = 0; q < UNR; ++q) j[q] = (s + q < ulen) ? ci[(int64_t)(s + q) * 64] : -1;
+#pragma unroll
+                for (int q = 0; q < UNR; ++q) {
+                    if (j[q] >= 0) {
+                        pj[q] = u[j[q]];
+#pragma unroll
+                        for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = av[((int64_t)(s + q) * EB + e2) * 64];
+                    } else {
+                        for (int t = 0; t < VP; ++t) pj[q].v[t] = (T)0;
+                        for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = (T)0;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < UNR; ++q)
+                    for (int r = 0; r < D; ++r)
+                        for (int c = 0; c < D; ++c) acc[r] += e[q][r * D + c] * pj[q].v[c];
+            }
+            for (; s < W; ++s) {
+                if (s < ulen) {
+                    const int j = ci[(int64_t)s * 64];
+                    const Vec<T, VP> pj = u[j];
+                    for (int r = 0; r < D; ++r)
+                        for (int c = 0; c < D; ++c) acc[r] += av[((int64_t)s * EB + r * D + c) * 64] * pj.v[c];
+                }
+            }
+        }
+        // lower blocks: A_ij = (A_ji)^T, read from row j's upper blocks
+        {
+            const int64_t base = sp_l[k];
+            const int W = (int)((sp_l[k + 1] - base) >> 6);
+            const int32_t *ci = lcol + base + lane;
+            const int32_t *pi = lpos + base + lane;
+            int s = 0;
+            for (; s + UNR <= W; s += UNR) {
+                int j[UNR];
+                int64_t q0[UNR];
+                Vec<T, VP> pj[UNR];
+                T e[UNR][EB];
+#pragma unroll
+                for (int q = 0; q < UNR; ++q) {
+                    const bool on = s + q < llen;
+                    j[q] = on ? ci[(int64_t)(s + q) * 64] : -1;
+                    const int64_t pq = on ? (int64_t)pi[(int64_t)(s + q) * 64] : 0;
+                    q0[q] = EB * (pq & ~(int64_t)63) + (pq & 63);
+                }
+#pragma unroll
+                for (int q = 0; q < UNR; ++q) {
+                    if (j[q] >= 0) {
+                        pj[q] = u[j[q]];
+#pragma unroll
+                        for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = uval[q0[q] + (int64_t)e2 * 64];
+                    } else {
+                        for (int t = 0; t < VP; ++t) pj[q].v[t] = (T)0;
+                        for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = (T)0;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < UNR; ++q)
+                    for (int r = 0; r < D; ++r)
+                        for (int c = 0; c < D; ++c) acc[r] += e[q][c * D + r] * pj[q].v[c];
+            }
+            for (; s < W; ++s) {
+                if (s < llen) {
+                    const int j = ci[(int64_t)s * 64];
+                    const int64_t pq = pi[(int64_t)s * 64];
+                    const int64_t b0 = EB * (pq & ~(int64_t)63) + (pq & 63);
+                    const Vec<T, VP> pj = u[j];
+                    for (int r = 0; r < D; ++r)
+                        for (int c = 0; c < D; ++c) acc[r] += uval[b0 + (int64_t)(c * D + r) * 64] * pj.v[c];
+                }
             }
         }
         if (row < n) {
@@ -383,6 +615,17 @@ template <typename T, int D, int VP>
 static int spmv_launch(lpm_ctx *c, bool fused, void *in, void *out) {
     typedef Vec<T, VP> V;
     const int g = spmv_grid(c);
+    if (c->sym_ok) {
+        if (fused)
+            hipLaunchKernelGGL((k_spmv_sym<T, D, VP, true, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->nslice, c->cptr,
+                               c->dslot, c->sp_u, c->ucol, (const T *)c->uval, c->sp_l, c->lcol, c->lpos, (const V *)in,
+                               (V *)out, c->partial + 2 * (int64_t)c->partial_cap, c->partial_cap, (const double *)c->scal, 1);
+        else
+            hipLaunchKernelGGL((k_spmv_sym<T, D, VP, false, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->nslice, c->cptr,
+                               c->dslot, c->sp_u, c->ucol, (const T *)c->uval, c->sp_l, c->lcol, c->lpos, (const V *)in,
+                               (V *)out, c->partial, c->partial_cap, (const double *)c->scal, 1);
+        return check_launch(c, "k_spmv_sym");
+    }
     if (fused)
         hipLaunchKernelGGL((k_spmv<T, D, VP, true, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->nslice, c->sp, c->cptr,
                            c->scol, (const T *)c->aval, (const V *)in, (V *)out, c->partial + 2 * (int64_t)c->partial_cap,
@@ -392,6 +635,91 @@ static int spmv_launch(lpm_ctx *c, bool fused, void *in, void *out) {
                            c->scol, (const T *)c->aval, (const V *)in, (V *)out, c->partial, c->partial_cap,
                            (const double *)c->scal, 1);
     return check_launch(c, "k_spmv");
+}
+
+// Index structure of the half store (once per context; the pattern never changes).
+static int sym_setup(lpm_ctx *c) {
+    if (c->dslot) return LPM_OK;
+    const int64_t n = c->n;
+    LPM_HIP(c, hipMalloc(&c->dslot, sizeof(int32_t) * (n > 0 ? n : 1)));
+    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_dslot, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->cptr, c->cidx, c->dslot, c->iflag);
+    int rc = check_launch(c, "k_dslot");
+    if (rc) return rc;
+    std::vector<int32_t> ds(n), cp(n + 1);
+    LPM_HIP(c, hipMemcpyAsync(ds.data(), c->dslot, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(cp.data(), c->cptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->h_iflag, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->h_iflag[0]) return lpm_fail(c, LPM_E_TOPO, "a pattern row lacks its diagonal block");
+    std::vector<int64_t> su(c->nslice + 1, 0), sl(c->nslice + 1, 0);
+    for (int64_t k = 0; k < c->nslice; ++k) {
+        int wu = 0, wl = 0;
+        for (int64_t i = 64 * k; i < std::min<int64_t>(n, 64 * k + 64); ++i) {
+            wu = std::max(wu, cp[i + 1] - cp[i] - ds[i]);
+            wl = std::max(wl, ds[i]);
+        }
+        su[k + 1] = su[k] + 64LL * wu;
+        sl[k + 1] = sl[k] + 64LL * wl;
+    }
+    c->nslots_u = su[c->nslice];
+    c->nslots_l = sl[c->nslice];
+    if (c->nslots_u >= 0x7fffffffLL) return lpm_fail(c, LPM_E_LIMIT, "upper store exceeds 32-bit block positions");
+    LPM_HIP(c, hipMalloc(&c->sp_u, sizeof(int64_t) * (c->nslice + 1)));
+    LPM_HIP(c, hipMalloc(&c->sp_l, sizeof(int64_t) * (c->nslice + 1)));
+    LPM_HIP(c, hipMalloc(&c->ucol, sizeof(int32_t) * std::max<int64_t>(c->nslots_u, 1)));
+    LPM_HIP(c, hipMalloc(&c->lcol, sizeof(int32_t) * std::max<int64_t>(c->nslots_l, 1)));
+    LPM_HIP(c, hipMalloc(&c->lpos, sizeof(int32_t) * std::max<int64_t>(c->nslots_l, 1)));
+    LPM_HIP(c, hipMemcpyAsync(c->sp_u, su.data(), sizeof(int64_t) * (c->nslice + 1), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->sp_l, sl.data(), sizeof(int64_t) * (c->nslice + 1), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_sym_cols, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->cptr, c->cidx, c->dslot, c->sp_u,
+                       c->sp_l, c->ucol, c->lcol, c->lpos, c->iflag);
+    rc = check_launch(c, "k_sym_cols");
+    if (rc) return rc;
+    LPM_HIP(c, hipMemcpyAsync(c->h_iflag, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->h_iflag[0]) return lpm_fail(c, LPM_E_TOPO, "pattern is not structurally symmetric");
+    return LPM_OK;
+}
+
+template <typename T, int D>
+static int sym_pack_t(lpm_ctx *c) {
+    hipLaunchKernelGGL((k_sym_pack<T, D>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->dslot,
+                       c->tslot, c->sp, (const T *)c->aval, c->sp_u, c->ucol, (T *)c->uval, c->iflag);
+    return check_launch(c, "k_sym_pack");
+}
+
+// Refresh the half store after the matrix changed; c->sym_ok says whether the SpMV may use it.
+int sym_prepare(lpm_ctx *c) {
+    if (!c->spmv_sym) {
+        c->sym_ok = false;
+        return LPM_OK;
+    }
+    if (c->sym_version == c->k_version && c->uval_prec == c->precision) return LPM_OK;
+    c->sym_ok = false;
+    int rc = sym_setup(c);
+    if (rc) return rc;
+    if (!c->tslot && (rc = launch_tslot(c))) return rc;
+    const size_t es = c->precision == 0 ? sizeof(float) : sizeof(double);
+    if (c->uval && c->uval_prec != c->precision) {
+        hipFree(c->uval);
+        c->uval = nullptr;
+    }
+    if (!c->uval) {
+        LPM_HIP(c, hipMalloc(&c->uval, es * c->dim * c->dim * std::max<int64_t>(c->nslots_u, 1)));
+        c->uval_prec = c->precision;
+    }
+    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+    if (c->precision == 0) rc = c->dim == 3 ? sym_pack_t<float, 3>(c) : sym_pack_t<float, 2>(c);
+    else rc = c->dim == 3 ? sym_pack_t<double, 3>(c) : sym_pack_t<double, 2>(c);
+    if (rc) return rc;
+    LPM_HIP(c, hipMemcpyAsync(c->h_iflag, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    c->sym_ok = c->h_iflag[0] == 0;
+    if (!c->sym_ok) c->st_sym_rejects++;
+    c->sym_version = c->k_version;
+    return LPM_OK;
 }
 
 template <typename T, int D, int VP>
@@ -466,6 +794,8 @@ int cg_solve(lpm_ctx *c, int *iters, double *error, int *success) {
     if (!c->k_ready) return lpm_fail(c, LPM_E_STATE, "solve before the stiffness matrix was assembled");
     int rc = launch_precond_setup(c);
     if (rc) return rc;
+    if ((rc = sym_prepare(c))) return rc;
+    if (c->sym_ok) c->st_sym_solves++;
     if (c->precision == 0)
         return c->dim == 3 ? cg_t<float, 3, 4>(c, iters, error, success) : cg_t<float, 2, 2>(c, iters, error, success);
     return c->dim == 3 ? cg_t<double, 3, 4>(c, iters, error, success) : cg_t<double, 2, 2>(c, iters, error, success);
@@ -508,6 +838,8 @@ static int bench_t(lpm_ctx *c, int reps, double *ms) {
 int spmv_bench(lpm_ctx *c, int reps, double *ms) {
     if (!c->k_ready) return lpm_fail(c, LPM_E_STATE, "no assembled matrix");
     if (reps <= 0) return lpm_fail(c, LPM_E_ARG, "reps must be positive");
+    int rc = sym_prepare(c);
+    if (rc) return rc;
     if (c->precision == 0) return c->dim == 3 ? bench_t<float, 3, 4>(c, reps, ms) : bench_t<float, 2, 2>(c, reps, ms);
     return c->dim == 3 ? bench_t<double, 3, 4>(c, reps, ms) : bench_t<double, 2, 2>(c, reps, ms);
 }

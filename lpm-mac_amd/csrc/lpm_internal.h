@@ -62,6 +62,18 @@ struct lpm_ctx {
     int64_t *sp = nullptr;
     int32_t *scol = nullptr;
     int64_t nslice = 0, nslots = 0;
+    // Symmetric half of the CG matrix, used by the SpMV when the stored matrix is symmetric:
+    // upper SELL-64 (blocks j >= i of row i, the diagonal first) holding the values, and a
+    // lower SELL-64 list (blocks j < i) of (column, position of block (j, i) in the upper store)
+    int spmv_sym = 0;                 // mode: 1 use it when the symmetry check passes, 0 never
+    int32_t *dslot = nullptr;         // slot of the diagonal block in each row of the pattern
+    int64_t *sp_u = nullptr, *sp_l = nullptr;
+    int32_t *ucol = nullptr, *lcol = nullptr, *lpos = nullptr;
+    int64_t nslots_u = 0, nslots_l = 0;
+    void *uval = nullptr;
+    int uval_prec = -1;
+    int64_t k_version = 0, sym_version = -1;  // assembly generation / generation packed
+    bool sym_ok = false;
     int32_t *nlptr = nullptr, *nlidx = nullptr;
     // particle state (device, FP64 unless noted)
     double *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr, *Pin = nullptr, *Pex = nullptr;
@@ -93,6 +105,7 @@ struct lpm_ctx {
     std::vector<Group> groups;
     // statistics
     int64_t st_newton = 0, st_cg_iters = 0, st_cg_solves = 0, st_asm = 0, st_spmv = 0, st_spmv_timed = 0;
+    int64_t st_sym_solves = 0, st_sym_rejects = 0;
     double st_spmv_ms = 0, last_cg_err = 0;
     int last_cg_iters = 0, timing = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -129,6 +142,7 @@ int cg_solve(lpm_ctx *c, int *iters, double *error, int *success);
 int launch_move(lpm_ctx *c);
 int spmv_bench(lpm_ctx *c, int reps, double *ms);
 int launch_precond_setup(lpm_ctx *c);
+int sym_prepare(lpm_ctx *c);
 
 // helpers
 static inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }

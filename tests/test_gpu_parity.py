@@ -198,6 +198,58 @@ def test_cg_fp64_matches_dense_solution():
     np.testing.assert_array_equal(p.g.get("xyz").reshape(-1, 3), x_exp + d.reshape(-1, 3))
 
 
+@pytest.mark.parametrize("lattice", ["sc", "hex"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_half_store_spmv_matches_full(lattice, precision):
+    """The half (upper) store that the SpMV uses for a symmetric matrix gives the CG solution
+    of the full SELL store on a deformed lattice; a matrix that is not symmetric (diagonal
+    K[1][0] left 0, derivative.h:38) is detected and solved from the full store."""
+    say("[gpu] half store", lattice, "precision", precision)
+    if lattice == "sc":
+        p, xyz, lo, hi = _bar(n=6)
+        axes, dim = "xyz", 3
+    else:
+        xyz, box = hex_plate(14, 14)
+        p = Pair(O.HEXAGON2D, 0.5, O.ELASTIC, xyz, box)
+        p.set_elastic(0.0, False, 69e3, 0.3, 1.0)
+        lo, hi = faces(xyz, 1)
+        axes, dim = "xy", 2
+    g = p.g
+    _, ls_g = p.load(disp=[(lo, a, 0.0) for a in axes] + [(hi, axes[-1], 0.01)])
+    x_def = deformed(xyz, 5, dim=dim)
+    tol = 1e-6 if precision == 0 else 1e-13
+    sols = {}
+    for sym in (1, 0):
+        g.set("xyz", x_def)
+        g.set("disp_constraint", np.zeros(3 * len(xyz), dtype=np.int32))
+        g.apply_bc(ls_g)
+        g.mode(cg_transpose=0, diag10_literal=0, cg_tol=tol, precision=precision, cg_maxit=20000, spmv_sym=sym)
+        g.update_geometry()
+        g.update_force_state()
+        g.update_rr()
+        g.calc_stiffness()
+        g.update_stiffness_disp_bc()
+        g.reset_stats()
+        it, err = g.solve_linear_system()
+        assert err <= tol
+        assert g.stat("sym_solves") == sym and g.stat("sym_rejects") == 0
+        sols[sym] = g.get("disp")
+        # the isolated SpMV runs on the same store
+        assert g.spmv_bench(3) > 0
+    assert rel(sols[1], sols[0]) < (2e-5 if precision == 0 else 1e-11)
+    # not symmetric: rejected, full store used, same answer as the oracle's dense solve
+    g.set("xyz", x_def)
+    g.mode(diag10_literal=1, spmv_sym=1)
+    g.update_geometry()
+    g.update_force_state()
+    g.update_rr()
+    g.calc_stiffness()
+    g.update_stiffness_disp_bc()
+    g.reset_stats()
+    g.solve_linear_system()
+    assert g.stat("sym_rejects") == 1 and g.stat("sym_solves") == 0
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_static_step_equilibrium_parity(precision):
     """One static load step (solver_static.h:97-149) with Dirichlet elimination, both sides
