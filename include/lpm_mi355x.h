@@ -113,12 +113,20 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *  "precision"        0: FP32 K, SpMV and CG (FP64 residual, positions, state) | 1: FP64 everywhere
  *  "cg_tol"           relative CG tolerance (Eigen default 2.22e-16; FP32 runs use ~1e-5)
  *  "cg_maxit"         CG iteration cap (-1: 2*ndof, Eigen default)
+ *  "cg_forcing"       0: every Newton iteration solves to cg_tol | 1: inexact Newton, the
+ *                        solve of iteration k stops at max(cg_tol, min(0.1, tol_NR*tol_mult/
+ *                        (2 ||r_k||))), i.e. it asks only for the reduction still needed to pass
+ *                        the Newton test; the Newton test itself is unchanged
  *  "precond"          0: Jacobi (Eigen DiagonalPreconditioner) | 1: 3x3 block Jacobi
  *  "spmv_sym"         0: SpMV on the full matrix | 1: after each assembly check that the
  *                        stored matrix is symmetric to rounding and, if so, run the SpMV from
  *                        its upper half (each block read from HBM once, the transposed half
  *                        gathered again through the caches; slower on 3D lattices whose
  *                        neighbour planes are further apart than an XCD's L2 holds)
+ *  "spmv_tune"        1: on the first solve of each matrix kind time every SpMV launch
+ *                        configuration and keep the fastest | 0: use the values below
+ *  "spmv_unr" (1,2,4), "spmv_wpb" (wavefronts per workgroup 1,2,4,8), "spmv_remap" (0/1:
+ *                        XCD-contiguous workgroup order) -- the SpMV launch configuration
  *  "max_NR_iter", "tol_NR"  Newton limits (Solver ctor arguments, solver.h:39-50)
  *  "verbose"          print per-iteration norms like the reference
  */
@@ -235,8 +243,10 @@ int lpm_solve_problem_cyclic(lpm_ctx *ctx, int32_t load_type, int32_t n_interval
  * "sym_rejects" (assemblies whose matrix failed the symmetry check), "spmv_sym_active". */
 double lpm_stat(const lpm_ctx *ctx, const char *name);
 int lpm_reset_stats(lpm_ctx *ctx);
-/* 1: time every SpMV launch with HIP events on the context stream */
-int lpm_set_timing(lpm_ctx *ctx, int on);
+/* every > 0: time the CG SpMV of every `every`-th iteration with HIP events on the context
+ * stream (1 = every launch; each event record idles the GPU for a few microseconds, so a
+ * sparse sample keeps the measurement out of the solve time); 0: off */
+int lpm_set_timing(lpm_ctx *ctx, int every);
 /* Run the CG SpMV kernel `reps` times on the stored matrix (x = current p vector) and return
  * the mean kernel time in milliseconds (HIP events on the context stream). */
 int lpm_spmv_bench(lpm_ctx *ctx, int32_t reps, double *ms);

@@ -233,7 +233,16 @@ int newton_iteration(lpm_ctx *c, int *n_iter) {
             return LPM_OK;
         }
         if (c->verbose) printf("|  |  Iteration-%d: ", ni);
-        RC(solve_linear_system(c, nullptr, nullptr));
+        const double cg_tol_set = c->cg_tol;
+        if (c->cg_forcing) {
+            // inexact Newton: ask CG only for the reduction that brings the residual under the
+            // Newton target (with a factor 2 margin), never less than the set cg_tol, at most 0.1
+            const double eta = 0.5 * c->tol_NR * tol_multiplier / norm_residual;
+            c->cg_tol = std::max(cg_tol_set, std::min(0.1, eta));
+        }
+        const int rc_solve = solve_linear_system(c, nullptr, nullptr);
+        c->cg_tol = cg_tol_set;
+        RC(rc_solve);
         RC(geometry_force_state(c));
         RC(launch_residual(c, &r2, nullptr));
         norm_residual = std::sqrt(r2);
@@ -454,7 +463,8 @@ int lpm_create(const lpm_topology *t, lpm_ctx **out) {
     AL(scal, SC_COUNT); AL(counter, 8); AL(iflag, 8);
 #undef AL
     c->partial_cap = std::max({blocks_for(n * c->dim, 256), blocks_for(n, 4), blocks_for(n, 256), 1});
-    if ((rc = dalloc(c, &c->partial, 3 * (int64_t)c->partial_cap))) return fail(rc);
+    // CG: (||r||^2, gamma) partials in two parity slots + the SpMV's delta partials
+    if ((rc = dalloc(c, &c->partial, 5 * (int64_t)c->partial_cap))) return fail(rc);
     if (hipHostMalloc((void **)&c->h_scal, sizeof(double) * SC_COUNT) != hipSuccess ||
         hipHostMalloc((void **)&c->h_iflag, sizeof(int32_t) * 8) != hipSuccess) {
         c->err = "hipHostMalloc failed";
@@ -556,6 +566,17 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "tol_NR") c->tol_NR = v;
     else if (w == "verbose") c->verbose = (int)v;
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
+    else if (w == "cg_forcing") c->cg_forcing = (int)v ? 1 : 0;
+    else if (w == "spmv_tune") {
+        c->spmv_tune = (int)v ? 1 : 0;
+        c->tune_key = -1;
+    } else if (w == "spmv_unr") {
+        if (v != 1 && v != 2 && v != 4) return lpm_fail(c, LPM_E_ARG, "spmv_unr must be 1, 2 or 4");
+        c->spmv_unr = (int)v;
+    } else if (w == "spmv_wpb") {
+        if (v != 1 && v != 2 && v != 4 && v != 8) return lpm_fail(c, LPM_E_ARG, "spmv_wpb must be 1, 2, 4 or 8");
+        c->spmv_wpb = (int)v;
+    } else if (w == "spmv_remap") c->spmv_remap = (int)v ? 1 : 0;
     else if (w == "spmv_sym") {
         c->spmv_sym = (int)v ? 1 : 0;
         c->sym_version = -1;
@@ -579,6 +600,11 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "verbose") return c->verbose;
     if (w == "cg_batch") return c->cg_batch;
     if (w == "spmv_sym") return c->spmv_sym;
+    if (w == "spmv_tune") return c->spmv_tune;
+    if (w == "cg_forcing") return c->cg_forcing;
+    if (w == "spmv_unr") return c->spmv_unr;
+    if (w == "spmv_wpb") return c->spmv_wpb;
+    if (w == "spmv_remap") return c->spmv_remap;
     return NAN;
 }
 
@@ -909,7 +935,7 @@ int lpm_reset_stats(lpm_ctx *c) {
 
 int lpm_set_timing(lpm_ctx *c, int on) {
     if (!c) return LPM_E_ARG;
-    c->timing = on ? 1 : 0;
+    c->timing = on > 0 ? on : 0;
     return LPM_OK;
 }
 

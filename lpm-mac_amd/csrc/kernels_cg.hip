@@ -8,10 +8,10 @@
 //
 // Solver: Jacobi (or 3x3 block-Jacobi) preconditioned CG in the single-reduction form of
 // Chronopoulos and Gear (J. Comput. Appl. Math. 25, 1989): per iteration one SpMV (which
-// also forms delta = (w, u)), one fused vector kernel (p, s, x, r, u updates plus ||r||^2
-// and gamma = (r, u)) and one one-block finalize that turns the partial sums into
-// alpha/beta and the stopping decision. The stopping rule and defaults are Eigen 3.4's:
-// x0 = 0, stop when ||r||^2 < max(tol^2 ||b||^2, DBL_MIN) or after maxIters iterations.
+// also forms delta = (w, u)) and one fused vector kernel whose workgroups first reduce the
+// partial sums to alpha/beta and the stopping decision, then do the p, s, x, r, u updates
+// and the partials of ||r||^2 and gamma = (r, u). The stopping rule and defaults are Eigen
+// 3.4's: x0 = 0, stop when ||r||^2 < max(tol^2 ||b||^2, DBL_MIN) or after maxIters iterations.
 // It reaches the same solution as Eigen's recurrence (the tests check the converged
 // vectors, not the iterates). Scalars are FP64 in device memory; vectors and matrix are
 // FP32 (precision 0) or FP64 (precision 1). No host synchronisation happens inside an
@@ -62,11 +62,10 @@ __global__ __launch_bounds__(256) void k_cg_init(int64_t n, const double *__rest
                                                   int block_pre, Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
                                                   Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
                                                   Vec<T, VP> *__restrict__ s, double *partial, int cap, double *scal) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // the prologue SpMV that follows checks the flag; clear what the previous solve left
-    if (i == 0) scal[SC_DONE] = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[SC_DONE] = 0.0;
     double v[2] = {0.0, 0.0};
-    if (i < n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         Vec<T, VP> rv, zv, zero;
         for (int k = 0; k < VP; ++k) { rv.v[k] = (T)0; zero.v[k] = (T)0; }
         for (int k = 0; k < D; ++k) rv.v[k] = (T)b64[D * i + k];
@@ -88,7 +87,7 @@ __global__ __launch_bounds__(256) void k_cg_init(int64_t n, const double *__rest
 // One wavefront per 64-row slice; lane l owns row 64k+l and walks its slots. UNR slots are
 // loaded before they are consumed so that each lane has several independent loads in flight.
 template <typename T, int D, int VP, bool FUSED, int UNR>
-__global__ __launch_bounds__(256) void k_spmv(int64_t n, int64_t nslice, const int64_t *__restrict__ sp,
+__global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const int64_t *__restrict__ sp,
                                                const int32_t *__restrict__ cptr, const int32_t *__restrict__ scol,
                                                const T *__restrict__ aval, const Vec<T, VP> *__restrict__ u,
                                                Vec<T, VP> *__restrict__ w, double *partial, int cap, const double *scal,
@@ -265,7 +264,7 @@ __global__ __launch_bounds__(256) void k_sym_pack(int64_t n, const int32_t *__re
 
 // w = A u from the half store (+ delta = (w, u) partials when FUSED)
 template <typename T, int D, int VP, bool FUSED, int UNR>
-__global__ __launch_bounds__(256) void k_spmv_sym(int64_t n, int64_t nslice, const int32_t *__restrict__ cptr,
+__global__ __launch_bounds__(512) void k_spmv_sym(int64_t n, int64_t nslice, const int32_t *__restrict__ cptr,
                                                    const int32_t *__restrict__ dslot, const int64_t *__restrict__ sp_u,
                                                    const int32_t *__restrict__ ucol, const T *__restrict__ uval,
                                                    const int64_t *__restrict__ sp_l, const int32_t *__restrict__ lcol,
@@ -384,20 +383,86 @@ __global__ __launch_bounds__(256) void k_spmv_sym(int64_t n, int64_t nslice, con
     block_partial<1>(v, partial, cap);
 }
 
-// ---------------------------------------------------------------- fused vector update
-// p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = M^-1 r;
-// partials ||r||^2 and gamma = (r, u)
+// ---------------------------------------------------------------- fused scalar step + vector update
+// Iteration `it` (the host passes it; every workgroup of the launch sees the same value):
+//  1. every workgroup sums the partials of ||r_it||^2 and gamma_it = (r_it, u_it) (written by
+//     the previous vector kernel, parity slot it&1) and delta_it = (w_it, u_it) (written by the
+//     SpMV) in one fixed order, so all workgroups reach the same stopping decision and the same
+//     alpha/beta without a separate finalize launch;
+//  2. stop if ||r_it||^2 < thr (it = 0: thr = max(tol^2 ||b||^2, DBL_MIN), zero rhs) or it >= maxit;
+//  3. else p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = M^-1 r, and write the
+//     partials of ||r||^2 and gamma for iteration it+1 into the other parity slot.
+// Workgroup 0 records the scalars; gamma and alpha live in per-parity slots so that the
+// workgroups of one launch read the previous iteration's values while workgroup 0 writes its own.
 template <typename T, int D, int VP>
 __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__ invd, int block_pre,
                                                  Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
                                                  Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
                                                  Vec<T, VP> *__restrict__ s, const Vec<T, VP> *__restrict__ w,
-                                                 double *partial, int cap, const double *scal) {
+                                                 double *partial, int cap, int np_vec, int np_spmv, double *scal, int it,
+                                                 double tol, double maxit) {
     if (scal[SC_DONE] != 0.0) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[3][4];
+    __shared__ double coef[3];  // alpha, beta, done
+    {
+        const double *pv = partial + (int64_t)(it & 1) * 2 * cap;
+        const double *pd = partial + (int64_t)4 * cap;
+        double t[3] = {0.0, 0.0, 0.0};
+        for (int b = threadIdx.x; b < np_vec; b += blockDim.x) {
+            t[0] += pv[b];
+            t[1] += pv[cap + b];
+        }
+        for (int b = threadIdx.x; b < np_spmv; b += blockDim.x) t[2] += pd[b];
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        for (int k = 0; k < 3; ++k) {
+            double v = t[k];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) red[k][wid] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int nw = blockDim.x >> 6;
+            double tot[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k < 3; ++k)
+                for (int q = 0; q < nw; ++q) tot[k] += red[k][q];
+            const double rr = tot[0], gamma = tot[1], delta = tot[2];
+            const int cur = it & 1, prev = cur ^ 1;
+            double alpha = 0.0, beta = 0.0, done = 0.0, thr;
+            if (it == 0) {
+                thr = fmax(tol * tol * rr, DBL_MIN);
+                done = (rr == 0.0) ? 3.0 : ((rr < thr) ? 1.0 : (maxit <= 0 ? 2.0 : 0.0));
+                alpha = gamma / delta;
+            } else {
+                thr = scal[SC_THR];
+                if (rr < thr) done = 1.0;
+                else if ((double)it >= maxit) done = 2.0;
+                else {
+                    beta = gamma / scal[SC_GAMMA_P + prev];
+                    alpha = gamma / (delta - beta * gamma / scal[SC_ALPHA_P + prev]);
+                }
+            }
+            coef[0] = alpha;
+            coef[1] = beta;
+            coef[2] = done;
+            if (blockIdx.x == 0) {
+                if (it == 0) {
+                    scal[SC_RHS2] = rr;
+                    scal[SC_THR] = thr;
+                    scal[SC_MAXIT] = maxit;
+                }
+                scal[SC_RR] = rr;
+                scal[SC_ITER] = it;
+                scal[SC_GAMMA_P + cur] = gamma;
+                scal[SC_ALPHA_P + cur] = alpha;
+                scal[SC_DONE] = done;
+            }
+        }
+        __syncthreads();
+    }
+    if (coef[2] != 0.0) return;
     double v[2] = {0.0, 0.0};
-    if (i < n) {
-        const T alpha = (T)scal[SC_ALPHA], beta = (T)scal[SC_BETA];
+    const T alpha = (T)coef[0], beta = (T)coef[1];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         Vec<T, VP> pv = p[i], sv = s[i], xv = x[i], rv = r[i], zv;
         const Vec<T, VP> uv = u[i], wv = w[i];
         for (int k = 0; k < D; ++k) {
@@ -417,62 +482,29 @@ __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__
             v[1] += (double)rv.v[k] * (double)zv.v[k];
         }
     }
-    block_partial<2>(v, partial, cap);
+    block_partial<2>(v, partial + (int64_t)((it & 1) ^ 1) * 2 * cap, cap);
 }
 
 // ---------------------------------------------------------------- one-block finalize
-// Sums the per-block partials in a fixed order and advances the scalars.
-//  FIN_INIT : parts (||b||^2, gamma0) + delta0 at offset 2   -> threshold, alpha0, done
-//  FIN_STEP : parts (||r||^2, gamma) + delta at offset 2    -> converged? / beta, alpha, count
-//  FIN_NORMS: (||r||^2, ||R||^2) for the Newton loop; FIN_DIAG: ||diag K||^2
-__global__ __launch_bounds__(1024) void k_finalize(int kind, int na, int nb, const double *__restrict__ partial, int cap,
-                                                   double *__restrict__ scal, double tol, double maxit) {
-    if (kind == FIN_STEP && scal[SC_DONE] != 0.0) return;
-    __shared__ double red[3][16];
+// Sums per-block partials in a fixed order: FIN_NORMS (||r||^2, ||R||^2) for the Newton loop,
+// FIN_DIAG ||diag K||^2 for the Dirichlet elimination.
+__global__ __launch_bounds__(1024) void k_finalize(int kind, int na, const double *__restrict__ partial, int cap,
+                                                   double *__restrict__ scal) {
+    __shared__ double red[2][16];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int nv = (kind == FIN_DIAG) ? 1 : ((kind == FIN_NORMS) ? 2 : 3);
+    const int nv = (kind == FIN_DIAG) ? 1 : 2;
     for (int k = 0; k < nv; ++k) {
-        const int np = (k < 2) ? na : nb;
         double t = 0;
-        for (int b = threadIdx.x; b < np; b += blockDim.x) t += partial[k * cap + b];
+        for (int b = threadIdx.x; b < na; b += blockDim.x) t += partial[k * cap + b];
         for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
         if (lane == 0) red[k][wid] = t;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    double tot[3] = {0.0, 0.0, 0.0};
+    double tot[2] = {0.0, 0.0};
     for (int k = 0; k < nv; ++k)
         for (int w = 0; w < nw; ++w) tot[k] += red[k][w];
-    if (kind == FIN_INIT) {
-        const double rhs2 = tot[0];
-        const double thr = fmax(tol * tol * rhs2, DBL_MIN);
-        scal[SC_RHS2] = rhs2;
-        scal[SC_RR] = rhs2;
-        scal[SC_ITER] = 0;
-        scal[SC_MAXIT] = maxit;
-        scal[SC_THR] = thr;
-        scal[SC_ABSNEW] = tot[1];
-        scal[SC_BETA] = 0.0;
-        scal[SC_ALPHA] = tot[1] / tot[2];
-        scal[SC_DONE] = (rhs2 == 0.0) ? 3.0 : ((rhs2 < thr) ? 1.0 : (maxit <= 0 ? 2.0 : 0.0));
-    } else if (kind == FIN_STEP) {
-        scal[SC_RR] = tot[0];
-        const double it = scal[SC_ITER] + 1.0;
-        scal[SC_ITER] = it;
-        if (tot[0] < scal[SC_THR]) {
-            scal[SC_DONE] = 1.0;
-        } else if (it >= scal[SC_MAXIT]) {
-            scal[SC_DONE] = 2.0;
-        } else {
-            const double gamma_old = scal[SC_ABSNEW], alpha_old = scal[SC_ALPHA];
-            const double gamma = tot[1], delta = tot[2];
-            const double beta = gamma / gamma_old;
-            scal[SC_BETA] = beta;
-            scal[SC_ALPHA] = gamma / (delta - beta * gamma / alpha_old);
-            scal[SC_ABSNEW] = gamma;
-            scal[SC_PT] = delta;
-        }
-    } else if (kind == FIN_NORMS) {
+    if (kind == FIN_NORMS) {
         scal[SC_NORM_R] = tot[0];
         scal[SC_NORM_REAC] = tot[1];
     } else {
@@ -562,10 +594,10 @@ static int check_launch(lpm_ctx *c, const char *what) {
     return LPM_OK;
 }
 
-int launch_finalize(lpm_ctx *c, int kind, int na, int nb, double tol, double maxit) {
-    if (na > c->partial_cap || nb > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "reduction exceeds partial buffer");
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, kind, na, nb, (const double *)c->partial,
-                       c->partial_cap, c->scal, tol, maxit);
+int launch_finalize(lpm_ctx *c, int kind, int na) {
+    if (na > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "reduction exceeds partial buffer");
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, kind, na, (const double *)c->partial,
+                       c->partial_cap, c->scal);
     return check_launch(c, "k_finalize");
 }
 
@@ -608,33 +640,91 @@ int launch_precond_setup(lpm_ctx *c) {
     return c->dim == 3 ? precond_t<double, 3, 4>(c) : precond_t<double, 2, 2>(c);
 }
 
-static int spmv_grid(const lpm_ctx *c) { return blocks_for(c->nslice, 4); }
+static int spmv_grid(const lpm_ctx *c) { return blocks_for(c->nslice, c->spmv_wpb); }
 
-// w = A u (fused: + delta partials at offset 2, skipped once the solve is done)
+template <typename T, int D, int VP, bool FUSED, int UNR>
+static void spmv_go(lpm_ctx *c, const void *in, void *out, double *part) {
+    typedef Vec<T, VP> V;
+    const int g = spmv_grid(c), tb = 64 * c->spmv_wpb;
+    if (c->sym_ok)
+        hipLaunchKernelGGL((k_spmv_sym<T, D, VP, FUSED, UNR>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice, c->cptr,
+                           c->dslot, c->sp_u, c->ucol, (const T *)c->uval, c->sp_l, c->lcol, c->lpos, (const V *)in, (V *)out,
+                           part, c->partial_cap, (const double *)c->scal, c->spmv_remap);
+    else
+        hipLaunchKernelGGL((k_spmv<T, D, VP, FUSED, UNR>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice, c->sp, c->cptr,
+                           c->scol, (const T *)c->aval, (const V *)in, (V *)out, part, c->partial_cap,
+                           (const double *)c->scal, c->spmv_remap);
+}
+
+// w = A u (fused: + delta partials, skipped once the solve is done)
 template <typename T, int D, int VP>
 static int spmv_launch(lpm_ctx *c, bool fused, void *in, void *out) {
-    typedef Vec<T, VP> V;
-    const int g = spmv_grid(c);
-    if (c->sym_ok) {
-        if (fused)
-            hipLaunchKernelGGL((k_spmv_sym<T, D, VP, true, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->nslice, c->cptr,
-                               c->dslot, c->sp_u, c->ucol, (const T *)c->uval, c->sp_l, c->lcol, c->lpos, (const V *)in,
-                               (V *)out, c->partial + 2 * (int64_t)c->partial_cap, c->partial_cap, (const double *)c->scal, 1);
-        else
-            hipLaunchKernelGGL((k_spmv_sym<T, D, VP, false, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->nslice, c->cptr,
-                               c->dslot, c->sp_u, c->ucol, (const T *)c->uval, c->sp_l, c->lcol, c->lpos, (const V *)in,
-                               (V *)out, c->partial, c->partial_cap, (const double *)c->scal, 1);
-        return check_launch(c, "k_spmv_sym");
+    double *part = fused ? c->partial + 4 * (int64_t)c->partial_cap : c->partial;
+    switch (c->spmv_unr) {
+    case 1:
+        fused ? spmv_go<T, D, VP, true, 1>(c, in, out, part) : spmv_go<T, D, VP, false, 1>(c, in, out, part);
+        break;
+    case 4:
+        fused ? spmv_go<T, D, VP, true, 4>(c, in, out, part) : spmv_go<T, D, VP, false, 4>(c, in, out, part);
+        break;
+    default:
+        fused ? spmv_go<T, D, VP, true, 2>(c, in, out, part) : spmv_go<T, D, VP, false, 2>(c, in, out, part);
     }
-    if (fused)
-        hipLaunchKernelGGL((k_spmv<T, D, VP, true, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->nslice, c->sp, c->cptr,
-                           c->scol, (const T *)c->aval, (const V *)in, (V *)out, c->partial + 2 * (int64_t)c->partial_cap,
-                           c->partial_cap, (const double *)c->scal, 1);
-    else
-        hipLaunchKernelGGL((k_spmv<T, D, VP, false, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->nslice, c->sp, c->cptr,
-                           c->scol, (const T *)c->aval, (const V *)in, (V *)out, c->partial, c->partial_cap,
-                           (const double *)c->scal, 1);
-    return check_launch(c, "k_spmv");
+    return check_launch(c, c->sym_ok ? "k_spmv_sym" : "k_spmv");
+}
+
+// Launch-configuration tuner: times the plain SpMV (v_p -> v_w; neither is live outside a
+// solve) in every configuration and keeps the fastest.
+template <typename T, int D, int VP>
+static int spmv_tune_t(lpm_ctx *c) {
+    // 4 or 8 wavefronts per workgroup: the fused SpMV leaves one partial per workgroup, which
+    // every workgroup of the next vector kernel sums, so fewer workgroups keep that cheap
+    static const int unrs[] = {1, 2, 4}, wpbs[] = {4, 8}, remaps[] = {0, 1};
+    hipEvent_t a, b;
+    LPM_HIP(c, hipEventCreate(&a));
+    LPM_HIP(c, hipEventCreate(&b));
+    double best = 1e300;
+    int bu = 2, bw = 4, br = 1;
+    const int reps = 4;
+    for (int un : unrs)
+        for (int wp : wpbs)
+            for (int rm : remaps) {
+                c->spmv_unr = un;
+                c->spmv_wpb = wp;
+                c->spmv_remap = rm;
+                int rc = spmv_launch<T, D, VP>(c, false, c->v_p, c->v_w);
+                if (rc) return rc;
+                LPM_HIP(c, hipEventRecord(a, c->stream));
+                for (int k = 0; k < reps; ++k)
+                    if ((rc = spmv_launch<T, D, VP>(c, false, c->v_p, c->v_w))) return rc;
+                LPM_HIP(c, hipEventRecord(b, c->stream));
+                LPM_HIP(c, hipEventSynchronize(b));
+                float ms = 0;
+                LPM_HIP(c, hipEventElapsedTime(&ms, a, b));
+                if (ms < best) {
+                    best = ms;
+                    bu = un;
+                    bw = wp;
+                    br = rm;
+                }
+            }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    c->spmv_unr = bu;
+    c->spmv_wpb = bw;
+    c->spmv_remap = br;
+    return LPM_OK;
+}
+
+static int spmv_tune(lpm_ctx *c) {
+    const int key = (c->sym_ok ? 2 : 0) + c->precision;
+    if (!c->spmv_tune || c->tune_key == key) return LPM_OK;
+    int rc;
+    if (c->precision == 0) rc = c->dim == 3 ? spmv_tune_t<float, 3, 4>(c) : spmv_tune_t<float, 2, 2>(c);
+    else rc = c->dim == 3 ? spmv_tune_t<double, 3, 4>(c) : spmv_tune_t<double, 2, 2>(c);
+    if (rc) return rc;
+    c->tune_key = key;
+    return LPM_OK;
 }
 
 // Index structure of the half store (once per context; the pattern never changes).
@@ -726,7 +816,8 @@ template <typename T, int D, int VP>
 static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     typedef Vec<T, VP> V;
     const int64_t n = c->n;
-    const int gv = blocks_for(n, 256), gs = spmv_grid(c);
+    // vector kernels: grid-stride over at most 512 workgroups (fewer partials to sum)
+    const int gv = std::max(1, std::min(blocks_for(n, 256), 512)), gs = spmv_grid(c);
     if (gv > c->partial_cap || gs > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "CG grid exceeds partial buffer");
     const int64_t maxit = c->cg_maxit < 0 ? 2 * n * c->dim : c->cg_maxit;
     hipLaunchKernelGGL((k_cg_init<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, c->residual, (const T *)c->invdiag,
@@ -736,55 +827,55 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     if (rc) return rc;
     rc = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
     if (rc) return rc;
-    rc = launch_finalize(c, FIN_INIT, gv, gs, c->cg_tol, (double)maxit);
-    if (rc) return rc;
-    const int batch = c->cg_batch > 0 ? c->cg_batch : 16;
+    // batches of `batch` iterations (an even number, so that a batch starts on parity 0);
+    // the device flag turns the rest of a batch into no-ops once the solve has stopped
+    const int batch = ((c->cg_batch > 0 ? c->cg_batch : 16) + 1) & ~1;
+    // timing: events around the SpMV of every `timing`-th iteration only (an event record costs
+    // a few microseconds of idle GPU time, so timing every launch would slow the solve)
     std::vector<hipEvent_t> evs;
     if (c->timing) {
         evs.resize(2 * batch);
         for (auto &e : evs) LPM_HIP(c, hipEventCreate(&e));
     }
-    int64_t launched = 0;
-    double it_prev = 0.0;
+    auto timed = [&](int64_t it) { return c->timing > 0 && it % c->timing == 0; };
+    int64_t it0 = 0;
     for (;;) {
-        LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
-        LPM_HIP(c, hipStreamSynchronize(c->stream));
-        if (launched > 0) {
-            // SpMVs of the batch that did work: one per completed iteration
-            int real = (int)(c->h_scal[SC_ITER] - it_prev);
-            real = real < 0 ? 0 : (real > batch ? batch : real);
-            c->st_spmv += real;
-            if (c->timing) {
-                for (int k = 0; k < real; ++k) {
-                    float ms = 0;
-                    if (hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]) == hipSuccess) c->st_spmv_ms += ms;
-                }
-                c->st_spmv_timed += real;
-            }
-        }
-        it_prev = c->h_scal[SC_ITER];
-        if (c->h_scal[SC_DONE] != 0.0) break;
-        if (launched > maxit + 2 * batch) break;  // the device flag caps the count; this only guards the loop
         for (int k = 0; k < batch; ++k) {
             hipLaunchKernelGGL((k_cg_vec<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (const T *)c->invdiag, c->precond,
                                (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s, (const V *)c->v_w, c->partial,
-                               c->partial_cap, (const double *)c->scal);
+                               c->partial_cap, gv, gs, c->scal, (int)(it0 + k), c->cg_tol, (double)maxit);
             rc = check_launch(c, "k_cg_vec");
             if (rc) return rc;
-            if (c->timing) LPM_HIP(c, hipEventRecord(evs[2 * k], c->stream));
+            if (timed(it0 + k)) LPM_HIP(c, hipEventRecord(evs[2 * k], c->stream));
             rc = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
             if (rc) return rc;
-            if (c->timing) LPM_HIP(c, hipEventRecord(evs[2 * k + 1], c->stream));
-            rc = launch_finalize(c, FIN_STEP, gv, gs, 0.0, 0.0);
-            if (rc) return rc;
+            if (timed(it0 + k)) LPM_HIP(c, hipEventRecord(evs[2 * k + 1], c->stream));
         }
-        launched += batch;
+        LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+        // SpMVs of the batch that did work: one per update (iterations it0 .. ITER-1 updated)
+        const bool done = c->h_scal[SC_DONE] != 0.0;
+        int real = done ? (int)(c->h_scal[SC_ITER] - (double)it0) : batch;
+        real = real < 0 ? 0 : (real > batch ? batch : real);
+        c->st_spmv += real;
+        for (int k = 0; k < real; ++k) {
+            if (!timed(it0 + k)) continue;
+            float ms = 0;
+            LPM_HIP(c, hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
+            c->st_spmv_ms += ms;
+            c->st_spmv_timed++;
+        }
+        if (done) break;
+        it0 += batch;
+        if (it0 > maxit + 2 * batch) break;  // the device cap stops earlier; this only guards the loop
     }
     for (auto &e : evs) hipEventDestroy(e);
     c->st_spmv += 1;  // the prologue SpMV
     const double rhs2 = c->h_scal[SC_RHS2];
     const double done = c->h_scal[SC_DONE];
-    *iters = (int)c->h_scal[SC_ITER];
+    // Eigen reports the updates done before the converging one (its loop breaks before i++)
+    const int64_t upd = (int64_t)c->h_scal[SC_ITER];
+    *iters = (int)((done == 1.0 && upd > 0) ? upd - 1 : upd);
     *error = (done == 3.0) ? 0.0 : sqrt(c->h_scal[SC_RR] / rhs2);
     *success = (done == 3.0) || (*error <= c->cg_tol);
     return LPM_OK;
@@ -795,6 +886,7 @@ int cg_solve(lpm_ctx *c, int *iters, double *error, int *success) {
     int rc = launch_precond_setup(c);
     if (rc) return rc;
     if ((rc = sym_prepare(c))) return rc;
+    if ((rc = spmv_tune(c))) return rc;
     if (c->sym_ok) c->st_sym_solves++;
     if (c->precision == 0)
         return c->dim == 3 ? cg_t<float, 3, 4>(c, iters, error, success) : cg_t<float, 2, 2>(c, iters, error, success);
@@ -840,6 +932,7 @@ int spmv_bench(lpm_ctx *c, int reps, double *ms) {
     if (reps <= 0) return lpm_fail(c, LPM_E_ARG, "reps must be positive");
     int rc = sym_prepare(c);
     if (rc) return rc;
+    if ((rc = spmv_tune(c))) return rc;
     if (c->precision == 0) return c->dim == 3 ? bench_t<float, 3, 4>(c, reps, ms) : bench_t<float, 2, 2>(c, reps, ms);
     return c->dim == 3 ? bench_t<double, 3, 4>(c, reps, ms) : bench_t<double, 2, 2>(c, reps, ms);
 }

@@ -691,7 +691,7 @@ int launch_residual(lpm_ctx *c, double *nr2, double *nR2) {
                        c->partial, c->partial_cap);
     int rc = check_launch(c, "k_residual");
     if (rc) return rc;
-    rc = launch_finalize(c, FIN_NORMS, g, 0, 0.0, 0.0);
+    rc = launch_finalize(c, FIN_NORMS, g);
     if (rc) return rc;
     if (nr2 || nR2) {
         LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
@@ -736,7 +736,7 @@ int launch_dirichlet(lpm_ctx *c) {
     hipLaunchKernelGGL(k_diag_norm, dim3(g), dim3(256), 0, c->stream, nd, c->diag64, c->partial, c->partial_cap);
     int rc = check_launch(c, "k_diag_norm");
     if (rc) return rc;
-    rc = launch_finalize(c, FIN_DIAG, g, 0, 0.0, 0.0);
+    rc = launch_finalize(c, FIN_DIAG, g);
     if (rc) return rc;
     const int g2 = blocks_for(c->n, 4);
     if (c->precision == 0) {

@@ -21,18 +21,16 @@
 enum {
     SC_RHS2 = 0,   // ||b||^2
     SC_THR,        // max(tol^2 ||b||^2, tiny)
-    SC_ABSNEW,     // r.z
-    SC_ALPHA,
-    SC_BETA,
-    SC_PT,         // p.t
-    SC_RR,         // ||r||^2
-    SC_ITER,       // completed CG iterations
+    SC_RR,         // ||r||^2 at the last stopping test
+    SC_ITER,       // CG updates done
     SC_DONE,       // 0 running, 1 converged, 2 iteration cap, 3 zero rhs
     SC_MAXIT,
     SC_NORM_R,     // residual ||.||^2
     SC_NORM_REAC,  // reaction ||.||^2
     SC_DIAGNORM,   // ||diag K||^2
     SC_FLAG,       // any-flag (bond broken / damage changed) as a double 0/1
+    SC_GAMMA_P,    // gamma = (r, M^-1 r) of iterations with parity 0 / 1 (two slots)
+    SC_ALPHA_P = SC_GAMMA_P + 2,  // alpha of iterations with parity 0 / 1 (two slots)
     SC_COUNT = 32
 };
 
@@ -54,6 +52,7 @@ struct lpm_ctx {
     int diag10_literal = 1, cg_transpose = 1, dirichlet_static = 0, rollback_pex = 0;
     int precision = 0, precond = 0, verbose = 0, max_NR_iter = 30, cg_batch = 16;
     double cg_tol = 2.220446049250313e-16, tol_NR = 1e-6;
+    int cg_forcing = 0;  // 1: per-Newton-iteration CG tolerance from the remaining reduction
     int64_t cg_maxit = -1;
     // topology (device)
     int32_t *bptr = nullptr, *bsplit = nullptr, *p2 = nullptr, *opp = nullptr, *nbarr = nullptr;
@@ -74,6 +73,10 @@ struct lpm_ctx {
     int uval_prec = -1;
     int64_t k_version = 0, sym_version = -1;  // assembly generation / generation packed
     bool sym_ok = false;
+    // SpMV launch configuration: slots unrolled per lane, wavefronts per workgroup, XCD remap;
+    // with spmv_tune the fastest of all configurations is measured on the first solve of each
+    // matrix kind (store, precision) and kept (the pattern never changes)
+    int spmv_unr = 2, spmv_wpb = 4, spmv_remap = 1, spmv_tune = 1, tune_key = -1;
     int32_t *nlptr = nullptr, *nlidx = nullptr;
     // particle state (device, FP64 unless noted)
     double *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr, *Pin = nullptr, *Pex = nullptr;
@@ -136,7 +139,7 @@ int launch_pattern(lpm_ctx *c);
 int launch_tslot(lpm_ctx *c);
 int launch_nonlocal_search(lpm_ctx *c, double ratio);
 // ---- launchers (kernels_cg.hip)
-int launch_finalize(lpm_ctx *c, int kind, int na, int nb, double tol, double maxit);
+int launch_finalize(lpm_ctx *c, int kind, int na);
 int cg_alloc(lpm_ctx *c);
 int cg_solve(lpm_ctx *c, int *iters, double *error, int *success);
 int launch_move(lpm_ctx *c);

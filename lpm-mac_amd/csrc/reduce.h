@@ -26,7 +26,7 @@ static __device__ __forceinline__ void block_partial(double (&v)[NV], double *pa
 }
 
 // finalize kinds
-enum { FIN_INIT = 0, FIN_STEP = 1, FIN_NORMS = 3, FIN_DIAG = 4 };
+enum { FIN_NORMS = 3, FIN_DIAG = 4 };
 
 // SELL-64 layout of the CG matrix: block rows are grouped in slices of 64 consecutive rows;
 // slice k starts at block slot sp[k] and is W_k slots wide (W_k = longest row of the slice).

@@ -275,6 +275,33 @@ def test_static_step_equilibrium_parity(precision):
     np.testing.assert_array_equal(p.g.get("xyz").reshape(-1, 3)[bottom], xyz[bottom])
 
 
+@pytest.mark.parametrize("forcing", [0, 1])
+def test_headline_settings_equilibrium_parity(forcing):
+    """The headline benchmark's solver settings (FP32 matrix/SpMV/CG with rtol 1e-3, optional
+    inexact-Newton forcing, Newton tol 1e-10) on a 12^3 SC bar in uniaxial tension, 3 load
+    steps: displacements and reactions within 1e-8 of the FP64 oracle solved tightly."""
+    say("[gpu] headline settings forcing", forcing)
+    p, xyz, bottom, top = _bar(n=12)
+    L = xyz[:, 2].max() - xyz[:, 2].min()
+    disp = [(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", 0.005 * L / 20)]
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=50)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-3, cg_forcing=forcing, tol_NR=1e-10, max_NR_iter=30, precision=0,
+             cg_transpose=0, diag10_literal=0, cg_maxit=20000)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+    for k in range(3):
+        ls_o, ls_g = p.load(disp=disp)
+        assert p.o.solve_step_static(ls_o, -1)
+        assert p.g.solve_step_static(ls_g, -1)
+        u_o = p.o.getd("xyz") - xyz.ravel()
+        u_g = p.g.get("xyz") - xyz.ravel()
+        say("   step", k, "disp rel err", rel(u_g, u_o), "reaction rel err",
+            rel(p.g.get("reaction"), p.o.getd("reaction")), "cg", p.g.stat("cg_iters"))
+        assert rel(u_g, u_o) < 1e-8
+        assert rel(p.g.get("reaction"), p.o.getd("reaction")) < 1e-8
+
+
 def test_literal_modes_reach_the_same_equilibrium():
     """Oracle in the reference's literal CG semantics (K^T, Q2) and GPU with a symmetric K both
     converge to the same equilibrium: the fixed point does not depend on K."""
