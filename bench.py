@@ -28,9 +28,15 @@ for p in (os.path.join(ROOT, "lpm-mac_amd"), os.path.join(ROOT, "oracle")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK = 8.0e12
-# the CG SpMV of every 16th iteration is bracketed by HIP events (sampling keeps the event
-# records' own cost out of the timed steps)
-TIMING_EVERY = 16
+# HBM traffic of one CG SpMV launch (full store, FP32, SC 64^3), measured with two rocprofv3
+# --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
+#   bench.py --steps 1 --warmup 0 --no-cpu-baseline --pmc-calibrate
+# Each reading is scaled by what the same counter reports for k_copy_calib, which moves a known
+# 256 MiB in and 256 MiB out: FETCH_SIZE reads 131078.9 KiB for it (x2.0000), WRITE_SIZE
+# 262144.0 KiB (x1.0000). Mean over the 596 fused SpMV launches that did work: FETCH_SIZE
+# 311296.8 KiB -> 637.5 MB; WRITE_SIZE 0 KiB (the 3 MB output stays in the write-back L2
+# until later kernels evict it).
+MEASURED_TRAFFIC = {(64, "k_spmv (CG SpMV, full store)"): 311296.8 * 1024 * 2.0 + 0.0}
 
 
 def sc_workload(n):
@@ -52,18 +58,19 @@ def spmv_bytes(ctx):
     """Algorithmic bytes of one FP32 3x3-block SpMV and the kernel that ran it.
 
     Full store: every block's 9 values + its column index, row pointers, one read of the
-    input vector and one write of the output (both padded to 16 B per particle).
+    input vector and one write of the output (4 B per DoF; the 16 B per particle padding of
+    the device vectors is not counted).
     Half store (symmetric matrix): the upper blocks (j >= i) with their column indices, for
     each lower block a column index and the position of its transpose, row pointers and
     diagonal slots, and the same vector traffic. A lower block's values are the transposed
     upper block, read a second time from cache, so they count once."""
     nblk = ctx.size("nconn")
     n = ctx.n
-    vec = 16 * n + 16 * n
+    vec = 8 * ctx.size("ndof")  # input read once + output written once, 4 B per DoF
     if ctx.stat("spmv_sym_active"):
         nup, nlo = (nblk + n) // 2, (nblk - n) // 2
-        return 40 * nup + 8 * nlo + 4 * (n + 1) + 4 * n + vec, "k_spmv_sym<float,3,4,true,2> (CG SpMV, half store)"
-    return 40 * nblk + 4 * (n + 1) + vec, "k_spmv<float,3,4,true,2> (CG SpMV, full store)"
+        return 40 * nup + 8 * nlo + 4 * (n + 1) + 4 * n + vec, "k_spmv_sym (CG SpMV, half store)"
+    return 40 * nblk + 4 * (n + 1) + vec, "k_spmv (CG SpMV, full store)"
 
 
 def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing):
@@ -156,6 +163,11 @@ def main():
     ap.add_argument("--spmv-sym", type=int, default=0, help="1: half store when K is symmetric, 0: full store")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--timing-every", type=int, default=16,
+                    help="bracket the CG SpMV of every k-th iteration with HIP events")
+    ap.add_argument("--pmc-calibrate", action="store_true",
+                    help="run the 256 MiB calibration copy (k_copy_calib) only 3 times, for rocprofv3 --pmc passes "
+                         "whose FETCH_SIZE / WRITE_SIZE readings it scales")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
@@ -174,7 +186,7 @@ def main():
         assert ok
     reset_state(ctx, xyz)
     ctx.reset_stats()
-    ctx.set_timing(TIMING_EVERY)
+    ctx.set_timing(args.timing_every)
     hip_sync(ctx)
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -187,6 +199,9 @@ def main():
     t = time.perf_counter() - t0
     ctx.set_timing(0)
     spmv_iso_ms = ctx.spmv_bench(50)
+    # practical ceiling: a plain streaming copy of the SpMV's size (read + write)
+    copy_bytes = 256 << 20
+    copy_ms = ctx.copy_bench(copy_bytes, 3 if args.pmc_calibrate else 20)
     K = args.steps
     spmv_n = ctx.stat("spmv_launches_timed")
     spmv_ms = ctx.stat("spmv_ms") / max(spmv_n, 1)
@@ -223,9 +238,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved * 1e9 / HBM_PEAK, "traffic": None,
+            "frac": achieved * 1e9 / HBM_PEAK, "traffic": MEASURED_TRAFFIC.get((args.n, kname)),
             "kernel": kname, "bytes_per_launch": B, "mean_kernel_ms": spmv_ms,
             "launches_timed": spmv_n, "isolated_spmv_ms": spmv_iso_ms,
+            "copy_kernel_GBps": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9,
         },
     }
     if not args.no_cpu_baseline:

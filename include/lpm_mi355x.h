@@ -250,6 +250,10 @@ int lpm_set_timing(lpm_ctx *ctx, int every);
 /* Run the CG SpMV kernel `reps` times on the stored matrix (x = current p vector) and return
  * the mean kernel time in milliseconds (HIP events on the context stream). */
 int lpm_spmv_bench(lpm_ctx *ctx, int32_t reps, double *ms);
+/* Copy `bytes` between two scratch device buffers `reps` times with a plain streaming kernel
+ * (k_copy_calib) and return the mean kernel time: the practical HBM ceiling, and a kernel of
+ * known traffic for scaling hardware counters. No reference counterpart (measurement only). */
+int lpm_copy_bench(lpm_ctx *ctx, int64_t bytes, int32_t reps, double *ms);
 /* Stream handle (hipStream_t) of the context, for callers that order their own work. */
 void *lpm_stream(lpm_ctx *ctx);
 

@@ -944,6 +944,11 @@ int lpm_spmv_bench(lpm_ctx *c, int32_t reps, double *ms) {
     return spmv_bench(c, reps, ms);
 }
 
+int lpm_copy_bench(lpm_ctx *c, int64_t bytes, int32_t reps, double *ms) {
+    if (!c || !ms) return LPM_E_ARG;
+    return copy_bench(c, bytes, reps, ms);
+}
+
 void *lpm_stream(lpm_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 }  // extern "C"

@@ -585,9 +585,22 @@ __global__ __launch_bounds__(256) void k_move(int64_t n, const Vec<T, VP> *__res
     xyz[3 * i + 2] = xyz[3 * i + 2] + d[2];
 }
 
+// ---------------------------------------------------------------- copy of known size
+// Streams `n16` 16-byte words from a to b: the reference kernel whose bytes (n16*16 read,
+// n16*16 written) calibrate hardware traffic counters and give the practical HBM ceiling.
+__global__ __launch_bounds__(256) void k_copy_calib(int64_t n16, const float4 *__restrict__ a, float4 *__restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 }  // namespace
 
 // =================================================================== host side
+// Events used only to time kernels: no system-scope fence, so recording one does not write
+// back and invalidate the caches (which would both stretch the measured interval and slow
+// the kernel that follows).
+static hipError_t timing_event(hipEvent_t *e) { return hipEventCreateWithFlags(e, hipEventDisableSystemFence); }
+
 static int check_launch(lpm_ctx *c, const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return lpm_fail(c, LPM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -681,8 +694,8 @@ static int spmv_tune_t(lpm_ctx *c) {
     // every workgroup of the next vector kernel sums, so fewer workgroups keep that cheap
     static const int unrs[] = {1, 2, 4}, wpbs[] = {4, 8}, remaps[] = {0, 1};
     hipEvent_t a, b;
-    LPM_HIP(c, hipEventCreate(&a));
-    LPM_HIP(c, hipEventCreate(&b));
+    LPM_HIP(c, timing_event(&a));
+    LPM_HIP(c, timing_event(&b));
     double best = 1e300;
     int bu = 2, bw = 4, br = 1;
     const int reps = 4;
@@ -835,7 +848,7 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     std::vector<hipEvent_t> evs;
     if (c->timing) {
         evs.resize(2 * batch);
-        for (auto &e : evs) LPM_HIP(c, hipEventCreate(&e));
+        for (auto &e : evs) LPM_HIP(c, timing_event(&e));
     }
     auto timed = [&](int64_t it) { return c->timing > 0 && it % c->timing == 0; };
     int64_t it0 = 0;
@@ -908,8 +921,8 @@ int launch_move(lpm_ctx *c) {
 template <typename T, int D, int VP>
 static int bench_t(lpm_ctx *c, int reps, double *ms) {
     hipEvent_t a, b;
-    LPM_HIP(c, hipEventCreate(&a));
-    LPM_HIP(c, hipEventCreate(&b));
+    LPM_HIP(c, timing_event(&a));
+    LPM_HIP(c, timing_event(&b));
     int rc = spmv_launch<T, D, VP>(c, false, c->v_p, c->v_w);  // warm
     if (rc) return rc;
     LPM_HIP(c, hipEventRecord(a, c->stream));
@@ -935,4 +948,39 @@ int spmv_bench(lpm_ctx *c, int reps, double *ms) {
     if ((rc = spmv_tune(c))) return rc;
     if (c->precision == 0) return c->dim == 3 ? bench_t<float, 3, 4>(c, reps, ms) : bench_t<float, 2, 2>(c, reps, ms);
     return c->dim == 3 ? bench_t<double, 3, 4>(c, reps, ms) : bench_t<double, 2, 2>(c, reps, ms);
+}
+
+int copy_bench(lpm_ctx *c, int64_t bytes, int reps, double *ms) {
+    if (bytes < 16 || reps <= 0) return lpm_fail(c, LPM_E_ARG, "copy_bench needs bytes >= 16 and reps > 0");
+    const int64_t n16 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    LPM_HIP(c, hipMalloc(&a, n16 * 16));
+    if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+        hipFree(a);
+        return lpm_fail(c, LPM_E_NOMEM, "copy_bench allocation");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LPM_OK;
+    const int g = (int)std::min<int64_t>(blocks_for(n16, 256), 8192);
+    if (hipMemsetAsync(a, 0, n16 * 16, c->stream) != hipSuccess || timing_event(&e0) != hipSuccess ||
+        timing_event(&e1) != hipSuccess) {
+        rc = lpm_fail(c, LPM_E_HIP, "copy_bench setup");
+    } else {
+        hipLaunchKernelGGL(k_copy_calib, dim3(g), dim3(256), 0, c->stream, n16, (const float4 *)a, (float4 *)b);
+        hipEventRecord(e0, c->stream);
+        for (int k = 0; k < reps; ++k)
+            hipLaunchKernelGGL(k_copy_calib, dim3(g), dim3(256), 0, c->stream, n16, (const float4 *)a, (float4 *)b);
+        hipEventRecord(e1, c->stream);
+        rc = check_launch(c, "k_copy_calib");
+        float t = 0;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            rc = lpm_fail(c, LPM_E_HIP, "copy_bench timing");
+        *ms = t / reps;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipStreamSynchronize(c->stream);
+    hipFree(a);
+    hipFree(b);
+    return rc;
 }

@@ -146,6 +146,7 @@ int launch_move(lpm_ctx *c);
 int spmv_bench(lpm_ctx *c, int reps, double *ms);
 int launch_precond_setup(lpm_ctx *c);
 int sym_prepare(lpm_ctx *c);
+int copy_bench(lpm_ctx *c, int64_t bytes, int reps, double *ms);
 
 // helpers
 static inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -113,6 +113,7 @@ struct Options {
     int elmap_literal = 1;     // reference: 1 (column-major read of el_mapping)
     double cg_tol = 1e-6;      // reference: 2.22e-16 (FP64)
     double cg_maxit = -1;      // reference: 2*ndof
+    int cg_forcing = 0;        // 1: inexact-Newton CG tolerance per Newton iteration (reference: 0)
     int verbose = 1;           // print like the reference
     int device = 0;
 };
@@ -735,6 +736,7 @@ class Assembly {
         lpm_set_mode(c, "precond", o.precond);
         lpm_set_mode(c, "cg_tol", o.cg_tol);
         lpm_set_mode(c, "cg_maxit", o.cg_maxit);
+        lpm_set_mode(c, "cg_forcing", o.cg_forcing);
         lpm_set_mode(c, "verbose", o.verbose);
     }
 

@@ -81,6 +81,7 @@ _SIGS = {
     "lpm_reset_stats": (_I, [_P]),
     "lpm_set_timing": (_I, [_P, _I]),
     "lpm_spmv_bench": (_I, [_P, ctypes.c_int32, _DP]),
+    "lpm_copy_bench": (_I, [_P, ctypes.c_int64, ctypes.c_int32, _DP]),
     "lpm_stream": (_P, [_P]),
 }
 
@@ -411,4 +412,9 @@ class Context:
     def spmv_bench(self, reps):
         ms = _D()
         self._chk(lib().lpm_spmv_bench(self.h, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def copy_bench(self, nbytes, reps):
+        ms = _D()
+        self._chk(lib().lpm_copy_bench(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
         return ms.value

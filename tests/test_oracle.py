@@ -556,3 +556,46 @@ def test_dump_format(tmp_path):
     assert lines[8].startswith("ITEM: ATOMS id type x y z dx dy dz s11")
     assert len(lines) == 9 + len(xyz)
     assert len(lines[9].split()) == 17
+
+
+def test_j2_return_mapping_textbook_cases():
+    """J2 radial return with linear mixed hardening (particle_j2plasticity.h:78-140) against
+    textbook facts, with the stress set directly: von Mises sigma_eq of uniaxial stress s is |s|
+    and of pure shear t is sqrt(3) t; the plastic multiplier is (sigma_eq - sigma_y)/(3G + H);
+    uniaxial plastic flow is (1, -1/2, -1/2) dlambda; the back stress moves by 2/3 xi H depsp;
+    triaxiality is 1/3 in uniaxial tension and 0 in shear; a bond of length dis along x
+    lengthens plastically by dis * depsp_xx; below yield nothing changes."""
+    s, xyz, box = sc_block(4, ptype=O.J2)
+    s.mode(elmap_literal=0)
+    s.set_j2(0.0, False, 200e3, 0.3, 300.0, 0.5, 2000.0, 0.2, 1.0)
+    E, mu, sy, xi, H, A = 200e3, 0.3, 300.0, 0.5, 2000.0, 0.2
+    G = E / (2 * (1 + mu))
+    s.update_geometry()
+    s.update_force_state()
+    n = len(xyz)
+    st = np.zeros((n, 6))
+    st[0, 0] = 450.0            # uniaxial tension above yield
+    st[1, 5] = 200.0            # pure shear (Voigt xy), sqrt(3)*200 = 346.4 > 300
+    st[2, 0] = 100.0            # uniaxial, elastic
+    s.setd("stress", st.ravel())
+    dlp0 = s.getd("dLp").copy()
+    s.update_state_var()
+    sv = s.getd("state_var").reshape(n, 7)
+    Dd = s.getd("Ddot_local")
+    dl_u = (450.0 - sy) / (3 * G + H)
+    dl_s = (np.sqrt(3.0) * 200.0 - sy) / (3 * G + H)
+    assert sv[0, 0] == pytest.approx(dl_u, rel=1e-13)
+    np.testing.assert_allclose(sv[0, 1:4], 2.0 / 3.0 * xi * H * dl_u * np.array([1.0, -0.5, -0.5]), rtol=1e-12)
+    assert Dd[0] == pytest.approx(dl_u * (1 + A / 3.0), rel=1e-12)
+    assert sv[1, 0] == pytest.approx(dl_s, rel=1e-13)
+    # shear flow: depsp_xy = 1.5 dlambda t / sigma_eq = dlambda sqrt(3)/2 (Voigt stores the tensor component)
+    assert sv[1, 6] == pytest.approx(2.0 / 3.0 * xi * H * dl_s * np.sqrt(3.0) / 2.0, rel=1e-12)
+    assert Dd[1] == pytest.approx(dl_s, rel=1e-12)
+    assert sv[2, 0] == 0.0 and Dd[2] == 0.0
+    # plastic bond lengthening of particle 0's bonds: dis * (n . depsp . n) with depsp = dl (1, -1/2, -1/2)
+    bptr = s.geti("bptr")
+    cs = s.getd("cs").reshape(-1, 3)
+    dis = s.getd("dis")
+    b = np.arange(bptr[0], bptr[1])
+    exp = dis[b] * dl_u * (cs[b, 0] ** 2 - 0.5 * cs[b, 1] ** 2 - 0.5 * cs[b, 2] ** 2)
+    np.testing.assert_allclose(s.getd("dLp")[b] - dlp0[b], exp, rtol=1e-12, atol=1e-18)
