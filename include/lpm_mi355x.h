@@ -235,6 +235,23 @@ int lpm_solve_problem_cyclic(lpm_ctx *ctx, int32_t load_type, int32_t n_interval
                              int64_t n_spectrum, const int32_t *groups, int32_t n_groups, int32_t time_mode,
                              double tau, int32_t undamaged_type, int32_t max_cycles_run, int32_t *cycles);
 
+/* ------------------------------------------------------------------ tangent and adjoint
+ * (no reference counterpart: sensitivities of the load-step result, frozen damage / broken
+ * bonds / plastic bond lengths, for parameter calibration)
+ * Exact directional derivative of the internal force of Assembly::updateGeometry +
+ * updateForceState (particle.h:163-215, bond.h:30-39, particle_elastic.h:38-50) at the current
+ * state: jv = (dPin/dx) v, both dim*n host arrays in DoF order. */
+int lpm_force_jvp(lpm_ctx *ctx, const double *v, double *jv);
+/* Adjoint sensitivity of F = sum over the particles of `group` of Pin[axis] (the group's
+ * reaction, Solver::updateRR, solver.h:108-128) at the current converged state, with respect
+ * to every bond's Kn and Tv (nbonds doubles each; either may be NULL). One adjoint solve
+ * J_ff lambda = (dF/du)_free on the exact force Jacobian J (symmetric), by iterative
+ * refinement whose inner solves run the CG on the assembled, Dirichlet-eliminated K; then
+ * dF/dKn_b = (w_i - w_j).cs_b dLe_b (1 - d_b), dF/dTv_b = (w_i - w_j).cs_b dLe_total(i,l_b) (1 - d_b)
+ * with w = e_group,axis - lambda. Reassembles K at the current state. */
+int lpm_adjoint_reaction(lpm_ctx *ctx, int32_t group, int32_t axis, double *dF_dKn, double *dF_dTv, double *F,
+                         int32_t *refinements, double *rel_residual);
+
 /* ------------------------------------------------------------------ statistics / measurement
  * Counters since the last reset: "newton_iters","cg_iters","cg_solves","assemblies",
  * "spmv_launches" (SpMVs that did work); "spmv_ms" / "spmv_launches_timed" (sum of SpMV kernel

@@ -177,6 +177,9 @@ int apply_bc(lpm_ctx *c, const lpm_load_step *s) {
     return LPM_OK;
 }
 
+}  // namespace
+
+// phases shared with the adjoint (kernels_adj.hip)
 int geometry_force_state(lpm_ctx *c) {
     RC(launch_geometry_force(c, true));
     return launch_particle_force(c);
@@ -197,6 +200,8 @@ int update_stiffness_disp_bc(lpm_ctx *c) {
     c->k_version++;
     return launch_dirichlet(c);
 }
+
+namespace {
 
 int solve_linear_system(lpm_ctx *c, int *iters, double *error) {
     int it = 0, ok = 0;
@@ -942,6 +947,35 @@ int lpm_set_timing(lpm_ctx *c, int on) {
 int lpm_spmv_bench(lpm_ctx *c, int32_t reps, double *ms) {
     if (!c || !ms) return LPM_E_ARG;
     return spmv_bench(c, reps, ms);
+}
+
+int lpm_force_jvp(lpm_ctx *c, const double *v, double *jv) {
+    if (!c || !v || !jv) return LPM_E_ARG;
+    const int64_t nd = c->n * c->dim;
+    double *d = nullptr;
+    LPM_HIP(c, hipMalloc(&d, sizeof(double) * 2 * std::max<int64_t>(nd, 1)));
+    int rc = LPM_OK;
+    if (hipMemcpyAsync(d, v, sizeof(double) * nd, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "jvp upload");
+    if (!rc) rc = force_jvp(c, d, d + nd);
+    if (!rc && hipMemcpyAsync(jv, d + nd, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "jvp download");
+    hipStreamSynchronize(c->stream);
+    hipFree(d);
+    return rc;
+}
+
+int lpm_adjoint_reaction(lpm_ctx *c, int32_t group, int32_t axis, double *dF_dKn, double *dF_dTv, double *F,
+                         int32_t *refinements, double *rel_residual) {
+    if (!c) return LPM_E_ARG;
+    if (axis < 0 || axis >= c->dim) return lpm_fail(c, LPM_E_ARG, "axis out of range");
+    RC(check_group(c, group));
+    int it = 0;
+    double rr = 0;
+    RC(adjoint_reaction(c, group, axis, dF_dKn, dF_dTv, F, &it, &rr));
+    if (refinements) *refinements = it;
+    if (rel_residual) *rel_residual = rr;
+    return LPM_OK;
 }
 
 int lpm_copy_bench(lpm_ctx *c, int64_t bytes, int32_t reps, double *ms) {

@@ -1,0 +1,265 @@
+// Tangent (Jacobian-vector product) of the internal force and the adjoint sensitivity of a
+// reaction sum with respect to the bond stiffness parameters Kn and Tv.
+//
+// The internal force Pin(x) is the one of Assembly::updateGeometry + updateForceState
+// (particle.h:163-215, bond.h:30-39, particle_elastic.h:38-50): with d_b = x_i - x_j,
+// dis_b = |d_b|, cs_b = d_b / dis_b, dLe_b = dis_b - dis0_b - dLp_b,
+// dLe_total(i, l) = sum of dLe_b over the layer-l bonds of i,
+// f_b = (2 Kn_b dLe_b + 2 Tv_b dLe_total(i, l_b)) (1 - bdamage_b) and
+// Pin_i = sum_b cs_b (f_b + f_opp(b)) / 2.
+// Its exact directional derivative (damage, broken bonds and plastic lengths dLp frozen):
+//   d dis_b = cs_b . (v_i - v_j),  d cs_b = ((v_i - v_j) - cs_b d dis_b) / dis_b,
+//   d f_b = (2 Kn_b d dis_b + 2 Tv_b sum_{layer} d dis) (1 - bdamage_b),
+//   d Pin_i = sum_b [ d cs_b (f_b + f_opp)/2 + cs_b (d f_b + d f_opp)/2 ].
+// The analytic stiffness of the reference (derivative.h) drops small terms (derivative.h:5),
+// so it differs from this Jacobian by O(strain); it serves as the preconditioner of the
+// adjoint solve, and the exact Jacobian above gives its residual.
+#include <algorithm>
+#include <vector>
+
+#include "lpm_internal.h"
+#include "reduce.h"
+
+namespace {
+
+// pass 1, one thread per particle: d dis, d cs and d f of each of its bonds
+__global__ __launch_bounds__(256) void k_jvp_bond(int64_t n, int dim, const int32_t *__restrict__ bptr,
+                                                   const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2,
+                                                   const double *__restrict__ dis, const double *__restrict__ cs,
+                                                   const double *__restrict__ Kn, const double *__restrict__ Tv,
+                                                   const double *__restrict__ bdamage, const double *__restrict__ v,
+                                                   double *__restrict__ dcs, double *__restrict__ dfb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b0 = bptr[i], bm = bsplit[i], b1 = bptr[i + 1];
+    double vi[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k) vi[k] = v[(int64_t)dim * i + k];
+    double tot[2] = {0.0, 0.0};
+    for (int b = b0; b < b1; ++b) {
+        const int64_t j = p2[b];
+        const bool broken = fabs(bdamage[b] - 1.0) < LPM_EPS;
+        double dd[3] = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) dd[k] = vi[k] - v[(int64_t)dim * j + k];
+        const double c0 = cs[3 * b], c1 = cs[3 * b + 1], c2 = cs[3 * b + 2];
+        const double ddis = broken ? 0.0 : (c0 * dd[0] + c1 * dd[1] + c2 * dd[2]);
+        const double inv = broken ? 0.0 : 1.0 / dis[b];
+        dcs[3 * b] = (dd[0] - c0 * ddis) * inv;
+        dcs[3 * b + 1] = (dd[1] - c1 * ddis) * inv;
+        dcs[3 * b + 2] = (dd[2] - c2 * ddis) * inv;
+        dfb[b] = ddis;  // d dLe for now
+        tot[b < bm ? 0 : 1] += ddis;
+    }
+    for (int b = b0; b < b1; ++b)
+        dfb[b] = (2.0 * Kn[b] * dfb[b] + 2.0 * Tv[b] * tot[b < bm ? 0 : 1]) * (1.0 - bdamage[b]);
+}
+
+// pass 2, one thread per particle: d Pin_i (dim entries per particle)
+__global__ __launch_bounds__(256) void k_jvp_particle(int64_t n, int dim, const int32_t *__restrict__ bptr,
+                                                       const int32_t *__restrict__ opp, const double *__restrict__ cs,
+                                                       const double *__restrict__ bforce, const double *__restrict__ dcs,
+                                                       const double *__restrict__ dfb, double *__restrict__ jv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double P[3] = {0, 0, 0};
+    for (int b = bptr[i]; b < bptr[i + 1]; ++b) {
+        const int o = opp[b];
+        const double f = 0.5 * (bforce[b] + bforce[o]), df = 0.5 * (dfb[b] + dfb[o]);
+        for (int k = 0; k < 3; ++k) P[k] += dcs[3 * b + k] * f + cs[3 * b + k] * df;
+    }
+    for (int k = 0; k < dim; ++k) jv[(int64_t)dim * i + k] = P[k];
+}
+
+// Zero the entries of DoFs that the Dirichlet elimination removes (constrained, or of a fully
+// damaged particle; stiffness.h:272-293).
+__global__ __launch_bounds__(256) void k_mask_free(int64_t nd, int dim, const int32_t *__restrict__ cons,
+                                                    const double *__restrict__ damage_visual, double *__restrict__ v) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    const int64_t i = d / dim;
+    const int k = (int)(d - i * dim);
+    if (cons[3 * i + k] == 1 || fabs(damage_visual[i] - 1.0) < LPM_EPS) v[d] = 0.0;
+}
+
+// indicator of (group particle, axis)
+__global__ __launch_bounds__(256) void k_indicator(int64_t m, const int32_t *__restrict__ ids, int dim, int axis,
+                                                    double *__restrict__ e) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) e[(int64_t)dim * ids[t] + axis] = 1.0;
+}
+
+// r = g - v ; partial ||r||^2
+__global__ __launch_bounds__(256) void k_sub_norm(int64_t nd, const double *__restrict__ g, const double *__restrict__ v,
+                                                   double *__restrict__ r, double *partial, int cap) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double x[1] = {0.0};
+    if (d < nd) {
+        const double t = g[d] - v[d];
+        r[d] = t;
+        x[0] = t * t;
+    }
+    block_partial<1>(x, partial, cap);
+}
+
+// lambda += CG solution (the CG vector is padded to VP entries per particle)
+template <typename T, int D, int VP>
+__global__ __launch_bounds__(256) void k_add_x(int64_t n, const T *__restrict__ x, double *__restrict__ lam) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < D; ++k) lam[D * i + k] += (double)x[VP * i + k];
+}
+
+// per-bond sensitivities with w = e - lambda:
+//   dF/dKn_b = (w_i - w_j) . cs_b  dLe_b (1 - bdamage_b)
+//   dF/dTv_b = (w_i - w_j) . cs_b  dLe_total(i, l_b) (1 - bdamage_b)
+__global__ __launch_bounds__(256) void k_param_grad(int64_t n, int dim, const int32_t *__restrict__ bptr,
+                                                     const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2,
+                                                     const double *__restrict__ cs, const double *__restrict__ dLe,
+                                                     const double *__restrict__ dLe_total, const double *__restrict__ bdamage,
+                                                     const double *__restrict__ e, const double *__restrict__ lam,
+                                                     double *__restrict__ gKn, double *__restrict__ gTv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double wi[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k) wi[k] = e[(int64_t)dim * i + k] - lam[(int64_t)dim * i + k];
+    for (int b = bptr[i]; b < bptr[i + 1]; ++b) {
+        const int64_t j = p2[b];
+        double s = 0.0;
+        for (int k = 0; k < dim; ++k) s += (wi[k] - (e[(int64_t)dim * j + k] - lam[(int64_t)dim * j + k])) * cs[3 * b + k];
+        const double w = 1.0 - bdamage[b];
+        gKn[b] = s * dLe[b] * w;
+        gTv[b] = s * dLe_total[2 * i + (b < bsplit[i] ? 0 : 1)] * w;
+    }
+}
+
+}  // namespace
+
+static int check(lpm_ctx *c, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return lpm_fail(c, LPM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return LPM_OK;
+}
+
+// jv = dPin/dx . v at the current geometry and forces (both dim*n, device)
+int force_jvp(lpm_ctx *c, const double *v, double *jv) {
+    const int g = blocks_for(c->n, 256);
+    double *dcs = nullptr, *dfb = nullptr;
+    LPM_HIP(c, hipMalloc(&dcs, sizeof(double) * 3 * std::max<int64_t>(c->nbond, 1)));
+    if (hipMalloc(&dfb, sizeof(double) * std::max<int64_t>(c->nbond, 1)) != hipSuccess) {
+        hipFree(dcs);
+        return lpm_fail(c, LPM_E_NOMEM, "jvp scratch");
+    }
+    hipLaunchKernelGGL(k_jvp_bond, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->bptr, c->bsplit, c->p2, c->dis, c->cs,
+                       c->Kn, c->Tv, c->bdamage, v, dcs, dfb);
+    int rc = check(c, "k_jvp_bond");
+    if (!rc) {
+        hipLaunchKernelGGL(k_jvp_particle, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->bptr, c->opp, c->cs, c->bforce,
+                           dcs, dfb, jv);
+        rc = check(c, "k_jvp_particle");
+    }
+    hipStreamSynchronize(c->stream);
+    hipFree(dcs);
+    hipFree(dfb);
+    return rc;
+}
+
+template <typename T, int D, int VP>
+static int add_x_t(lpm_ctx *c, double *lam) {
+    hipLaunchKernelGGL((k_add_x<T, D, VP>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, (const T *)c->v_x, lam);
+    return check(c, "k_add_x");
+}
+
+static int add_x(lpm_ctx *c, double *lam) {
+    if (c->precision == 0) return c->dim == 3 ? add_x_t<float, 3, 4>(c, lam) : add_x_t<float, 2, 2>(c, lam);
+    return c->dim == 3 ? add_x_t<double, 3, 4>(c, lam) : add_x_t<double, 2, 2>(c, lam);
+}
+
+// Adjoint of F = sum over the group's particles of Pin[axis] at the current state.
+int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, double *F, int *refinements,
+                     double *rel_residual) {
+    const int64_t n = c->n, nd = n * c->dim;
+    const Group &grp = c->groups[gid];
+    int rc = geometry_force_state(c);
+    if (rc) return rc;
+    // K with constrained DoFs eliminated: the preconditioner of the exact-Jacobian solve
+    if ((rc = calc_stiffness(c))) return rc;
+    if ((rc = update_stiffness_disp_bc(c))) return rc;
+    double *buf = nullptr;  // e, g, lambda, r, jv
+    LPM_HIP(c, hipMalloc(&buf, sizeof(double) * 5 * nd));
+    double *e = buf, *g = buf + nd, *lam = buf + 2 * nd, *r = buf + 3 * nd, *jv = buf + 4 * nd;
+    double *saved_res = c->residual;
+    const double saved_tol = c->cg_tol;
+    auto done = [&](int code) {
+        c->residual = saved_res;
+        c->cg_tol = saved_tol;
+        hipStreamSynchronize(c->stream);
+        hipFree(buf);
+        return code;
+    };
+    const int gd = blocks_for(nd, 256);
+    if (gd > c->partial_cap) return done(lpm_fail(c, LPM_E_LIMIT, "adjoint grid exceeds partial buffer"));
+    if (hipMemsetAsync(buf, 0, sizeof(double) * 5 * nd, c->stream) != hipSuccess)
+        return done(lpm_fail(c, LPM_E_HIP, "adjoint clear"));
+    if (grp.n > 0)
+        hipLaunchKernelGGL(k_indicator, dim3(blocks_for(grp.n, 256)), dim3(256), 0, c->stream, grp.n, grp.ids, c->dim, axis, e);
+    if ((rc = check(c, "k_indicator"))) return done(rc);
+    // g = (dF/du) restricted to the free DoFs = (J^T e)_free = (J e)_free (J is symmetric)
+    if ((rc = force_jvp(c, e, g))) return done(rc);
+    hipLaunchKernelGGL(k_mask_free, dim3(gd), dim3(256), 0, c->stream, nd, c->dim, c->cons, c->damage_visual, g);
+    if ((rc = check(c, "k_mask_free"))) return done(rc);
+    // J_ff lambda = g by iterative refinement: r = g - J_ff lambda (exact, FP64), K_ff d = r (CG), lambda += d
+    double g2 = -1.0, r2 = 0.0;
+    int it = 0;
+    const int max_ref = 40;
+    c->cg_tol = std::max(saved_tol, c->precision == 0 ? 1e-6 : 1e-12);
+    for (;; ++it) {
+        if ((rc = force_jvp(c, lam, jv))) return done(rc);
+        hipLaunchKernelGGL(k_mask_free, dim3(gd), dim3(256), 0, c->stream, nd, c->dim, c->cons, c->damage_visual, jv);
+        hipLaunchKernelGGL(k_sub_norm, dim3(gd), dim3(256), 0, c->stream, nd, g, jv, r, c->partial, c->partial_cap);
+        if ((rc = check(c, "k_sub_norm"))) return done(rc);
+        if ((rc = launch_finalize(c, FIN_DIAG, gd))) return done(rc);
+        LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+        r2 = c->h_scal[SC_DIAGNORM];
+        if (g2 < 0) g2 = r2;
+        if (g2 == 0.0 || r2 <= 1e-26 * g2 || it >= max_ref) break;
+        c->residual = r;
+        int cg_it = 0, ok = 0;
+        double err = 0;
+        rc = cg_solve(c, &cg_it, &err, &ok);
+        c->residual = saved_res;
+        if (rc) return done(rc);
+        if ((rc = add_x(c, lam))) return done(rc);
+        hipLaunchKernelGGL(k_mask_free, dim3(gd), dim3(256), 0, c->stream, nd, c->dim, c->cons, c->damage_visual, lam);
+        if ((rc = check(c, "k_mask_free"))) return done(rc);
+    }
+    if (refinements) *refinements = it;
+    if (rel_residual) *rel_residual = g2 > 0 ? sqrt(r2 / g2) : 0.0;
+    // per-bond sensitivities, written straight into the caller's host arrays via a device buffer
+    double *gb = nullptr;
+    if (hipMalloc(&gb, sizeof(double) * 2 * std::max<int64_t>(c->nbond, 1)) != hipSuccess)
+        return done(lpm_fail(c, LPM_E_NOMEM, "adjoint output"));
+    hipLaunchKernelGGL(k_param_grad, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->dim, c->bptr, c->bsplit, c->p2,
+                       c->cs, c->dLe, c->dLe_total, c->bdamage, e, lam, gb, gb + c->nbond);
+    rc = check(c, "k_param_grad");
+    if (!rc && gKn) rc = hipMemcpyAsync(gKn, gb, sizeof(double) * c->nbond, hipMemcpyDeviceToHost, c->stream) == hipSuccess
+                             ? LPM_OK : lpm_fail(c, LPM_E_HIP, "adjoint copy");
+    if (!rc && gTv) rc = hipMemcpyAsync(gTv, gb + c->nbond, sizeof(double) * c->nbond, hipMemcpyDeviceToHost, c->stream) == hipSuccess
+                             ? LPM_OK : lpm_fail(c, LPM_E_HIP, "adjoint copy");
+    hipStreamSynchronize(c->stream);
+    hipFree(gb);
+    if (rc) return done(rc);
+    if (F) {
+        // F from the current Pin (the reaction the step reports, solver.h:108-128)
+        std::vector<double> pin(3 * n);
+        std::vector<int32_t> ids(grp.n);
+        LPM_HIP(c, hipMemcpyAsync(pin.data(), c->Pin, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+        if (grp.n) LPM_HIP(c, hipMemcpyAsync(ids.data(), grp.ids, sizeof(int32_t) * grp.n, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+        std::sort(ids.begin(), ids.end());
+        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+        double s = 0;
+        for (int32_t id : ids) s += pin[3 * (int64_t)id + axis];
+        *F = s;
+    }
+    return done(LPM_OK);
+}

@@ -147,6 +147,14 @@ int spmv_bench(lpm_ctx *c, int reps, double *ms);
 int launch_precond_setup(lpm_ctx *c);
 int sym_prepare(lpm_ctx *c);
 int copy_bench(lpm_ctx *c, int64_t bytes, int reps, double *ms);
+// ---- host phases (ctx.hip)
+int geometry_force_state(lpm_ctx *c);
+int calc_stiffness(lpm_ctx *c);
+int update_stiffness_disp_bc(lpm_ctx *c);
+// ---- tangent and adjoint (kernels_adj.hip)
+int force_jvp(lpm_ctx *c, const double *v, double *jv);
+int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, double *F, int *refinements,
+                     double *rel_residual);
 
 // helpers
 static inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -82,6 +82,8 @@ _SIGS = {
     "lpm_set_timing": (_I, [_P, _I]),
     "lpm_spmv_bench": (_I, [_P, ctypes.c_int32, _DP]),
     "lpm_copy_bench": (_I, [_P, ctypes.c_int64, ctypes.c_int32, _DP]),
+    "lpm_force_jvp": (_I, [_P, _DP, _DP]),
+    "lpm_adjoint_reaction": (_I, [_P, ctypes.c_int32, ctypes.c_int32, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
     "lpm_stream": (_P, [_P]),
 }
 
@@ -413,6 +415,28 @@ class Context:
         ms = _D()
         self._chk(lib().lpm_spmv_bench(self.h, reps, ctypes.byref(ms)))
         return ms.value
+
+    def force_jvp(self, v):
+        """Exact directional derivative of the internal force: (dPin/dx) v, DoF order."""
+        nd = self.n * self.size("dim")
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        assert v.size == nd
+        out = np.zeros(nd)
+        self._chk(lib().lpm_force_jvp(self.h, v.ctypes.data_as(_DP), out.ctypes.data_as(_DP)))
+        return out
+
+    def adjoint_reaction(self, group, axis):
+        """dF/dKn and dF/dTv per bond for F = sum of Pin[axis] over the group (a group id from
+        load_step / group()). Returns (F, dKn, dTv, refinements, relative residual)."""
+        nb = self.size("nbonds")
+        gk, gt = np.zeros(nb), np.zeros(nb)
+        F = _D()
+        it = ctypes.c_int32()
+        rr = _D()
+        ax = "xyz".index(axis) if isinstance(axis, str) else int(axis)
+        self._chk(lib().lpm_adjoint_reaction(self.h, int(group), ax, gk.ctypes.data_as(_DP), gt.ctypes.data_as(_DP),
+                                             ctypes.byref(F), ctypes.byref(it), ctypes.byref(rr)))
+        return F.value, gk, gt, it.value, rr.value
 
     def copy_bench(self, nbytes, reps):
         ms = _D()
