@@ -442,6 +442,17 @@ int lpm_create(const lpm_topology *t, lpm_ctx **out) {
     c->n = n;
     c->nbond = nbond;
     c->maxbond = maxb;
+    for (int64_t p0 = 0; p0 < n; p0 += 32)
+        c->gp_bonds = std::max<int>(c->gp_bonds, bp[std::min<int64_t>(n, p0 + 32)] - bp[p0]);
+    // 2-hop records of a row: for each bond i->k, the bonds of k in the same layer
+    for (int64_t i = 0; i < n; ++i) {
+        int hops = 0;
+        for (int b = bp[i]; b < bp[i + 1]; ++b) {
+            const int k = p2[b];
+            hops += (b < bs[i]) ? bs[k] - bp[k] : bp[k + 1] - bs[k];
+        }
+        c->asm_hops = std::max(c->asm_hops, hops);
+    }
     c->vpad = c->dim == 3 ? 4 : 2;
     c->precision = 0;
     c->cg_tol = 1e-6;

@@ -29,9 +29,14 @@ __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b 
 // ---------------------------------------------------------------- geometry + bond force
 // Particle::updateBondsGeometry + Bond::updatebGeometry (particle.h:163-181, bond.h:30-39),
 // then updateBondsForce of the model (particle_elastic.h:38-50, particle_elastic_damage.h:74-87,
-// particle_j2plasticity.h:63-76, particle_fatigue_hcf.h:65-80). One thread per particle; the
-// per-layer sums run in bond order.
-__global__ __launch_bounds__(256) void k_geometry_force(
+// particle_j2plasticity.h:63-76, particle_fatigue_hcf.h:65-80).
+// A workgroup of one wavefront owns GP consecutive particles and their contiguous bond range.
+// Per-bond work runs lane-per-bond (coalesced loads and stores of the bond arrays); the
+// per-particle sums then run thread-per-particle over the bond values staged in LDS, in bond
+// order, so every sum is formed in the reference's order.
+constexpr int GP = 32;  // particles per workgroup of the bond kernels
+
+__global__ __launch_bounds__(64) void k_geometry_force(
     int64_t n, int model, int do_force, const int32_t *__restrict__ bptr, const int32_t *__restrict__ bsplit,
     const int32_t *__restrict__ p2, const double *__restrict__ xyz, const double *__restrict__ dis0,
     const double *__restrict__ dLp, const double *__restrict__ Kn, const double *__restrict__ Tv,
@@ -39,50 +44,74 @@ __global__ __launch_bounds__(256) void k_geometry_force(
     double *__restrict__ bstrain, double *__restrict__ cs, double *__restrict__ dLe_total,
     double *__restrict__ TdLe_total, double *__restrict__ cs_sum, double *__restrict__ bforce,
     double *__restrict__ bdamage) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2];
-    const int b0 = bptr[i], bm = bsplit[i], b1 = bptr[i + 1];
-    double tot[2];
-    for (int l = 0; l < 2; ++l) {
-        const int lo = l == 0 ? b0 : bm, hi = l == 0 ? bm : b1;
-        double dt = 0, tdt = 0, sx = 0, sy = 0, sz = 0;
-        for (int b = lo; b < hi; ++b) {
-            const int64_t j = p2[b];
-            const double dx = x0 - xyz[3 * j], dy = y0 - xyz[3 * j + 1], dz = z0 - xyz[3 * j + 2];
-            const double d = sqrt(dx * dx + dy * dy + dz * dz);
-            const bool broken = fabs(bdamage[b] - 1.0) < LPM_EPS;
-            const double d0 = dis0[b];
-            const double dl = d - d0;
-            const double bs = broken ? 0.0 : (dl / d);
-            const double le = broken ? 0.0 : (d - d0 - dLp[b]);
-            const double cx = broken ? 0.0 : (dx / d), cy = broken ? 0.0 : (dy / d), cz = broken ? 0.0 : (dz / d);
-            dis[b] = d;
-            bstrain[b] = bs;
-            dLe[b] = le;
-            cs[3 * b] = cx;
-            cs[3 * b + 1] = cy;
-            cs[3 * b + 2] = cz;
-            dt += le;
-            tdt += Tv[b] * le;
-            sx += cx;
-            sy += cy;
-            sz += cz;
-        }
-        dLe_total[2 * i + l] = dt;
-        TdLe_total[2 * i + l] = tdt;
-        cs_sum[6 * i + 3 * l] = sx;
-        cs_sum[6 * i + 3 * l + 1] = sy;
-        cs_sum[6 * i + 3 * l + 2] = sz;
-        tot[l] = dt;
+    extern __shared__ double lds[];
+    __shared__ int sbp[GP + 1], ssp[GP];
+    __shared__ double stot[GP][2];
+    const int tid = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * GP;
+    const int np = (int)((n - p0) < GP ? (n - p0) : GP);
+    if (tid <= np) sbp[tid] = bptr[p0 + tid];
+    if (tid < np) ssp[tid] = bsplit[p0 + tid];
+    __syncthreads();
+    const int B0 = sbp[0], nbk = sbp[np] - B0;
+    double *sle = lds, *stle = sle + nbk, *scx = stle + nbk, *scy = scx + nbk, *scz = scy + nbk;
+    int *own = (int *)(scz + nbk);
+    if (tid < np)
+        for (int b = sbp[tid]; b < sbp[tid + 1]; ++b) own[b - B0] = tid;
+    __syncthreads();
+    for (int t = tid; t < nbk; t += blockDim.x) {
+        const int b = B0 + t;
+        const int64_t i = p0 + own[t];
+        const int64_t j = p2[b];
+        const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1], dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        const bool broken = fabs(bdamage[b] - 1.0) < LPM_EPS;
+        const double d0 = dis0[b];
+        const double dl = d - d0;
+        const double bs = broken ? 0.0 : (dl / d);
+        const double le = broken ? 0.0 : (d - d0 - dLp[b]);
+        const double cx = broken ? 0.0 : (dx / d), cy = broken ? 0.0 : (dy / d), cz = broken ? 0.0 : (dz / d);
+        dis[b] = d;
+        bstrain[b] = bs;
+        dLe[b] = le;
+        cs[3 * b] = cx;
+        cs[3 * b + 1] = cy;
+        cs[3 * b + 2] = cz;
+        sle[t] = le;
+        stle[t] = Tv[b] * le;
+        scx[t] = cx;
+        scy[t] = cy;
+        scz[t] = cz;
     }
+    __syncthreads();
+    if (tid < np) {
+        const int64_t i = p0 + tid;
+        for (int l = 0; l < 2; ++l) {
+            const int lo = (l == 0 ? sbp[tid] : ssp[tid]) - B0, hi = (l == 0 ? ssp[tid] : sbp[tid + 1]) - B0;
+            double dt = 0, tdt = 0, sx = 0, sy = 0, sz = 0;
+            for (int t = lo; t < hi; ++t) {
+                dt += sle[t];
+                tdt += stle[t];
+                sx += scx[t];
+                sy += scy[t];
+                sz += scz[t];
+            }
+            dLe_total[2 * i + l] = dt;
+            TdLe_total[2 * i + l] = tdt;
+            cs_sum[6 * i + 3 * l] = sx;
+            cs_sum[6 * i + 3 * l + 1] = sy;
+            cs_sum[6 * i + 3 * l + 2] = sz;
+            stot[tid][l] = dt;
+        }
+    }
+    __syncthreads();
     if (!do_force) return;
-    const double Di = damage[i];
-    for (int b = b0; b < b1; ++b) {
-        double f = 2. * Kn[b] * dLe[b] + 2. * Tv[b] * tot[b < bm ? 0 : 1];
+    for (int t = tid; t < nbk; t += blockDim.x) {
+        const int b = B0 + t, p = own[t];
+        double f = 2. * Kn[b] * sle[t] + 2. * Tv[b] * stot[p][b < ssp[p] ? 0 : 1];
         double bd = bdamage[b];
         if (model != ELASTIC) {
-            bd = dmax(bd, dmax(Di, damage[p2[b]]));
+            bd = dmax(bd, dmax(damage[p0 + p], damage[p2[b]]));
             bdamage[b] = bd;
         }
         f *= (1.0 - bd);
@@ -114,34 +143,55 @@ __global__ __launch_bounds__(256) void k_bond_force(int64_t n, int model, const 
 }
 
 // Particle::updateParticleForce / updateParticleStress / updateParticleDamageVisual
-// (particle.h:150-161, 197-240); the reverse bond comes from the precomputed opp[].
-__global__ __launch_bounds__(256) void k_particle_force(int64_t n, double volume, int nneighbors,
-                                                         const int32_t *__restrict__ bptr, const int32_t *__restrict__ opp,
-                                                         const double *__restrict__ cs, const double *__restrict__ dis,
-                                                         const double *__restrict__ bforce, const double *__restrict__ bdamage,
-                                                         double *__restrict__ Pin, double *__restrict__ stress,
-                                                         double *__restrict__ damage_visual) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int b0 = bptr[i], b1 = bptr[i + 1];
-    const int nb = b1 - b0;
+// (particle.h:150-161, 197-240); the reverse bond comes from the precomputed opp[]. Same
+// workgroup layout as k_geometry_force: lane-per-bond loads into LDS, then per-particle sums in
+// bond order.
+__global__ __launch_bounds__(64) void k_particle_force(int64_t n, double volume, int nneighbors,
+                                                        const int32_t *__restrict__ bptr, const int32_t *__restrict__ opp,
+                                                        const double *__restrict__ cs, const double *__restrict__ dis,
+                                                        const double *__restrict__ bforce, const double *__restrict__ bdamage,
+                                                        double *__restrict__ Pin, double *__restrict__ stress,
+                                                        double *__restrict__ damage_visual) {
+    extern __shared__ double lds[];
+    __shared__ int sbp[GP + 1];
+    const int tid = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * GP;
+    const int np = (int)((n - p0) < GP ? (n - p0) : GP);
+    if (tid <= np) sbp[tid] = bptr[p0 + tid];
+    __syncthreads();
+    const int B0 = sbp[0], nbk = sbp[np] - B0;
+    double *sff = lds, *scx = sff + nbk, *scy = scx + nbk, *scz = scy + nbk, *sdis = scz + nbk, *sbd = sdis + nbk;
+    for (int t = tid; t < nbk; t += blockDim.x) {
+        const int b = B0 + t;
+        sff[t] = bforce[b] + bforce[opp[b]];
+        scx[t] = cs[3 * b];
+        scy[t] = cs[3 * b + 1];
+        scz[t] = cs[3 * b + 2];
+        sdis[t] = dis[b];
+        sbd[t] = bdamage[b];
+    }
+    __syncthreads();
+    if (tid >= np) return;
+    const int64_t i = p0 + tid;
+    const int lo = sbp[tid] - B0, hi = sbp[tid + 1] - B0;
+    const int nb = hi - lo;
     const double V_m = volume * nb / nneighbors;
     double P0 = 0., P1 = 0., P2 = 0.;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, dv = 0;
-    for (int b = b0; b < b1; ++b) {
-        const double ff = bforce[b] + bforce[opp[b]];
-        const double cx = cs[3 * b], cy = cs[3 * b + 1], cz = cs[3 * b + 2];
+    for (int t = lo; t < hi; ++t) {
+        const double ff = sff[t];
+        const double cx = scx[t], cy = scy[t], cz = scz[t];
         P0 += cx * 0.5 * ff;
         P1 += cy * 0.5 * ff;
         P2 += cz * 0.5 * ff;
-        const double w = 0.5 / V_m * dis[b] * 0.5 * ff;
+        const double w = 0.5 / V_m * sdis[t] * 0.5 * ff;
         s0 += w * cx * cx;
         s1 += w * cy * cy;
         s2 += w * cz * cz;
         s3 += w * cy * cz;
         s4 += w * cx * cz;
         s5 += w * cx * cy;
-        dv += bdamage[b];
+        dv += sbd[t];
     }
     Pin[3 * i] = P0;
     Pin[3 * i + 1] = P1;
@@ -176,32 +226,96 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, int dim, const int3
 // ---------------------------------------------------------------- stiffness assembly
 // Stiffness::calcStiffness with localStiffnessANA (stiffness.h:67-101, 228-263) and
 // fdu2dxyz / fdu2dxyz1 / fdu2dxyz2 (derivative.h:7-239). One wavefront per particle row i,
-// one lane per block (i, j in conns(i)). Every lane of the wave walks the same bonds of i and
-// of i's neighbours (uniform, broadcast loads) and keeps only the terms that land on its own
-// j, in the reference's loop order, so each FP64 block equals the reference's block bit for
-// bit before it is rounded to the storage precision.
+// one lane per block (i, j in conns(i)).
+// The wavefront first stages, with all lanes loading in parallel, what the row needs in LDS:
+// the bonds of i (partner, cosines, Kn, Tv, bdamage) and, for each of them, the same-layer
+// bonds of the partner k (the 2-hop paths). Every lane then walks these lists in the
+// reference's loop order and keeps only the terms that land on its own j, so each FP64 block
+// equals the reference's block bit for bit before it is rounded to the storage precision.
+// LDS per wavefront: LPM_MAXB bond records + `hops` 2-hop records (asm_lds_bytes).
+struct AsmBond {
+    double c[3], tv, kn, bd;
+    int p2, pad;
+};
+struct AsmHop {
+    double c[3], tv, bd;
+    int p2, pad;
+};
+
+__host__ __device__ inline size_t asm_lds_bytes(int hops) {
+    return sizeof(AsmBond) * LPM_MAXB + sizeof(int) * 2 * (LPM_MAXB + 1) + sizeof(AsmHop) * (size_t)hops;
+}
+
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_assemble(
-    int64_t n, int diag10_literal, int transpose, const int32_t *__restrict__ bptr,
+    int64_t n, int hops, int diag10_literal, int transpose, const int32_t *__restrict__ bptr,
     const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2, const int32_t *__restrict__ cptr,
     const int32_t *__restrict__ cidx, const int32_t *__restrict__ tslot, const int64_t *__restrict__ sp,
     const double *__restrict__ cs, const double *__restrict__ cs_sum, const double *__restrict__ Kn,
     const double *__restrict__ Tv, const double *__restrict__ bdamage, T *__restrict__ aval, double *__restrict__ diag64) {
-    const int lane = threadIdx.x & 63;
-    const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (i >= n) return;
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    char *base = (char *)lds + (size_t)wid * asm_lds_bytes(hops);
+    AsmBond *B = (AsmBond *)base;
+    int *soff = (int *)(B + LPM_MAXB);  // 2-hop segment offsets of each bond of i (+ end)
+    int *slo = soff + LPM_MAXB + 1;     // first same-layer bond of the partner
+    AsmHop *H = (AsmHop *)(slo + LPM_MAXB + 1);
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid;
+    const bool row = i < n;
+    const int ib0 = row ? bptr[i] : 0, ibm = row ? bsplit[i] : 0, ib1 = row ? bptr[i + 1] : 0;
+    const int nbi = ib1 - ib0, tm = ibm - ib0;
+    // stage the bonds of i and the length of each bond's 2-hop list
+    if (lane < nbi) {
+        const int b = ib0 + lane;
+        const int k = p2[b];
+        AsmBond r;
+        r.c[0] = cs[3 * b];
+        r.c[1] = cs[3 * b + 1];
+        r.c[2] = cs[3 * b + 2];
+        r.tv = Tv[b];
+        r.kn = Kn[b];
+        r.bd = bdamage[b];
+        r.p2 = k;
+        r.pad = 0;
+        B[lane] = r;
+        const int klo = lane < tm ? bptr[k] : bsplit[k], khi = lane < tm ? bsplit[k] : bptr[k + 1];
+        slo[lane] = klo;
+        soff[lane + 1] = khi - klo;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        soff[0] = 0;
+        for (int t = 0; t < nbi; ++t) soff[t + 1] += soff[t];
+    }
+    __syncthreads();
+    const int E = soff[nbi];
+    for (int e = lane; e < E; e += 64) {
+        int t = 0;
+        while (soff[t + 1] <= e) ++t;
+        const int b3 = slo[t] + (e - soff[t]);
+        AsmHop h;
+        h.c[0] = cs[3 * b3];
+        h.c[1] = cs[3 * b3 + 1];
+        h.c[2] = cs[3 * b3 + 2];
+        h.tv = Tv[b3];
+        h.bd = bdamage[b3];
+        h.p2 = p2[b3];
+        h.pad = 0;
+        H[e] = h;
+    }
+    __syncthreads();
+    if (!row) return;
     const int c0 = cptr[i], nci = cptr[i + 1] - c0;
-    const int ib0 = bptr[i], ibm = bsplit[i], ib1 = bptr[i + 1];
     for (int slot = lane; slot < nci; slot += 64) {
         const int j = cidx[c0 + slot];
         double K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         if (j == i) {
             for (int l = 0; l < 2; ++l) {
-                const int lo = l == 0 ? ib0 : ibm, hi = l == 0 ? ibm : ib1;
+                const int lo = l == 0 ? 0 : tm, hi = l == 0 ? tm : nbi;
                 const double S0 = cs_sum[6 * i + 3 * l], S1 = cs_sum[6 * i + 3 * l + 1], S2 = cs_sum[6 * i + 3 * l + 2];
                 for (int b = lo; b < hi; ++b) {
-                    const double w = (1 - bdamage[b]), kn = Kn[b], tv = Tv[b];
-                    const double c0_ = cs[3 * b], c1 = cs[3 * b + 1], c2 = cs[3 * b + 2];
+                    const double w = (1 - B[b].bd), kn = B[b].kn, tv = B[b].tv;
+                    const double c0_ = B[b].c[0], c1 = B[b].c[1], c2 = B[b].c[2];
                     K[0][0] += w * ((kn * c0_ + tv * S0) * c0_ + (kn + tv) * (c0_ * c0_));
                     K[0][1] += w * ((kn * c1 + tv * S1) * c0_ + (kn + tv) * c1 * c0_);
                     K[1][1] += w * ((kn * c1 + tv * S1) * c1 + (kn + tv) * (c1 * c1));
@@ -222,21 +336,21 @@ __global__ __launch_bounds__(256) void k_assemble(
             for (int l = 0; l < 2; ++l) {
                 for (int r = 0; r < 3; ++r)
                     for (int q = 0; q < 3; ++q) Kl[l][r][q] = 0.0;
-                const int lo = l == 0 ? ib0 : ibm, hi = l == 0 ? ibm : ib1;
+                const int lo = l == 0 ? 0 : tm, hi = l == 0 ? tm : nbi;
                 int bj = -1;  // first bond i->j of this layer
                 for (int b = lo; b < hi; ++b)
-                    if (bj < 0 && p2[b] == j) bj = b;
+                    if (bj < 0 && B[b].p2 == j) bj = b;
                 bool touched = bj >= 0;
                 if (bj >= 0) {  // derivative.h:53-119
-                    const double n1[3] = {cs[3 * bj], cs[3 * bj + 1], cs[3 * bj + 2]};
-                    const double wT1 = (1 - bdamage[bj]) * Tv[bj];
+                    const double n1[3] = {B[bj].c[0], B[bj].c[1], B[bj].c[2]};
+                    const double wT1 = (1 - B[bj].bd) * B[bj].tv;
                     const double S[3] = {cs_sum[6 * j + 3 * l], cs_sum[6 * j + 3 * l + 1], cs_sum[6 * j + 3 * l + 2]};
                     for (int b2 = lo; b2 < hi; ++b2) {
-                        const int k = p2[b2];
-                        const double n2[3] = {cs[3 * b2], cs[3 * b2 + 1], cs[3 * b2 + 2]};
-                        const double wT2 = (1 - bdamage[b2]) * Tv[b2];
+                        const int k = B[b2].p2;
+                        const double n2[3] = {B[b2].c[0], B[b2].c[1], B[b2].c[2]};
+                        const double wT2 = (1 - B[b2].bd) * B[b2].tv;
                         if (k == j) {
-                            const double w = (1 - bdamage[b2]), kn = Kn[b2], kt = Kn[b2] + Tv[b2], tv = Tv[b2];
+                            const double w = (1 - B[b2].bd), kn = B[b2].kn, kt = B[b2].kn + B[b2].tv, tv = B[b2].tv;
                             for (int r = 0; r < D; ++r)
                                 for (int q = 0; q < D; ++q) {
                                     const double pr = (r == q) ? kt * (n2[r] * n2[r])
@@ -247,11 +361,10 @@ __global__ __launch_bounds__(256) void k_assemble(
                             const double a = 0.5 * (wT1 + wT2);
                             for (int r = 0; r < D; ++r)
                                 for (int q = 0; q < D; ++q) Kl[l][r][q] -= a * n2[r] * n1[q];
-                            const int klo = l == 0 ? bptr[k] : bsplit[k], khi = l == 0 ? bsplit[k] : bptr[k + 1];
-                            for (int b3 = klo; b3 < khi; ++b3) {
-                                if (p2[b3] == j) {
-                                    const double n3[3] = {cs[3 * b3], cs[3 * b3 + 1], cs[3 * b3 + 2]};
-                                    const double a3 = 0.5 * ((1 - bdamage[b3]) * Tv[b3] + wT2);
+                            for (int e = soff[b2]; e < soff[b2 + 1]; ++e) {
+                                if (H[e].p2 == j) {
+                                    const double n3[3] = {H[e].c[0], H[e].c[1], H[e].c[2]};
+                                    const double a3 = 0.5 * ((1 - H[e].bd) * H[e].tv + wT2);
                                     for (int r = 0; r < D; ++r)
                                         for (int q = 0; q < D; ++q) Kl[l][r][q] -= a3 * n2[r] * n3[q];
                                 }
@@ -260,14 +373,12 @@ __global__ __launch_bounds__(256) void k_assemble(
                     }
                 } else {  // derivative.h:122-139
                     for (int b1 = lo; b1 < hi; ++b1) {
-                        const int k = p2[b1];
-                        const int klo = l == 0 ? bptr[k] : bsplit[k], khi = l == 0 ? bsplit[k] : bptr[k + 1];
-                        for (int b2 = klo; b2 < khi; ++b2) {
-                            if (p2[b2] == j) {
+                        for (int e = soff[b1]; e < soff[b1 + 1]; ++e) {
+                            if (H[e].p2 == j) {
                                 touched = true;
-                                const double n1[3] = {cs[3 * b1], cs[3 * b1 + 1], cs[3 * b1 + 2]};
-                                const double n2[3] = {cs[3 * b2], cs[3 * b2 + 1], cs[3 * b2 + 2]};
-                                const double a = 0.5 * ((1 - bdamage[b2]) * Tv[b2] + (1 - bdamage[b1]) * Tv[b1]);
+                                const double n1[3] = {B[b1].c[0], B[b1].c[1], B[b1].c[2]};
+                                const double n2[3] = {H[e].c[0], H[e].c[1], H[e].c[2]};
+                                const double a = 0.5 * ((1 - H[e].bd) * H[e].tv + (1 - B[b1].bd) * B[b1].tv);
                                 for (int r = 0; r < D; ++r)
                                     for (int q = 0; q < D; ++q) Kl[l][r][q] -= a * n1[r] * n2[q];
                             }
@@ -664,7 +775,8 @@ static int check_launch(lpm_ctx *c, const char *what) {
 }
 
 int launch_geometry_force(lpm_ctx *c, bool force) {
-    hipLaunchKernelGGL(k_geometry_force, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->model, force ? 1 : 0,
+    const size_t lds = (size_t)c->gp_bonds * (5 * sizeof(double) + sizeof(int));
+    hipLaunchKernelGGL(k_geometry_force, dim3(blocks_for(c->n, GP)), dim3(64), lds, c->stream, c->n, c->model, force ? 1 : 0,
                        c->bptr, c->bsplit, c->p2, c->xyz, c->dis0, c->dLp, c->Kn, c->Tv, c->damage, c->dis, c->dLe,
                        c->bstrain, c->cs, c->dLe_total, c->TdLe_total, c->cs_sum, c->bforce, c->bdamage);
     return check_launch(c, "k_geometry_force");
@@ -677,7 +789,8 @@ int launch_bond_force(lpm_ctx *c) {
 }
 
 int launch_particle_force(lpm_ctx *c) {
-    hipLaunchKernelGGL(k_particle_force, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->volume,
+    const size_t lds = (size_t)c->gp_bonds * 6 * sizeof(double);
+    hipLaunchKernelGGL(k_particle_force, dim3(blocks_for(c->n, GP)), dim3(64), lds, c->stream, c->n, c->volume,
                        c->nneighbors, c->bptr, c->opp, c->cs, c->dis, c->bforce, c->bdamage, c->Pin, c->stress,
                        c->damage_visual);
     return check_launch(c, "k_particle_force");
@@ -703,29 +816,24 @@ int launch_residual(lpm_ctx *c, double *nr2, double *nR2) {
 }
 
 int launch_assemble(lpm_ctx *c) {
-    const int g = blocks_for(c->n, 4);
     if (c->cg_transpose && !c->tslot) return lpm_fail(c, LPM_E_STATE, "transpose map missing");
+    // wavefronts per workgroup: as many as fit 64 KB of LDS, at most 4
+    const size_t per = asm_lds_bytes(c->asm_hops);
+    const int W = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per));
+    const int g = blocks_for(c->n, W);
+    const size_t lds = per * W;
+#define ASM_LAUNCH(T, D)                                                                                               \
+    hipLaunchKernelGGL((k_assemble<T, D>), dim3(g), dim3(64 * W), lds, c->stream, c->n, c->asm_hops, c->diag10_literal,   \
+                       c->cg_transpose, c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum,  \
+                       c->Kn, c->Tv, c->bdamage, (T *)c->aval, c->diag64)
     if (c->precision == 0) {
-        float *a = (float *)c->aval;
-        if (c->dim == 3)
-            hipLaunchKernelGGL((k_assemble<float, 3>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
-                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum, c->Kn, c->Tv,
-                               c->bdamage, a, c->diag64);
-        else
-            hipLaunchKernelGGL((k_assemble<float, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
-                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum, c->Kn, c->Tv,
-                               c->bdamage, a, c->diag64);
+        if (c->dim == 3) ASM_LAUNCH(float, 3);
+        else ASM_LAUNCH(float, 2);
     } else {
-        double *a = (double *)c->aval;
-        if (c->dim == 3)
-            hipLaunchKernelGGL((k_assemble<double, 3>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
-                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum, c->Kn, c->Tv,
-                               c->bdamage, a, c->diag64);
-        else
-            hipLaunchKernelGGL((k_assemble<double, 2>), dim3(g), dim3(256), 0, c->stream, c->n, c->diag10_literal, c->cg_transpose,
-                               c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum, c->Kn, c->Tv,
-                               c->bdamage, a, c->diag64);
+        if (c->dim == 3) ASM_LAUNCH(double, 3);
+        else ASM_LAUNCH(double, 2);
     }
+#undef ASM_LAUNCH
     return check_launch(c, "k_assemble");
 }
 

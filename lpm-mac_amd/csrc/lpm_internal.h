@@ -48,6 +48,8 @@ struct lpm_ctx {
     double radius = 0, volume = 0, cutoff[2] = {0, 0};
     int64_t n = 0, nbond = 0, nconn = 0, nnl = 0;
     int maxconn = 0, maxbond = 0;
+    int gp_bonds = 0;  // most bonds of any group of 32 consecutive particles (LDS of the bond kernels)
+    int asm_hops = 0;  // most 2-hop bond records of any particle (LDS of the assembly)
     // modes
     int diag10_literal = 1, cg_transpose = 1, dirichlet_static = 0, rollback_pex = 0;
     int precision = 0, precond = 0, verbose = 0, max_NR_iter = 30, cg_batch = 16;
