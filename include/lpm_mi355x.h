@@ -117,7 +117,13 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        solve of iteration k stops at max(cg_tol, min(0.1, tol_NR*tol_mult/
  *                        (2 ||r_k||))), i.e. it asks only for the reduction still needed to pass
  *                        the Newton test; the Newton test itself is unchanged
- *  "precond"          0: Jacobi (Eigen DiagonalPreconditioner) | 1: 3x3 block Jacobi
+ *  "precond"          0: Jacobi (Eigen DiagonalPreconditioner) | 1: 3x3 block Jacobi |
+ *                        2: Jacobi + additive coarse correction on the rigid-body modes of cubic
+ *                        aggregates of particles (two-level; iteration count no longer grows
+ *                        with the lattice width). Changes the CG's rate, not the equilibrium.
+ *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 (0: smallest of 8, 12,
+ *                        16, 24, ... giving at most 512 coarse DoFs); get_mode("ncoarse") reads
+ *                        the coarse dimension once built
  *  "spmv_sym"         0: SpMV on the full matrix | 1: after each assembly check that the
  *                        stored matrix is symmetric to rounding and, if so, run the SpMV from
  *                        its upper half (each block read from HBM once, the transposed half

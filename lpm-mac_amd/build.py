@@ -27,7 +27,7 @@ def run(cmd):
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     objs = []
-    for src in ("kernels_phys.hip", "kernels_cg.hip", "kernels_adj.hip", "ctx.hip"):
+    for src in ("kernels_phys.hip", "kernels_cg.hip", "kernels_coarse.hip", "kernels_adj.hip", "ctx.hip"):
         o = os.path.join(OBJ, src + ".o")
         run(["hipcc", "-c", *HIP_FLAGS, *EXACT.get(src, []), "-o", o, os.path.join(CSRC, src)])
         objs.append(o)

@@ -537,6 +537,7 @@ int lpm_destroy(lpm_ctx *c) {
                     c->dslot, c->sp_u, c->sp_l, c->ucol, c->lcol, c->lpos, c->uval};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    coarse_free(c);
     for (auto &g : c->groups)
         if (g.ids) hipFree(g.ids);
     if (c->h_scal) hipHostFree(c->h_scal);
@@ -577,7 +578,13 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
         c->precision = (int)v ? 1 : 0;
     } else if (w == "cg_tol") c->cg_tol = v;
     else if (w == "cg_maxit") c->cg_maxit = (int64_t)v;
-    else if (w == "precond") c->precond = (int)v ? 1 : 0;
+    else if (w == "precond") {
+        if (v != 0 && v != 1 && v != 2) return lpm_fail(c, LPM_E_ARG, "precond must be 0, 1 or 2");
+        c->precond = (int)v;
+    } else if (w == "coarse_cells") {
+        if (c->agg_of) return lpm_fail(c, LPM_E_STATE, "coarse_cells is fixed once the coarse space exists");
+        c->coarse_cells = std::max(0, (int)v);
+    }
     else if (w == "max_NR_iter") c->max_NR_iter = (int)v;
     else if (w == "tol_NR") c->tol_NR = v;
     else if (w == "verbose") c->verbose = (int)v;
@@ -618,6 +625,8 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "spmv_sym") return c->spmv_sym;
     if (w == "spmv_tune") return c->spmv_tune;
     if (w == "cg_forcing") return c->cg_forcing;
+    if (w == "coarse_cells") return c->coarse_cells;
+    if (w == "ncoarse") return c->ncoarse;
     if (w == "spmv_unr") return c->spmv_unr;
     if (w == "spmv_wpb") return c->spmv_wpb;
     if (w == "spmv_remap") return c->spmv_remap;

@@ -385,15 +385,72 @@ __global__ __launch_bounds__(512) void k_spmv_sym(int64_t n, int64_t nslice, con
 
 // ---------------------------------------------------------------- fused scalar step + vector update
 // Iteration `it` (the host passes it; every workgroup of the launch sees the same value):
-//  1. every workgroup sums the partials of ||r_it||^2 and gamma_it = (r_it, u_it) (written by
-//     the previous vector kernel, parity slot it&1) and delta_it = (w_it, u_it) (written by the
-//     SpMV) in one fixed order, so all workgroups reach the same stopping decision and the same
-//     alpha/beta without a separate finalize launch;
+//  1. every workgroup sums the partials of ||r_it||^2, gamma_it = (r_it, u_it) (written by the
+//     previous vector kernel) and delta_it = (w_it, u_it) (written by the SpMV) in one fixed
+//     order, so all workgroups reach the same stopping decision and the same alpha/beta without
+//     a separate finalize launch;
 //  2. stop if ||r_it||^2 < thr (it = 0: thr = max(tol^2 ||b||^2, DBL_MIN), zero rhs) or it >= maxit;
 //  3. else p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = M^-1 r, and write the
 //     partials of ||r||^2 and gamma for iteration it+1 into the other parity slot.
 // Workgroup 0 records the scalars; gamma and alpha live in per-parity slots so that the
 // workgroups of one launch read the previous iteration's values while workgroup 0 writes its own.
+// Returns alpha, beta and the stop code in coef (shared).
+__device__ void cg_scalar_step(const double *prr, const double *pg, const double *pd, int np_vec, int np_spmv,
+                               double *scal, int it, double tol, double maxit, double *coef) {
+    __shared__ double red[3][4];
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < np_vec; b += blockDim.x) {
+        t[0] += prr[b];
+        t[1] += pg[b];
+    }
+    for (int b = threadIdx.x; b < np_spmv; b += blockDim.x) t[2] += pd[b];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < 3; ++k) {
+        double v = t[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[k][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        double tot[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 3; ++k)
+            for (int q = 0; q < nw; ++q) tot[k] += red[k][q];
+        const double rr = tot[0], gamma = tot[1], delta = tot[2];
+        const int cur = it & 1, prev = cur ^ 1;
+        double alpha = 0.0, beta = 0.0, done = 0.0, thr;
+        if (it == 0) {
+            thr = fmax(tol * tol * rr, DBL_MIN);
+            done = (rr == 0.0) ? 3.0 : ((rr < thr) ? 1.0 : (maxit <= 0 ? 2.0 : 0.0));
+            alpha = gamma / delta;
+        } else {
+            thr = scal[SC_THR];
+            if (rr < thr) done = 1.0;
+            else if ((double)it >= maxit) done = 2.0;
+            else {
+                beta = gamma / scal[SC_GAMMA_P + prev];
+                alpha = gamma / (delta - beta * gamma / scal[SC_ALPHA_P + prev]);
+            }
+        }
+        coef[0] = alpha;
+        coef[1] = beta;
+        coef[2] = done;
+        if (blockIdx.x == 0) {
+            if (it == 0) {
+                scal[SC_RHS2] = rr;
+                scal[SC_THR] = thr;
+                scal[SC_MAXIT] = maxit;
+            }
+            scal[SC_RR] = rr;
+            scal[SC_ITER] = it;
+            scal[SC_GAMMA_P + cur] = gamma;
+            scal[SC_ALPHA_P + cur] = alpha;
+            scal[SC_DONE] = done;
+        }
+    }
+    __syncthreads();
+}
+
 template <typename T, int D, int VP>
 __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__ invd, int block_pre,
                                                  Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
@@ -402,63 +459,9 @@ __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__
                                                  double *partial, int cap, int np_vec, int np_spmv, double *scal, int it,
                                                  double tol, double maxit) {
     if (scal[SC_DONE] != 0.0) return;
-    __shared__ double red[3][4];
     __shared__ double coef[3];  // alpha, beta, done
-    {
-        const double *pv = partial + (int64_t)(it & 1) * 2 * cap;
-        const double *pd = partial + (int64_t)4 * cap;
-        double t[3] = {0.0, 0.0, 0.0};
-        for (int b = threadIdx.x; b < np_vec; b += blockDim.x) {
-            t[0] += pv[b];
-            t[1] += pv[cap + b];
-        }
-        for (int b = threadIdx.x; b < np_spmv; b += blockDim.x) t[2] += pd[b];
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        for (int k = 0; k < 3; ++k) {
-            double v = t[k];
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (lane == 0) red[k][wid] = v;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int nw = blockDim.x >> 6;
-            double tot[3] = {0.0, 0.0, 0.0};
-            for (int k = 0; k < 3; ++k)
-                for (int q = 0; q < nw; ++q) tot[k] += red[k][q];
-            const double rr = tot[0], gamma = tot[1], delta = tot[2];
-            const int cur = it & 1, prev = cur ^ 1;
-            double alpha = 0.0, beta = 0.0, done = 0.0, thr;
-            if (it == 0) {
-                thr = fmax(tol * tol * rr, DBL_MIN);
-                done = (rr == 0.0) ? 3.0 : ((rr < thr) ? 1.0 : (maxit <= 0 ? 2.0 : 0.0));
-                alpha = gamma / delta;
-            } else {
-                thr = scal[SC_THR];
-                if (rr < thr) done = 1.0;
-                else if ((double)it >= maxit) done = 2.0;
-                else {
-                    beta = gamma / scal[SC_GAMMA_P + prev];
-                    alpha = gamma / (delta - beta * gamma / scal[SC_ALPHA_P + prev]);
-                }
-            }
-            coef[0] = alpha;
-            coef[1] = beta;
-            coef[2] = done;
-            if (blockIdx.x == 0) {
-                if (it == 0) {
-                    scal[SC_RHS2] = rr;
-                    scal[SC_THR] = thr;
-                    scal[SC_MAXIT] = maxit;
-                }
-                scal[SC_RR] = rr;
-                scal[SC_ITER] = it;
-                scal[SC_GAMMA_P + cur] = gamma;
-                scal[SC_ALPHA_P + cur] = alpha;
-                scal[SC_DONE] = done;
-            }
-        }
-        __syncthreads();
-    }
+    const double *pv = partial + (int64_t)(it & 1) * 2 * cap;
+    cg_scalar_step(pv, pv + cap, partial + (int64_t)4 * cap, np_vec, np_spmv, scal, it, tol, maxit, coef);
     if (coef[2] != 0.0) return;
     double v[2] = {0.0, 0.0};
     const T alpha = (T)coef[0], beta = (T)coef[1];
@@ -483,6 +486,67 @@ __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__
         }
     }
     block_partial<2>(v, partial + (int64_t)((it & 1) ^ 1) * 2 * cap, cap);
+}
+
+// Two-level variant, first half: the same scalar step (gamma partials at the fixed offset cap),
+// then p, s, x, r updates and the ||r||^2 partials; u = M^-1 r waits for the coarse correction.
+template <typename T, int D, int VP>
+__global__ __launch_bounds__(256) void k_cg_vec_a(int64_t n, Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
+                                                   const Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
+                                                   Vec<T, VP> *__restrict__ s, const Vec<T, VP> *__restrict__ w,
+                                                   double *partial, int cap, int np_vec, int np_spmv, double *scal, int it,
+                                                   double tol, double maxit) {
+    if (scal[SC_DONE] != 0.0) return;
+    __shared__ double coef[3];
+    cg_scalar_step(partial + (int64_t)(it & 1) * 2 * cap, partial + cap, partial + (int64_t)4 * cap, np_vec, np_spmv, scal,
+                   it, tol, maxit, coef);
+    if (coef[2] != 0.0) return;
+    double v[1] = {0.0};
+    const T alpha = (T)coef[0], beta = (T)coef[1];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        Vec<T, VP> pv = p[i], sv = s[i], xv = x[i], rv = r[i];
+        const Vec<T, VP> uv = u[i], wv = w[i];
+        for (int k = 0; k < D; ++k) {
+            pv.v[k] = uv.v[k] + beta * pv.v[k];
+            sv.v[k] = wv.v[k] + beta * sv.v[k];
+            xv.v[k] += alpha * pv.v[k];
+            rv.v[k] -= alpha * sv.v[k];
+        }
+        p[i] = pv;
+        s[i] = sv;
+        x[i] = xv;
+        r[i] = rv;
+        for (int k = 0; k < D; ++k) v[0] += (double)rv.v[k] * (double)rv.v[k];
+    }
+    block_partial<1>(v, partial + (int64_t)((it & 1) ^ 1) * 2 * cap, cap);
+}
+
+// Two-level variant, second half: u = D^-1 r + P Ac^-1 P^T r (the coarse correction cy is
+// already formed) and the gamma = (r, u) partials at offset cap.
+template <typename T, int D, int VP>
+__global__ __launch_bounds__(256) void k_cg_vec_b(int64_t n, const T *__restrict__ invd, int block_pre,
+                                                   const Vec<T, VP> *__restrict__ r, Vec<T, VP> *__restrict__ u,
+                                                   const int32_t *__restrict__ agg_of, const double *__restrict__ agg_d,
+                                                   const uint8_t *__restrict__ mask, const double *__restrict__ cy, int M,
+                                                   double *partial, int cap, const double *scal) {
+    if (scal[SC_DONE] != 0.0) return;
+    double v[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const Vec<T, VP> rv = r[i];
+        Vec<T, VP> zv;
+        precond_apply<T, D, VP>(invd, block_pre, i, rv, zv);
+        double P[3][6];
+        rigid_modes<D>(&agg_d[3 * i], mask[i], P);
+        const double *y = cy + (int64_t)agg_of[i] * M;
+        for (int k = 0; k < D; ++k) {
+            double t = 0.0;
+            for (int m = 0; m < M; ++m) t += P[k][m] * y[m];
+            zv.v[k] += (T)t;
+        }
+        u[i] = zv;
+        for (int k = 0; k < D; ++k) v[0] += (double)rv.v[k] * (double)zv.v[k];
+    }
+    block_partial<1>(v, partial + cap, cap);
 }
 
 // ---------------------------------------------------------------- one-block finalize
@@ -644,7 +708,7 @@ int cg_alloc(lpm_ctx *c) {
 template <typename T, int D, int VP>
 static int precond_t(lpm_ctx *c) {
     hipLaunchKernelGGL((k_precond<T, D>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp,
-                       (const T *)c->aval, c->precond, (T *)c->invdiag);
+                       (const T *)c->aval, c->precond == 1, (T *)c->invdiag);
     return check_launch(c, "k_precond");
 }
 
@@ -834,10 +898,21 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     if (gv > c->partial_cap || gs > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "CG grid exceeds partial buffer");
     const int64_t maxit = c->cg_maxit < 0 ? 2 * n * c->dim : c->cg_maxit;
     hipLaunchKernelGGL((k_cg_init<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, c->residual, (const T *)c->invdiag,
-                       c->precond, (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s, c->partial,
+                       c->precond == 1, (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s, c->partial,
                        c->partial_cap, c->scal);
     int rc = check_launch(c, "k_cg_init");
     if (rc) return rc;
+    const bool coarse = c->precond == 2;
+    // two-level: u = D^-1 r + P Ac^-1 P^T r (restriction + coarse solve, then the fused update)
+    auto coarse_u = [&]() -> int {
+        int r2 = launch_coarse_apply(c);
+        if (r2) return r2;
+        hipLaunchKernelGGL((k_cg_vec_b<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (const T *)c->invdiag, 0,
+                           (const V *)c->v_r, (V *)c->v_u, c->agg_of, c->agg_d, c->dofmask, (const double *)c->cy, c->nmode,
+                           c->partial, c->partial_cap, (const double *)c->scal);
+        return check_launch(c, "k_cg_vec_b");
+    };
+    if (coarse && (rc = coarse_u())) return rc;
     rc = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
     if (rc) return rc;
     // batches of `batch` iterations (an even number, so that a batch starts on parity 0);
@@ -854,10 +929,19 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     int64_t it0 = 0;
     for (;;) {
         for (int k = 0; k < batch; ++k) {
-            hipLaunchKernelGGL((k_cg_vec<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (const T *)c->invdiag, c->precond,
-                               (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s, (const V *)c->v_w, c->partial,
-                               c->partial_cap, gv, gs, c->scal, (int)(it0 + k), c->cg_tol, (double)maxit);
-            rc = check_launch(c, "k_cg_vec");
+            if (coarse) {
+                hipLaunchKernelGGL((k_cg_vec_a<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (V *)c->v_x, (V *)c->v_r,
+                                   (const V *)c->v_u, (V *)c->v_p, (V *)c->v_s, (const V *)c->v_w, c->partial, c->partial_cap,
+                                   gv, gs, c->scal, (int)(it0 + k), c->cg_tol, (double)maxit);
+                rc = check_launch(c, "k_cg_vec_a");
+                if (!rc) rc = coarse_u();
+            } else {
+                hipLaunchKernelGGL((k_cg_vec<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (const T *)c->invdiag,
+                                   c->precond == 1, (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s,
+                                   (const V *)c->v_w, c->partial, c->partial_cap, gv, gs, c->scal, (int)(it0 + k), c->cg_tol,
+                                   (double)maxit);
+                rc = check_launch(c, "k_cg_vec");
+            }
             if (rc) return rc;
             if (timed(it0 + k)) LPM_HIP(c, hipEventRecord(evs[2 * k], c->stream));
             rc = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
@@ -898,6 +982,7 @@ int cg_solve(lpm_ctx *c, int *iters, double *error, int *success) {
     if (!c->k_ready) return lpm_fail(c, LPM_E_STATE, "solve before the stiffness matrix was assembled");
     int rc = launch_precond_setup(c);
     if (rc) return rc;
+    if ((rc = coarse_prepare(c))) return rc;
     if ((rc = sym_prepare(c))) return rc;
     if ((rc = spmv_tune(c))) return rc;
     if (c->sym_ok) c->st_sym_solves++;

@@ -1,0 +1,412 @@
+// Two-level preconditioner for the CG: Jacobi plus an additive coarse-space correction,
+//   M^-1 r = D^-1 r + P Ac^-1 P^T r,   Ac = P^T K P,
+// where the columns of P are the rigid-body modes (translations and infinitesimal rotations
+// about the aggregate centre) of cubic aggregates of particles, with the rows of eliminated
+// DoFs set to zero. The Jacobi part alone lets the CG iteration count grow with the lattice
+// width; the coarse space removes the smooth, nearly rigid components that cause that growth.
+// It changes only the rate of the inner linear solve: the Newton test and the equilibrium it
+// converges to are the same as with Jacobi.
+//
+// Setup (once per context, the pattern never changes): aggregates from the initial positions,
+// and for every coarse block (g, h) the list of row entries that contribute to it.
+// Per assembly: contributions P_i^T K_ij P_j summed per (row i, aggregate h) in slot order,
+// then per coarse block in row order (no atomics: the result does not depend on scheduling),
+// then an in-place Gauss-Jordan inversion of Ac in one workgroup.
+// Per CG iteration: one workgroup per aggregate forms P^T r, one small kernel forms Ac^-1 P^T r;
+// the prolongation is fused into the CG's preconditioner step (kernels_cg.hip).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+#include "lpm_internal.h"
+#include "reduce.h"
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_dofmask(int64_t n, int dim, const int32_t *__restrict__ cons,
+                                                  const double *__restrict__ damage_visual, uint8_t *__restrict__ mask) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned m = 0;
+    const bool dead = fabs(damage_visual[i] - 1.0) < LPM_EPS;
+    for (int k = 0; k < dim; ++k)
+        if (cons[3 * i + k] == 1 || dead) m |= 1u << k;
+    mask[i] = (uint8_t)m;
+}
+
+// Stage 1, one thread per row i: for each aggregate h among the columns of row i (in the
+// row's aggregate list), V = P_i^T (sum over slots with agg(j) = h, in slot order, of K_ij P_j).
+// With P_j = [I | R(d_j)], K_ij P_j = [K_ij | K_ij R(d_j)].
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_coarse_rows(int64_t n, const int32_t *__restrict__ cptr,
+                                                      const int32_t *__restrict__ cidx, const int64_t *__restrict__ sp,
+                                                      const T *__restrict__ aval, const int32_t *__restrict__ agg_of,
+                                                      const double *__restrict__ agg_d, const uint8_t *__restrict__ mask,
+                                                      const int32_t *__restrict__ ra_ptr, const int32_t *__restrict__ ra_id,
+                                                      double *__restrict__ vbuf) {
+    constexpr int EB = D * D, M = D == 3 ? 6 : 3;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c0 = cptr[i], nc = cptr[i + 1] - c0;
+    double Pi[3][6];
+    rigid_modes<D>(&agg_d[3 * i], mask[i], Pi);
+    for (int f = ra_ptr[i]; f < ra_ptr[i + 1]; ++f) {
+        const int h = ra_id[f];
+        double W[D][M];
+        for (int r = 0; r < D; ++r)
+            for (int m = 0; m < M; ++m) W[r][m] = 0.0;
+        for (int c = 0; c < nc; ++c) {
+            const int64_t j = cidx[c0 + c];
+            if (agg_of[j] != h) continue;
+            double Pj[3][6];
+            rigid_modes<D>(&agg_d[3 * j], mask[j], Pj);
+            double Kb[D][D];
+            for (int r = 0; r < D; ++r)
+                for (int q = 0; q < D; ++q) Kb[r][q] = (double)aval[sell_off(sp, i, c, EB, r * D + q)];
+            for (int r = 0; r < D; ++r)
+                for (int m = 0; m < M; ++m) {
+                    double t = 0.0;
+                    for (int q = 0; q < D; ++q) t += Kb[r][q] * Pj[q][m];
+                    W[r][m] += t;
+                }
+        }
+        double *out = vbuf + (int64_t)f * M * M;
+        for (int a = 0; a < M; ++a)
+            for (int b = 0; b < M; ++b) {
+                double t = 0.0;
+                for (int r = 0; r < D; ++r) t += Pi[r][a] * W[r][b];
+                out[a * M + b] = t;
+            }
+    }
+}
+
+// Stage 2, one wavefront per coarse block q = (g, h): lanes take the block's row entries in a
+// fixed interleave, each sums its entries' M x M contributions, and a fixed shuffle tree adds
+// the lanes; Ac[g M + a][h M + b] is then the sum over the block's entries.
+template <int M>
+__global__ __launch_bounds__(256) void k_coarse_blocks(int64_t ncq, int nc, const int32_t *__restrict__ cq_ptr,
+                                                        const int32_t *__restrict__ cq_f, const int32_t *__restrict__ cq_gh,
+                                                        const double *__restrict__ vbuf, double *__restrict__ Ac) {
+    constexpr int MM = M * M;
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (q >= ncq) return;
+    double acc[MM];
+    for (int e = 0; e < MM; ++e) acc[e] = 0.0;
+    for (int k = cq_ptr[q] + lane; k < cq_ptr[q + 1]; k += 64) {
+        const double *v = vbuf + (int64_t)cq_f[k] * MM;
+        for (int e = 0; e < MM; ++e) acc[e] += v[e];
+    }
+    for (int e = 0; e < MM; ++e)
+        for (int o = 32; o > 0; o >>= 1) acc[e] += __shfl_xor(acc[e], o, 64);
+    const int g = cq_gh[2 * q], h = cq_gh[2 * q + 1];
+    for (int e = lane; e < MM; e += 64) {
+        double val = 0.0;
+        for (int t = 0; t < MM; ++t)
+            if (t == e) val = acc[t];
+        Ac[(int64_t)(g * M + e / M) * nc + (h * M + e % M)] = val;
+    }
+}
+
+// Gauss-Jordan inversion of the symmetrised Ac (SPD, so no pivoting), one launch per pivot k,
+// reading A_in and writing A_out (no element is read after another thread wrote it). A coarse
+// DoF whose pivot vanishes (an aggregate whose DoFs are all eliminated) is dropped: its row and
+// column become zero, so it receives no correction.
+__global__ __launch_bounds__(256) void k_coarse_sym(int nc, double *__restrict__ A, double *__restrict__ thr) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < (int64_t)nc * nc) {
+        const int i = (int)(idx / nc), j = (int)(idx % nc);
+        if (i < j) {
+            const double s = 0.5 * (A[(int64_t)i * nc + j] + A[(int64_t)j * nc + i]);
+            A[(int64_t)i * nc + j] = s;
+            A[(int64_t)j * nc + i] = s;
+        }
+    }
+    if (idx == 0) {
+        double dm = 0.0;
+        for (int k = 0; k < nc; ++k) dm = fmax(dm, fabs(A[(int64_t)k * nc + k]));
+        thr[0] = 1e-13 * dm;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gj_pivot(int nc, int k, const double *__restrict__ Ain, double *__restrict__ Aout,
+                                                   const double *__restrict__ thr) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nc * nc) return;
+    const int i = (int)(idx / nc), j = (int)(idx % nc);
+    const double piv = Ain[(int64_t)k * nc + k];
+    if (fabs(piv) <= thr[0]) {
+        Aout[idx] = (i == k || j == k) ? 0.0 : Ain[idx];
+        return;
+    }
+    const double p = 1.0 / piv;
+    const double rk = (j == k) ? 1.0 : Ain[(int64_t)k * nc + j];
+    if (i == k) Aout[idx] = rk * p;
+    else Aout[idx] = ((j == k) ? 0.0 : Ain[idx]) - Ain[(int64_t)i * nc + k] * rk * p;
+}
+
+// Partial restriction: workgroup (g, s) sums (P_i^T r_i)[m] over the s-th chunk of 256 members
+// of aggregate g into czp[(g S + s) M + m].
+template <typename T, int D, int VP>
+__global__ __launch_bounds__(256) void k_coarse_restrict(int S, const int32_t *__restrict__ agg_ptr,
+                                                          const int32_t *__restrict__ agg_idx, const double *__restrict__ agg_d,
+                                                          const uint8_t *__restrict__ mask, const T *__restrict__ r,
+                                                          double *__restrict__ czp, const double *__restrict__ scal) {
+    if (scal[SC_DONE] != 0.0) return;
+    constexpr int M = D == 3 ? 6 : 3;
+    __shared__ double red[M][4];
+    const int g = blockIdx.x / S, sidx = blockIdx.x % S;
+    double acc[M];
+    for (int a = 0; a < M; ++a) acc[a] = 0.0;
+    const int m0 = agg_ptr[g] + sidx * blockDim.x + threadIdx.x;
+    if (m0 < agg_ptr[g + 1]) {
+        const int64_t i = agg_idx[m0];
+        double P[3][6];
+        rigid_modes<D>(&agg_d[3 * i], mask[i], P);
+        double ri[3] = {0, 0, 0};
+        for (int k = 0; k < D; ++k) ri[k] = (double)r[VP * i + k];
+        for (int a = 0; a < M; ++a)
+            for (int k = 0; k < D; ++k) acc[a] += P[k][a] * ri[k];
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int a = 0; a < M; ++a) {
+        double v = acc[a];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[a][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < M) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[threadIdx.x][w];
+        czp[(int64_t)blockIdx.x * M + threadIdx.x] = s;
+    }
+}
+
+// cy = Ac^-1 cz: every workgroup first sums the restriction partials into cz (LDS, fixed order),
+// then each wavefront forms 4 rows, lanes striding the row (Ac^-1 is symmetric, row = column).
+__global__ __launch_bounds__(256) void k_coarse_gemv(int nc, int M, int S, const double *__restrict__ Ainv,
+                                                      const double *__restrict__ czp, double *__restrict__ cy,
+                                                      const double *__restrict__ scal) {
+    if (scal[SC_DONE] != 0.0) return;
+    __shared__ double cz[1024];
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        const int g = c / M, m = c % M;
+        double s = 0.0;
+        for (int t = 0; t < S; ++t) s += czp[((int64_t)g * S + t) * M + m];
+        cz[c] = s;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (int rr = 0; rr < 4; ++rr) {
+        const int r = 4 * w + rr;
+        if (r >= nc) break;
+        double s = 0.0;
+        for (int c = lane; c < nc; c += 64) s += Ainv[(int64_t)r * nc + c] * cz[c];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) cy[r] = s;
+    }
+}
+
+}  // namespace
+
+static int ck(lpm_ctx *c, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return lpm_fail(c, LPM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return LPM_OK;
+}
+
+void coarse_free(lpm_ctx *c) {
+    void *ptrs[] = {c->agg_of, c->agg_ptr, c->agg_idx, c->agg_d, c->ra_ptr, c->ra_id, c->cq_ptr,
+                    c->cq_f,   c->cq_gh,   c->cvbuf,   c->Ac,    c->cz,     c->cy,    c->dofmask};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+}
+
+// Aggregates and index lists (host, once per context).
+static int coarse_setup(lpm_ctx *c) {
+    if (c->agg_of) return LPM_OK;
+    const int64_t n = c->n;
+    const int D = c->dim, M = D == 3 ? 6 : 3;
+    std::vector<double> x(3 * n);
+    std::vector<int32_t> cp(n + 1), ci(c->nconn);
+    LPM_HIP(c, hipMemcpyAsync(x.data(), c->xyz0, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(cp.data(), c->cptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(ci.data(), c->cidx, sizeof(int32_t) * c->nconn, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], x[3 * i + k]);
+            hi[k] = std::max(hi[k], x[3 * i + k]);
+        }
+    const double spacing = 2.0 * c->radius;
+    // aggregate edge: the set value, or the smallest of 8, 12, 16, ... lattice spacings that keeps
+    // the coarse space at most 512 DoFs (the inversion runs in one workgroup)
+    std::vector<int32_t> agg(n);
+    int nagg = 0;
+    for (int cells = c->coarse_cells > 0 ? c->coarse_cells : 8;; cells += (cells < 16 ? 4 : 8)) {
+        const double edge = cells * spacing;
+        int64_t dims[3] = {1, 1, 1};
+        for (int k = 0; k < D; ++k) dims[k] = (int64_t)std::floor((hi[k] - lo[k]) / edge) + 1;
+        std::vector<int32_t> id(dims[0] * dims[1] * dims[2], -1);
+        nagg = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t cell[3] = {0, 0, 0};
+            for (int k = 0; k < D; ++k) cell[k] = std::min<int64_t>(dims[k] - 1, (int64_t)((x[3 * i + k] - lo[k]) / edge));
+            const int64_t key = cell[0] + dims[0] * (cell[1] + dims[1] * cell[2]);
+            if (id[key] < 0) id[key] = nagg++;
+            agg[i] = id[key];
+        }
+        if (c->coarse_cells > 0 || (int64_t)nagg * M <= 512) break;
+    }
+    if ((int64_t)nagg * M > 1024) return lpm_fail(c, LPM_E_LIMIT, "coarse space above 1024 DoFs: raise coarse_cells");
+    const int nc = nagg * M;
+    // members, centres, offsets
+    std::vector<int32_t> aptr(nagg + 1, 0), aidx(n);
+    for (int64_t i = 0; i < n; ++i) aptr[agg[i] + 1]++;
+    for (int g = 0; g < nagg; ++g) aptr[g + 1] += aptr[g];
+    {
+        std::vector<int32_t> fill(aptr.begin(), aptr.end() - 1);
+        for (int64_t i = 0; i < n; ++i) aidx[fill[agg[i]]++] = (int32_t)i;
+    }
+    std::vector<double> cen(3 * nagg, 0.0), d(3 * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) cen[3 * agg[i] + k] += x[3 * i + k];
+    for (int g = 0; g < nagg; ++g)
+        for (int k = 0; k < 3; ++k) cen[3 * g + k] /= (double)(aptr[g + 1] - aptr[g]);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) d[3 * i + k] = x[3 * i + k] - cen[3 * agg[i] + k];
+    // distinct aggregates of each row's columns (ascending)
+    std::vector<int32_t> rptr(n + 1, 0), rid;
+    rid.reserve(2 * n);
+    for (int64_t i = 0; i < n; ++i) {
+        std::vector<int32_t> hs;
+        for (int q = cp[i]; q < cp[i + 1]; ++q) hs.push_back(agg[ci[q]]);
+        std::sort(hs.begin(), hs.end());
+        hs.erase(std::unique(hs.begin(), hs.end()), hs.end());
+        rid.insert(rid.end(), hs.begin(), hs.end());
+        rptr[i + 1] = (int32_t)rid.size();
+    }
+    // coarse blocks (g, h) and their contributing row entries f, in row order
+    std::vector<int32_t> qgh;
+    {
+        std::vector<std::pair<int64_t, int32_t>> kf;  // (g*nagg + h, f)
+        kf.reserve(rid.size());
+        for (int64_t i = 0; i < n; ++i)
+            for (int f = rptr[i]; f < rptr[i + 1]; ++f) kf.push_back({(int64_t)agg[i] * nagg + rid[f], f});
+        std::stable_sort(kf.begin(), kf.end(), [](const std::pair<int64_t, int32_t> &a, const std::pair<int64_t, int32_t> &b) {
+            return a.first < b.first;
+        });
+        std::vector<int32_t> qptr(1, 0), qf;
+        for (size_t t = 0; t < kf.size(); ++t) {
+            if (t > 0 && kf[t].first != kf[t - 1].first) qptr.push_back((int32_t)qf.size());
+            if (t == 0 || kf[t].first != kf[t - 1].first) {
+                qgh.push_back((int32_t)(kf[t].first / nagg));
+                qgh.push_back((int32_t)(kf[t].first % nagg));
+            }
+            qf.push_back(kf[t].second);
+        }
+        qptr.push_back((int32_t)qf.size());
+        c->ncq = (int64_t)qgh.size() / 2;
+        LPM_HIP(c, hipMalloc(&c->cq_ptr, sizeof(int32_t) * qptr.size()));
+        LPM_HIP(c, hipMalloc(&c->cq_f, sizeof(int32_t) * std::max<size_t>(qf.size(), 1)));
+        LPM_HIP(c, hipMemcpyAsync(c->cq_ptr, qptr.data(), sizeof(int32_t) * qptr.size(), hipMemcpyHostToDevice, c->stream));
+        LPM_HIP(c, hipMemcpyAsync(c->cq_f, qf.data(), sizeof(int32_t) * qf.size(), hipMemcpyHostToDevice, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    c->nagg = nagg;
+    c->nmode = M;
+    c->ncoarse = nc;
+    c->nra = (int64_t)rid.size();
+    LPM_HIP(c, hipMalloc(&c->cq_gh, sizeof(int32_t) * std::max<size_t>(qgh.size(), 1)));
+    LPM_HIP(c, hipMalloc(&c->agg_of, sizeof(int32_t) * n));
+    LPM_HIP(c, hipMalloc(&c->agg_ptr, sizeof(int32_t) * (nagg + 1)));
+    LPM_HIP(c, hipMalloc(&c->agg_idx, sizeof(int32_t) * n));
+    LPM_HIP(c, hipMalloc(&c->agg_d, sizeof(double) * 3 * n));
+    LPM_HIP(c, hipMalloc(&c->ra_ptr, sizeof(int32_t) * (n + 1)));
+    LPM_HIP(c, hipMalloc(&c->ra_id, sizeof(int32_t) * std::max<size_t>(rid.size(), 1)));
+    LPM_HIP(c, hipMalloc(&c->cvbuf, sizeof(double) * M * M * std::max<size_t>(rid.size(), 1)));
+    int maxm = 1;
+    for (int g = 0; g < nagg; ++g) maxm = std::max(maxm, aptr[g + 1] - aptr[g]);
+    c->coarse_S = (maxm + 255) / 256;
+    LPM_HIP(c, hipMalloc(&c->Ac, sizeof(double) * (2 * (size_t)nc * nc + 1)));
+    LPM_HIP(c, hipMalloc(&c->cz, sizeof(double) * (size_t)nagg * c->coarse_S * M));
+    LPM_HIP(c, hipMalloc(&c->cy, sizeof(double) * nc));
+    LPM_HIP(c, hipMalloc(&c->dofmask, n));
+    LPM_HIP(c, hipMemsetAsync(c->cz, 0, sizeof(double) * (size_t)nagg * c->coarse_S * M, c->stream));
+    LPM_HIP(c, hipMemsetAsync(c->cy, 0, sizeof(double) * nc, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->cq_gh, qgh.data(), sizeof(int32_t) * qgh.size(), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->agg_of, agg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->agg_ptr, aptr.data(), sizeof(int32_t) * (nagg + 1), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->agg_idx, aidx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->agg_d, d.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->ra_ptr, rptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->ra_id, rid.data(), sizeof(int32_t) * rid.size(), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+template <typename T, int D>
+static int coarse_rows_t(lpm_ctx *c) {
+    hipLaunchKernelGGL((k_coarse_rows<T, D>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr,
+                       c->cidx, c->sp, (const T *)c->aval, c->agg_of, c->agg_d, c->dofmask, c->ra_ptr, c->ra_id, c->cvbuf);
+    return ck(c, "k_coarse_rows");
+}
+
+// Rebuild and invert Ac after the matrix changed.
+int coarse_prepare(lpm_ctx *c) {
+    if (c->precond != 2) return LPM_OK;
+    int rc = coarse_setup(c);
+    if (rc) return rc;
+    if (c->coarse_version == c->k_version) return LPM_OK;
+    const int64_t n = c->n;
+    hipLaunchKernelGGL(k_dofmask, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->dim, c->cons, c->damage_visual,
+                       c->dofmask);
+    if ((rc = ck(c, "k_dofmask"))) return rc;
+    if (c->precision == 0) rc = c->dim == 3 ? coarse_rows_t<float, 3>(c) : coarse_rows_t<float, 2>(c);
+    else rc = c->dim == 3 ? coarse_rows_t<double, 3>(c) : coarse_rows_t<double, 2>(c);
+    if (rc) return rc;
+    const int nc = c->ncoarse, M = c->nmode;
+    LPM_HIP(c, hipMemsetAsync(c->Ac, 0, sizeof(double) * (size_t)nc * nc, c->stream));
+    if (c->ncq > 0) {
+        const int gq = (int)((c->ncq + 3) / 4);
+        if (M == 6)
+            hipLaunchKernelGGL((k_coarse_blocks<6>), dim3(gq), dim3(256), 0, c->stream, c->ncq, nc, c->cq_ptr, c->cq_f,
+                               c->cq_gh, c->cvbuf, c->Ac);
+        else
+            hipLaunchKernelGGL((k_coarse_blocks<3>), dim3(gq), dim3(256), 0, c->stream, c->ncq, nc, c->cq_ptr, c->cq_f,
+                               c->cq_gh, c->cvbuf, c->Ac);
+        if ((rc = ck(c, "k_coarse_blocks"))) return rc;
+    }
+    const int64_t nn = (int64_t)nc * nc;
+    double *A0 = c->Ac, *A1 = c->Ac + nn, *thr = c->Ac + 2 * nn;
+    hipLaunchKernelGGL(k_coarse_sym, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, A0, thr);
+    if ((rc = ck(c, "k_coarse_sym"))) return rc;
+    for (int k = 0; k < nc; ++k) {
+        hipLaunchKernelGGL(k_gj_pivot, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, k, (const double *)((k & 1) ? A1 : A0),
+                           (k & 1) ? A0 : A1, (const double *)thr);
+        if ((rc = ck(c, "k_gj_pivot"))) return rc;
+    }
+    c->Ainv = (nc & 1) ? A1 : A0;
+    c->coarse_version = c->k_version;
+    return LPM_OK;
+}
+
+template <typename T, int D, int VP>
+static int coarse_apply_t(lpm_ctx *c) {
+    hipLaunchKernelGGL((k_coarse_restrict<T, D, VP>), dim3(c->nagg * c->coarse_S), dim3(256), 0, c->stream, c->coarse_S,
+                       c->agg_ptr, c->agg_idx, c->agg_d, c->dofmask, (const T *)c->v_r, c->cz, (const double *)c->scal);
+    int rc = ck(c, "k_coarse_restrict");
+    if (rc) return rc;
+    // 4 wavefronts per workgroup, 4 rows per wavefront
+    hipLaunchKernelGGL(k_coarse_gemv, dim3(blocks_for(c->ncoarse, 16)), dim3(256), 0, c->stream, c->ncoarse, c->nmode,
+                       c->coarse_S, (const double *)c->Ainv, (const double *)c->cz, c->cy, (const double *)c->scal);
+    return ck(c, "k_coarse_gemv");
+}
+
+int launch_coarse_apply(lpm_ctx *c) {
+    if (c->precision == 0) return c->dim == 3 ? coarse_apply_t<float, 3, 4>(c) : coarse_apply_t<float, 2, 2>(c);
+    return c->dim == 3 ? coarse_apply_t<double, 3, 4>(c) : coarse_apply_t<double, 2, 2>(c);
+}

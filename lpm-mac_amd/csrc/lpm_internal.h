@@ -79,6 +79,23 @@ struct lpm_ctx {
     // with spmv_tune the fastest of all configurations is measured on the first solve of each
     // matrix kind (store, precision) and kept (the pattern never changes)
     int spmv_unr = 2, spmv_wpb = 4, spmv_remap = 1, spmv_tune = 1, tune_key = -1;
+    // Two-level preconditioner (precond == 2): Jacobi plus an additive coarse correction on
+    // rigid-body modes of cubic aggregates of particles, M^-1 = D^-1 + P Ac^-1 P^T, Ac = P^T K P.
+    int coarse_cells = 0;             // aggregate edge in lattice spacings (0: automatic)
+    int nagg = 0, nmode = 0, ncoarse = 0;
+    int32_t *agg_of = nullptr, *agg_ptr = nullptr, *agg_idx = nullptr;  // particle -> aggregate; members
+    double *agg_d = nullptr;          // x0_i - centre of its aggregate (3 per particle)
+    int32_t *ra_ptr = nullptr, *ra_id = nullptr;  // distinct aggregates of the columns of each row
+    int64_t nra = 0;
+    int32_t *cq_ptr = nullptr, *cq_f = nullptr, *cq_gh = nullptr;  // coarse blocks: contributing row entries
+    int64_t ncq = 0;
+    double *cvbuf = nullptr;          // per (row, aggregate) contributions P_i^T K_ij P_j (nmode^2 each)
+    double *Ac = nullptr;             // 2 ncoarse^2 + 1 FP64: Gauss-Jordan ping-pong buffers + pivot threshold
+    double *Ainv = nullptr;           // the buffer of Ac that holds Ac^-1 (inside Ac)
+    double *cz = nullptr, *cy = nullptr;  // restriction partials (nagg * coarse_S * nmode) / correction
+    int coarse_S = 1;                 // restriction workgroups per aggregate
+    uint8_t *dofmask = nullptr;       // per particle: bit k set when DoF k is eliminated
+    int64_t coarse_version = -1;
     int32_t *nlptr = nullptr, *nlidx = nullptr;
     // particle state (device, FP64 unless noted)
     double *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr, *Pin = nullptr, *Pex = nullptr;
@@ -148,6 +165,10 @@ int launch_move(lpm_ctx *c);
 int spmv_bench(lpm_ctx *c, int reps, double *ms);
 int launch_precond_setup(lpm_ctx *c);
 int sym_prepare(lpm_ctx *c);
+// ---- two-level preconditioner (kernels_coarse.hip)
+int coarse_prepare(lpm_ctx *c);
+int launch_coarse_apply(lpm_ctx *c);  // cz = P^T r, cy = Ac^-1 cz (skipped once the CG is done)
+void coarse_free(lpm_ctx *c);
 int copy_bench(lpm_ctx *c, int64_t bytes, int reps, double *ms);
 // ---- host phases (ctx.hip)
 int geometry_force_state(lpm_ctx *c);

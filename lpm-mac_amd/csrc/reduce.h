@@ -36,3 +36,31 @@ enum { FIN_NORMS = 3, FIN_DIAG = 4 };
 __device__ __forceinline__ int64_t sell_off(const int64_t *sp, int64_t i, int c, int EB, int e) {
     return (int64_t)EB * sp[i >> 6] + ((int64_t)c * EB + e) * 64 + (i & 63);
 }
+
+// Rigid-body modes of one particle at offset d from its aggregate centre: P = [I_D | R(d)] with
+// u = t + w x d (3D, 6 modes) or u = t + theta e_z x d (2D, 3 modes); row k is zero when bit k
+// of `mask` marks DoF k as eliminated.
+template <int D>
+__device__ __forceinline__ void rigid_modes(const double *d, unsigned mask, double (&P)[3][6]) {
+    for (int k = 0; k < 3; ++k)
+        for (int m = 0; m < 6; ++m) P[k][m] = 0.0;
+    if (D == 3) {
+        P[0][0] = 1.0;
+        P[0][4] = d[2];
+        P[0][5] = -d[1];
+        P[1][1] = 1.0;
+        P[1][3] = -d[2];
+        P[1][5] = d[0];
+        P[2][2] = 1.0;
+        P[2][3] = d[1];
+        P[2][4] = -d[0];
+    } else {
+        P[0][0] = 1.0;
+        P[0][2] = -d[1];
+        P[1][1] = 1.0;
+        P[1][2] = d[0];
+    }
+    for (int k = 0; k < D; ++k)
+        if (mask & (1u << k))
+            for (int m = 0; m < 6; ++m) P[k][m] = 0.0;
+}

@@ -275,18 +275,19 @@ def test_static_step_equilibrium_parity(precision):
     np.testing.assert_array_equal(p.g.get("xyz").reshape(-1, 3)[bottom], xyz[bottom])
 
 
-@pytest.mark.parametrize("forcing", [0, 1])
-def test_headline_settings_equilibrium_parity(forcing):
+@pytest.mark.parametrize("forcing,precond", [(0, 0), (1, 0), (1, 2)])
+def test_headline_settings_equilibrium_parity(forcing, precond):
     """The headline benchmark's solver settings (FP32 matrix/SpMV/CG with rtol 1e-3, optional
-    inexact-Newton forcing, Newton tol 1e-10) on a 12^3 SC bar in uniaxial tension, 3 load
-    steps: displacements and reactions within 1e-8 of the FP64 oracle solved tightly."""
-    say("[gpu] headline settings forcing", forcing)
+    inexact-Newton forcing, Jacobi or two-level preconditioner, Newton tol 1e-10) on a 12^3 SC
+    bar in uniaxial tension, 3 load steps: displacements and reactions within 1e-8 of the FP64
+    oracle solved tightly."""
+    say("[gpu] headline settings forcing", forcing, "precond", precond)
     p, xyz, bottom, top = _bar(n=12)
     L = xyz[:, 2].max() - xyz[:, 2].min()
     disp = [(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", 0.005 * L / 20)]
     p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=50)
     p.g.mode(dirichlet_static=1, cg_tol=1e-3, cg_forcing=forcing, tol_NR=1e-10, max_NR_iter=30, precision=0,
-             cg_transpose=0, diag10_literal=0, cg_maxit=20000)
+             cg_transpose=0, diag10_literal=0, cg_maxit=20000, precond=precond, coarse_cells=4 if precond == 2 else 0)
     for s in (p.o, p.g):
         s.update_geometry()
         s.update_force_state()
@@ -300,6 +301,42 @@ def test_headline_settings_equilibrium_parity(forcing):
             rel(p.g.get("reaction"), p.o.getd("reaction")), "cg", p.g.stat("cg_iters"))
         assert rel(u_g, u_o) < 1e-8
         assert rel(p.g.get("reaction"), p.o.getd("reaction")) < 1e-8
+
+
+@pytest.mark.parametrize("lattice", ["sc", "hex"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_two_level_preconditioner(lattice, precision):
+    """precond 2 (Jacobi + rigid-body aggregate coarse correction) reaches the oracle's
+    equilibrium to 1e-8 and needs fewer CG iterations than Jacobi on the same step."""
+    say("[gpu] two-level", lattice, "precision", precision)
+    iters = {}
+    for pc in (0, 2):
+        if lattice == "sc":
+            p, xyz, lo, hi = _bar(n=12)
+            axes = "xyz"
+        else:
+            xyz, box = hex_plate(40, 40)
+            p = Pair(O.HEXAGON2D, 0.5, O.ELASTIC, xyz, box)
+            p.set_elastic(0.0, True, 69e3, 0.3, 1.0)
+            lo, hi = faces(xyz, 1)
+            axes = "xy"
+        disp = [(lo, a, 0.0) for a in axes] + [(hi, axes[-1], 0.02)]
+        ls_o, ls_g = p.load(disp=disp)
+        p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=50)
+        p.g.mode(dirichlet_static=1, cg_tol=1e-6 if precision == 0 else 1e-12, tol_NR=1e-11, max_NR_iter=50,
+                 precision=precision, cg_transpose=0, diag10_literal=0, cg_maxit=20000, precond=pc, coarse_cells=4)
+        for s in (p.o, p.g):
+            s.update_geometry()
+            s.update_force_state()
+        assert p.o.solve_step_static(ls_o, -1)
+        assert p.g.solve_step_static(ls_g, -1)
+        u_o = p.o.getd("xyz") - xyz.ravel()
+        u_g = p.g.get("xyz") - xyz.ravel()
+        iters[pc] = p.g.stat("cg_iters")
+        say("   precond", pc, "ncoarse", p.g.get_mode("ncoarse"), "disp rel err", rel(u_g, u_o), "cg", iters[pc])
+        assert rel(u_g, u_o) < 1e-8
+        assert rel(p.g.get("reaction"), p.o.getd("reaction")) < 1e-8
+    assert iters[2] < iters[0]
 
 
 def test_literal_modes_reach_the_same_equilibrium():
