@@ -146,16 +146,16 @@ __global__ __launch_bounds__(256) void k_gj_pivot(int nc, int k, const double *_
     else Aout[idx] = ((j == k) ? 0.0 : Ain[idx]) - Ain[(int64_t)i * nc + k] * rk * p;
 }
 
-// Partial restriction: workgroup (g, s) sums (P_i^T r_i)[m] over the s-th chunk of 256 members
+// Partial restriction: workgroup (g, s) sums (P_i^T r_i)[m] over the s-th chunk of 1024 members
 // of aggregate g into czp[(g S + s) M + m].
 template <typename T, int D, int VP>
-__global__ __launch_bounds__(256) void k_coarse_restrict(int S, const int32_t *__restrict__ agg_ptr,
+__global__ __launch_bounds__(1024) void k_coarse_restrict(int S, const int32_t *__restrict__ agg_ptr,
                                                           const int32_t *__restrict__ agg_idx, const double *__restrict__ agg_d,
                                                           const uint8_t *__restrict__ mask, const T *__restrict__ r,
                                                           double *__restrict__ czp, const double *__restrict__ scal) {
     if (scal[SC_DONE] != 0.0) return;
     constexpr int M = D == 3 ? 6 : 3;
-    __shared__ double red[M][4];
+    __shared__ double red[M][16];
     const int g = blockIdx.x / S, sidx = blockIdx.x % S;
     double acc[M];
     for (int a = 0; a < M; ++a) acc[a] = 0.0;
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void k_coarse_restrict(int S, const int32_t *_
 }
 
 // cy = Ac^-1 cz: every workgroup first sums the restriction partials into cz (LDS, fixed order),
-// then each wavefront forms 4 rows, lanes striding the row (Ac^-1 is symmetric, row = column).
+// then each wavefront forms one row, lanes striding the row (Ac^-1 is symmetric, row = column).
 __global__ __launch_bounds__(256) void k_coarse_gemv(int nc, int M, int S, const double *__restrict__ Ainv,
                                                       const double *__restrict__ czp, double *__restrict__ cy,
                                                       const double *__restrict__ scal) {
@@ -198,15 +198,12 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(int nc, int M, int S, const
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    for (int rr = 0; rr < 4; ++rr) {
-        const int r = 4 * w + rr;
-        if (r >= nc) break;
-        double s = 0.0;
-        for (int c = lane; c < nc; c += 64) s += Ainv[(int64_t)r * nc + c] * cz[c];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) cy[r] = s;
-    }
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (r >= nc) return;
+    double s = 0.0;
+    for (int c = lane; c < nc; c += 64) s += Ainv[(int64_t)r * nc + c] * cz[c];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) cy[r] = s;
 }
 
 }  // namespace
@@ -218,6 +215,7 @@ static int ck(lpm_ctx *c, const char *what) {
 }
 
 void coarse_free(lpm_ctx *c) {
+    if (c->gj_graph) hipGraphExecDestroy(c->gj_graph);
     void *ptrs[] = {c->agg_of, c->agg_ptr, c->agg_idx, c->agg_d, c->ra_ptr, c->ra_id, c->cq_ptr,
                     c->cq_f,   c->cq_gh,   c->cvbuf,   c->Ac,    c->cz,     c->cy,    c->dofmask};
     for (void *p : ptrs)
@@ -330,7 +328,7 @@ static int coarse_setup(lpm_ctx *c) {
     LPM_HIP(c, hipMalloc(&c->cvbuf, sizeof(double) * M * M * std::max<size_t>(rid.size(), 1)));
     int maxm = 1;
     for (int g = 0; g < nagg; ++g) maxm = std::max(maxm, aptr[g + 1] - aptr[g]);
-    c->coarse_S = (maxm + 255) / 256;
+    c->coarse_S = (maxm + 1023) / 1024;
     LPM_HIP(c, hipMalloc(&c->Ac, sizeof(double) * (2 * (size_t)nc * nc + 1)));
     LPM_HIP(c, hipMalloc(&c->cz, sizeof(double) * (size_t)nagg * c->coarse_S * M));
     LPM_HIP(c, hipMalloc(&c->cy, sizeof(double) * nc));
@@ -384,11 +382,28 @@ int coarse_prepare(lpm_ctx *c) {
     double *A0 = c->Ac, *A1 = c->Ac + nn, *thr = c->Ac + 2 * nn;
     hipLaunchKernelGGL(k_coarse_sym, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, A0, thr);
     if ((rc = ck(c, "k_coarse_sym"))) return rc;
-    for (int k = 0; k < nc; ++k) {
-        hipLaunchKernelGGL(k_gj_pivot, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, k, (const double *)((k & 1) ? A1 : A0),
-                           (k & 1) ? A0 : A1, (const double *)thr);
-        if ((rc = ck(c, "k_gj_pivot"))) return rc;
+    // the nc pivot launches are tiny (a few microseconds each): issued one by one, the host's
+    // launch rate would leave the GPU idle between them, so they are captured once into a graph
+    if (!c->gj_graph) {
+        hipGraph_t graph = nullptr;
+        LPM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < nc; ++k)
+            hipLaunchKernelGGL(k_gj_pivot, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, k,
+                               (const double *)((k & 1) ? A1 : A0), (k & 1) ? A0 : A1, (const double *)thr);
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            return lpm_fail(c, LPM_E_HIP, std::string("inversion graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        const hipError_t ie = hipGraphInstantiate(&c->gj_graph, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            c->gj_graph = nullptr;
+            return lpm_fail(c, LPM_E_HIP, std::string("inversion graph: ") + hipGetErrorString(ie));
+        }
     }
+    LPM_HIP(c, hipGraphLaunch(c->gj_graph, c->stream));
     c->Ainv = (nc & 1) ? A1 : A0;
     c->coarse_version = c->k_version;
     return LPM_OK;
@@ -396,12 +411,12 @@ int coarse_prepare(lpm_ctx *c) {
 
 template <typename T, int D, int VP>
 static int coarse_apply_t(lpm_ctx *c) {
-    hipLaunchKernelGGL((k_coarse_restrict<T, D, VP>), dim3(c->nagg * c->coarse_S), dim3(256), 0, c->stream, c->coarse_S,
+    hipLaunchKernelGGL((k_coarse_restrict<T, D, VP>), dim3(c->nagg * c->coarse_S), dim3(1024), 0, c->stream, c->coarse_S,
                        c->agg_ptr, c->agg_idx, c->agg_d, c->dofmask, (const T *)c->v_r, c->cz, (const double *)c->scal);
     int rc = ck(c, "k_coarse_restrict");
     if (rc) return rc;
-    // 4 wavefronts per workgroup, 4 rows per wavefront
-    hipLaunchKernelGGL(k_coarse_gemv, dim3(blocks_for(c->ncoarse, 16)), dim3(256), 0, c->stream, c->ncoarse, c->nmode,
+    // 4 wavefronts per workgroup, one row per wavefront
+    hipLaunchKernelGGL(k_coarse_gemv, dim3(blocks_for(c->ncoarse, 4)), dim3(256), 0, c->stream, c->ncoarse, c->nmode,
                        c->coarse_S, (const double *)c->Ainv, (const double *)c->cz, c->cy, (const double *)c->scal);
     return ck(c, "k_coarse_gemv");
 }

@@ -94,6 +94,7 @@ struct lpm_ctx {
     double *Ainv = nullptr;           // the buffer of Ac that holds Ac^-1 (inside Ac)
     double *cz = nullptr, *cy = nullptr;  // restriction partials (nagg * coarse_S * nmode) / correction
     int coarse_S = 1;                 // restriction workgroups per aggregate
+    hipGraphExec_t gj_graph = nullptr;  // the ncoarse pivot launches of the inversion, captured once
     uint8_t *dofmask = nullptr;       // per particle: bit k set when DoF k is eliminated
     int64_t coarse_version = -1;
     int32_t *nlptr = nullptr, *nlidx = nullptr;
