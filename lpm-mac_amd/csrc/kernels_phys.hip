@@ -227,44 +227,53 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, int dim, const int3
 // Stiffness::calcStiffness with localStiffnessANA (stiffness.h:67-101, 228-263) and
 // fdu2dxyz / fdu2dxyz1 / fdu2dxyz2 (derivative.h:7-239). One wavefront per particle row i,
 // one lane per block (i, j in conns(i)).
-// The wavefront first stages, with all lanes loading in parallel, what the row needs in LDS:
-// the bonds of i (partner, cosines, Kn, Tv, bdamage) and, for each of them, the same-layer
-// bonds of the partner k (the 2-hop paths). Every lane then walks these lists in the
-// reference's loop order and keeps only the terms that land on its own j, so each FP64 block
-// equals the reference's block bit for bit before it is rounded to the storage precision.
-// LDS per wavefront: LPM_MAXB bond records + `hops` 2-hop records (asm_lds_bytes).
+// The wavefront first stages in LDS, with all lanes loading in parallel, what the row needs:
+// the bonds of i (partner, cosines, Kn, Tv, bdamage), for each of them the same-layer bonds of
+// the partner k (the 2-hop paths i -> k -> j2, in bond order), and, for each block of the row,
+// the list of 2-hop paths that end on its column (bucketed by a counting sort and then sorted,
+// so each list is in path order). Every lane then adds, in the reference's loop order, only
+// the terms of its own block, so each FP64 block equals the reference's block bit for bit
+// before it is rounded to the storage precision.
 struct AsmBond {
     double c[3], tv, kn, bd;
     int p2, pad;
 };
 struct AsmHop {
     double c[3], tv, bd;
-    int p2, pad;
+    int p2, owner;  // owner: index of the bond of i the path starts with
 };
 
-__host__ __device__ inline size_t asm_lds_bytes(int hops) {
-    return sizeof(AsmBond) * LPM_MAXB + sizeof(int) * 2 * (LPM_MAXB + 1) + sizeof(AsmHop) * (size_t)hops;
+__host__ __device__ inline size_t asm_lds_bytes(int hops, int maxconn) {
+    return sizeof(AsmBond) * LPM_MAXB + sizeof(int) * 2 * (LPM_MAXB + 1) + sizeof(AsmHop) * (size_t)hops +
+           sizeof(int) * 2 * (size_t)hops + sizeof(int) * 4 * (size_t)(maxconn + 1);
 }
 
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_assemble(
-    int64_t n, int hops, int diag10_literal, int transpose, const int32_t *__restrict__ bptr,
+    int64_t n, int hops, int maxconn, int diag10_literal, int transpose, const int32_t *__restrict__ bptr,
     const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2, const int32_t *__restrict__ cptr,
     const int32_t *__restrict__ cidx, const int32_t *__restrict__ tslot, const int64_t *__restrict__ sp,
     const double *__restrict__ cs, const double *__restrict__ cs_sum, const double *__restrict__ Kn,
     const double *__restrict__ Tv, const double *__restrict__ bdamage, T *__restrict__ aval, double *__restrict__ diag64) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    char *base = (char *)lds + (size_t)wid * asm_lds_bytes(hops);
+    char *base = (char *)lds + (size_t)wid * asm_lds_bytes(hops, maxconn);
     AsmBond *B = (AsmBond *)base;
     int *soff = (int *)(B + LPM_MAXB);  // 2-hop segment offsets of each bond of i (+ end)
     int *slo = soff + LPM_MAXB + 1;     // first same-layer bond of the partner
     AsmHop *H = (AsmHop *)(slo + LPM_MAXB + 1);
+    int *hslot = (int *)(H + hops);     // block slot each path ends on (-1: none)
+    int *hlist = hslot + hops;          // paths grouped by slot, in path order
+    int *rcol = hlist + hops;           // columns of row i
+    int *lcnt = rcol + (maxconn + 1);   // paths per slot
+    int *lbeg = lcnt + (maxconn + 1);   // start of each slot's list
+    int *lfil = lbeg + (maxconn + 1);   // fill cursor
     const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid;
     const bool row = i < n;
     const int ib0 = row ? bptr[i] : 0, ibm = row ? bsplit[i] : 0, ib1 = row ? bptr[i + 1] : 0;
     const int nbi = ib1 - ib0, tm = ibm - ib0;
-    // stage the bonds of i and the length of each bond's 2-hop list
+    const int c0 = row ? cptr[i] : 0, nci = row ? cptr[i + 1] - c0 : 0;
+    // bonds of i and the length of each bond's 2-hop list; the row's columns
     if (lane < nbi) {
         const int b = ib0 + lane;
         const int k = p2[b];
@@ -282,6 +291,11 @@ __global__ __launch_bounds__(256) void k_assemble(
         slo[lane] = klo;
         soff[lane + 1] = khi - klo;
     }
+    for (int q = lane; q < nci; q += 64) {
+        rcol[q] = cidx[c0 + q];
+        lcnt[q] = 0;
+        lfil[q] = 0;
+    }
     __syncthreads();
     if (lane == 0) {
         soff[0] = 0;
@@ -289,6 +303,7 @@ __global__ __launch_bounds__(256) void k_assemble(
     }
     __syncthreads();
     const int E = soff[nbi];
+    // the paths, and the slot of the column each one ends on (binary search in the sorted row)
     for (int e = lane; e < E; e += 64) {
         int t = 0;
         while (soff[t + 1] <= e) ++t;
@@ -300,14 +315,49 @@ __global__ __launch_bounds__(256) void k_assemble(
         h.tv = Tv[b3];
         h.bd = bdamage[b3];
         h.p2 = p2[b3];
-        h.pad = 0;
+        h.owner = t;
         H[e] = h;
+        int lo = 0, hi = nci - 1, sl = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const int v = rcol[mid];
+            if (v == h.p2) { sl = mid; break; }
+            if (v < h.p2) lo = mid + 1;
+            else hi = mid - 1;
+        }
+        hslot[e] = sl;
+        if (sl >= 0) atomicAdd(&lcnt[sl], 1);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int acc = 0;
+        for (int q = 0; q < nci; ++q) {
+            lbeg[q] = acc;
+            acc += lcnt[q];
+        }
+    }
+    __syncthreads();
+    for (int e = lane; e < E; e += 64) {
+        const int sl = hslot[e];
+        if (sl >= 0) hlist[lbeg[sl] + atomicAdd(&lfil[sl], 1)] = e;
+    }
+    __syncthreads();
+    for (int q = lane; q < nci; q += 64) {  // restore path order within each list
+        int *L = hlist + lbeg[q];
+        for (int a = 1; a < lcnt[q]; ++a) {
+            const int v = L[a];
+            int b = a - 1;
+            while (b >= 0 && L[b] > v) {
+                L[b + 1] = L[b];
+                --b;
+            }
+            L[b + 1] = v;
+        }
     }
     __syncthreads();
     if (!row) return;
-    const int c0 = cptr[i], nci = cptr[i + 1] - c0;
     for (int slot = lane; slot < nci; slot += 64) {
-        const int j = cidx[c0 + slot];
+        const int j = rcol[slot];
         double K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         if (j == i) {
             for (int l = 0; l < 2; ++l) {
@@ -331,12 +381,16 @@ __global__ __launch_bounds__(256) void k_assemble(
             K[2][1] = K[1][2];
             for (int r = 0; r < D; ++r) diag64[(int64_t)D * i + r] = K[r][r];
         } else {
+            const int *L = hlist + lbeg[slot];
+            const int nL = lcnt[slot];
+            int q = 0;  // cursor in this block's path list (layer-0 paths come first)
             double Kl[2][3][3];
             bool flag[2];
             for (int l = 0; l < 2; ++l) {
                 for (int r = 0; r < 3; ++r)
-                    for (int q = 0; q < 3; ++q) Kl[l][r][q] = 0.0;
+                    for (int c = 0; c < 3; ++c) Kl[l][r][c] = 0.0;
                 const int lo = l == 0 ? 0 : tm, hi = l == 0 ? tm : nbi;
+                const int eend = soff[hi];  // paths of this layer end here
                 int bj = -1;  // first bond i->j of this layer
                 for (int b = lo; b < hi; ++b)
                     if (bj < 0 && B[b].p2 == j) bj = b;
@@ -352,55 +406,58 @@ __global__ __launch_bounds__(256) void k_assemble(
                         if (k == j) {
                             const double w = (1 - B[b2].bd), kn = B[b2].kn, kt = B[b2].kn + B[b2].tv, tv = B[b2].tv;
                             for (int r = 0; r < D; ++r)
-                                for (int q = 0; q < D; ++q) {
-                                    const double pr = (r == q) ? kt * (n2[r] * n2[r])
-                                                               : kt * n2[r > q ? r : q] * n2[r > q ? q : r];
-                                    Kl[l][r][q] += w * ((-kn * n2[q] + tv * S[q]) * n2[r] - pr);
+                                for (int c = 0; c < D; ++c) {
+                                    const double pr = (r == c) ? kt * (n2[r] * n2[r])
+                                                               : kt * n2[r > c ? r : c] * n2[r > c ? c : r];
+                                    Kl[l][r][c] += w * ((-kn * n2[c] + tv * S[c]) * n2[r] - pr);
                                 }
+                            // a path through j itself cannot end on j (no self bonds)
                         } else {
                             const double a = 0.5 * (wT1 + wT2);
                             for (int r = 0; r < D; ++r)
-                                for (int q = 0; q < D; ++q) Kl[l][r][q] -= a * n2[r] * n1[q];
-                            for (int e = soff[b2]; e < soff[b2 + 1]; ++e) {
-                                if (H[e].p2 == j) {
-                                    const double n3[3] = {H[e].c[0], H[e].c[1], H[e].c[2]};
-                                    const double a3 = 0.5 * ((1 - H[e].bd) * H[e].tv + wT2);
-                                    for (int r = 0; r < D; ++r)
-                                        for (int q = 0; q < D; ++q) Kl[l][r][q] -= a3 * n2[r] * n3[q];
-                                }
+                                for (int c = 0; c < D; ++c) Kl[l][r][c] -= a * n2[r] * n1[c];
+                            const int e1 = soff[b2 + 1];
+                            for (; q < nL && L[q] < e1; ++q) {
+                                const int e = L[q];
+                                const double n3[3] = {H[e].c[0], H[e].c[1], H[e].c[2]};
+                                const double a3 = 0.5 * ((1 - H[e].bd) * H[e].tv + wT2);
+                                for (int r = 0; r < D; ++r)
+                                    for (int c = 0; c < D; ++c) Kl[l][r][c] -= a3 * n2[r] * n3[c];
                             }
                         }
+                        // paths of b2 that the reference skips (k == j) are not in j's list
+                        const int e1 = soff[b2 + 1];
+                        while (q < nL && L[q] < e1) ++q;
                     }
                 } else {  // derivative.h:122-139
-                    for (int b1 = lo; b1 < hi; ++b1) {
-                        for (int e = soff[b1]; e < soff[b1 + 1]; ++e) {
-                            if (H[e].p2 == j) {
-                                touched = true;
-                                const double n1[3] = {B[b1].c[0], B[b1].c[1], B[b1].c[2]};
-                                const double n2[3] = {H[e].c[0], H[e].c[1], H[e].c[2]};
-                                const double a = 0.5 * ((1 - H[e].bd) * H[e].tv + (1 - B[b1].bd) * B[b1].tv);
-                                for (int r = 0; r < D; ++r)
-                                    for (int q = 0; q < D; ++q) Kl[l][r][q] -= a * n1[r] * n2[q];
-                            }
-                        }
+                    for (; q < nL && L[q] < eend; ++q) {
+                        const int e = L[q];
+                        const int b1 = H[e].owner;
+                        touched = true;
+                        const double n1[3] = {B[b1].c[0], B[b1].c[1], B[b1].c[2]};
+                        const double n2[3] = {H[e].c[0], H[e].c[1], H[e].c[2]};
+                        const double a = 0.5 * ((1 - H[e].bd) * H[e].tv + (1 - B[b1].bd) * B[b1].tv);
+                        for (int r = 0; r < D; ++r)
+                            for (int c = 0; c < D; ++c) Kl[l][r][c] -= a * n1[r] * n2[c];
                     }
                 }
+                while (q < nL && L[q] < eend) ++q;
                 flag[l] = touched;
             }
             for (int r = 0; r < D; ++r)
-                for (int q = 0; q < D; ++q) {
-                    if (flag[0] && flag[1]) K[r][q] = Kl[0][r][q] + Kl[1][r][q];
-                    else if (flag[0]) K[r][q] = Kl[0][r][q];
-                    else if (flag[1]) K[r][q] = Kl[1][r][q];
+                for (int c = 0; c < D; ++c) {
+                    if (flag[0] && flag[1]) K[r][c] = Kl[0][r][c] + Kl[1][r][c];
+                    else if (flag[0]) K[r][c] = Kl[0][r][c];
+                    else if (flag[1]) K[r][c] = Kl[1][r][c];
                 }
         }
         if (!transpose) {
             for (int r = 0; r < D; ++r)
-                for (int q = 0; q < D; ++q) aval[sell_off(sp, i, slot, D * D, r * D + q)] = (T)K[r][q];
+                for (int c = 0; c < D; ++c) aval[sell_off(sp, i, slot, D * D, r * D + c)] = (T)K[r][c];
         } else {
             const int ts = tslot[c0 + slot];
             for (int r = 0; r < D; ++r)
-                for (int q = 0; q < D; ++q) aval[sell_off(sp, j, ts, D * D, r * D + q)] = (T)K[q][r];
+                for (int c = 0; c < D; ++c) aval[sell_off(sp, j, ts, D * D, r * D + c)] = (T)K[c][r];
         }
     }
 }
@@ -818,12 +875,13 @@ int launch_residual(lpm_ctx *c, double *nr2, double *nR2) {
 int launch_assemble(lpm_ctx *c) {
     if (c->cg_transpose && !c->tslot) return lpm_fail(c, LPM_E_STATE, "transpose map missing");
     // wavefronts per workgroup: as many as fit 64 KB of LDS, at most 4
-    const size_t per = asm_lds_bytes(c->asm_hops);
+    const size_t per = asm_lds_bytes(c->asm_hops, c->maxconn);
     const int W = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per));
     const int g = blocks_for(c->n, W);
     const size_t lds = per * W;
 #define ASM_LAUNCH(T, D)                                                                                               \
-    hipLaunchKernelGGL((k_assemble<T, D>), dim3(g), dim3(64 * W), lds, c->stream, c->n, c->asm_hops, c->diag10_literal,   \
+    hipLaunchKernelGGL((k_assemble<T, D>), dim3(g), dim3(64 * W), lds, c->stream, c->n, c->asm_hops, c->maxconn,          \
+                       c->diag10_literal,                                                                                 \
                        c->cg_transpose, c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp, c->cs, c->cs_sum,  \
                        c->Kn, c->Tv, c->bdamage, (T *)c->aval, c->diag64)
     if (c->precision == 0) {
