@@ -73,7 +73,7 @@ def spmv_bytes(ctx):
     return 40 * nblk + 4 * (n + 1) + vec, "k_spmv (CG SpMV, full store)"
 
 
-def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing):
+def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0):
     import lpmgpu as G
 
     xyz, box, r, L = sc_workload(n)
@@ -81,7 +81,8 @@ def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing):
     ctx = G.Context(G.SC3D, r, G.ELASTIC, xyz)
     ctx.set_elastic(0.0, False, 200e3, 0.3, 1.0, elmap_literal=0)
     ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, cg_tol=cg_tol, tol_NR=tol_nr,
-             max_NR_iter=30, cg_maxit=20000, precond=precond, spmv_sym=spmv_sym, cg_forcing=cg_forcing)
+             max_NR_iter=30, cg_maxit=20000, precond=precond, spmv_sym=spmv_sym, cg_forcing=cg_forcing,
+             coarse_cells=coarse_cells)
     z = xyz[:, 2]
     bottom = np.nonzero(z < z.min() + 1e-6)[0]
     top = np.nonzero(z > z.max() - 1e-6)[0]
@@ -163,6 +164,7 @@ def main():
     ap.add_argument("--precond", type=int, default=2,
                     help="0 Jacobi (as the reference's Eigen CG), 1 block Jacobi, 2 Jacobi + rigid-body coarse correction")
     ap.add_argument("--spmv-sym", type=int, default=0, help="1: half store when K is symmetric, 0: full store")
+    ap.add_argument("--coarse-cells", type=int, default=0, help="aggregate edge of the two-level preconditioner (0: auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--timing-every", type=int, default=16,
@@ -179,7 +181,8 @@ def main():
     import lpmgpu as G
 
     G.lib()
-    ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond, args.spmv_sym, args.cg_forcing)
+    ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond, args.spmv_sym, args.cg_forcing,
+                                      args.coarse_cells)
     if args.verbose:
         print(f"setup {t_setup:.2f} s, n={ctx.n} bonds={ctx.size('nbonds')} blocks={ctx.size('nconn')}", flush=True)
     # warmup steps, then back to the initial state so that the timed steps are steps 1..K

@@ -121,9 +121,9 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        2: Jacobi + additive coarse correction on the rigid-body modes of cubic
  *                        aggregates of particles (two-level; iteration count no longer grows
  *                        with the lattice width). Changes the CG's rate, not the equilibrium.
- *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 (0: smallest of 8, 12,
- *                        16, 24, ... giving at most 512 coarse DoFs); get_mode("ncoarse") reads
- *                        the coarse dimension once built
+ *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 (0: smallest of 8, 9,
+ *                        10, ... giving at most 768 coarse DoFs); get_mode("ncoarse") reads the
+ *                        coarse dimension once built
  *  "spmv_sym"         0: SpMV on the full matrix | 1: after each assembly check that the
  *                        stored matrix is symmetric to rounding and, if so, run the SpMV from
  *                        its upper half (each block read from HBM once, the transposed half

@@ -750,8 +750,8 @@ static int spmv_launch(lpm_ctx *c, bool fused, void *in, void *out) {
     return check_launch(c, c->sym_ok ? "k_spmv_sym" : "k_spmv");
 }
 
-// Launch-configuration tuner: times the plain SpMV (v_p -> v_w; neither is live outside a
-// solve) in every configuration and keeps the fastest.
+// Launch-configuration tuner: times the fused SpMV the CG runs (v_p -> v_w, delta partials;
+// none of them is live outside a solve) in every configuration and keeps the fastest.
 template <typename T, int D, int VP>
 static int spmv_tune_t(lpm_ctx *c) {
     // 4 or 8 wavefronts per workgroup: the fused SpMV leaves one partial per workgroup, which
@@ -762,18 +762,20 @@ static int spmv_tune_t(lpm_ctx *c) {
     LPM_HIP(c, timing_event(&b));
     double best = 1e300;
     int bu = 2, bw = 4, br = 1;
-    const int reps = 4;
+    const int reps = 6;
+    // the fused kernel returns at once while the stop flag of the last solve is still set
+    LPM_HIP(c, hipMemsetAsync(c->scal + SC_DONE, 0, sizeof(double), c->stream));
     for (int un : unrs)
         for (int wp : wpbs)
             for (int rm : remaps) {
                 c->spmv_unr = un;
                 c->spmv_wpb = wp;
                 c->spmv_remap = rm;
-                int rc = spmv_launch<T, D, VP>(c, false, c->v_p, c->v_w);
+                int rc = spmv_launch<T, D, VP>(c, true, c->v_p, c->v_w);
                 if (rc) return rc;
                 LPM_HIP(c, hipEventRecord(a, c->stream));
                 for (int k = 0; k < reps; ++k)
-                    if ((rc = spmv_launch<T, D, VP>(c, false, c->v_p, c->v_w))) return rc;
+                    if ((rc = spmv_launch<T, D, VP>(c, true, c->v_p, c->v_w))) return rc;
                 LPM_HIP(c, hipEventRecord(b, c->stream));
                 LPM_HIP(c, hipEventSynchronize(b));
                 float ms = 0;

@@ -240,11 +240,11 @@ static int coarse_setup(lpm_ctx *c) {
             hi[k] = std::max(hi[k], x[3 * i + k]);
         }
     const double spacing = 2.0 * c->radius;
-    // aggregate edge: the set value, or the smallest of 8, 12, 16, ... lattice spacings that keeps
-    // the coarse space at most 512 DoFs (the inversion runs in one workgroup)
+    // aggregate edge: the set value, or the smallest of 8, 9, 10, ... lattice spacings that keeps
+    // the coarse space at most 768 DoFs (the inversion's cost grows with its cube)
     std::vector<int32_t> agg(n);
     int nagg = 0;
-    for (int cells = c->coarse_cells > 0 ? c->coarse_cells : 8;; cells += (cells < 16 ? 4 : 8)) {
+    for (int cells = c->coarse_cells > 0 ? c->coarse_cells : 8;; cells += (cells < 32 ? 1 : 8)) {
         const double edge = cells * spacing;
         int64_t dims[3] = {1, 1, 1};
         for (int k = 0; k < D; ++k) dims[k] = (int64_t)std::floor((hi[k] - lo[k]) / edge) + 1;
@@ -257,7 +257,7 @@ static int coarse_setup(lpm_ctx *c) {
             if (id[key] < 0) id[key] = nagg++;
             agg[i] = id[key];
         }
-        if (c->coarse_cells > 0 || (int64_t)nagg * M <= 512) break;
+        if (c->coarse_cells > 0 || (int64_t)nagg * M <= 768) break;
     }
     if ((int64_t)nagg * M > 1024) return lpm_fail(c, LPM_E_LIMIT, "coarse space above 1024 DoFs: raise coarse_cells");
     const int nc = nagg * M;
