@@ -32,11 +32,11 @@ HBM_PEAK = 8.0e12
 # --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
 #   bench.py --steps 1 --warmup 0 --no-cpu-baseline --pmc-calibrate
 # Each reading is scaled by what the same counter reports for k_copy_calib, which moves a known
-# 256 MiB in and 256 MiB out: FETCH_SIZE reads 131078.9 KiB for it (x2.0000), WRITE_SIZE
-# 262144.0 KiB (x1.0000). Mean over the 596 fused SpMV launches that did work: FETCH_SIZE
-# 311296.8 KiB -> 637.5 MB; WRITE_SIZE 0 KiB (the 3 MB output stays in the write-back L2
-# until later kernels evict it).
-MEASURED_TRAFFIC = {(64, "k_spmv (CG SpMV, full store)"): 311296.8 * 1024 * 2.0 + 0.0}
+# 256 MiB in and 256 MiB out: FETCH_SIZE reads 131082.8 KiB for it (x1.9999), WRITE_SIZE
+# 262144.0 KiB (x1.0000). Mean over the fused CG SpMV launches that did work: FETCH_SIZE
+# 313761.5 KiB -> 642.5 MB; WRITE_SIZE ~0 (the 3 MB output stays in the write-back L2 until
+# later kernels evict it).
+MEASURED_TRAFFIC = {(64, "k_spmv (CG SpMV, full store)"): 313761.5 * 1024 * 262144.0 / 131082.8}
 
 
 def sc_workload(n):
