@@ -204,6 +204,7 @@ def main():
     t = time.perf_counter() - t0
     ctx.set_timing(0)
     spmv_iso_ms = ctx.spmv_bench(50)
+    tuned_ms = ctx.stat("spmv_tuned_ms")
     # practical ceiling: a plain streaming copy of the SpMV's size (read + write)
     copy_bytes = 256 << 20
     copy_ms = ctx.copy_bench(copy_bytes, 3 if args.pmc_calibrate else 20)
@@ -270,7 +271,7 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved * 1e9 / HBM_PEAK, "traffic": MEASURED_TRAFFIC.get((args.n, kname)),
             "kernel": kname, "bytes_per_launch": B, "mean_kernel_ms": spmv_ms,
-            "launches_timed": spmv_n, "isolated_spmv_ms": spmv_iso_ms,
+            "launches_timed": spmv_n, "isolated_spmv_ms": spmv_iso_ms, "isolated_fused_spmv_ms": tuned_ms,
             "copy_kernel_GBps": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9,
         },
     }

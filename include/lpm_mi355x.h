@@ -263,7 +263,8 @@ int lpm_adjoint_reaction(lpm_ctx *ctx, int32_t group, int32_t axis, double *dF_d
  * "spmv_launches" (SpMVs that did work); "spmv_ms" / "spmv_launches_timed" (sum of SpMV kernel
  * time from HIP events and the number of launches it covers, when timing is on);
  * "last_cg_iters","last_cg_error"; "sym_solves" (solves that ran on the half store),
- * "sym_rejects" (assemblies whose matrix failed the symmetry check), "spmv_sym_active". */
+ * "sym_rejects" (assemblies whose matrix failed the symmetry check), "spmv_sym_active",
+ * "spmv_tuned_ms" (mean time of the fused CG SpMV in the configuration the tuner kept). */
 double lpm_stat(const lpm_ctx *ctx, const char *name);
 int lpm_reset_stats(lpm_ctx *ctx);
 /* every > 0: time the CG SpMV of every `every`-th iteration with HIP events on the context

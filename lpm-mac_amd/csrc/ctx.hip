@@ -947,6 +947,7 @@ double lpm_stat(const lpm_ctx *c, const char *name) {
     if (w == "sym_solves") return (double)c->st_sym_solves;
     if (w == "sym_rejects") return (double)c->st_sym_rejects;
     if (w == "spmv_sym_active") return c->sym_ok ? 1.0 : 0.0;
+    if (w == "spmv_tuned_ms") return c->tune_best_ms;
     return NAN;
 }
 

@@ -792,6 +792,7 @@ static int spmv_tune_t(lpm_ctx *c) {
     c->spmv_unr = bu;
     c->spmv_wpb = bw;
     c->spmv_remap = br;
+    c->tune_best_ms = best / reps;
     return LPM_OK;
 }
 

@@ -11,7 +11,7 @@
 // and for every coarse block (g, h) the list of row entries that contribute to it.
 // Per assembly: contributions P_i^T K_ij P_j summed per (row i, aggregate h) in slot order,
 // then per coarse block in row order (no atomics: the result does not depend on scheduling),
-// then an in-place Gauss-Jordan inversion of Ac in one workgroup.
+// then a blocked Gauss-Jordan inversion of Ac (32 pivots per step, captured in a graph).
 // Per CG iteration: one workgroup per aggregate forms P^T r, one small kernel forms Ac^-1 P^T r;
 // the prolongation is fused into the CG's preconditioner step (kernels_cg.hip).
 #include <algorithm>
@@ -109,11 +109,13 @@ __global__ __launch_bounds__(256) void k_coarse_blocks(int64_t ncq, int nc, cons
     }
 }
 
-// Gauss-Jordan inversion of the symmetrised Ac (SPD, so no pivoting), one launch per pivot k,
-// reading A_in and writing A_out (no element is read after another thread wrote it). A coarse
-// DoF whose pivot vanishes (an aggregate whose DoFs are all eliminated) is dropped: its row and
-// column become zero, so it receives no correction.
-__global__ __launch_bounds__(256) void k_coarse_sym(int nc, double *__restrict__ A, double *__restrict__ thr) {
+// Blocked Gauss-Jordan inversion of the symmetrised Ac (SPD, so no pivoting), NB pivots per
+// block step, two launches per step (reading A_in, writing A_out, so no element is read after
+// another thread wrote it). A coarse DoF whose row is zero (an aggregate whose DoFs are all
+// eliminated) is set to the identity before and to zero after, so it receives no correction.
+constexpr int GJ_NB = 32;
+
+__global__ __launch_bounds__(256) void k_coarse_sym(int nc, double *__restrict__ A, uint8_t *__restrict__ dead) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < (int64_t)nc * nc) {
         const int i = (int)(idx / nc), j = (int)(idx % nc);
@@ -123,27 +125,84 @@ __global__ __launch_bounds__(256) void k_coarse_sym(int nc, double *__restrict__
             A[(int64_t)j * nc + i] = s;
         }
     }
-    if (idx == 0) {
-        double dm = 0.0;
-        for (int k = 0; k < nc; ++k) dm = fmax(dm, fabs(A[(int64_t)k * nc + k]));
-        thr[0] = 1e-13 * dm;
+}
+
+// zero rows -> identity (and flag them); runs after k_coarse_sym, one thread per row
+__global__ __launch_bounds__(256) void k_coarse_dead(int nc, double *__restrict__ A, uint8_t *__restrict__ dead) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double m = 0.0;
+    for (int j = 0; j < nc; ++j) m = fmax(m, fabs(A[(int64_t)i * nc + j]));
+    dead[i] = m == 0.0;
+    if (m == 0.0) A[(int64_t)i * nc + i] = 1.0;
+}
+
+// Block step kb, part 1 (one workgroup): Dinv = A[K,K]^-1 (Gauss-Jordan in LDS) and the new row
+// panel R[k][j] = sum_m Dinv[k][m] A[m][j] for j outside K, stored transposed (Rt[j][k]).
+__global__ __launch_bounds__(1024) void k_gj_block_diag(int nc, int kb, const double *__restrict__ A,
+                                                         double *__restrict__ Dinv, double *__restrict__ Rt) {
+    __shared__ double D[GJ_NB][GJ_NB];
+    const int nb = min(GJ_NB, nc - kb);
+    const int tid = threadIdx.x;
+    if (tid < nb * nb) D[tid / nb][tid % nb] = A[(int64_t)(kb + tid / nb) * nc + kb + tid % nb];
+    __syncthreads();
+    for (int k = 0; k < nb; ++k) {
+        double rk = 0.0, ck = 0.0, piv = 0.0;
+        const int i = tid / nb, j = tid % nb;
+        if (tid < nb * nb) {
+            piv = D[k][k];
+            rk = (j == k) ? 1.0 : D[k][j];
+            ck = D[i][k];
+        }
+        __syncthreads();
+        if (tid < nb * nb) {
+            const double p = 1.0 / piv;
+            if (i == k) D[i][j] = rk * p;
+            else D[i][j] = ((j == k) ? 0.0 : D[i][j]) - ck * rk * p;
+        }
+        __syncthreads();
+    }
+    if (tid < nb * nb) Dinv[tid] = D[tid / nb][tid % nb];
+    for (int64_t t = tid; t < (int64_t)nc * nb; t += blockDim.x) {
+        const int j = (int)(t / nb), k = (int)(t % nb);
+        double s = 0.0;
+        if (j < kb || j >= kb + nb)
+            for (int m = 0; m < nb; ++m) s += D[k][m] * A[(int64_t)(kb + m) * nc + j];
+        Rt[t] = s;
     }
 }
 
-__global__ __launch_bounds__(256) void k_gj_pivot(int nc, int k, const double *__restrict__ Ain, double *__restrict__ Aout,
-                                                   const double *__restrict__ thr) {
+// Block step kb, part 2: every element of the matrix.
+__global__ __launch_bounds__(256) void k_gj_block_update(int nc, int kb, const double *__restrict__ Ain,
+                                                          double *__restrict__ Aout, const double *__restrict__ Dinv,
+                                                          const double *__restrict__ Rt) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nc * nc) return;
+    const int nb = min(GJ_NB, nc - kb);
+    const int i = (int)(idx / nc), j = (int)(idx % nc);
+    const bool ik = i >= kb && i < kb + nb, jk = j >= kb && j < kb + nb;
+    double v;
+    if (ik && jk) {
+        v = Dinv[(i - kb) * nb + (j - kb)];
+    } else if (ik) {
+        v = Rt[(int64_t)j * nb + (i - kb)];
+    } else if (jk) {
+        double s = 0.0;
+        for (int m = 0; m < nb; ++m) s += Ain[(int64_t)i * nc + kb + m] * Dinv[m * nb + (j - kb)];
+        v = -s;
+    } else {
+        double s = 0.0;
+        for (int m = 0; m < nb; ++m) s += Ain[(int64_t)i * nc + kb + m] * Rt[(int64_t)j * nb + m];
+        v = Ain[idx] - s;
+    }
+    Aout[idx] = v;
+}
+
+__global__ __launch_bounds__(256) void k_coarse_undead(int nc, double *__restrict__ A, const uint8_t *__restrict__ dead) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)nc * nc) return;
     const int i = (int)(idx / nc), j = (int)(idx % nc);
-    const double piv = Ain[(int64_t)k * nc + k];
-    if (fabs(piv) <= thr[0]) {
-        Aout[idx] = (i == k || j == k) ? 0.0 : Ain[idx];
-        return;
-    }
-    const double p = 1.0 / piv;
-    const double rk = (j == k) ? 1.0 : Ain[(int64_t)k * nc + j];
-    if (i == k) Aout[idx] = rk * p;
-    else Aout[idx] = ((j == k) ? 0.0 : Ain[idx]) - Ain[(int64_t)i * nc + k] * rk * p;
+    if (dead[i] || dead[j]) A[idx] = 0.0;
 }
 
 // Partial restriction: workgroup (g, s) sums (P_i^T r_i)[m] over the s-th chunk of 1024 members
@@ -329,7 +388,8 @@ static int coarse_setup(lpm_ctx *c) {
     int maxm = 1;
     for (int g = 0; g < nagg; ++g) maxm = std::max(maxm, aptr[g + 1] - aptr[g]);
     c->coarse_S = (maxm + 1023) / 1024;
-    LPM_HIP(c, hipMalloc(&c->Ac, sizeof(double) * (2 * (size_t)nc * nc + 1)));
+    // two nc^2 ping-pong buffers, the block inverse, the row panel and the dead-row flags
+    LPM_HIP(c, hipMalloc(&c->Ac, sizeof(double) * (2 * (size_t)nc * nc + GJ_NB * GJ_NB + (size_t)nc * GJ_NB) + nc));
     LPM_HIP(c, hipMalloc(&c->cz, sizeof(double) * (size_t)nagg * c->coarse_S * M));
     LPM_HIP(c, hipMalloc(&c->cy, sizeof(double) * nc));
     LPM_HIP(c, hipMalloc(&c->dofmask, n));
@@ -379,17 +439,26 @@ int coarse_prepare(lpm_ctx *c) {
         if ((rc = ck(c, "k_coarse_blocks"))) return rc;
     }
     const int64_t nn = (int64_t)nc * nc;
-    double *A0 = c->Ac, *A1 = c->Ac + nn, *thr = c->Ac + 2 * nn;
-    hipLaunchKernelGGL(k_coarse_sym, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, A0, thr);
-    if ((rc = ck(c, "k_coarse_sym"))) return rc;
-    // the nc pivot launches are tiny (a few microseconds each): issued one by one, the host's
-    // launch rate would leave the GPU idle between them, so they are captured once into a graph
+    double *A0 = c->Ac, *A1 = c->Ac + nn, *Dinv = c->Ac + 2 * nn, *Rt = Dinv + GJ_NB * GJ_NB;
+    uint8_t *dead = (uint8_t *)(Rt + (int64_t)nc * GJ_NB);
+    const int nsteps = (nc + GJ_NB - 1) / GJ_NB;
+    // the block steps are small launches (a few microseconds each): issued one by one, the host's
+    // launch rate would leave the GPU idle between them, so the whole inversion is captured once
+    // into a graph
     if (!c->gj_graph) {
         hipGraph_t graph = nullptr;
         LPM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < nc; ++k)
-            hipLaunchKernelGGL(k_gj_pivot, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, k,
-                               (const double *)((k & 1) ? A1 : A0), (k & 1) ? A0 : A1, (const double *)thr);
+        hipLaunchKernelGGL(k_coarse_sym, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, A0, dead);
+        hipLaunchKernelGGL(k_coarse_dead, dim3(blocks_for(nc, 256)), dim3(256), 0, c->stream, nc, A0, dead);
+        for (int st = 0; st < nsteps; ++st) {
+            const double *Ain = (st & 1) ? A1 : A0;
+            double *Aout = (st & 1) ? A0 : A1;
+            hipLaunchKernelGGL(k_gj_block_diag, dim3(1), dim3(1024), 0, c->stream, nc, st * GJ_NB, Ain, Dinv, Rt);
+            hipLaunchKernelGGL(k_gj_block_update, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, st * GJ_NB, Ain,
+                               Aout, (const double *)Dinv, (const double *)Rt);
+        }
+        hipLaunchKernelGGL(k_coarse_undead, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc,
+                           (nsteps & 1) ? A1 : A0, (const uint8_t *)dead);
         const hipError_t le = hipGetLastError();
         const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -404,7 +473,7 @@ int coarse_prepare(lpm_ctx *c) {
         }
     }
     LPM_HIP(c, hipGraphLaunch(c->gj_graph, c->stream));
-    c->Ainv = (nc & 1) ? A1 : A0;
+    c->Ainv = (nsteps & 1) ? A1 : A0;
     c->coarse_version = c->k_version;
     return LPM_OK;
 }
