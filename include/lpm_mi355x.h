@@ -258,6 +258,18 @@ int lpm_force_jvp(lpm_ctx *ctx, const double *v, double *jv);
 int lpm_adjoint_reaction(lpm_ctx *ctx, int32_t group, int32_t axis, double *dF_dKn, double *dF_dTv, double *F,
                          int32_t *refinements, double *rel_residual);
 
+/* ------------------------------------------------------------------ specimen ensembles
+ * (no reference counterpart: the reference runs one specimen per process, particle.h:17)
+ * Declare the context's particles to be nspec independent specimens, specimen s owning ids
+ * first[s] .. first[s+1]-1 (first[0] = 0, first[nspec] = n; no bond may join two specimens).
+ * Every kernel then runs all specimens in the same launches; Solver::NewtonIteration
+ * (solver.h:53-89) keeps each specimen's own tolerance multiplier, residual and iteration
+ * count, and the nonlocal search and the two-level preconditioner's aggregates stay within a
+ * specimen. nspec = 0 returns to a single system. Set before the first two-level solve. */
+int lpm_set_specimens(lpm_ctx *ctx, int32_t nspec, const int64_t *first);
+/* Per-specimen results of the last static step: "newton_iters", "converged" (nspec doubles). */
+int lpm_specimen_stat(const lpm_ctx *ctx, const char *name, double *out);
+
 /* ------------------------------------------------------------------ statistics / measurement
  * Counters since the last reset: "newton_iters","cg_iters","cg_solves","assemblies",
  * "spmv_launches" (SpMVs that did work); "spmv_ms" / "spmv_launches_timed" (sum of SpMV kernel

@@ -222,7 +222,75 @@ int solve_linear_system(lpm_ctx *c, int *iters, double *error) {
 }
 
 // Solver::NewtonIteration (solver.h:53-89)
+// Solver::NewtonIteration (solver.h:53-89) for every specimen of an ensemble at once: each
+// specimen keeps its own tolerance multiplier, residual norm and iteration count. One CG solves
+// all specimens still iterating; each one's right-hand side is scaled to unit norm, so the CG
+// tolerance means the same for all of them, and converged or failed specimens get a zero
+// right-hand side (no displacement). *n_iter = the largest count, or max_NR_iter if a specimen failed.
+static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
+    const int S = c->nspec;
+    std::vector<double> r2, R2;
+    RC(geometry_force_state(c));
+    RC(launch_residual_spec(c, r2, R2));
+    std::vector<double> tolm(S), nr(S), w(S);
+    std::vector<char> active(S), failed(S, 0);
+    c->spec_newton.assign(S, 0);
+    for (int s = 0; s < S; ++s) {
+        nr[s] = std::sqrt(r2[s]);
+        tolm[s] = std::max(nr[s], std::sqrt(R2[s]));
+        active[s] = nr[s] > c->tol_NR * tolm[s];
+    }
+    for (;;) {
+        bool any = false;
+        for (int s = 0; s < S; ++s) {
+            if (!active[s]) continue;
+            if (++c->spec_newton[s] > c->max_NR_iter) {
+                active[s] = 0;
+                failed[s] = 1;
+                c->spec_newton[s] = c->max_NR_iter;
+                continue;
+            }
+            any = true;
+        }
+        if (!any) break;
+        for (int s = 0; s < S; ++s) w[s] = active[s] ? 1.0 / nr[s] : 0.0;
+        LPM_HIP(c, hipMemcpyAsync(c->spec_w, w.data(), sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
+        RC(launch_spec_scale(c, c->residual));
+        int it = 0, ok = 0;
+        double err = 0;
+        RC(cg_solve(c, &it, &err, &ok));
+        c->st_cg_iters += it;
+        c->st_cg_solves++;
+        c->last_cg_iters = it;
+        c->last_cg_err = err;
+        for (int s = 0; s < S; ++s) w[s] = active[s] ? nr[s] : 0.0;
+        LPM_HIP(c, hipMemcpyAsync(c->spec_w, w.data(), sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
+        RC(launch_scale_x(c));
+        RC(launch_move(c));
+        RC(geometry_force_state(c));
+        RC(launch_residual_spec(c, r2, R2));
+        c->st_newton++;
+        for (int s = 0; s < S; ++s) {
+            nr[s] = std::sqrt(r2[s]);
+            if (active[s] && !(nr[s] > c->tol_NR * tolm[s])) active[s] = 0;
+        }
+    }
+    int mx = 0;
+    bool anyfail = false;
+    c->spec_ok.assign(S, 1);
+    for (int s = 0; s < S; ++s) {
+        mx = std::max(mx, c->spec_newton[s]);
+        if (failed[s]) {
+            anyfail = true;
+            c->spec_ok[s] = 0;
+        }
+    }
+    *n_iter = anyfail ? c->max_NR_iter : mx;
+    return LPM_OK;
+}
+
 int newton_iteration(lpm_ctx *c, int *n_iter) {
+    if (c->nspec > 0) return newton_iteration_spec(c, n_iter);
     double r2 = 0, R2 = 0;
     RC(geometry_force_state(c));
     RC(launch_residual(c, &r2, &R2));
@@ -534,7 +602,8 @@ int lpm_destroy(lpm_ctx *c) {
                     c->cs_sum, c->type, c->cons, c->dis0, c->dis, c->dis_last, c->dLe, c->dLp, c->dLp_last, c->bstrain, c->Kn,
                     c->Tv, c->cs, c->bforce, c->bforce_last, c->bdamage, c->bdamage_last, c->residual, c->disp, c->diag64,
                     c->aval, c->invdiag, c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w, c->scal, c->partial, c->counter, c->iflag, c->sp, c->scol,
-                    c->dslot, c->sp_u, c->sp_l, c->ucol, c->lcol, c->lpos, c->uval};
+                    c->dslot, c->sp_u, c->sp_l, c->ucol, c->lcol, c->lpos, c->uval,
+                    c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part};
     for (void *p : ptrs)
         if (p) hipFree(p);
     coarse_free(c);
@@ -996,6 +1065,64 @@ int lpm_adjoint_reaction(lpm_ctx *c, int32_t group, int32_t axis, double *dF_dKn
     RC(adjoint_reaction(c, group, axis, dF_dKn, dF_dTv, F, &it, &rr));
     if (refinements) *refinements = it;
     if (rel_residual) *rel_residual = rr;
+    return LPM_OK;
+}
+
+int lpm_set_specimens(lpm_ctx *c, int32_t nspec, const int64_t *first) {
+    if (!c || nspec < 0 || (nspec > 0 && !first)) return LPM_E_ARG;
+    if (nspec == 0) {
+        c->nspec = 0;
+        return LPM_OK;
+    }
+    if (first[0] != 0 || first[nspec] != c->n) return lpm_fail(c, LPM_E_ARG, "specimen offsets must run from 0 to n");
+    for (int s = 0; s < nspec; ++s)
+        if (first[s + 1] <= first[s]) return lpm_fail(c, LPM_E_ARG, "specimen offsets must increase");
+    // no bond may join two specimens
+    std::vector<int32_t> bp(c->n + 1), p2(c->nbond);
+    LPM_HIP(c, hipMemcpyAsync(bp.data(), c->bptr, sizeof(int32_t) * (c->n + 1), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(p2.data(), c->p2, sizeof(int32_t) * c->nbond, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<int32_t> sof(c->n);
+    for (int s = 0; s < nspec; ++s)
+        for (int64_t i = first[s]; i < first[s + 1]; ++i) sof[i] = s;
+    for (int64_t i = 0; i < c->n; ++i)
+        for (int b = bp[i]; b < bp[i + 1]; ++b)
+            if (sof[p2[b]] != sof[i]) return lpm_fail(c, LPM_E_TOPO, "a bond joins two specimens");
+    if (c->agg_of) return lpm_fail(c, LPM_E_STATE, "set the specimens before the first two-level solve");
+    std::vector<int32_t> blk(nspec + 1, 0);
+    for (int s = 0; s < nspec; ++s) blk[s + 1] = blk[s] + blocks_for((first[s + 1] - first[s]) * c->dim, 256);
+    void *old[] = {c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part};
+    for (void *p : old)
+        if (p) hipFree(p);
+    c->spec_of = nullptr;
+    c->spec_w = c->spec_part = nullptr;
+    c->spec_blk = nullptr;
+    c->spec_first_d = nullptr;
+    LPM_HIP(c, hipMalloc(&c->spec_of, sizeof(int32_t) * c->n));
+    LPM_HIP(c, hipMalloc(&c->spec_w, sizeof(double) * nspec));
+    LPM_HIP(c, hipMalloc(&c->spec_blk, sizeof(int32_t) * (nspec + 1)));
+    LPM_HIP(c, hipMalloc(&c->spec_first_d, sizeof(int64_t) * (nspec + 1)));
+    LPM_HIP(c, hipMalloc(&c->spec_part, sizeof(double) * 2 * std::max(blk[nspec], 1)));
+    LPM_HIP(c, hipMemcpyAsync(c->spec_of, sof.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->spec_blk, blk.data(), sizeof(int32_t) * (nspec + 1), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->spec_first_d, first, sizeof(int64_t) * (nspec + 1), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    c->nspec = nspec;
+    c->spec_nblk = blk[nspec];
+    c->spec_first.assign(first, first + nspec + 1);
+    c->spec_newton.assign(nspec, 0);
+    c->spec_ok.assign(nspec, 1);
+    return LPM_OK;
+}
+
+int lpm_specimen_stat(const lpm_ctx *c, const char *name, double *out) {
+    if (!c || !name || !out) return LPM_E_ARG;
+    const std::string w(name);
+    for (int s = 0; s < c->nspec; ++s) {
+        if (w == "newton_iters") out[s] = c->spec_newton.empty() ? 0.0 : c->spec_newton[s];
+        else if (w == "converged") out[s] = c->spec_ok.empty() ? 1.0 : c->spec_ok[s];
+        else return LPM_E_ARG;
+    }
     return LPM_OK;
 }
 

@@ -994,6 +994,30 @@ int cg_solve(lpm_ctx *c, int *iters, double *error, int *success) {
     return c->dim == 3 ? cg_t<double, 3, 4>(c, iters, error, success) : cg_t<double, 2, 2>(c, iters, error, success);
 }
 
+// x_i *= w[specimen of i] (undoes the per-specimen right-hand-side scaling of an ensemble solve)
+template <typename T, int D, int VP>
+__global__ __launch_bounds__(256) void k_scale_x(int64_t n, const int32_t *__restrict__ spec_of, const double *__restrict__ w,
+                                                  Vec<T, VP> *__restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double f = w[spec_of[i]];
+    Vec<T, VP> v = x[i];
+    for (int k = 0; k < D; ++k) v.v[k] = (T)((double)v.v[k] * f);
+    x[i] = v;
+}
+
+template <typename T, int D, int VP>
+static int scale_x_t(lpm_ctx *c) {
+    hipLaunchKernelGGL((k_scale_x<T, D, VP>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n,
+                       (const int32_t *)c->spec_of, (const double *)c->spec_w, (Vec<T, VP> *)c->v_x);
+    return check_launch(c, "k_scale_x");
+}
+
+int launch_scale_x(lpm_ctx *c) {
+    if (c->precision == 0) return c->dim == 3 ? scale_x_t<float, 3, 4>(c) : scale_x_t<float, 2, 2>(c);
+    return c->dim == 3 ? scale_x_t<double, 3, 4>(c) : scale_x_t<double, 2, 2>(c);
+}
+
 template <typename T, int D, int VP>
 static int move_t(lpm_ctx *c) {
     hipLaunchKernelGGL((k_move<T, D, VP>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n,

@@ -301,18 +301,21 @@ static int coarse_setup(lpm_ctx *c) {
     const double spacing = 2.0 * c->radius;
     // aggregate edge: the set value, or the smallest of 8, 9, 10, ... lattice spacings that keeps
     // the coarse space at most 768 DoFs (the inversion's cost grows with its cube)
-    std::vector<int32_t> agg(n);
+    std::vector<int32_t> agg(n), sof(n, 0);  // aggregates never span two specimens of an ensemble
+    for (int s = 0; s < c->nspec; ++s)
+        for (int64_t i = c->spec_first[s]; i < c->spec_first[s + 1]; ++i) sof[i] = s;
+    const int64_t nsp = std::max(c->nspec, 1);
     int nagg = 0;
     for (int cells = c->coarse_cells > 0 ? c->coarse_cells : 8;; cells += (cells < 32 ? 1 : 8)) {
         const double edge = cells * spacing;
         int64_t dims[3] = {1, 1, 1};
         for (int k = 0; k < D; ++k) dims[k] = (int64_t)std::floor((hi[k] - lo[k]) / edge) + 1;
-        std::vector<int32_t> id(dims[0] * dims[1] * dims[2], -1);
+        std::vector<int32_t> id(dims[0] * dims[1] * dims[2] * nsp, -1);
         nagg = 0;
         for (int64_t i = 0; i < n; ++i) {
             int64_t cell[3] = {0, 0, 0};
             for (int k = 0; k < D; ++k) cell[k] = std::min<int64_t>(dims[k] - 1, (int64_t)((x[3 * i + k] - lo[k]) / edge));
-            const int64_t key = cell[0] + dims[0] * (cell[1] + dims[1] * cell[2]);
+            const int64_t key = cell[0] + dims[0] * (cell[1] + dims[1] * (cell[2] + dims[2] * (int64_t)sof[i]));
             if (id[key] < 0) id[key] = nagg++;
             agg[i] = id[key];
         }

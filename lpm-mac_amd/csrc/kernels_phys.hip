@@ -223,6 +223,39 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, int dim, const int3
     block_partial<2>(v, partial, cap);
 }
 
+// Solver::updateRR per specimen of an ensemble: the same residual and, per block of one
+// specimen's DoFs, the partial squared norms of residual and reaction (blocks of specimen s
+// are spec_blk[s] .. spec_blk[s+1]-1, so the host sums each specimen's partials in order).
+__global__ __launch_bounds__(256) void k_residual_spec(int nspec, int dim, const int64_t *__restrict__ first,
+                                                        const int32_t *__restrict__ spec_blk, const int32_t *__restrict__ cons,
+                                                        const double *__restrict__ Pex, const double *__restrict__ Pin,
+                                                        double *__restrict__ residual, double *partial, int cap) {
+    int lo = 0, hi = nspec - 1;
+    while (lo < hi) {  // specimen of this block
+        const int mid = (lo + hi + 1) >> 1;
+        if (spec_blk[mid] <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t d = first[lo] * dim + (int64_t)(blockIdx.x - spec_blk[lo]) * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (d < first[lo + 1] * dim) {
+        const int64_t i = d / dim;
+        const int k = (int)(d - i * dim);
+        const int c = cons[3 * i + k];
+        const double r = (1 - c) * (Pex[3 * i + k] - Pin[3 * i + k]);
+        residual[d] = r;
+        v[0] = r * r;
+        if (c == 1) v[1] = Pin[3 * i + k] * Pin[3 * i + k];
+    }
+    block_partial<2>(v, partial, cap);
+}
+
+__global__ __launch_bounds__(256) void k_spec_scale(int64_t nd, int dim, const int32_t *__restrict__ spec_of,
+                                                     const double *__restrict__ w, double *__restrict__ v) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < nd) v[d] *= w[spec_of[d / dim]];
+}
+
 // ---------------------------------------------------------------- stiffness assembly
 // Stiffness::calcStiffness with localStiffnessANA (stiffness.h:67-101, 228-263) and
 // fdu2dxyz / fdu2dxyz1 / fdu2dxyz2 (derivative.h:7-239). One wavefront per particle row i,
@@ -794,25 +827,33 @@ __global__ __launch_bounds__(256) void k_rowof(int64_t n, const int32_t *__restr
 // Assembly::searchNonlocalNeighbors (assembly.h:117-130): all j (self included) with
 // |x_i - x_j| < ratio * L_i, in id order. Positions are staged through LDS tiles.
 __global__ __launch_bounds__(256) void k_nonlocal_search(int64_t n, double ratio, const double *__restrict__ xyz,
-                                                          const double *__restrict__ L, int32_t *__restrict__ counts,
-                                                          const int32_t *__restrict__ nlptr, int32_t *__restrict__ nlidx) {
+                                                          const double *__restrict__ L, const int32_t *__restrict__ spec_of,
+                                                          int32_t *__restrict__ counts, const int32_t *__restrict__ nlptr,
+                                                          int32_t *__restrict__ nlidx) {
     __shared__ double tile[3 * 256];
+    __shared__ int32_t tspec[256];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n;
     double x0 = 0, y0 = 0, z0 = 0, cut = 0;
     if (act) { x0 = xyz[3 * i]; y0 = xyz[3 * i + 1]; z0 = xyz[3 * i + 2]; cut = ratio * L[i]; }
     int cnt = 0;
     int64_t o = (act && nlidx) ? nlptr[i] : 0;
+    const int32_t si = (act && spec_of) ? spec_of[i] : 0;  // an ensemble's specimens do not see each other
     for (int64_t base = 0; base < n; base += 256) {
         const int64_t j = base + threadIdx.x;
         __syncthreads();
-        if (j < n) { tile[3 * threadIdx.x] = xyz[3 * j]; tile[3 * threadIdx.x + 1] = xyz[3 * j + 1]; tile[3 * threadIdx.x + 2] = xyz[3 * j + 2]; }
+        if (j < n) {
+            tile[3 * threadIdx.x] = xyz[3 * j];
+            tile[3 * threadIdx.x + 1] = xyz[3 * j + 1];
+            tile[3 * threadIdx.x + 2] = xyz[3 * j + 2];
+            tspec[threadIdx.x] = spec_of ? spec_of[j] : 0;
+        }
         __syncthreads();
         const int m = (int)((n - base) < 256 ? (n - base) : 256);
         if (act) {
             for (int q = 0; q < m; ++q) {
                 const double dx = x0 - tile[3 * q], dy = y0 - tile[3 * q + 1], dz = z0 - tile[3 * q + 2];
-                if (sqrt(dx * dx + dy * dy + dz * dz) < cut) {
+                if (tspec[q] == si && sqrt(dx * dx + dy * dy + dz * dz) < cut) {
                     if (nlidx) nlidx[o++] = (int32_t)(base + q);
                     ++cnt;
                 }
@@ -870,6 +911,37 @@ int launch_residual(lpm_ctx *c, double *nr2, double *nR2) {
         if (nR2) *nR2 = c->h_scal[SC_NORM_REAC];
     }
     return LPM_OK;
+}
+
+int launch_residual_spec(lpm_ctx *c, std::vector<double> &r2, std::vector<double> &R2) {
+    const int g = c->spec_nblk;
+    hipLaunchKernelGGL(k_residual_spec, dim3(g), dim3(256), 0, c->stream, c->nspec, c->dim, (const int64_t *)c->spec_first_d,
+                       (const int32_t *)c->spec_blk, c->cons, c->Pex, c->Pin, c->residual, c->spec_part, g);
+    int rc = check_launch(c, "k_residual_spec");
+    if (rc) return rc;
+    std::vector<double> h(2 * (size_t)g);
+    LPM_HIP(c, hipMemcpyAsync(h.data(), c->spec_part, sizeof(double) * 2 * g, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    r2.assign(c->nspec, 0.0);
+    R2.assign(c->nspec, 0.0);
+    // the blocks of specimen s cover DoFs of s only; sum them in block order
+    int64_t b0 = 0;
+    for (int sp = 0; sp < c->nspec; ++sp) {
+        const int64_t nb = blocks_for((c->spec_first[sp + 1] - c->spec_first[sp]) * c->dim, 256);
+        for (int64_t b = b0; b < b0 + nb; ++b) {
+            r2[sp] += h[b];
+            R2[sp] += h[g + b];
+        }
+        b0 += nb;
+    }
+    return LPM_OK;
+}
+
+int launch_spec_scale(lpm_ctx *c, double *v) {
+    const int64_t nd = c->n * c->dim;
+    hipLaunchKernelGGL(k_spec_scale, dim3(blocks_for(nd, 256)), dim3(256), 0, c->stream, nd, c->dim, (const int32_t *)c->spec_of,
+                       (const double *)c->spec_w, v);
+    return check_launch(c, "k_spec_scale");
 }
 
 int launch_assemble(lpm_ctx *c) {
@@ -1042,7 +1114,7 @@ int launch_nonlocal_search(lpm_ctx *c, double ratio) {
     int32_t *counts = nullptr;
     LPM_HIP(c, hipMalloc(&counts, sizeof(int32_t) * (c->n + 1)));
     hipLaunchKernelGGL(k_nonlocal_search, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, ratio, c->xyz,
-                       c->nonlocal_L, counts, (const int32_t *)nullptr, (int32_t *)nullptr);
+                       c->nonlocal_L, (const int32_t *)c->spec_of, counts, (const int32_t *)nullptr, (int32_t *)nullptr);
     int rc = check_launch(c, "k_nonlocal_search(count)");
     if (rc) { hipFree(counts); return rc; }
     std::vector<int32_t> h(c->n);
@@ -1061,7 +1133,7 @@ int launch_nonlocal_search(lpm_ctx *c, double ratio) {
     LPM_HIP(c, hipMalloc(&c->nlidx, sizeof(int32_t) * (c->nnl > 0 ? c->nnl : 1)));
     LPM_HIP(c, hipMemcpyAsync(c->nlptr, p.data(), sizeof(int32_t) * (c->n + 1), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_nonlocal_search, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, ratio, c->xyz,
-                       c->nonlocal_L, counts, (const int32_t *)c->nlptr, c->nlidx);
+                       c->nonlocal_L, (const int32_t *)c->spec_of, counts, (const int32_t *)c->nlptr, c->nlidx);
     rc = check_launch(c, "k_nonlocal_search(fill)");
     LPM_HIP(c, hipStreamSynchronize(c->stream));
     hipFree(counts);

@@ -98,6 +98,18 @@ struct lpm_ctx {
     hipGraphExec_t gj_graph = nullptr;  // the ncoarse pivot launches of the inversion, captured once
     uint8_t *dofmask = nullptr;       // per particle: bit k set when DoF k is eliminated
     int64_t coarse_version = -1;
+    // Specimen ensemble: independent particle systems (contiguous id ranges, no bond between
+    // them) solved in the same launches, each with its own Newton test.
+    int nspec = 0;                    // 0: one system
+    std::vector<int64_t> spec_first;  // nspec + 1 particle offsets
+    int32_t *spec_of = nullptr;       // specimen of each particle (device)
+    double *spec_w = nullptr;         // per-specimen factor (device): rhs scale / displacement scale
+    int32_t *spec_blk = nullptr;      // first specimen-residual block of each specimen (device, nspec + 1)
+    int64_t *spec_first_d = nullptr;  // spec_first on the device
+    int spec_nblk = 0;
+    double *spec_part = nullptr;      // per-block partials of the specimen residual norms (2 x spec_nblk)
+    std::vector<int> spec_newton;     // Newton iterations of each specimen in the last step
+    std::vector<char> spec_ok;        // converged in the last step
     int32_t *nlptr = nullptr, *nlidx = nullptr;
     // particle state (device, FP64 unless noted)
     double *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr, *Pin = nullptr, *Pex = nullptr;
@@ -159,11 +171,14 @@ int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any);
 int launch_pattern(lpm_ctx *c);
 int launch_tslot(lpm_ctx *c);
 int launch_nonlocal_search(lpm_ctx *c, double ratio);
+int launch_residual_spec(lpm_ctx *c, std::vector<double> &r2, std::vector<double> &R2);
+int launch_spec_scale(lpm_ctx *c, double *v);  // v[dof] *= spec_w[specimen]
 // ---- launchers (kernels_cg.hip)
 int launch_finalize(lpm_ctx *c, int kind, int na);
 int cg_alloc(lpm_ctx *c);
 int cg_solve(lpm_ctx *c, int *iters, double *error, int *success);
 int launch_move(lpm_ctx *c);
+int launch_scale_x(lpm_ctx *c);  // x_i *= spec_w[specimen of i]
 int spmv_bench(lpm_ctx *c, int reps, double *ms);
 int launch_precond_setup(lpm_ctx *c);
 int sym_prepare(lpm_ctx *c);

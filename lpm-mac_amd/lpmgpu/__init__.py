@@ -83,6 +83,8 @@ _SIGS = {
     "lpm_spmv_bench": (_I, [_P, ctypes.c_int32, _DP]),
     "lpm_copy_bench": (_I, [_P, ctypes.c_int64, ctypes.c_int32, _DP]),
     "lpm_force_jvp": (_I, [_P, _DP, _DP]),
+    "lpm_set_specimens": (_I, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "lpm_specimen_stat": (_I, [_P, _S, _DP]),
     "lpm_adjoint_reaction": (_I, [_P, ctypes.c_int32, ctypes.c_int32, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
     "lpm_stream": (_P, [_P]),
 }
@@ -415,6 +417,17 @@ class Context:
         ms = _D()
         self._chk(lib().lpm_spmv_bench(self.h, reps, ctypes.byref(ms)))
         return ms.value
+
+    def set_specimens(self, first):
+        """Declare an ensemble: specimen s owns particle ids first[s] .. first[s+1]-1."""
+        first = np.ascontiguousarray(first, dtype=np.int64)
+        self._nspec = len(first) - 1
+        self._chk(lib().lpm_set_specimens(self.h, self._nspec, first.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+
+    def specimen_stat(self, name):
+        out = np.zeros(self._nspec)
+        self._chk(lib().lpm_specimen_stat(self.h, name.encode(), out.ctypes.data_as(_DP)))
+        return out
 
     def force_jvp(self, v):
         """Exact directional derivative of the internal force: (dPin/dx) v, DoF order."""

@@ -1,0 +1,82 @@
+"""Specimen ensembles: several independent particle systems of uneven size in one context
+(one launch per kernel for all of them), each with its own Newton test. Every specimen must
+reach the equilibrium the FP64 oracle computes for it alone; the specimens are placed on top of
+each other in space, so any cross-talk (bonds, nonlocal neighbours, coarse aggregates) would
+show."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from scenarios import G, hex_plate, rel, sc_cube
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _union(parts):
+    """Concatenate specimens: positions, bond CSR with shifted ids, and particle offsets."""
+    xyz, bp, bs, p2, first = [], [0], [], [], [0]
+    for x, (b_ptr, b_split, b_p2) in parts:
+        off = first[-1]
+        boff = bp[-1]
+        xyz.append(x)
+        bp.extend(list(np.asarray(b_ptr[1:]) + boff))
+        bs.extend(list(np.asarray(b_split) + boff))
+        p2.extend(list(np.asarray(b_p2) + off))
+        first.append(off + len(x))
+    return (np.vstack(xyz), (np.array(bp, dtype=np.int32), np.array(bs, dtype=np.int32), np.array(p2, dtype=np.int32)),
+            np.array(first, dtype=np.int64))
+
+
+@pytest.mark.parametrize("precond,precision", [(0, 1), (2, 0)])
+def test_hex_ensemble_matches_single_specimen_oracle(precond, precision):
+    say("[gpu] ensemble hex precond", precond, "precision", precision)
+    sizes = [(12, 10), (24, 16), (16, 30), (40, 40)]
+    pulls = [0.001, 0.002, 0.0015, 0.001]
+    parts, cases = [], []
+    for (nx, ny), pull in zip(sizes, pulls):
+        x, box = hex_plate(nx, ny)
+        parts.append((x, G.build_bonds(G.HEXAGON2D, 0.5, x)))
+        cases.append((x, box, pull))
+    xyz, bonds, first = _union(parts)
+    g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=bonds)
+    g.set_specimens(first)
+    g.set_elastic(0.0, True, 69e3, 0.3, 1.0, elmap_literal=0)
+    g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=precision, precond=precond, coarse_cells=4,
+           cg_tol=1e-12 if precision == 1 else 1e-5, tol_NR=1e-10, max_NR_iter=50, cg_maxit=50000)
+    disp = []
+    for s, (x, box, pull) in enumerate(cases):
+        ids = np.arange(first[s], first[s + 1])
+        y = x[:, 1]
+        lo, hi = ids[y < y.min() + 1e-6], ids[y > y.max() - 1e-6]
+        disp += [(lo, "x", 0.0), (lo, "y", 0.0), (hi, "y", pull)]
+    ls = g.load_step(disp=disp)
+    g.update_geometry()
+    g.update_force_state()
+    steps = 3
+    for _ in range(steps):
+        assert g.solve_step_static(ls, -1)
+    its = g.specimen_stat("newton_iters")
+    assert np.all(g.specimen_stat("converged") == 1)
+    xg = g.get("xyz").reshape(-1, 3)
+    for s, (x, box, pull) in enumerate(cases):
+        o = O.System(O.HEXAGON2D, 0.5, O.ELASTIC, x, box)
+        o.mode(elmap_literal=0, dirichlet_static=1, cg_tol=1e-12, tol_NR=1e-10, max_NR_iter=50)
+        o.set_elastic(0.0, True, 69e3, 0.3, 1.0)
+        y = x[:, 1]
+        lo, hi = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
+        o.update_geometry()
+        o.update_force_state()
+        for _ in range(steps):
+            assert o.solve_step_static(O.LoadStep().add_disp(lo, "x", 0.0).add_disp(lo, "y", 0.0).add_disp(hi, "y", pull), -1)
+        u_o = o.getd("xyz").reshape(-1, 3) - x
+        u_g = xg[first[s]:first[s + 1]] - x
+        err = rel(u_g, u_o)
+        say(f"   specimen {s} ({len(x)} particles): newton {its[s]:.0f} disp rel err {err:.3e}")
+        assert err < 1e-8
