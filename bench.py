@@ -167,6 +167,8 @@ def main():
     ap.add_argument("--coarse-cells", type=int, default=0, help="aggregate edge of the two-level preconditioner (0: auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="instead of the headline, time this many uneven hex specimens in one context")
     ap.add_argument("--timing-every", type=int, default=16,
                     help="bracket the CG SpMV of every k-th iteration with HIP events")
     ap.add_argument("--pmc-calibrate", action="store_true",
@@ -181,6 +183,11 @@ def main():
     import lpmgpu as G
 
     G.lib()
+    if args.ensemble > 0:
+        if args.steps == 20:
+            args.steps = 5
+        ensemble_bench(args)
+        return
     ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond, args.spmv_sym, args.cg_forcing,
                                       args.coarse_cells)
     if args.verbose:
@@ -277,6 +284,108 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.n, jac_counts, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())))
+    print(json.dumps(out), flush=True)
+
+
+def ensemble_bench(args):
+    """Specimen ensemble (the shape of the 64-dogbone calibration workload): 64 hex plates of
+    uniform random size in [5k, 60k] particles (seed 5), Ti-like E = 110 GPa, nu = 0.34, plane
+    stress, bottom edge clamped, top edge pulled each step by its own strain increment (uniform in
+    [1e-4, 3e-4] of the specimen height), `steps` static load steps. Timed once as one context holding all specimens (one launch per kernel) and once
+    as one context per specimen run back to back; prints one JSON line."""
+    import lpmgpu as G
+
+    rng = np.random.default_rng(5)
+    S = args.ensemble
+    sizes = rng.integers(5000, 60001, S)
+    strains = rng.uniform(1e-4, 3e-4, S)
+    parts, pulls = [], []
+    for m, e in zip(sizes, strains):
+        nx = int(round(math.sqrt(m)))
+        box = [0.0, float(nx), 0.0, 0.8660254037844386 * nx, 0.0, 1.0]
+        x = G.generate_lattice(G.HEXAGON2D, box, 0.5)
+        parts.append(x)
+        pulls.append(e * (x[:, 1].max() - x[:, 1].min()))
+
+    def make(xyzs, bonds=None):
+        ctx = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, np.vstack(xyzs), bonds=bonds)
+        ctx.set_elastic(0.0, True, 110e3, 0.34, 1.0, elmap_literal=0)
+        ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond,
+                 cg_tol=args.cg_tol, tol_NR=args.tol_nr, max_NR_iter=30, cg_maxit=50000)
+        return ctx
+
+    def bcs(xyzs, offs, pulls_):
+        disp = []
+        for x, off, pull in zip(xyzs, offs, pulls_):
+            y = x[:, 1]
+            ids = np.arange(off, off + len(x))
+            lo, hi = ids[y < y.min() + 1e-6], ids[y > y.max() - 1e-6]
+            disp += [(lo, "x", 0.0), (lo, "y", 0.0), (hi, "y", float(pull))]
+        return disp
+
+    # one context for all specimens: concatenated positions, bond CSR with shifted ids
+    bp, bs, p2, first = [np.zeros(1, np.int64)], [], [], [0]
+    for x in parts:
+        b_ptr, b_split, b_p2 = (np.asarray(a, dtype=np.int64) for a in G.build_bonds(G.HEXAGON2D, 0.5, x))
+        boff = int(bp[-1][-1])
+        bp.append(b_ptr[1:] + boff)
+        bs.append(b_split + boff)
+        p2.append(b_p2 + first[-1])
+        first.append(first[-1] + len(x))
+    union = tuple(np.concatenate(a).astype(np.int32) for a in (bp, bs, p2))
+    ctx = make(parts, union)
+    ctx.set_specimens(first)
+    ls = ctx.load_step(disp=bcs(parts, first[:-1], pulls))
+    ctx.update_geometry()
+    ctx.update_force_state()
+    ctx.solve_step_static(ls, -1)  # warm-up step (tuner, coarse space)
+    hip_sync(ctx)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if not ctx.solve_step_static(ls, -1):
+            raise RuntimeError("ensemble step did not converge")
+    hip_sync(ctx)
+    t_batch = time.perf_counter() - t0
+    newton = ctx.specimen_stat("newton_iters")
+    del ctx
+    # the same specimens, one context each, back to back
+    ctxs = []
+    for x, pull in zip(parts, pulls):
+        c1 = make([x])
+        l1 = c1.load_step(disp=bcs([x], [0], [pull]))
+        c1.update_geometry()
+        c1.update_force_state()
+        c1.solve_step_static(l1, -1)
+        ctxs.append((c1, l1))
+    for c1, _ in ctxs:
+        hip_sync(c1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for c1, l1 in ctxs:
+            if not c1.solve_step_static(l1, -1):
+                raise RuntimeError("single-specimen step did not converge")
+    for c1, _ in ctxs:
+        hip_sync(c1)
+    t_seq = time.perf_counter() - t0
+    total = int(first[-1])
+    out = {
+        "metric": "specimen load steps/s, 64 uneven hex specimens in one context",
+        "value": S * args.steps / t_batch,
+        "unit": "specimen_load_steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": 1e3 * t_batch / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-mixed (FP32 K/SpMV/CG, FP64 residual/positions)",
+        "data": "synthetic",
+        "config": {"workload": f"{S} hex plates, {total} particles, sizes {int(sizes.min())}-{int(sizes.max())}",
+                   "precond": args.precond, "newton_iters_per_step_max": float(newton.max()),
+                   "one_context_per_specimen_specimen_steps_per_s": S * args.steps / t_seq,
+                   "speedup_vs_one_context_per_specimen": t_seq / t_batch},
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -241,7 +241,8 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
         active[s] = nr[s] > c->tol_NR * tolm[s];
     }
     for (;;) {
-        bool any = false;
+        int nact = 0;
+        double eta = 0.1;
         for (int s = 0; s < S; ++s) {
             if (!active[s]) continue;
             if (++c->spec_newton[s] > c->max_NR_iter) {
@@ -250,15 +251,25 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
                 c->spec_newton[s] = c->max_NR_iter;
                 continue;
             }
-            any = true;
+            ++nact;
+            eta = std::min(eta, 0.5 * c->tol_NR * tolm[s] / nr[s]);
         }
-        if (!any) break;
+        if (nact == 0) break;
         for (int s = 0; s < S; ++s) w[s] = active[s] ? 1.0 / nr[s] : 0.0;
         LPM_HIP(c, hipMemcpyAsync(c->spec_w, w.data(), sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
         RC(launch_spec_scale(c, c->residual));
         int it = 0, ok = 0;
         double err = 0;
-        RC(cg_solve(c, &it, &err, &ok));
+        // Every active specimen's right-hand side has norm 1, and the CG tests the whole
+        // residual: a tolerance of tau / sqrt(active) bounds each specimen's own relative
+        // residual by tau. tau is the set cg_tol or, with forcing, the inexact-Newton reduction
+        // the strictest active specimen needs.
+        const double cg_tol_set = c->cg_tol;
+        const double tau = c->cg_forcing ? std::max(cg_tol_set, eta) : cg_tol_set;
+        c->cg_tol = tau / std::sqrt((double)nact);
+        const int rc_cg = cg_solve(c, &it, &err, &ok);
+        c->cg_tol = cg_tol_set;
+        RC(rc_cg);
         c->st_cg_iters += it;
         c->st_cg_solves++;
         c->last_cg_iters = it;
@@ -270,10 +281,19 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
         RC(geometry_force_state(c));
         RC(launch_residual_spec(c, r2, R2));
         c->st_newton++;
+        int nleft = 0, worst = -1;
+        double wrel = 0.0;
         for (int s = 0; s < S; ++s) {
             nr[s] = std::sqrt(r2[s]);
             if (active[s] && !(nr[s] > c->tol_NR * tolm[s])) active[s] = 0;
+            if (active[s]) {
+                ++nleft;
+                if (nr[s] / tolm[s] > wrel) wrel = nr[s] / tolm[s], worst = s;
+            }
         }
+        if (c->verbose)
+            printf("|  |  specimens: CG %d iterations (error %.3e), %d still active, worst relative residual %.3e (specimen %d)\n",
+                   it, err, nleft, wrel, worst);
     }
     int mx = 0;
     bool anyfail = false;

@@ -11,7 +11,10 @@
 // and for every coarse block (g, h) the list of row entries that contribute to it.
 // Per assembly: contributions P_i^T K_ij P_j summed per (row i, aggregate h) in slot order,
 // then per coarse block in row order (no atomics: the result does not depend on scheduling),
-// then a blocked Gauss-Jordan inversion of Ac (32 pivots per step, captured in a graph).
+// then a blocked Gauss-Jordan inversion of Ac (32 pivots per step, captured in a graph). Ac is
+// block diagonal with one block per specimen of an ensemble (aggregates never span two
+// specimens), so every specimen gets a coarse space of its own size and all blocks are inverted
+// in the same launches.
 // Per CG iteration: one workgroup per aggregate forms P^T r, one small kernel forms Ac^-1 P^T r;
 // the prolongation is fused into the CG's preconditioner step (kernels_cg.hip).
 #include <algorithm>
@@ -85,9 +88,11 @@ __global__ __launch_bounds__(256) void k_coarse_rows(int64_t n, const int32_t *_
 // fixed interleave, each sums its entries' M x M contributions, and a fixed shuffle tree adds
 // the lanes; Ac[g M + a][h M + b] is then the sum over the block's entries.
 template <int M>
-__global__ __launch_bounds__(256) void k_coarse_blocks(int64_t ncq, int nc, const int32_t *__restrict__ cq_ptr,
+__global__ __launch_bounds__(256) void k_coarse_blocks(int64_t ncq, const int32_t *__restrict__ cq_ptr,
                                                         const int32_t *__restrict__ cq_f, const int32_t *__restrict__ cq_gh,
-                                                        const double *__restrict__ vbuf, double *__restrict__ Ac) {
+                                                        const double *__restrict__ vbuf, const int32_t *__restrict__ agg_grp,
+                                                        const int32_t *__restrict__ cb, const int64_t *__restrict__ off,
+                                                        double *__restrict__ Ac) {
     constexpr int MM = M * M;
     const int lane = threadIdx.x & 63;
     const int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -100,12 +105,15 @@ __global__ __launch_bounds__(256) void k_coarse_blocks(int64_t ncq, int nc, cons
     }
     for (int e = 0; e < MM; ++e)
         for (int o = 32; o > 0; o >>= 1) acc[e] += __shfl_xor(acc[e], o, 64);
+    // g and h belong to the same group: aggregates never span two specimens, nor do matrix rows
     const int g = cq_gh[2 * q], h = cq_gh[2 * q + 1];
+    const int b = agg_grp[g], base = cb[b], nb = cb[b + 1] - base;
+    double *A = Ac + off[b];
     for (int e = lane; e < MM; e += 64) {
         double val = 0.0;
         for (int t = 0; t < MM; ++t)
             if (t == e) val = acc[t];
-        Ac[(int64_t)(g * M + e / M) * nc + (h * M + e % M)] = val;
+        A[(int64_t)(g * M + e / M - base) * nb + (h * M + e % M - base)] = val;
     }
 }
 
@@ -113,9 +121,19 @@ __global__ __launch_bounds__(256) void k_coarse_blocks(int64_t ncq, int nc, cons
 // block step, two launches per step (reading A_in, writing A_out, so no element is read after
 // another thread wrote it). A coarse DoF whose row is zero (an aggregate whose DoFs are all
 // eliminated) is set to the identity before and to zero after, so it receives no correction.
+// A coarse DoF whose pivot (its Schur complement against the pivots before it) falls to
+// GJ_DROP times its own diagonal is a combination of earlier modes (an aggregate whose
+// unmasked DoFs cannot tell a rotation from a translation): it is skipped as a pivot and its row
+// and column of the inverse are zero, which inverts Ac on the span of the other modes.
 constexpr int GJ_NB = 32;
+constexpr double GJ_DROP = 1e-9;
 
-__global__ __launch_bounds__(256) void k_coarse_sym(int nc, double *__restrict__ A, uint8_t *__restrict__ dead) {
+// The kernels below take the group from blockIdx.y (the Gauss-Jordan steps of all groups run in
+// the same launches) unless they say otherwise; nc is then the group's block size.
+__global__ __launch_bounds__(256) void k_coarse_sym(const int32_t *__restrict__ cb, const int64_t *__restrict__ off,
+                                                    double *__restrict__ A) {
+    const int b = blockIdx.y, nc = cb[b + 1] - cb[b];
+    A += off[b];
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < (int64_t)nc * nc) {
         const int i = (int)(idx / nc), j = (int)(idx % nc);
@@ -127,41 +145,70 @@ __global__ __launch_bounds__(256) void k_coarse_sym(int nc, double *__restrict__
     }
 }
 
-// zero rows -> identity (and flag them); runs after k_coarse_sym, one thread per row
-__global__ __launch_bounds__(256) void k_coarse_dead(int nc, double *__restrict__ A, uint8_t *__restrict__ dead) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nc) return;
+// zero rows -> identity (and flag them), keep every diagonal for the pivot test; runs after
+// k_coarse_sym, one thread per coarse DoF r of the whole system
+__global__ __launch_bounds__(256) void k_coarse_dead(int ncoarse, int M, const int32_t *__restrict__ agg_grp,
+                                                     const int32_t *__restrict__ cb, const int64_t *__restrict__ off,
+                                                     double *__restrict__ Ac, double *__restrict__ odiag,
+                                                     uint8_t *__restrict__ dead) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ncoarse) return;
+    const int b = agg_grp[r / M], nc = cb[b + 1] - cb[b], i = r - cb[b];
+    double *A = Ac + off[b];
     double m = 0.0;
     for (int j = 0; j < nc; ++j) m = fmax(m, fabs(A[(int64_t)i * nc + j]));
-    dead[i] = m == 0.0;
+    dead[r] = m == 0.0;
     if (m == 0.0) A[(int64_t)i * nc + i] = 1.0;
+    odiag[r] = A[(int64_t)i * nc + i];
 }
 
 // Block step kb, part 1 (one workgroup): Dinv = A[K,K]^-1 (Gauss-Jordan in LDS) and the new row
 // panel R[k][j] = sum_m Dinv[k][m] A[m][j] for j outside K, stored transposed (Rt[j][k]).
-__global__ __launch_bounds__(1024) void k_gj_block_diag(int nc, int kb, const double *__restrict__ A,
-                                                         double *__restrict__ Dinv, double *__restrict__ Rt) {
+// Group b's block inverse is Dinv + b NB^2, its panel Rt + cb[b] NB.
+__global__ __launch_bounds__(1024) void k_gj_block_diag(int kb, const int32_t *__restrict__ cb,
+                                                         const int64_t *__restrict__ off, const double *__restrict__ Ain,
+                                                         const double *__restrict__ odiag_all, uint8_t *__restrict__ dead_all,
+                                                         double *__restrict__ Dinv_all, double *__restrict__ Rt_all) {
     __shared__ double D[GJ_NB][GJ_NB];
+    __shared__ uint8_t skip[GJ_NB];
+    const int b = blockIdx.y, base = cb[b], nc = cb[b + 1] - base;
+    if (kb >= nc) return;  // this group's inversion has finished
+    const double *A = Ain + off[b];
+    const double *odiag = odiag_all + base;
+    uint8_t *dead = dead_all + base;
+    double *Dinv = Dinv_all + (int64_t)b * GJ_NB * GJ_NB, *Rt = Rt_all + (int64_t)base * GJ_NB;
     const int nb = min(GJ_NB, nc - kb);
     const int tid = threadIdx.x;
     if (tid < nb * nb) D[tid / nb][tid % nb] = A[(int64_t)(kb + tid / nb) * nc + kb + tid % nb];
+    if (tid < GJ_NB) skip[tid] = 0;
     __syncthreads();
     for (int k = 0; k < nb; ++k) {
-        double rk = 0.0, ck = 0.0, piv = 0.0;
+        // every thread reads the same pivot, so the skip branch is uniform across the workgroup
+        const double pk = D[k][k];
+        if (!(pk > GJ_DROP * odiag[kb + k])) {
+            if (tid == 0) {
+                skip[k] = 1;
+                dead[kb + k] = 1;
+            }
+            continue;
+        }
+        double rk = 0.0, ck = 0.0;
         const int i = tid / nb, j = tid % nb;
         if (tid < nb * nb) {
-            piv = D[k][k];
             rk = (j == k) ? 1.0 : D[k][j];
             ck = D[i][k];
         }
         __syncthreads();
         if (tid < nb * nb) {
-            const double p = 1.0 / piv;
+            const double p = 1.0 / pk;
             if (i == k) D[i][j] = rk * p;
             else D[i][j] = ((j == k) ? 0.0 : D[i][j]) - ck * rk * p;
         }
         __syncthreads();
     }
+    __syncthreads();
+    if (tid < nb * nb && (skip[tid / nb] || skip[tid % nb])) D[tid / nb][tid % nb] = 0.0;
+    __syncthreads();
     if (tid < nb * nb) Dinv[tid] = D[tid / nb][tid % nb];
     for (int64_t t = tid; t < (int64_t)nc * nb; t += blockDim.x) {
         const int j = (int)(t / nb), k = (int)(t % nb);
@@ -172,12 +219,22 @@ __global__ __launch_bounds__(1024) void k_gj_block_diag(int nc, int kb, const do
     }
 }
 
-// Block step kb, part 2: every element of the matrix.
-__global__ __launch_bounds__(256) void k_gj_block_update(int nc, int kb, const double *__restrict__ Ain,
-                                                          double *__restrict__ Aout, const double *__restrict__ Dinv,
-                                                          const double *__restrict__ Rt) {
+// Block step kb, part 2: every element of the group's block (a group whose inversion has
+// finished is copied, so that every block ends in the same buffer).
+__global__ __launch_bounds__(256) void k_gj_block_update(int kb, const int32_t *__restrict__ cb,
+                                                          const int64_t *__restrict__ off, const double *__restrict__ Ain_all,
+                                                          double *__restrict__ Aout_all, const double *__restrict__ Dinv_all,
+                                                          const double *__restrict__ Rt_all) {
+    const int b = blockIdx.y, base = cb[b], nc = cb[b + 1] - base;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)nc * nc) return;
+    const double *Ain = Ain_all + off[b];
+    double *Aout = Aout_all + off[b];
+    if (kb >= nc) {
+        Aout[idx] = Ain[idx];
+        return;
+    }
+    const double *Dinv = Dinv_all + (int64_t)b * GJ_NB * GJ_NB, *Rt = Rt_all + (int64_t)base * GJ_NB;
     const int nb = min(GJ_NB, nc - kb);
     const int i = (int)(idx / nc), j = (int)(idx % nc);
     const bool ik = i >= kb && i < kb + nb, jk = j >= kb && j < kb + nb;
@@ -198,7 +255,11 @@ __global__ __launch_bounds__(256) void k_gj_block_update(int nc, int kb, const d
     Aout[idx] = v;
 }
 
-__global__ __launch_bounds__(256) void k_coarse_undead(int nc, double *__restrict__ A, const uint8_t *__restrict__ dead) {
+__global__ __launch_bounds__(256) void k_coarse_undead(const int32_t *__restrict__ cb, const int64_t *__restrict__ off,
+                                                       double *__restrict__ Ac, const uint8_t *__restrict__ dead_all) {
+    const int b = blockIdx.y, nc = cb[b + 1] - cb[b];
+    double *A = Ac + off[b];
+    const uint8_t *dead = dead_all + cb[b];
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)nc * nc) return;
     const int i = (int)(idx / nc), j = (int)(idx % nc);
@@ -242,15 +303,19 @@ __global__ __launch_bounds__(1024) void k_coarse_restrict(int S, const int32_t *
     }
 }
 
-// cy = Ac^-1 cz: every workgroup first sums the restriction partials into cz (LDS, fixed order),
-// then each wavefront forms one row, lanes striding the row (Ac^-1 is symmetric, row = column).
-__global__ __launch_bounds__(256) void k_coarse_gemv(int nc, int M, int S, const double *__restrict__ Ainv,
+// cy = Ac^-1 cz for group blockIdx.y: every workgroup first sums the group's restriction
+// partials into cz (LDS, fixed order), then each wavefront forms one row of the group's block,
+// lanes striding the row (Ac^-1 is symmetric, row = column).
+__global__ __launch_bounds__(256) void k_coarse_gemv(int M, int S, const int32_t *__restrict__ cb,
+                                                      const int64_t *__restrict__ off, const double *__restrict__ Ainv_all,
                                                       const double *__restrict__ czp, double *__restrict__ cy,
                                                       const double *__restrict__ scal) {
     if (scal[SC_DONE] != 0.0) return;
+    const int b = blockIdx.y, base = cb[b], nc = cb[b + 1] - base;
+    if ((int)(blockIdx.x * (blockDim.x >> 6)) >= nc) return;  // uniform over the workgroup
     __shared__ double cz[1024];
     for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-        const int g = c / M, m = c % M;
+        const int g = (base + c) / M, m = (base + c) % M;
         double s = 0.0;
         for (int t = 0; t < S; ++t) s += czp[((int64_t)g * S + t) * M + m];
         cz[c] = s;
@@ -259,10 +324,11 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(int nc, int M, int S, const
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (r >= nc) return;
+    const double *Ainv = Ainv_all + off[b];
     double s = 0.0;
     for (int c = lane; c < nc; c += 64) s += Ainv[(int64_t)r * nc + c] * cz[c];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) cy[r] = s;
+    if (lane == 0) cy[base + r] = s;
 }
 
 }  // namespace
@@ -275,8 +341,9 @@ static int ck(lpm_ctx *c, const char *what) {
 
 void coarse_free(lpm_ctx *c) {
     if (c->gj_graph) hipGraphExecDestroy(c->gj_graph);
-    void *ptrs[] = {c->agg_of, c->agg_ptr, c->agg_idx, c->agg_d, c->ra_ptr, c->ra_id, c->cq_ptr,
-                    c->cq_f,   c->cq_gh,   c->cvbuf,   c->Ac,    c->cz,     c->cy,    c->dofmask};
+    void *ptrs[] = {c->agg_of, c->agg_ptr, c->agg_idx, c->agg_d, c->ra_ptr, c->ra_id,  c->cq_ptr,
+                    c->cq_f,   c->cq_gh,   c->cvbuf,   c->Ac,    c->cz,     c->cy,     c->dofmask,
+                    c->cgrp_cb, c->cgrp_off, c->agg_grp};
     for (void *p : ptrs)
         if (p) hipFree(p);
 }
@@ -292,36 +359,62 @@ static int coarse_setup(lpm_ctx *c) {
     LPM_HIP(c, hipMemcpyAsync(cp.data(), c->cptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
     LPM_HIP(c, hipMemcpyAsync(ci.data(), c->cidx, sizeof(int32_t) * c->nconn, hipMemcpyDeviceToHost, c->stream));
     LPM_HIP(c, hipStreamSynchronize(c->stream));
-    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int64_t i = 0; i < n; ++i)
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = std::min(lo[k], x[3 * i + k]);
-            hi[k] = std::max(hi[k], x[3 * i + k]);
-        }
     const double spacing = 2.0 * c->radius;
-    // aggregate edge: the set value, or the smallest of 8, 9, 10, ... lattice spacings that keeps
-    // the coarse space at most 768 DoFs (the inversion's cost grows with its cube)
-    std::vector<int32_t> agg(n), sof(n, 0);  // aggregates never span two specimens of an ensemble
-    for (int s = 0; s < c->nspec; ++s)
-        for (int64_t i = c->spec_first[s]; i < c->spec_first[s + 1]; ++i) sof[i] = s;
+    // Aggregates are the cells of a grid laid over each specimen's own bounding box (the whole
+    // system when there are no specimens), so they never span two specimens, and each specimen is
+    // one coarse group (one diagonal block of Ac). Each box edge is cut into round(extent /
+    // target) equal cells: a remainder cell one or two lattice rows thick would make an aggregate
+    // whose modes are dependent once its prescribed DoFs are masked. The target edge is the set
+    // value, or, per group, the smallest of 8, 9, 10, ... lattice spacings that keeps the group's
+    // block at most 768 DoFs (the inversion's cost grows with its cube).
     const int64_t nsp = std::max(c->nspec, 1);
-    int nagg = 0;
-    for (int cells = c->coarse_cells > 0 ? c->coarse_cells : 8;; cells += (cells < 32 ? 1 : 8)) {
-        const double edge = cells * spacing;
-        int64_t dims[3] = {1, 1, 1};
-        for (int k = 0; k < D; ++k) dims[k] = (int64_t)std::floor((hi[k] - lo[k]) / edge) + 1;
-        std::vector<int32_t> id(dims[0] * dims[1] * dims[2] * nsp, -1);
-        nagg = 0;
-        for (int64_t i = 0; i < n; ++i) {
-            int64_t cell[3] = {0, 0, 0};
-            for (int k = 0; k < D; ++k) cell[k] = std::min<int64_t>(dims[k] - 1, (int64_t)((x[3 * i + k] - lo[k]) / edge));
-            const int64_t key = cell[0] + dims[0] * (cell[1] + dims[1] * (cell[2] + dims[2] * (int64_t)sof[i]));
-            if (id[key] < 0) id[key] = nagg++;
-            agg[i] = id[key];
+    std::vector<int64_t> first(nsp + 1, 0);
+    if (c->nspec > 0)
+        for (int64_t s = 0; s <= nsp; ++s) first[s] = c->spec_first[s];
+    else
+        first[1] = n;
+    std::vector<int32_t> agg(n), agg_grp;
+    std::vector<int32_t> cb(nsp + 1, 0);
+    std::vector<int64_t> off(nsp + 1, 0);
+    int nagg = 0, nbmax = 0;
+    for (int64_t s = 0; s < nsp; ++s) {
+        double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        for (int64_t i = first[s]; i < first[s + 1]; ++i)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], x[3 * i + k]);
+                hi[k] = std::max(hi[k], x[3 * i + k]);
+            }
+        int ns = 0;  // aggregates of this group, numbered from nagg in order of first member
+        for (int cells = c->coarse_cells > 0 ? c->coarse_cells : 8; first[s + 1] > first[s]; cells += (cells < 32 ? 1 : 8)) {
+            const double target = cells * spacing;
+            int64_t dims[3] = {1, 1, 1};
+            double edge[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k < D; ++k) {
+                dims[k] = std::max<int64_t>(1, (int64_t)std::llround((hi[k] - lo[k]) / target));
+                edge[k] = (hi[k] - lo[k]) / (double)dims[k];
+            }
+            std::vector<int32_t> id(dims[0] * dims[1] * dims[2], -1);
+            ns = 0;
+            for (int64_t i = first[s]; i < first[s + 1]; ++i) {
+                int64_t key = 0;
+                for (int k = D - 1; k >= 0; --k) {
+                    const int64_t ck =
+                        edge[k] > 0.0 ? std::min<int64_t>(dims[k] - 1, (int64_t)((x[3 * i + k] - lo[k]) / edge[k])) : 0;
+                    key = key * dims[k] + ck;
+                }
+                if (id[key] < 0) id[key] = ns++;
+                agg[i] = nagg + id[key];
+            }
+            if (c->coarse_cells > 0 || (int64_t)ns * M <= 768) break;
         }
-        if (c->coarse_cells > 0 || (int64_t)nagg * M <= 768) break;
+        if ((int64_t)ns * M > 1024)
+            return lpm_fail(c, LPM_E_LIMIT, "coarse block above 1024 DoFs: raise coarse_cells");
+        nagg += ns;
+        agg_grp.insert(agg_grp.end(), ns, (int32_t)s);
+        cb[s + 1] = cb[s] + ns * M;
+        off[s + 1] = off[s] + (int64_t)ns * M * ns * M;
+        nbmax = std::max(nbmax, ns * M);
     }
-    if ((int64_t)nagg * M > 1024) return lpm_fail(c, LPM_E_LIMIT, "coarse space above 1024 DoFs: raise coarse_cells");
     const int nc = nagg * M;
     // members, centres, offsets
     std::vector<int32_t> aptr(nagg + 1, 0), aidx(n);
@@ -369,6 +462,10 @@ static int coarse_setup(lpm_ctx *c) {
             qf.push_back(kf[t].second);
         }
         qptr.push_back((int32_t)qf.size());
+        // every coarse block must lie inside one group's diagonal block (the kernels index it so)
+        for (size_t t = 0; t < qgh.size(); t += 2)
+            if (agg_grp[qgh[t]] != agg_grp[qgh[t + 1]])
+                return lpm_fail(c, LPM_E_TOPO, "the matrix pattern joins two specimens");
         c->ncq = (int64_t)qgh.size() / 2;
         LPM_HIP(c, hipMalloc(&c->cq_ptr, sizeof(int32_t) * qptr.size()));
         LPM_HIP(c, hipMalloc(&c->cq_f, sizeof(int32_t) * std::max<size_t>(qf.size(), 1)));
@@ -391,8 +488,19 @@ static int coarse_setup(lpm_ctx *c) {
     int maxm = 1;
     for (int g = 0; g < nagg; ++g) maxm = std::max(maxm, aptr[g + 1] - aptr[g]);
     c->coarse_S = (maxm + 1023) / 1024;
-    // two nc^2 ping-pong buffers, the block inverse, the row panel and the dead-row flags
-    LPM_HIP(c, hipMalloc(&c->Ac, sizeof(double) * (2 * (size_t)nc * nc + GJ_NB * GJ_NB + (size_t)nc * GJ_NB) + nc));
+    // two ping-pong buffers of the packed blocks, the groups' NB x NB block inverses, their row
+    // panels, the original diagonal and the dead-row flags
+    c->ncgrp = (int)nsp;
+    c->cgrp_nbmax = nbmax;
+    c->ac_elems = off[nsp];
+    LPM_HIP(c, hipMalloc(&c->Ac, sizeof(double) * (2 * (size_t)c->ac_elems + (size_t)nsp * GJ_NB * GJ_NB +
+                                                   (size_t)nc * GJ_NB + nc) + nc));
+    LPM_HIP(c, hipMalloc(&c->cgrp_cb, sizeof(int32_t) * (nsp + 1)));
+    LPM_HIP(c, hipMalloc(&c->cgrp_off, sizeof(int64_t) * (nsp + 1)));
+    LPM_HIP(c, hipMalloc(&c->agg_grp, sizeof(int32_t) * std::max(nagg, 1)));
+    LPM_HIP(c, hipMemcpyAsync(c->cgrp_cb, cb.data(), sizeof(int32_t) * (nsp + 1), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->cgrp_off, off.data(), sizeof(int64_t) * (nsp + 1), hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->agg_grp, agg_grp.data(), sizeof(int32_t) * nagg, hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMalloc(&c->cz, sizeof(double) * (size_t)nagg * c->coarse_S * M));
     LPM_HIP(c, hipMalloc(&c->cy, sizeof(double) * nc));
     LPM_HIP(c, hipMalloc(&c->dofmask, n));
@@ -429,39 +537,47 @@ int coarse_prepare(lpm_ctx *c) {
     if (c->precision == 0) rc = c->dim == 3 ? coarse_rows_t<float, 3>(c) : coarse_rows_t<float, 2>(c);
     else rc = c->dim == 3 ? coarse_rows_t<double, 3>(c) : coarse_rows_t<double, 2>(c);
     if (rc) return rc;
-    const int nc = c->ncoarse, M = c->nmode;
-    LPM_HIP(c, hipMemsetAsync(c->Ac, 0, sizeof(double) * (size_t)nc * nc, c->stream));
+    const int nc = c->ncoarse, M = c->nmode, G = c->ncgrp, nbmax = c->cgrp_nbmax;
+    const int64_t ne = c->ac_elems;
+    LPM_HIP(c, hipMemsetAsync(c->Ac, 0, sizeof(double) * (size_t)ne, c->stream));
     if (c->ncq > 0) {
         const int gq = (int)((c->ncq + 3) / 4);
         if (M == 6)
-            hipLaunchKernelGGL((k_coarse_blocks<6>), dim3(gq), dim3(256), 0, c->stream, c->ncq, nc, c->cq_ptr, c->cq_f,
-                               c->cq_gh, c->cvbuf, c->Ac);
+            hipLaunchKernelGGL((k_coarse_blocks<6>), dim3(gq), dim3(256), 0, c->stream, c->ncq, c->cq_ptr, c->cq_f, c->cq_gh,
+                               c->cvbuf, c->agg_grp, c->cgrp_cb, c->cgrp_off, c->Ac);
         else
-            hipLaunchKernelGGL((k_coarse_blocks<3>), dim3(gq), dim3(256), 0, c->stream, c->ncq, nc, c->cq_ptr, c->cq_f,
-                               c->cq_gh, c->cvbuf, c->Ac);
+            hipLaunchKernelGGL((k_coarse_blocks<3>), dim3(gq), dim3(256), 0, c->stream, c->ncq, c->cq_ptr, c->cq_f, c->cq_gh,
+                               c->cvbuf, c->agg_grp, c->cgrp_cb, c->cgrp_off, c->Ac);
         if ((rc = ck(c, "k_coarse_blocks"))) return rc;
     }
-    const int64_t nn = (int64_t)nc * nc;
-    double *A0 = c->Ac, *A1 = c->Ac + nn, *Dinv = c->Ac + 2 * nn, *Rt = Dinv + GJ_NB * GJ_NB;
-    uint8_t *dead = (uint8_t *)(Rt + (int64_t)nc * GJ_NB);
-    const int nsteps = (nc + GJ_NB - 1) / GJ_NB;
+    double *A0 = c->Ac, *A1 = c->Ac + ne, *Dinv = c->Ac + 2 * ne, *Rt = Dinv + (int64_t)G * GJ_NB * GJ_NB;
+    double *odiag = Rt + (int64_t)nc * GJ_NB;
+    uint8_t *dead = (uint8_t *)(odiag + nc);
+    const int nsteps = (nbmax + GJ_NB - 1) / GJ_NB;
+    // every group's block in the same launches (grid y = group, x over the largest block)
+    const dim3 gsq(blocks_for((int64_t)nbmax * nbmax, 256), G);
     // the block steps are small launches (a few microseconds each): issued one by one, the host's
     // launch rate would leave the GPU idle between them, so the whole inversion is captured once
     // into a graph
-    if (!c->gj_graph) {
+    if (!c->gj_graph && nbmax > 0) {
         hipGraph_t graph = nullptr;
         LPM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipLaunchKernelGGL(k_coarse_sym, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, A0, dead);
-        hipLaunchKernelGGL(k_coarse_dead, dim3(blocks_for(nc, 256)), dim3(256), 0, c->stream, nc, A0, dead);
+        hipLaunchKernelGGL(k_coarse_sym, gsq, dim3(256), 0, c->stream, (const int32_t *)c->cgrp_cb,
+                           (const int64_t *)c->cgrp_off, A0);
+        hipLaunchKernelGGL(k_coarse_dead, dim3(blocks_for(nc, 256)), dim3(256), 0, c->stream, nc, M,
+                           (const int32_t *)c->agg_grp, (const int32_t *)c->cgrp_cb, (const int64_t *)c->cgrp_off, A0, odiag,
+                           dead);
         for (int st = 0; st < nsteps; ++st) {
             const double *Ain = (st & 1) ? A1 : A0;
             double *Aout = (st & 1) ? A0 : A1;
-            hipLaunchKernelGGL(k_gj_block_diag, dim3(1), dim3(1024), 0, c->stream, nc, st * GJ_NB, Ain, Dinv, Rt);
-            hipLaunchKernelGGL(k_gj_block_update, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc, st * GJ_NB, Ain,
-                               Aout, (const double *)Dinv, (const double *)Rt);
+            hipLaunchKernelGGL(k_gj_block_diag, dim3(1, G), dim3(1024), 0, c->stream, st * GJ_NB,
+                               (const int32_t *)c->cgrp_cb, (const int64_t *)c->cgrp_off, Ain, (const double *)odiag, dead,
+                               Dinv, Rt);
+            hipLaunchKernelGGL(k_gj_block_update, gsq, dim3(256), 0, c->stream, st * GJ_NB, (const int32_t *)c->cgrp_cb,
+                               (const int64_t *)c->cgrp_off, Ain, Aout, (const double *)Dinv, (const double *)Rt);
         }
-        hipLaunchKernelGGL(k_coarse_undead, dim3(blocks_for(nn, 256)), dim3(256), 0, c->stream, nc,
-                           (nsteps & 1) ? A1 : A0, (const uint8_t *)dead);
+        hipLaunchKernelGGL(k_coarse_undead, gsq, dim3(256), 0, c->stream, (const int32_t *)c->cgrp_cb,
+                           (const int64_t *)c->cgrp_off, (nsteps & 1) ? A1 : A0, (const uint8_t *)dead);
         const hipError_t le = hipGetLastError();
         const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -475,7 +591,7 @@ int coarse_prepare(lpm_ctx *c) {
             return lpm_fail(c, LPM_E_HIP, std::string("inversion graph: ") + hipGetErrorString(ie));
         }
     }
-    LPM_HIP(c, hipGraphLaunch(c->gj_graph, c->stream));
+    if (c->gj_graph) LPM_HIP(c, hipGraphLaunch(c->gj_graph, c->stream));
     c->Ainv = (nsteps & 1) ? A1 : A0;
     c->coarse_version = c->k_version;
     return LPM_OK;
@@ -487,9 +603,10 @@ static int coarse_apply_t(lpm_ctx *c) {
                        c->agg_ptr, c->agg_idx, c->agg_d, c->dofmask, (const T *)c->v_r, c->cz, (const double *)c->scal);
     int rc = ck(c, "k_coarse_restrict");
     if (rc) return rc;
-    // 4 wavefronts per workgroup, one row per wavefront
-    hipLaunchKernelGGL(k_coarse_gemv, dim3(blocks_for(c->ncoarse, 4)), dim3(256), 0, c->stream, c->ncoarse, c->nmode,
-                       c->coarse_S, (const double *)c->Ainv, (const double *)c->cz, c->cy, (const double *)c->scal);
+    // 4 wavefronts per workgroup, one row of a group's block per wavefront
+    hipLaunchKernelGGL(k_coarse_gemv, dim3(blocks_for(std::max(c->cgrp_nbmax, 1), 4), c->ncgrp), dim3(256), 0, c->stream,
+                       c->nmode, c->coarse_S, (const int32_t *)c->cgrp_cb, (const int64_t *)c->cgrp_off,
+                       (const double *)c->Ainv, (const double *)c->cz, c->cy, (const double *)c->scal);
     return ck(c, "k_coarse_gemv");
 }
 

@@ -91,7 +91,14 @@ struct lpm_ctx {
     int32_t *cq_ptr = nullptr, *cq_f = nullptr, *cq_gh = nullptr;  // coarse blocks: contributing row entries
     int64_t ncq = 0;
     double *cvbuf = nullptr;          // per (row, aggregate) contributions P_i^T K_ij P_j (nmode^2 each)
-    double *Ac = nullptr;             // 2 ncoarse^2 + 1 FP64: Gauss-Jordan ping-pong buffers + pivot threshold
+    // Ac is block diagonal, one dense block per coarse group (a specimen of an ensemble, or the
+    // whole system): block b covers coarse DoFs [cgrp_cb[b], cgrp_cb[b+1]) and is stored row-major
+    // at cgrp_off[b] in each of the two Gauss-Jordan buffers.
+    int ncgrp = 1, cgrp_nbmax = 0;
+    int64_t ac_elems = 0;             // sum of the squared block sizes
+    int32_t *cgrp_cb = nullptr, *agg_grp = nullptr;  // block bases (ncgrp + 1) / group of each aggregate
+    int64_t *cgrp_off = nullptr;      // block offsets (ncgrp + 1)
+    double *Ac = nullptr;             // Gauss-Jordan ping-pong buffers, block inverses, row panels, diagonal, flags
     double *Ainv = nullptr;           // the buffer of Ac that holds Ac^-1 (inside Ac)
     double *cz = nullptr, *cy = nullptr;  // restriction partials (nagg * coarse_S * nmode) / correction
     int coarse_S = 1;                 // restriction workgroups per aggregate
