@@ -162,7 +162,8 @@ def main():
     ap.add_argument("--cg-tol", type=float, default=1e-3)
     ap.add_argument("--cg-forcing", type=int, default=1, help="1: inexact-Newton CG tolerance per iteration")
     ap.add_argument("--precond", type=int, default=2,
-                    help="0 Jacobi (as the reference's Eigen CG), 1 block Jacobi, 2 Jacobi + rigid-body coarse correction")
+                    help="0 Jacobi (as the reference's Eigen CG), 1 block Jacobi, 2 Jacobi + rigid-body coarse correction, "
+                         "3 block Jacobi + rigid-body coarse correction")
     ap.add_argument("--spmv-sym", type=int, default=0, help="1: half store when K is symmetric, 0: full store")
     ap.add_argument("--coarse-cells", type=int, default=0, help="aggregate edge of the two-level preconditioner (0: auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -262,7 +263,8 @@ def main():
             "workload": f"SC {args.n}^3 elastic uniaxial tension, {ctx.n} particles, 0.5% strain in 20 steps",
             "particles": ctx.n, "bonds": ctx.size("nbonds"), "bsr_blocks": ctx.size("nconn"),
             "newton_tol": args.tol_nr, "cg_rtol": args.cg_tol, "cg_forcing": args.cg_forcing,
-            "precond": {0: "jacobi", 1: "block-jacobi", 2: "two-level (jacobi + rigid-body aggregate coarse space)"}[args.precond],
+            "precond": {0: "jacobi", 1: "block-jacobi", 2: "two-level (jacobi + rigid-body aggregate coarse space)",
+                        3: "two-level (3x3 block jacobi + rigid-body aggregate coarse space)"}[args.precond],
             "coarse_dofs": int(ncoarse),
             "spmv_store": "half (symmetric)" if spmv_sym_active else "full",
             "spmv_launch": {"slots_unrolled": int(ctx.get_mode("spmv_unr")),

@@ -120,8 +120,9 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *  "precond"          0: Jacobi (Eigen DiagonalPreconditioner) | 1: 3x3 block Jacobi |
  *                        2: Jacobi + additive coarse correction on the rigid-body modes of cubic
  *                        aggregates of particles (two-level; iteration count no longer grows
- *                        with the lattice width). Changes the CG's rate, not the equilibrium.
- *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 (0: smallest of 8, 9,
+ *                        with the lattice width) | 3: as 2 with 3x3 block Jacobi in place of
+ *                        Jacobi. Changes the CG's rate, not the equilibrium.
+ *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 and 3 (0: smallest of 8, 9,
  *                        10, ... giving at most 768 coarse DoFs); get_mode("ncoarse") reads the
  *                        coarse dimension once built
  *  "spmv_sym"         0: SpMV on the full matrix | 1: after each assembly check that the

@@ -668,7 +668,7 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     } else if (w == "cg_tol") c->cg_tol = v;
     else if (w == "cg_maxit") c->cg_maxit = (int64_t)v;
     else if (w == "precond") {
-        if (v != 0 && v != 1 && v != 2) return lpm_fail(c, LPM_E_ARG, "precond must be 0, 1 or 2");
+        if (v != 0 && v != 1 && v != 2 && v != 3) return lpm_fail(c, LPM_E_ARG, "precond must be 0, 1, 2 or 3");
         c->precond = (int)v;
     } else if (w == "coarse_cells") {
         if (c->agg_of) return lpm_fail(c, LPM_E_STATE, "coarse_cells is fixed once the coarse space exists");

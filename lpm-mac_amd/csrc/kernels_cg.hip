@@ -527,9 +527,10 @@ template <typename T, int D, int VP>
 __global__ __launch_bounds__(256) void k_cg_vec_b(int64_t n, const T *__restrict__ invd, int block_pre,
                                                    const Vec<T, VP> *__restrict__ r, Vec<T, VP> *__restrict__ u,
                                                    const int32_t *__restrict__ agg_of, const double *__restrict__ agg_d,
-                                                   const uint8_t *__restrict__ mask, const double *__restrict__ cy, int M,
+                                                   const uint8_t *__restrict__ mask, const double *__restrict__ cy,
                                                    double *partial, int cap, const double *scal) {
     if (scal[SC_DONE] != 0.0) return;
+    constexpr int M = D == 3 ? 6 : 3;  // compile-time, so that P stays in registers
     double v[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const Vec<T, VP> rv = r[i];
@@ -708,7 +709,7 @@ int cg_alloc(lpm_ctx *c) {
 template <typename T, int D, int VP>
 static int precond_t(lpm_ctx *c) {
     hipLaunchKernelGGL((k_precond<T, D>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp,
-                       (const T *)c->aval, c->precond == 1, (T *)c->invdiag);
+                       (const T *)c->aval, block_jacobi(c), (T *)c->invdiag);
     return check_launch(c, "k_precond");
 }
 
@@ -901,17 +902,17 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     if (gv > c->partial_cap || gs > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "CG grid exceeds partial buffer");
     const int64_t maxit = c->cg_maxit < 0 ? 2 * n * c->dim : c->cg_maxit;
     hipLaunchKernelGGL((k_cg_init<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, c->residual, (const T *)c->invdiag,
-                       c->precond == 1, (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s, c->partial,
+                       block_jacobi(c), (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s, c->partial,
                        c->partial_cap, c->scal);
     int rc = check_launch(c, "k_cg_init");
     if (rc) return rc;
-    const bool coarse = c->precond == 2;
+    const bool coarse = two_level(c);
     // two-level: u = D^-1 r + P Ac^-1 P^T r (restriction + coarse solve, then the fused update)
     auto coarse_u = [&]() -> int {
         int r2 = launch_coarse_apply(c);
         if (r2) return r2;
-        hipLaunchKernelGGL((k_cg_vec_b<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (const T *)c->invdiag, 0,
-                           (const V *)c->v_r, (V *)c->v_u, c->agg_of, c->agg_d, c->dofmask, (const double *)c->cy, c->nmode,
+        hipLaunchKernelGGL((k_cg_vec_b<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (const T *)c->invdiag, block_jacobi(c),
+                           (const V *)c->v_r, (V *)c->v_u, c->agg_of, c->agg_d, c->dofmask, (const double *)c->cy,
                            c->partial, c->partial_cap, (const double *)c->scal);
         return check_launch(c, "k_cg_vec_b");
     };
@@ -940,7 +941,7 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
                 if (!rc) rc = coarse_u();
             } else {
                 hipLaunchKernelGGL((k_cg_vec<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (const T *)c->invdiag,
-                                   c->precond == 1, (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s,
+                                   block_jacobi(c), (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s,
                                    (const V *)c->v_w, c->partial, c->partial_cap, gv, gs, c->scal, (int)(it0 + k), c->cg_tol,
                                    (double)maxit);
                 rc = check_launch(c, "k_cg_vec");

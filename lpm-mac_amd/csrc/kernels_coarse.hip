@@ -146,60 +146,55 @@ __global__ __launch_bounds__(256) void k_coarse_sym(const int32_t *__restrict__ 
 }
 
 // zero rows -> identity (and flag them), keep every diagonal for the pivot test; runs after
-// k_coarse_sym, one thread per coarse DoF r of the whole system
+// k_coarse_sym, one wavefront per coarse DoF r of the whole system
 __global__ __launch_bounds__(256) void k_coarse_dead(int ncoarse, int M, const int32_t *__restrict__ agg_grp,
                                                      const int32_t *__restrict__ cb, const int64_t *__restrict__ off,
                                                      double *__restrict__ Ac, double *__restrict__ odiag,
                                                      uint8_t *__restrict__ dead) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (r >= ncoarse) return;
     const int b = agg_grp[r / M], nc = cb[b + 1] - cb[b], i = r - cb[b];
     double *A = Ac + off[b];
     double m = 0.0;
-    for (int j = 0; j < nc; ++j) m = fmax(m, fabs(A[(int64_t)i * nc + j]));
-    dead[r] = m == 0.0;
-    if (m == 0.0) A[(int64_t)i * nc + i] = 1.0;
-    odiag[r] = A[(int64_t)i * nc + i];
+    for (int j = lane; j < nc; j += 64) m = fmax(m, fabs(A[(int64_t)i * nc + j]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (lane == 0) {
+        dead[r] = m == 0.0;
+        if (m == 0.0) A[(int64_t)i * nc + i] = 1.0;
+        odiag[r] = A[(int64_t)i * nc + i];
+    }
 }
 
-// Block step kb, part 1 (one workgroup): Dinv = A[K,K]^-1 (Gauss-Jordan in LDS) and the new row
-// panel R[k][j] = sum_m Dinv[k][m] A[m][j] for j outside K, stored transposed (Rt[j][k]).
-// Group b's block inverse is Dinv + b NB^2, its panel Rt + cb[b] NB.
-__global__ __launch_bounds__(1024) void k_gj_block_diag(int kb, const int32_t *__restrict__ cb,
-                                                         const int64_t *__restrict__ off, const double *__restrict__ Ain,
-                                                         const double *__restrict__ odiag_all, uint8_t *__restrict__ dead_all,
-                                                         double *__restrict__ Dinv_all, double *__restrict__ Rt_all) {
-    __shared__ double D[GJ_NB][GJ_NB];
+// Block step kb, part 1 (one workgroup per group): Dinv = A[K,K]^-1 by Gauss-Jordan in LDS, with
+// the pivot drop described above. Dinv is stored NB x NB (row stride NB), zero outside nb x nb
+// and in the rows and columns of dropped pivots.
+__global__ __launch_bounds__(1024) void k_gj_diag(int kb, const int32_t *__restrict__ cb, const int64_t *__restrict__ off,
+                                                   const double *__restrict__ Ain, const double *__restrict__ odiag_all,
+                                                   uint8_t *__restrict__ dead_all, double *__restrict__ Dinv_all) {
+    __shared__ double D[GJ_NB][GJ_NB + 1];
     __shared__ uint8_t skip[GJ_NB];
     const int b = blockIdx.y, base = cb[b], nc = cb[b + 1] - base;
     if (kb >= nc) return;  // this group's inversion has finished
     const double *A = Ain + off[b];
-    const double *odiag = odiag_all + base;
-    uint8_t *dead = dead_all + base;
-    double *Dinv = Dinv_all + (int64_t)b * GJ_NB * GJ_NB, *Rt = Rt_all + (int64_t)base * GJ_NB;
     const int nb = min(GJ_NB, nc - kb);
-    const int tid = threadIdx.x;
-    if (tid < nb * nb) D[tid / nb][tid % nb] = A[(int64_t)(kb + tid / nb) * nc + kb + tid % nb];
+    const int tid = threadIdx.x, i = tid / GJ_NB, j = tid % GJ_NB;  // blockDim = NB * NB
+    const bool in = i < nb && j < nb;
+    D[i][j] = in ? A[(int64_t)(kb + i) * nc + kb + j] : 0.0;
     if (tid < GJ_NB) skip[tid] = 0;
     __syncthreads();
     for (int k = 0; k < nb; ++k) {
         // every thread reads the same pivot, so the skip branch is uniform across the workgroup
         const double pk = D[k][k];
-        if (!(pk > GJ_DROP * odiag[kb + k])) {
+        if (!(pk > GJ_DROP * odiag_all[base + kb + k])) {
             if (tid == 0) {
                 skip[k] = 1;
-                dead[kb + k] = 1;
+                dead_all[base + kb + k] = 1;
             }
             continue;
         }
-        double rk = 0.0, ck = 0.0;
-        const int i = tid / nb, j = tid % nb;
-        if (tid < nb * nb) {
-            rk = (j == k) ? 1.0 : D[k][j];
-            ck = D[i][k];
-        }
+        const double rk = (j == k) ? 1.0 : D[k][j], ck = D[i][k];
         __syncthreads();
-        if (tid < nb * nb) {
+        if (in) {
             const double p = 1.0 / pk;
             if (i == k) D[i][j] = rk * p;
             else D[i][j] = ((j == k) ? 0.0 : D[i][j]) - ck * rk * p;
@@ -207,52 +202,91 @@ __global__ __launch_bounds__(1024) void k_gj_block_diag(int kb, const int32_t *_
         __syncthreads();
     }
     __syncthreads();
-    if (tid < nb * nb && (skip[tid / nb] || skip[tid % nb])) D[tid / nb][tid % nb] = 0.0;
-    __syncthreads();
-    if (tid < nb * nb) Dinv[tid] = D[tid / nb][tid % nb];
-    for (int64_t t = tid; t < (int64_t)nc * nb; t += blockDim.x) {
-        const int j = (int)(t / nb), k = (int)(t % nb);
-        double s = 0.0;
-        if (j < kb || j >= kb + nb)
-            for (int m = 0; m < nb; ++m) s += D[k][m] * A[(int64_t)(kb + m) * nc + j];
-        Rt[t] = s;
-    }
+    Dinv_all[(int64_t)b * GJ_NB * GJ_NB + tid] = (in && !skip[i] && !skip[j]) ? D[i][j] : 0.0;
 }
 
-// Block step kb, part 2: every element of the group's block (a group whose inversion has
-// finished is copied, so that every block ends in the same buffer).
-__global__ __launch_bounds__(256) void k_gj_block_update(int kb, const int32_t *__restrict__ cb,
-                                                          const int64_t *__restrict__ off, const double *__restrict__ Ain_all,
-                                                          double *__restrict__ Aout_all, const double *__restrict__ Dinv_all,
-                                                          const double *__restrict__ Rt_all) {
+// Block step kb, part 2: the row panel Rt[j][k] = sum_m Dinv[k][m] A[kb+m][j] for j outside K
+// (zero for j in K), row stride NB; four threads per column j, eight k each, Dinv staged in LDS.
+__global__ __launch_bounds__(256) void k_gj_panel(int kb, const int32_t *__restrict__ cb, const int64_t *__restrict__ off,
+                                                   const double *__restrict__ Ain, const double *__restrict__ Dinv_all,
+                                                   double *__restrict__ Rt_all) {
+    __shared__ double Ds[GJ_NB][GJ_NB + 1];
     const int b = blockIdx.y, base = cb[b], nc = cb[b + 1] - base;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)nc * nc) return;
+    if (kb >= nc || (int)(blockIdx.x * (blockDim.x >> 2)) >= nc) return;  // uniform over the workgroup
+    const double *Dinv = Dinv_all + (int64_t)b * GJ_NB * GJ_NB;
+    for (int e = threadIdx.x; e < GJ_NB * GJ_NB; e += blockDim.x) Ds[e / GJ_NB][e % GJ_NB] = Dinv[e];
+    __syncthreads();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, j = t >> 2, k0 = (t & 3) * 8;
+    if (j >= nc) return;
+    const double *A = Ain + off[b];
+    const int nb = min(GJ_NB, nc - kb);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (j < kb || j >= kb + nb)
+        for (int m = 0; m < nb; ++m) {
+            const double a = A[(int64_t)(kb + m) * nc + j];
+            for (int q = 0; q < 8; ++q) acc[q] += Ds[k0 + q][m] * a;
+        }
+    double *Rt = Rt_all + (int64_t)(base + j) * GJ_NB + k0;
+    for (int q = 0; q < 8; ++q) Rt[q] = acc[q];
+}
+
+// Block step kb, part 3: the rank-NB update of the group's block, one 32 x 32 tile per
+// workgroup (tiles are aligned with the pivot blocks, so a tile lies wholly inside or outside K
+// in each direction):
+//   tile in K x K:      Dinv
+//   row tile in K:      Rt^T
+//   column tile in K:   -A[:,K] Dinv
+//   else:               A - A[:,K] Rt^T
+// A group whose inversion has finished is copied, so that every block ends in the same buffer.
+__global__ __launch_bounds__(256) void k_gj_update(int kb, int T, const int32_t *__restrict__ cb,
+                                                    const int64_t *__restrict__ off, const double *__restrict__ Ain_all,
+                                                    double *__restrict__ Aout_all, const double *__restrict__ Dinv_all,
+                                                    const double *__restrict__ Rt_all) {
+    static_assert(GJ_NB == 32, "tiles are 32 x 32 pivot blocks");
+    __shared__ double L[32][33], R[32][33];
+    const int b = blockIdx.y, base = cb[b], nc = cb[b + 1] - base;
+    const int i0 = (blockIdx.x / T) * 32, j0 = (blockIdx.x % T) * 32;
+    if (i0 >= nc || j0 >= nc) return;  // uniform over the workgroup
     const double *Ain = Ain_all + off[b];
     double *Aout = Aout_all + off[b];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     if (kb >= nc) {
-        Aout[idx] = Ain[idx];
+        for (int q = 0; q < 4; ++q) {
+            const int i = i0 + ty + 8 * q, j = j0 + tx;
+            if (i < nc && j < nc) Aout[(int64_t)i * nc + j] = Ain[(int64_t)i * nc + j];
+        }
         return;
     }
     const double *Dinv = Dinv_all + (int64_t)b * GJ_NB * GJ_NB, *Rt = Rt_all + (int64_t)base * GJ_NB;
     const int nb = min(GJ_NB, nc - kb);
-    const int i = (int)(idx / nc), j = (int)(idx % nc);
-    const bool ik = i >= kb && i < kb + nb, jk = j >= kb && j < kb + nb;
-    double v;
+    const bool ik = i0 == kb, jk = j0 == kb;
     if (ik && jk) {
-        v = Dinv[(i - kb) * nb + (j - kb)];
-    } else if (ik) {
-        v = Rt[(int64_t)j * nb + (i - kb)];
-    } else if (jk) {
-        double s = 0.0;
-        for (int m = 0; m < nb; ++m) s += Ain[(int64_t)i * nc + kb + m] * Dinv[m * nb + (j - kb)];
-        v = -s;
-    } else {
-        double s = 0.0;
-        for (int m = 0; m < nb; ++m) s += Ain[(int64_t)i * nc + kb + m] * Rt[(int64_t)j * nb + m];
-        v = Ain[idx] - s;
+        for (int q = 0; q < 4; ++q) {
+            const int ii = ty + 8 * q, i = i0 + ii, j = j0 + tx;
+            if (i < nc && j < nc) Aout[(int64_t)i * nc + j] = Dinv[ii * GJ_NB + tx];
+        }
+        return;
     }
-    Aout[idx] = v;
+    for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+        const int r = e >> 5, m = e & 31;
+        // R[jj][m]: Rt rows j0.. (row tile in K and general case) or Dinv^T (column tile in K)
+        R[r][m] = jk ? Dinv[m * GJ_NB + r] : ((j0 + r < nc) ? Rt[(int64_t)(j0 + r) * GJ_NB + m] : 0.0);
+        L[r][m] = (!ik && i0 + r < nc && m < nb) ? Ain[(int64_t)(i0 + r) * nc + kb + m] : 0.0;
+    }
+    __syncthreads();
+    for (int q = 0; q < 4; ++q) {
+        const int ii = ty + 8 * q, i = i0 + ii, j = j0 + tx;
+        if (i >= nc || j >= nc) continue;
+        double v;
+        if (ik) {
+            v = R[tx][ii];
+        } else {
+            double s = 0.0;
+            for (int m = 0; m < 32; ++m) s += L[ii][m] * R[tx][m];
+            v = jk ? -s : Ain[(int64_t)i * nc + j] - s;
+        }
+        Aout[(int64_t)i * nc + j] = v;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_coarse_undead(const int32_t *__restrict__ cb, const int64_t *__restrict__ off,
@@ -526,7 +560,7 @@ static int coarse_rows_t(lpm_ctx *c) {
 
 // Rebuild and invert Ac after the matrix changed.
 int coarse_prepare(lpm_ctx *c) {
-    if (c->precond != 2) return LPM_OK;
+    if (!two_level(c)) return LPM_OK;
     int rc = coarse_setup(c);
     if (rc) return rc;
     if (c->coarse_version == c->k_version) return LPM_OK;
@@ -553,7 +587,7 @@ int coarse_prepare(lpm_ctx *c) {
     double *A0 = c->Ac, *A1 = c->Ac + ne, *Dinv = c->Ac + 2 * ne, *Rt = Dinv + (int64_t)G * GJ_NB * GJ_NB;
     double *odiag = Rt + (int64_t)nc * GJ_NB;
     uint8_t *dead = (uint8_t *)(odiag + nc);
-    const int nsteps = (nbmax + GJ_NB - 1) / GJ_NB;
+    const int nsteps = (nbmax + GJ_NB - 1) / GJ_NB, T = nsteps;  // T x T tiles of 32 cover the largest block
     // every group's block in the same launches (grid y = group, x over the largest block)
     const dim3 gsq(blocks_for((int64_t)nbmax * nbmax, 256), G);
     // the block steps are small launches (a few microseconds each): issued one by one, the host's
@@ -564,17 +598,21 @@ int coarse_prepare(lpm_ctx *c) {
         LPM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(k_coarse_sym, gsq, dim3(256), 0, c->stream, (const int32_t *)c->cgrp_cb,
                            (const int64_t *)c->cgrp_off, A0);
-        hipLaunchKernelGGL(k_coarse_dead, dim3(blocks_for(nc, 256)), dim3(256), 0, c->stream, nc, M,
+        hipLaunchKernelGGL(k_coarse_dead, dim3(blocks_for(nc, 4)), dim3(256), 0, c->stream, nc, M,
                            (const int32_t *)c->agg_grp, (const int32_t *)c->cgrp_cb, (const int64_t *)c->cgrp_off, A0, odiag,
                            dead);
         for (int st = 0; st < nsteps; ++st) {
             const double *Ain = (st & 1) ? A1 : A0;
             double *Aout = (st & 1) ? A0 : A1;
-            hipLaunchKernelGGL(k_gj_block_diag, dim3(1, G), dim3(1024), 0, c->stream, st * GJ_NB,
+            hipLaunchKernelGGL(k_gj_diag, dim3(1, G), dim3(GJ_NB * GJ_NB), 0, c->stream, st * GJ_NB,
                                (const int32_t *)c->cgrp_cb, (const int64_t *)c->cgrp_off, Ain, (const double *)odiag, dead,
-                               Dinv, Rt);
-            hipLaunchKernelGGL(k_gj_block_update, gsq, dim3(256), 0, c->stream, st * GJ_NB, (const int32_t *)c->cgrp_cb,
-                               (const int64_t *)c->cgrp_off, Ain, Aout, (const double *)Dinv, (const double *)Rt);
+                               Dinv);
+            hipLaunchKernelGGL(k_gj_panel, dim3(blocks_for((int64_t)nbmax * 4, 256), G), dim3(256), 0, c->stream,
+                               st * GJ_NB, (const int32_t *)c->cgrp_cb, (const int64_t *)c->cgrp_off, Ain,
+                               (const double *)Dinv, Rt);
+            hipLaunchKernelGGL(k_gj_update, dim3(T * T, G), dim3(256), 0, c->stream, st * GJ_NB, T,
+                               (const int32_t *)c->cgrp_cb, (const int64_t *)c->cgrp_off, Ain, Aout, (const double *)Dinv,
+                               (const double *)Rt);
         }
         hipLaunchKernelGGL(k_coarse_undead, gsq, dim3(256), 0, c->stream, (const int32_t *)c->cgrp_cb,
                            (const int64_t *)c->cgrp_off, (nsteps & 1) ? A1 : A0, (const uint8_t *)dead);

@@ -188,6 +188,10 @@ int launch_move(lpm_ctx *c);
 int launch_scale_x(lpm_ctx *c);  // x_i *= spec_w[specimen of i]
 int spmv_bench(lpm_ctx *c, int reps, double *ms);
 int launch_precond_setup(lpm_ctx *c);
+// precond 1 and 3 smooth with the inverse 3x3 (2x2) diagonal blocks, 0 and 2 with the inverse
+// diagonal; 2 and 3 add the coarse correction
+static inline bool block_jacobi(const lpm_ctx *c) { return c->precond == 1 || c->precond == 3; }
+static inline bool two_level(const lpm_ctx *c) { return c->precond == 2 || c->precond == 3; }
 int sym_prepare(lpm_ctx *c);
 // ---- two-level preconditioner (kernels_coarse.hip)
 int coarse_prepare(lpm_ctx *c);
