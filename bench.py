@@ -170,6 +170,7 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--ensemble", type=int, default=0,
                     help="instead of the headline, time this many uneven hex specimens in one context")
+    ap.add_argument("--fatigue", action="store_true", help="with --ensemble: HCF dogbones through fatigue cycles")
     ap.add_argument("--timing-every", type=int, default=16,
                     help="bracket the CG SpMV of every k-th iteration with HIP events")
     ap.add_argument("--pmc-calibrate", action="store_true",
@@ -185,6 +186,9 @@ def main():
 
     G.lib()
     if args.ensemble > 0:
+        if args.fatigue:
+            fatigue_ensemble_bench(args)
+            return
         if args.steps == 20:
             args.steps = 5
         ensemble_bench(args)
@@ -386,6 +390,136 @@ def ensemble_bench(args):
         "config": {"workload": f"{S} hex plates, {total} particles, sizes {int(sizes.min())}-{int(sizes.max())}",
                    "precond": args.precond, "newton_iters_per_step_max": float(newton.max()),
                    "one_context_per_specimen_specimen_steps_per_s": S * args.steps / t_seq,
+                   "speedup_vs_one_context_per_specimen": t_seq / t_batch},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def dogbone(lattice_mod, m):
+    """Positions of a 2D hex dogbone of about m particles, loaded along y: grips of width W over
+    the top and bottom quarter of the height H = 2W, a gauge of width W/2 over the middle 30%,
+    straight tapers between (a mask over a hex plate from the lattice generator)."""
+    G = lattice_mod
+    W = math.sqrt(m * 0.8660254037844386 / 1.6)
+    H = 2.0 * W
+    x = G.generate_lattice(G.HEXAGON2D, [0.0, W, 0.0, H, 0.0, 1.0], 0.5)
+    y = (x[:, 1] - x[:, 1].min()) / (x[:, 1].max() - x[:, 1].min())
+    d = np.abs(y - 0.5)  # 0 at mid-height, 0.5 at the ends
+    width = np.where(d <= 0.15, 0.5, np.where(d >= 0.25, 1.0, 0.5 + 0.5 * (d - 0.15) / 0.10)) * W
+    keep = np.abs(x[:, 0] - 0.5 * W) <= 0.5 * width + 1e-9
+    return np.ascontiguousarray(x[keep])
+
+
+def fatigue_ensemble_bench(args):
+    """HCF fatigue ensemble (the shape of the 64-dogbone calibration workload): 64 2D hex
+    dogbones of uniform random size in [5k, 60k] particles (seed 5), Ti-like E = 110 GPa,
+    nu = 0.34, plane stress, HCF damage with per-specimen A (uniform in [3e-5, 8e-5]), B = 2.019,
+    d = 0.5, nonlocal length 4; bottom row clamped, top row displaced (LoadType dogbone
+    displacement) with R = 0.1 cycles of amplitude 1e-3 of each specimen's height, 200 cycles
+    with the linear time map (tau = 0.05: 20 cycles per simulated cycle). Timed once as one
+    context holding all specimens and once as one context per specimen run back to back."""
+    import lpmgpu as G
+
+    rng = np.random.default_rng(5)
+    S = args.ensemble
+    sizes = rng.integers(5000, 60001, S)
+    A = rng.uniform(3e-5, 8e-5, S)
+    parts = [dogbone(G, m) for m in sizes]
+    heights = [x[:, 1].max() - x[:, 1].min() for x in parts]
+    amp = [1e-3 * h for h in heights]
+    cycles, tau = 200, 0.05
+    spec = [0.1] + [1.0, 0.1] * cycles
+
+    def rows(x, off):
+        y = x[:, 1]
+        ids = np.arange(off, off + len(x))
+        return ids[y > y.max() - 1e-6], ids[y < y.min() + 1e-6]
+
+    def make(xyzs, offs, bonds=None, nspec=False):
+        ctx = G.Context(G.HEXAGON2D, 0.5, G.HCF, np.vstack(xyzs), bonds=bonds)
+        if nspec:
+            ctx.set_specimens(offs + [offs[-1] + len(xyzs[-1])])
+        t = np.zeros(sum(len(x) for x in xyzs), dtype=np.int32)
+        groups = []
+        for x, off in zip(xyzs, offs):
+            top, bottom = rows(x, off)
+            t[top] = 4
+            t[bottom] = 4
+            groups.append([top, bottom])
+        ctx.set("type", t)
+        ctx.set_hcf(4.0, True, 110e3, 0.34, 5e-5, 2.019, 0.5, elmap_literal=0)
+        mat = ctx.get("mat").reshape(-1, 10)
+        return ctx, groups, mat
+
+    def setup(ctx):
+        ctx.search_nonlocal(1.5)
+        ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond,
+                 cg_tol=args.cg_tol, cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=30, cg_maxit=50000)
+        ctx.update_geometry()
+        ctx.update_force_state()
+
+    # one context for all specimens
+    bp, bs, p2, first = [np.zeros(1, np.int64)], [], [], [0]
+    for x in parts:
+        b_ptr, b_split, b_p2 = (np.asarray(a, dtype=np.int64) for a in G.build_bonds(G.HEXAGON2D, 0.5, x))
+        boff = int(bp[-1][-1])
+        bp.append(b_ptr[1:] + boff)
+        bs.append(b_split + boff)
+        p2.append(b_p2 + first[-1])
+        first.append(first[-1] + len(x))
+    union = tuple(np.concatenate(a).astype(np.int32) for a in (bp, bs, p2))
+    ctx, groups, mat = make(parts, first[:-1], union, nspec=True)
+    for s in range(S):
+        mat[first[s]:first[s + 1], 0] = A[s]
+    ctx.set("mat", mat)
+    setup(ctx)
+    disp = []
+    for (top, bottom), a in zip(groups, amp):
+        disp += [(top, "x", 0.0), (top, "y", 0.1 * a), (top, "z", 0.0), (bottom, "x", 0.0), (bottom, "y", 0.0),
+                 (bottom, "z", 0.0)]
+    hip_sync(ctx)
+    t0 = time.perf_counter()
+    ctx.solve_problem_fatigue_static([ctx.load_step(disp=disp)])
+    n_done = ctx.solve_problem_cyclic_specimens(G.LOAD_DOGBONE_DISP, 1, spec, groups, amp, tau, 4)
+    hip_sync(ctx)
+    t_batch = time.perf_counter() - t0
+    dmax = float(ctx.get("damage").max())
+    steps = ctx.stat("cg_solves")
+    del ctx
+    # the same specimens, one context each, back to back
+    t_seq = 0.0
+    for s, x in enumerate(parts):
+        c1, g1, m1 = make([x], [0])
+        m1[:, 0] = A[s]
+        c1.set("mat", m1)
+        setup(c1)
+        top, bottom = g1[0]
+        d1 = [(top, "x", 0.0), (top, "y", 0.1 * amp[s]), (top, "z", 0.0), (bottom, "x", 0.0), (bottom, "y", 0.0),
+              (bottom, "z", 0.0)]
+        hip_sync(c1)
+        t0 = time.perf_counter()
+        c1.solve_problem_fatigue_static([c1.load_step(disp=d1)])
+        c1.solve_problem_cyclic(G.LOAD_DOGBONE_DISP, 1, [v * amp[s] for v in spec], [top, bottom], tau, 4)
+        hip_sync(c1)
+        t_seq += time.perf_counter() - t0
+        c1.close()
+    out = {
+        "metric": "specimen fatigue cycles/s, 64 uneven HCF dogbones in one context",
+        "value": S * n_done / t_batch,
+        "unit": "specimen_cycles/s",
+        "n_gpus": 1,
+        "steps": int(n_done),
+        "warmup": 0,
+        "ms_per_step": 1e3 * t_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-mixed (FP32 K/SpMV/CG, FP64 residual/positions/damage)",
+        "data": "synthetic",
+        "config": {"workload": f"{S} hex dogbones, {first[-1]} particles, sizes {int(sizes.min())}-{int(sizes.max())}, "
+                               f"R=0.1, {cycles} cycles, tau={tau}",
+                   "precond": args.precond, "fatigue_steps": steps, "max_damage": dmax,
+                   "one_context_per_specimen_specimen_cycles_per_s": S * n_done / t_seq,
                    "speedup_vs_one_context_per_specimen": t_seq / t_batch},
     }
     print(json.dumps(out), flush=True)

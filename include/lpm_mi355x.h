@@ -268,6 +268,16 @@ int lpm_adjoint_reaction(lpm_ctx *ctx, int32_t group, int32_t axis, double *dF_d
  * count, and the nonlocal search and the two-level preconditioner's aggregates stay within a
  * specimen. nspec = 0 returns to a single system. Set before the first two-level solve. */
 int lpm_set_specimens(lpm_ctx *ctx, int32_t nspec, const int64_t *first);
+/* lpm_solve_problem_cyclic for an ensemble: the load template is instantiated once per
+ * specimen, on that specimen's own groups (groups[need*s .. need*s + need-1], need = 3, or 2
+ * for LPM_LOAD_DOGBONE_DISP) with the spectrum's increments times scale[s] (NULL: 1), so that a
+ * force splits over each specimen's own group and every specimen gets its own amplitude. All
+ * specimens run the same cycles and cycle jumps (the time map does not depend on the state),
+ * each fatigue step in the same launches with per-specimen Newton tests. */
+int lpm_solve_problem_cyclic_specimens(lpm_ctx *ctx, int32_t load_type, int32_t n_interval, const double *spectrum,
+                                       int64_t n_spectrum, const int32_t *groups, const double *scale,
+                                       int32_t time_mode, double tau, int32_t undamaged_type, int32_t max_cycles_run,
+                                       int32_t *cycles);
 /* Per-specimen results of the last static step: "newton_iters", "converged" (nspec doubles). */
 int lpm_specimen_stat(const lpm_ctx *ctx, const char *name, double *out);
 

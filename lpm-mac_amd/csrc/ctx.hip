@@ -474,6 +474,56 @@ OwnedStep fatigue_template(int type, double v, const int32_t *g) {
     return s;
 }
 
+// SolverFatigue::solveProblemCyclic + generateLoadCycle + solveProblemOneCycle
+// (solver_fatigue.h:128-241). nset template instances run side by side: instance k applies the
+// template to groups[need k .. need k + need - 1] with the spectrum's increments times scale[k]
+// (one instance for a single system, one per specimen for an ensemble, which then advances all
+// specimens through the same cycles and cycle jumps in the same launches).
+int solve_cyclic(lpm_ctx *c, int32_t load_type, int32_t n_interval, const double *spec, int64_t nspec,
+                 const int32_t *groups, const double *scale, int nset, int32_t tmode, double tau, int32_t undamaged,
+                 int32_t max_cycles_run, int32_t *cycles) {
+    if (!spec || n_interval <= 0 || tau <= 0 || load_type < 0 || load_type > 3) return lpm_fail(c, LPM_E_ARG, "bad cyclic load");
+    const int need = load_type == LPM_LOAD_DOGBONE_DISP ? 2 : 3;
+    for (int k = 0; k < need * nset; ++k) RC(check_group(c, groups[k]));
+    const double n_cycles = 0.5 * (nspec - 1);
+    double t = 0, dNdt = 1;
+    int N = 0, ran = 0;
+    auto half = [&](double df) {
+        lpm_load_step empty{0, nullptr, 0, nullptr};
+        OwnedStep st(empty);
+        for (int k = 0; k < nset; ++k) {
+            const OwnedStep one = fatigue_template(load_type, df * scale[k], groups + need * k);
+            st.disp.insert(st.disp.end(), one.disp.begin(), one.disp.end());
+            st.force.insert(st.force.end(), one.force.begin(), one.force.end());
+        }
+        return st;
+    };
+    do {
+        if (2 * (int64_t)N + 2 >= nspec) return lpm_fail(c, LPM_E_ARG, "load spectrum too short for the cycle count");
+        const double f0 = spec[2 * N], f1 = spec[2 * N + 1], f2 = spec[2 * N + 2];
+        std::vector<OwnedStep> cyc;
+        for (int k = 0; k < n_interval; ++k) cyc.push_back(half((f1 - f0) / n_interval));
+        for (int k = 0; k < n_interval; ++k) cyc.push_back(half((f2 - f1) / n_interval));
+        dNdt = (tmode == LPM_TIME_LINEAR) ? (int)(1 / tau) : (int)(1 / tau * std::exp(t++ / tau));
+        for (auto &st : cyc) {
+            int ok = 0;
+            lpm_load_step v = st.view();
+            RC(solve_step_fatigue(c, &v, &ok));
+            RC(launch_state_var(c));
+        }
+        int any = 0;
+        RC(launch_fatigue_damage(c, dNdt, undamaged, &any));
+        RC(store_state(c));
+        RC(geometry_force_state(c));
+        N += (int)dNdt;
+        ++ran;
+        if (max_cycles_run > 0 && ran >= max_cycles_run) break;
+    } while (N < n_cycles);
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    if (cycles) *cycles = N;
+    return LPM_OK;
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -987,38 +1037,22 @@ int lpm_solve_problem_fatigue_static(lpm_ctx *c, const lpm_load_step *steps, int
 int lpm_solve_problem_cyclic(lpm_ctx *c, int32_t load_type, int32_t n_interval, const double *spec, int64_t nspec,
                              const int32_t *groups, int32_t ngroups, int32_t tmode, double tau, int32_t undamaged,
                              int32_t max_cycles_run, int32_t *cycles) {
-    if (!c || !spec || !groups || n_interval <= 0 || tau <= 0) return LPM_E_ARG;
+    if (!c || !groups) return LPM_E_ARG;
     const int need = load_type == LPM_LOAD_DOGBONE_DISP ? 2 : 3;
-    if (ngroups < need || load_type < 0 || load_type > 3) return lpm_fail(c, LPM_E_ARG, "load template needs more groups");
-    for (int k = 0; k < need; ++k) RC(check_group(c, groups[k]));
-    const double n_cycles = 0.5 * (nspec - 1);
-    double t = 0, dNdt = 1;
-    int N = 0, ran = 0;
-    do {
-        if (2 * (int64_t)N + 2 >= nspec) return lpm_fail(c, LPM_E_ARG, "load spectrum too short for the cycle count");
-        const double f0 = spec[2 * N], f1 = spec[2 * N + 1], f2 = spec[2 * N + 2];
-        std::vector<OwnedStep> cyc;
-        for (int k = 0; k < n_interval; ++k) cyc.push_back(fatigue_template(load_type, (f1 - f0) / n_interval, groups));
-        for (int k = 0; k < n_interval; ++k) cyc.push_back(fatigue_template(load_type, (f2 - f1) / n_interval, groups));
-        dNdt = (tmode == LPM_TIME_LINEAR) ? (int)(1 / tau) : (int)(1 / tau * std::exp(t++ / tau));
-        // SolverFatigue::solveProblemOneCycle (solver_fatigue.h:220-241)
-        for (auto &s : cyc) {
-            int ok = 0;
-            lpm_load_step v = s.view();
-            RC(solve_step_fatigue(c, &v, &ok));
-            RC(launch_state_var(c));
-        }
-        int any = 0;
-        RC(launch_fatigue_damage(c, dNdt, undamaged, &any));
-        RC(store_state(c));
-        RC(geometry_force_state(c));
-        N += (int)dNdt;
-        ++ran;
-        if (max_cycles_run > 0 && ran >= max_cycles_run) break;
-    } while (N < n_cycles);
-    LPM_HIP(c, hipStreamSynchronize(c->stream));
-    if (cycles) *cycles = N;
-    return LPM_OK;
+    if (ngroups < need) return lpm_fail(c, LPM_E_ARG, "load template needs more groups");
+    const double one = 1.0;
+    return solve_cyclic(c, load_type, n_interval, spec, nspec, groups, &one, 1, tmode, tau, undamaged, max_cycles_run,
+                        cycles);
+}
+
+int lpm_solve_problem_cyclic_specimens(lpm_ctx *c, int32_t load_type, int32_t n_interval, const double *spec,
+                                       int64_t nspec, const int32_t *groups, const double *scale, int32_t tmode,
+                                       double tau, int32_t undamaged, int32_t max_cycles_run, int32_t *cycles) {
+    if (!c || !groups) return LPM_E_ARG;
+    if (c->nspec <= 0) return lpm_fail(c, LPM_E_STATE, "no specimens declared (lpm_set_specimens)");
+    std::vector<double> ones(c->nspec, 1.0);
+    return solve_cyclic(c, load_type, n_interval, spec, nspec, groups, scale ? scale : ones.data(), c->nspec, tmode, tau,
+                        undamaged, max_cycles_run, cycles);
 }
 
 double lpm_stat(const lpm_ctx *c, const char *name) {

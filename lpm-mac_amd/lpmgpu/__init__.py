@@ -85,6 +85,8 @@ _SIGS = {
     "lpm_force_jvp": (_I, [_P, _DP, _DP]),
     "lpm_set_specimens": (_I, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "lpm_specimen_stat": (_I, [_P, _S, _DP]),
+    "lpm_solve_problem_cyclic_specimens": (_I, [_P, ctypes.c_int32, ctypes.c_int32, _DP, _L, _IP, _DP, ctypes.c_int32,
+                                                _D, ctypes.c_int32, ctypes.c_int32, _IP]),
     "lpm_adjoint_reaction": (_I, [_P, ctypes.c_int32, ctypes.c_int32, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
     "lpm_stream": (_P, [_P]),
 }
@@ -401,6 +403,18 @@ class Context:
         out = ctypes.c_int32(0)
         self._chk(lib().lpm_solve_problem_cyclic(self.h, ltype, n_interval, _dp(spec), len(spec), _ip(g), len(g), tmode,
                                                  tau, undamaged, max_cycles_run, ctypes.byref(out)))
+        return out.value
+
+    def solve_problem_cyclic_specimens(self, ltype, n_interval, spectrum, groups, scale, tau, undamaged, tmode=0,
+                                       max_cycles_run=0):
+        """groups: per specimen, the template's groups (id arrays); scale: per-specimen amplitude factors."""
+        spec = np.ascontiguousarray(spectrum, dtype=np.float64)
+        g = np.array([self.group(x) for gs in groups for x in gs], dtype=np.int32)
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        out = ctypes.c_int32(0)
+        self._chk(lib().lpm_solve_problem_cyclic_specimens(self.h, ltype, n_interval, _dp(spec), len(spec), _ip(g),
+                                                           _dp(sc), tmode, tau, undamaged, max_cycles_run,
+                                                           ctypes.byref(out)))
         return out.value
 
     # ---- stats

@@ -80,3 +80,68 @@ def test_hex_ensemble_matches_single_specimen_oracle(precond, precision):
         err = rel(u_g, u_o)
         say(f"   specimen {s} ({len(x)} particles): newton {its[s]:.0f} disp rel err {err:.3e}")
         assert err < 1e-8
+
+
+def test_hcf_ensemble_cycles_match_single_specimen_oracle():
+    """Three HCF specimens of different sizes, each with its own fatigue parameters (A, B, d) and
+    force amplitude, through a fatigue static step and three load cycles (R = 0.1) in one
+    context (lpm_solve_problem_cyclic_specimens). Each specimen's damage and HCF state must
+    match the oracle run on that specimen alone."""
+    say("[gpu] HCF ensemble cycles")
+    cases = [((10, 16), (5.206e-5, 2.019, 0.5), 400.0), ((14, 12), (8e-5, 2.019, 0.5), 500.0),
+             ((8, 20), (3e-5, 2.1, 0.45), 300.0)]
+    parts, per = [], []
+    for s, ((nx, ny), abd, fmax) in enumerate(cases):
+        x, box = hex_plate(nx, ny)
+        parts.append((x, G.build_bonds(G.HEXAGON2D, 0.5, x)))
+        left = np.nonzero(x[:, 0] < x[:, 0].min() + 0.5)[0]
+        y = x[:, 1]
+        bottom, top = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
+        t = np.zeros(len(x), dtype=np.int32)
+        t[top] = 4
+        t[bottom] = 4
+        per.append((x, box, abd, fmax, left, bottom, top, t))
+    xyz, bonds, first = _union(parts)
+    g = G.Context(G.HEXAGON2D, 0.5, G.HCF, xyz, bonds=bonds)
+    g.set_specimens(first)
+    g.set("type", np.concatenate([p[7] for p in per]))
+    g.set_hcf(4.0, True, 110e3, 0.34, 5.206e-5, 2.019, 0.5, elmap_literal=0)
+    mat = g.get("mat").reshape(-1, 10)  # per-specimen fatigue parameters (A, B, d) per particle
+    for s, p in enumerate(per):
+        mat[first[s]:first[s + 1], 0:3] = p[2]
+    g.set("mat", mat)
+    g.search_nonlocal(1.5)
+    g.mode(precision=1, cg_tol=1e-12, tol_NR=1e-10, max_NR_iter=60, cg_maxit=5000, cg_transpose=0, diag10_literal=0)
+    g.update_geometry()
+    g.update_force_state()
+    disp, force, groups = [], [], []
+    for s, p in enumerate(per):
+        off = first[s]
+        left, bottom, top = p[4] + off, p[5] + off, p[6] + off
+        disp += [(left, "x", 0.0), (bottom, "y", 0.0)]
+        force += [(top, 0.0, 0.1 * p[3], 0.0)]
+        groups.append([left, top, bottom])
+    g.solve_problem_fatigue_static([g.load_step(disp=disp, force=force)])
+    spec = [0.1] + [1.0, 0.1] * 3
+    Ng = g.solve_problem_cyclic_specimens(G.LOAD_UNIAXIAL_FORCE, 1, spec, groups, [p[3] for p in per], 1.0, 4)
+    assert Ng == 3
+    Dg = g.get("damage")
+    svg = g.get("state_var").reshape(-1, 7)[:, :3]
+    for s, (x, box, abd, fmax, left, bottom, top, t) in enumerate(per):
+        o = O.System(O.HEXAGON2D, 0.5, O.HCF, x, box)
+        o.mode(elmap_literal=0, cg_tol=1e-13, tol_NR=1e-10, max_NR_iter=60, cg_transpose=0, diag10_literal=0)
+        o.seti("type", t)
+        o.set_hcf(4.0, True, 110e3, 0.34, *abd)
+        o.search_nonlocal(1.5)
+        o.update_geometry()
+        o.update_force_state()
+        o.solve_problem_fatigue_static([O.LoadStep().add_disp(left, "x", 0.0).add_disp(bottom, "y", 0.0)
+                                        .add_force(top, 0.0, 0.1 * fmax, 0.0)])
+        No = o.solve_problem_cyclic(1, 1, [v * fmax for v in spec], [left, top, bottom], 1.0, 4)
+        assert No == 3
+        Do = o.getd("damage")
+        sl = slice(first[s], first[s + 1])
+        say(f"   specimen {s} ({len(x)} particles): damage max {Do.max():.4e} diff {np.abs(Dg[sl] - Do).max():.3e}")
+        assert Do.max() > 0
+        np.testing.assert_allclose(Dg[sl], Do, rtol=1e-7, atol=1e-15)
+        np.testing.assert_allclose(svg[sl], o.getd("state_var").reshape(-1, 7)[:, :3], rtol=1e-7, atol=1e-12)
