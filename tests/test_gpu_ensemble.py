@@ -145,3 +145,54 @@ def test_hcf_ensemble_cycles_match_single_specimen_oracle():
         assert Do.max() > 0
         np.testing.assert_allclose(Dg[sl], Do, rtol=1e-7, atol=1e-15)
         np.testing.assert_allclose(svg[sl], o.getd("state_var").reshape(-1, 7)[:, :3], rtol=1e-7, atol=1e-12)
+
+
+def test_batched_adjoint_matches_single_specimen_adjoints():
+    """One adjoint solve on an ensemble, with the group holding every specimen's top row, gives
+    F = sum of the specimens' reactions; no bond joins two specimens, so the gradient on specimen
+    s's bonds is d(F_s)/d(Kn, Tv) of that specimen alone. It must match the adjoint of the
+    specimen's own context (itself checked against oracle finite differences elsewhere)."""
+    say("[gpu] batched adjoint")
+    sizes = [(12, 10), (20, 14), (9, 16)]
+    pulls = [0.001, 0.0015, 0.002]
+    parts, cases = [], []
+    for (nx, ny), pull in zip(sizes, pulls):
+        x, box = hex_plate(nx, ny)
+        parts.append((x, G.build_bonds(G.HEXAGON2D, 0.5, x)))
+        cases.append((x, pull))
+    xyz, bonds, first = _union(parts)
+
+    def run(ctx, specs):
+        ctx.set_elastic(0.0, True, 69e3, 0.3, 1.0, elmap_literal=0)
+        ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=1, cg_tol=1e-12, tol_NR=1e-11,
+                 max_NR_iter=50, cg_maxit=50000)
+        disp, tops = [], []
+        for off, x, pull in specs:
+            y = x[:, 1]
+            ids = np.arange(off, off + len(x))
+            lo, hi = ids[y < y.min() + 1e-6], ids[y > y.max() - 1e-6]
+            disp += [(lo, "x", 0.0), (lo, "y", 0.0), (hi, "y", pull)]
+            tops.append(hi)
+        ls = ctx.load_step(disp=disp)
+        ctx.update_geometry()
+        ctx.update_force_state()
+        for _ in range(2):
+            assert ctx.solve_step_static(ls, -1)
+        return ctx.adjoint_reaction(ctx.group(np.concatenate(tops)), "y")
+
+    g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=bonds)
+    g.set_specimens(first)
+    F, gKn, gTv, nref, rres = run(g, [(first[s], x, pull) for s, (x, pull) in enumerate(cases)])
+    say("   ensemble F", F, "refinements", nref, "rel residual", rres)
+    assert rres < 1e-9
+    bptr = g.get("bond_ptr")
+    Fs = 0.0
+    for s, (x, pull) in enumerate(cases):
+        c1 = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, x)
+        F1, k1, t1, _, r1 = run(c1, [(0, x, pull)])
+        Fs += F1
+        b0, b1 = bptr[first[s]], bptr[first[s + 1]]
+        ek, et = rel(gKn[b0:b1], k1), rel(gTv[b0:b1], t1)
+        say(f"   specimen {s}: F {F1:.10e} dKn rel err {ek:.2e} dTv rel err {et:.2e}")
+        assert ek < 1e-8 and et < 1e-8
+    assert F == pytest.approx(Fs, rel=1e-10)
