@@ -72,7 +72,7 @@ def spmv_bytes(ctx):
     return 40 * nblk + 4 * (n + 1) + vec, "k_spmv (CG SpMV, full store)"
 
 
-def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0):
+def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0, predictor=0):
     import lpmgpu as G
 
     xyz, box, r, L = sc_workload(n)
@@ -81,7 +81,7 @@ def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0):
     ctx.set_elastic(0.0, False, 200e3, 0.3, 1.0, elmap_literal=0)
     ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, cg_tol=cg_tol, tol_NR=tol_nr,
              max_NR_iter=30, cg_maxit=20000, precond=precond, spmv_sym=spmv_sym, cg_forcing=cg_forcing,
-             coarse_cells=coarse_cells)
+             coarse_cells=coarse_cells, predictor=predictor)
     z = xyz[:, 2]
     bottom = np.nonzero(z < z.min() + 1e-6)[0]
     top = np.nonzero(z > z.max() - 1e-6)[0]
@@ -165,6 +165,9 @@ def main():
                          "3 block Jacobi + rigid-body coarse correction")
     ap.add_argument("--spmv-sym", type=int, default=0, help="1: half store when K is symmetric, 0: full store")
     ap.add_argument("--coarse-cells", type=int, default=0, help="aggregate edge of the two-level preconditioner (0: auto)")
+    ap.add_argument("--predictor", type=int, default=1,
+                    help="1: start each static step from the previous step's converged increment (Newton runs to the "
+                         "same test); 0: the reference's start")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--ensemble", type=int, default=0,
@@ -193,7 +196,7 @@ def main():
         ensemble_bench(args)
         return
     ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond, args.spmv_sym, args.cg_forcing,
-                                      args.coarse_cells)
+                                      args.coarse_cells, args.predictor)
     if args.verbose:
         print(f"setup {t_setup:.2f} s, n={ctx.n} bonds={ctx.size('nbonds')} blocks={ctx.size('nconn')}", flush=True)
     # warmup steps, then back to the initial state so that the timed steps are steps 1..K
@@ -230,25 +233,31 @@ def main():
     ncoarse = ctx.get_mode("ncoarse")
     # equilibrium check of the last step (size-independent): residual and reaction balance
     nr, nR = ctx.update_rr()
-    # the same K steps once more with the Jacobi preconditioner (the reference's): its iteration
-    # counts drive the CPU projection, and its rate is reported beside the headline
-    jac = None
-    if args.precond != 0 and not args.no_cpu_baseline:
+    def rerun(**modes):
         reset_state(ctx, xyz)
         ctx.reset_stats()
-        ctx.mode(precond=0)
+        ctx.mode(**modes)
         hip_sync(ctx)
-        tj = time.perf_counter()
+        tr = time.perf_counter()
         for _ in range(K):
             if not ctx.solve_step_static(ls, -1):
-                raise RuntimeError("Jacobi reference run did not converge")
+                raise RuntimeError(f"comparison run {modes} did not converge")
         hip_sync(ctx)
-        tj = time.perf_counter() - tj
-        jac = {"load_steps_per_s": K / tj, "newton_iters_per_step": ctx.stat("newton_iters") / K,
-               "cg_iters_per_step": ctx.stat("cg_iters") / K}
-        jac_counts = (ctx.stat("newton_iters") / K, ctx.stat("cg_iters") / K, ctx.stat("assemblies") / K)
-    else:
-        jac_counts = counts
+        tr = time.perf_counter() - tr
+        return ({"load_steps_per_s": K / tr, "newton_iters_per_step": ctx.stat("newton_iters") / K,
+                 "cg_iters_per_step": ctx.stat("cg_iters") / K},
+                (ctx.stat("newton_iters") / K, ctx.stat("cg_iters") / K, ctx.stat("assemblies") / K))
+
+    # The same K steps again: without the predictor (the reference's Newton start), and with the
+    # reference's algorithm (Jacobi, no predictor), whose iteration counts drive the CPU
+    # projection; both rates are reported beside the headline.
+    jac = nopred = None
+    jac_counts = counts
+    if not args.no_cpu_baseline:
+        if args.predictor:
+            nopred, _ = rerun(predictor=0)
+        if args.precond != 0 or args.predictor:
+            jac, jac_counts = rerun(precond=0, predictor=0)
     out = {
         "metric": "load steps/s + CG SpMV HBM GB/s (% peak), 3D SC 64^3, FP32-mixed vs FP64",
         "value": K / t,
@@ -266,6 +275,7 @@ def main():
             "workload": f"SC {args.n}^3 elastic uniaxial tension, {ctx.n} particles, 0.5% strain in 20 steps",
             "particles": ctx.n, "bonds": ctx.size("nbonds"), "bsr_blocks": ctx.size("nconn"),
             "newton_tol": args.tol_nr, "cg_rtol": args.cg_tol, "cg_forcing": args.cg_forcing,
+            "predictor": "previous step's converged increment" if args.predictor else "none (the reference's start)",
             "precond": {0: "jacobi", 1: "block-jacobi", 2: "two-level (jacobi + rigid-body aggregate coarse space)",
                         3: "two-level (3x3 block jacobi + rigid-body aggregate coarse space)"}[args.precond],
             "coarse_dofs": int(ncoarse),
@@ -277,7 +287,8 @@ def main():
             "newton_iters_per_step": counts[0], "cg_iters_per_step": counts[1],
             "final_residual_over_reaction": nr / nR if nR > 0 else None,
             "parallelism": "single GPU",
-            "same_steps_with_jacobi_preconditioner": jac,
+            "same_steps_without_predictor": nopred,
+            "same_steps_reference_algorithm_jacobi_no_predictor": jac,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -316,7 +327,8 @@ def ensemble_bench(args):
         ctx = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, np.vstack(xyzs), bonds=bonds)
         ctx.set_elastic(0.0, True, 110e3, 0.34, 1.0, elmap_literal=0)
         ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond,
-                 cg_tol=args.cg_tol, tol_NR=args.tol_nr, max_NR_iter=30, cg_maxit=50000)
+                 cg_tol=args.cg_tol, cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=30, cg_maxit=50000,
+                 predictor=args.predictor)
         return ctx
 
     def bcs(xyzs, offs, pulls_):

@@ -122,6 +122,13 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        aggregates of particles (two-level; iteration count no longer grows
  *                        with the lattice width) | 3: as 2 with 3x3 block Jacobi in place of
  *                        Jacobi. Changes the CG's rate, not the equilibrium.
+ *  "predictor"        1: a static step starts its free DoFs from the converged positions plus the
+ *                        previous converged step's increment (after the step's boundary
+ *                        conditions); Newton then runs to the same test. 0 (default): the
+ *                        reference's start, the converged positions. Reset by a halving restart
+ *                        and by uploading positions.
+ *  "coarse_graph"     1: replay the coarse inversion's launches from a HIP graph captured once |
+ *                        0 (default): launch them directly
  *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 and 3 (0: smallest of 8, 9,
  *                        10, ... giving at most 768 coarse DoFs); get_mode("ncoarse") reads the
  *                        coarse dimension once built

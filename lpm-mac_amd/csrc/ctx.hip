@@ -46,6 +46,18 @@ __global__ void k_disp_bc(int64_t m, const int32_t *__restrict__ ids, int serial
 }
 
 // Solver::updateForceBC (solver.h:168-190) for one ForceBC
+// Load-step predictor: every DoF that the step does not prescribe starts from its converged
+// position plus the previous step's converged increment (xyz_last - xyz_prev); xyz_prev then
+// takes the converged position the increment of the next step will be measured from.
+__global__ void k_predict(int64_t n3, const int32_t *__restrict__ cons, const double *__restrict__ xyz_last,
+                          double *__restrict__ xyz_prev, double *__restrict__ xyz, int apply) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n3) return;
+    const double last = xyz_last[q];
+    if (apply && cons[q] != 1) xyz[q] = xyz[q] + (last - xyz_prev[q]);
+    xyz_prev[q] = last;
+}
+
 __global__ void k_force_bc(int64_t m, const int32_t *__restrict__ ids, int serial, int mode, double fx, double fy,
                            double fz, double *__restrict__ Pex) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -361,6 +373,7 @@ int store_state(lpm_ctx *c) {
 
 int reset_state(lpm_ctx *c, bool reset_xyz) {
     const size_t n = c->n, nb = c->nbond, D = sizeof(double);
+    c->pred_valid = false;  // the step is re-run (halved): no increment to extrapolate
     if (reset_xyz) RC(copy_d2d(c, c->xyz, c->xyz_last, 3 * n * D));
     RC(copy_d2d(c, c->damage, c->damage_last, n * D));
     RC(copy_d2d(c, c->sv, c->sv_last, LPM_NSV * n * D));
@@ -372,8 +385,26 @@ int reset_state(lpm_ctx *c, bool reset_xyz) {
 }
 
 // SolverStatic::solveProblemStep (solver_static.h:97-149)
+// predictor (mode "predictor", static steps only: fatigue substeps alternate loading and
+// unloading): after the step's boundary conditions, move the free DoFs by the previous converged
+// step's increment; Newton then corrects from there to the same test
+int predict(lpm_ctx *c) {
+    if (!c->predictor) return LPM_OK;
+    const int64_t n3 = 3 * c->n;
+    if (!c->xyz_prev) {
+        LPM_HIP(c, hipMalloc(&c->xyz_prev, sizeof(double) * n3));
+        c->pred_valid = false;
+    }
+    hipLaunchKernelGGL(k_predict, dim3(blocks_for(n3, 256)), dim3(256), 0, c->stream, n3, (const int32_t *)c->cons,
+                       (const double *)c->xyz_last, c->xyz_prev, c->xyz, c->pred_valid ? 1 : 0);
+    LPM_HIP(c, hipGetLastError());
+    c->pred_valid = true;
+    return LPM_OK;
+}
+
 int solve_step_static(lpm_ctx *c, const lpm_load_step *s, int undamaged, int *converged) {
     RC(apply_bc(c, s));
+    RC(predict(c));
     int n_newton = 0, any = 0;
     bool new_damaged = false;
     do {
@@ -673,7 +704,7 @@ int lpm_destroy(lpm_ctx *c) {
                     c->Tv, c->cs, c->bforce, c->bforce_last, c->bdamage, c->bdamage_last, c->residual, c->disp, c->diag64,
                     c->aval, c->invdiag, c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w, c->scal, c->partial, c->counter, c->iflag, c->sp, c->scol,
                     c->dslot, c->sp_u, c->sp_l, c->ucol, c->lcol, c->lpos, c->uval,
-                    c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part};
+                    c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part, c->xyz_prev};
     for (void *p : ptrs)
         if (p) hipFree(p);
     coarse_free(c);
@@ -729,6 +760,11 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "verbose") c->verbose = (int)v;
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
     else if (w == "cg_forcing") c->cg_forcing = (int)v ? 1 : 0;
+    else if (w == "coarse_graph") c->coarse_graph = (int)v ? 1 : 0;
+    else if (w == "predictor") {
+        c->predictor = (int)v ? 1 : 0;
+        c->pred_valid = false;
+    }
     else if (w == "spmv_tune") {
         c->spmv_tune = (int)v ? 1 : 0;
         c->tune_key = -1;
@@ -764,6 +800,8 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "spmv_sym") return c->spmv_sym;
     if (w == "spmv_tune") return c->spmv_tune;
     if (w == "cg_forcing") return c->cg_forcing;
+    if (w == "coarse_graph") return c->coarse_graph;
+    if (w == "predictor") return c->predictor;
     if (w == "coarse_cells") return c->coarse_cells;
     if (w == "ncoarse") return c->ncoarse;
     if (w == "spmv_unr") return c->spmv_unr;
@@ -784,7 +822,7 @@ int lpm_upload(lpm_ctx *c, const char *name, const void *host, int64_t count) {
     if (count == 0) return LPM_OK;
     LPM_HIP(c, hipMemcpyAsync(f.p, host, f.es * count, hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipStreamSynchronize(c->stream));
-    if (w == "disp_constraint" || w == "mat") c->k_ready = c->k_ready;  // no derived data to invalidate
+    if (w == "xyz" || w == "xyz_last") c->pred_valid = false;  // positions set by hand: nothing to extrapolate
     return LPM_OK;
 }
 

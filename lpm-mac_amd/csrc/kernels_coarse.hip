@@ -590,12 +590,7 @@ int coarse_prepare(lpm_ctx *c) {
     const int nsteps = (nbmax + GJ_NB - 1) / GJ_NB, T = nsteps;  // T x T tiles of 32 cover the largest block
     // every group's block in the same launches (grid y = group, x over the largest block)
     const dim3 gsq(blocks_for((int64_t)nbmax * nbmax, 256), G);
-    // the block steps are small launches (a few microseconds each): issued one by one, the host's
-    // launch rate would leave the GPU idle between them, so the whole inversion is captured once
-    // into a graph
-    if (!c->gj_graph && nbmax > 0) {
-        hipGraph_t graph = nullptr;
-        LPM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    auto issue = [&]() {
         hipLaunchKernelGGL(k_coarse_sym, gsq, dim3(256), 0, c->stream, (const int32_t *)c->cgrp_cb,
                            (const int64_t *)c->cgrp_off, A0);
         hipLaunchKernelGGL(k_coarse_dead, dim3(blocks_for(nc, 4)), dim3(256), 0, c->stream, nc, M,
@@ -616,20 +611,33 @@ int coarse_prepare(lpm_ctx *c) {
         }
         hipLaunchKernelGGL(k_coarse_undead, gsq, dim3(256), 0, c->stream, (const int32_t *)c->cgrp_cb,
                            (const int64_t *)c->cgrp_off, (nsteps & 1) ? A1 : A0, (const uint8_t *)dead);
-        const hipError_t le = hipGetLastError();
-        const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
-        if (le != hipSuccess || ce != hipSuccess) {
-            if (graph) hipGraphDestroy(graph);
-            return lpm_fail(c, LPM_E_HIP, std::string("inversion graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+    };
+    if (nbmax > 0 && !c->coarse_graph) {
+        // issued one by one: each step's kernels run for longer than the host takes to launch them
+        issue();
+        if ((rc = ck(c, "coarse inversion"))) return rc;
+    } else if (nbmax > 0) {
+        // captured once into a graph (for block steps too short to keep up with host launches)
+        if (!c->gj_graph) {
+            hipGraph_t graph = nullptr;
+            LPM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            issue();
+            const hipError_t le = hipGetLastError();
+            const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (graph) hipGraphDestroy(graph);
+                return lpm_fail(c, LPM_E_HIP,
+                                std::string("inversion graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+            }
+            const hipError_t ie = hipGraphInstantiate(&c->gj_graph, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                c->gj_graph = nullptr;
+                return lpm_fail(c, LPM_E_HIP, std::string("inversion graph: ") + hipGetErrorString(ie));
+            }
         }
-        const hipError_t ie = hipGraphInstantiate(&c->gj_graph, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
-        if (ie != hipSuccess) {
-            c->gj_graph = nullptr;
-            return lpm_fail(c, LPM_E_HIP, std::string("inversion graph: ") + hipGetErrorString(ie));
-        }
+        LPM_HIP(c, hipGraphLaunch(c->gj_graph, c->stream));
     }
-    if (c->gj_graph) LPM_HIP(c, hipGraphLaunch(c->gj_graph, c->stream));
     c->Ainv = (nsteps & 1) ? A1 : A0;
     c->coarse_version = c->k_version;
     return LPM_OK;

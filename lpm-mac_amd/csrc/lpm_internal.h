@@ -102,7 +102,13 @@ struct lpm_ctx {
     double *Ainv = nullptr;           // the buffer of Ac that holds Ac^-1 (inside Ac)
     double *cz = nullptr, *cy = nullptr;  // restriction partials (nagg * coarse_S * nmode) / correction
     int coarse_S = 1;                 // restriction workgroups per aggregate
-    hipGraphExec_t gj_graph = nullptr;  // the ncoarse pivot launches of the inversion, captured once
+    hipGraphExec_t gj_graph = nullptr;  // the inversion's launches, captured once (coarse_graph = 1)
+    int coarse_graph = 0;             // 1: replay the inversion from a HIP graph, 0: launch directly
+    // Load-step predictor (mode "predictor"): free DoFs start each step from the converged
+    // position plus the previous step's converged increment
+    int predictor = 0;
+    bool pred_valid = false;          // xyz_prev holds the converged positions of the step before
+    double *xyz_prev = nullptr;
     uint8_t *dofmask = nullptr;       // per particle: bit k set when DoF k is eliminated
     int64_t coarse_version = -1;
     // Specimen ensemble: independent particle systems (contiguous id ranges, no bond between

@@ -275,32 +275,55 @@ def test_static_step_equilibrium_parity(precision):
     np.testing.assert_array_equal(p.g.get("xyz").reshape(-1, 3)[bottom], xyz[bottom])
 
 
-@pytest.mark.parametrize("forcing,precond", [(0, 0), (1, 0), (1, 2)])
-def test_headline_settings_equilibrium_parity(forcing, precond):
+_HEADLINE_ORACLE = {}
+
+
+def _headline_oracle_trajectory():
+    """The FP64 oracle's displacements and reactions after each of 5 load steps of the 12^3 bar
+    (reference start, solved tightly); computed once and shared by the settings below."""
+    if not _HEADLINE_ORACLE:
+        p, xyz, bottom, top = _bar(n=12)
+        L = xyz[:, 2].max() - xyz[:, 2].min()
+        disp = [(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", 0.005 * L / 20)]
+        p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=50)
+        p.o.update_geometry()
+        p.o.update_force_state()
+        traj = []
+        for _ in range(5):
+            ls_o, _ = p.load(disp=disp)
+            assert p.o.solve_step_static(ls_o, -1)
+            traj.append((p.o.getd("xyz") - xyz.ravel(), p.o.getd("reaction")))
+        _HEADLINE_ORACLE["traj"] = traj
+    return _HEADLINE_ORACLE["traj"]
+
+
+@pytest.mark.parametrize("forcing,precond,predictor", [(0, 0, 0), (1, 0, 0), (1, 2, 0), (1, 2, 1), (1, 0, 1)])
+def test_headline_settings_equilibrium_parity(forcing, precond, predictor):
     """The headline benchmark's solver settings (FP32 matrix/SpMV/CG with rtol 1e-3, optional
-    inexact-Newton forcing, Jacobi or two-level preconditioner, Newton tol 1e-10) on a 12^3 SC
-    bar in uniaxial tension, 3 load steps: displacements and reactions within 1e-8 of the FP64
-    oracle solved tightly."""
-    say("[gpu] headline settings forcing", forcing, "precond", precond)
-    p, xyz, bottom, top = _bar(n=12)
+    inexact-Newton forcing, Jacobi or two-level preconditioner, optional load-step predictor,
+    Newton tol 1e-10) on a 12^3 SC bar in uniaxial tension, 5 load steps: displacements and
+    reactions within 1e-8 of the FP64 oracle (reference start, no predictor) solved tightly."""
+    say("[gpu] headline settings forcing", forcing, "precond", precond, "predictor", predictor)
+    traj = _headline_oracle_trajectory()
+    xyz, box = sc_cube(12)
+    bottom, top = faces(xyz, 2)
     L = xyz[:, 2].max() - xyz[:, 2].min()
-    disp = [(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", 0.005 * L / 20)]
-    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=50)
-    p.g.mode(dirichlet_static=1, cg_tol=1e-3, cg_forcing=forcing, tol_NR=1e-10, max_NR_iter=30, precision=0,
-             cg_transpose=0, diag10_literal=0, cg_maxit=20000, precond=precond, coarse_cells=4 if precond == 2 else 0)
-    for s in (p.o, p.g):
-        s.update_geometry()
-        s.update_force_state()
-    for k in range(3):
-        ls_o, ls_g = p.load(disp=disp)
-        assert p.o.solve_step_static(ls_o, -1)
-        assert p.g.solve_step_static(ls_g, -1)
-        u_o = p.o.getd("xyz") - xyz.ravel()
-        u_g = p.g.get("xyz") - xyz.ravel()
-        say("   step", k, "disp rel err", rel(u_g, u_o), "reaction rel err",
-            rel(p.g.get("reaction"), p.o.getd("reaction")), "cg", p.g.stat("cg_iters"))
+    g = G.Context(G.SC3D, 0.5, G.ELASTIC, xyz)
+    g.set_elastic(0.0, False, 200e3, 0.3, 1.0, elmap_literal=0)
+    g.mode(dirichlet_static=1, cg_tol=1e-3, cg_forcing=forcing, tol_NR=1e-10, max_NR_iter=30, precision=0,
+           cg_transpose=0, diag10_literal=0, cg_maxit=20000, precond=precond, coarse_cells=4 if precond == 2 else 0,
+           predictor=predictor)
+    ls = g.load_step(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", 0.005 * L / 20)])
+    g.update_geometry()
+    g.update_force_state()
+    for k, (u_o, R_o) in enumerate(traj):
+        g.reset_stats()
+        assert g.solve_step_static(ls, -1)
+        u_g = g.get("xyz") - xyz.ravel()
+        say("   step", k, "disp rel err", rel(u_g, u_o), "reaction rel err", rel(g.get("reaction"), R_o),
+            "newton", g.stat("newton_iters"), "cg", g.stat("cg_iters"))
         assert rel(u_g, u_o) < 1e-8
-        assert rel(p.g.get("reaction"), p.o.getd("reaction")) < 1e-8
+        assert rel(g.get("reaction"), R_o) < 1e-8
 
 
 @pytest.mark.parametrize("lattice", ["sc", "hex"])
