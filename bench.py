@@ -77,7 +77,7 @@ def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0, 
 
     xyz, box, r, L = sc_workload(n)
     t0 = time.time()
-    ctx = G.Context(G.SC3D, r, G.ELASTIC, xyz)
+    ctx = G.Context(G.SC3D, r, G.ELASTIC, xyz, bonds="device")  # bond search on the device
     ctx.set_elastic(0.0, False, 200e3, 0.3, 1.0, elmap_literal=0)
     ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, cg_tol=cg_tol, tol_NR=tol_nr,
              max_NR_iter=30, cg_maxit=20000, precond=precond, spmv_sym=spmv_sym, cg_forcing=cg_forcing,
@@ -287,6 +287,7 @@ def main():
             "newton_iters_per_step": counts[0], "cg_iters_per_step": counts[1],
             "final_residual_over_reaction": nr / nR if nR > 0 else None,
             "parallelism": "single GPU",
+            "setup_seconds": t_setup,
             "same_steps_without_predictor": nopred,
             "same_steps_reference_algorithm_jacobi_no_predictor": jac,
         },

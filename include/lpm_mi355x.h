@@ -90,12 +90,18 @@ typedef struct {
     int32_t model;             /* LPM_ELASTIC .. LPM_FATIGUE_HCF (Assembly ptype) */
     int64_t n;                 /* particles */
     const double *xyz;         /* [3n] initial positions (also xyz_initial / xyz_last) */
-    const int32_t *bond_ptr;   /* [n+1] */
+    const int32_t *bond_ptr;   /* [n+1], or NULL: search the bonds on the device (see lpm_create) */
     const int32_t *bond_split; /* [n] first layer-1 bond of each particle */
     const int32_t *bond_p2;    /* [bond_ptr[n]] partner ids */
     int32_t device;            /* HIP device ordinal */
 } lpm_topology;
 
+/* With bond lists: validated on the host (Assembly(xyz, bonds), assembly.h:88-106 with
+ * readBond). With bond_ptr == NULL: Assembly::createBonds (assembly.h:212-234) runs on the
+ * device with the same criterion, layer test and order as lpm_build_bonds (a cell grid, each
+ * particle's lists sorted by partner id), followed by the reverse bonds and the initial bond
+ * geometry, also on the device. The device search bonds every pair within the cutoff, so
+ * ensembles whose specimens overlap in space pass their own lists. */
 int lpm_create(const lpm_topology *topo, lpm_ctx **out);
 int lpm_destroy(lpm_ctx *ctx);
 const char *lpm_last_error(const lpm_ctx *ctx); /* ctx may be NULL: last error of lpm_create */

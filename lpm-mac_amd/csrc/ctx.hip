@@ -565,7 +565,9 @@ const char *lpm_last_error(const lpm_ctx *c) { return c ? c->err.c_str() : g_cre
 int lpm_create(const lpm_topology *t, lpm_ctx **out) {
     if (!t || !out) return lpm_fail(nullptr, LPM_E_ARG, "null argument");
     *out = nullptr;
-    if (t->n <= 0 || !t->xyz || !t->bond_ptr || !t->bond_split || (!t->bond_p2 && t->bond_ptr[t->n] > 0))
+    // bond_ptr == NULL: search the bonds on the device; else the caller's lists, validated here
+    const bool search = t->bond_ptr == nullptr;
+    if (t->n <= 0 || !t->xyz || (!search && (!t->bond_split || (!t->bond_p2 && t->bond_ptr[t->n] > 0))))
         return lpm_fail(nullptr, LPM_E_ARG, "empty or incomplete topology");
     if (t->model < 0 || t->model > 3) return lpm_fail(nullptr, LPM_E_ARG, "unknown particle model");
     if (t->n > 0x7ffffffeLL) return lpm_fail(nullptr, LPM_E_LIMIT, "too many particles");
@@ -573,29 +575,34 @@ int lpm_create(const lpm_topology *t, lpm_ctx **out) {
     if (lpm_unit_cell(t->lattice, t->radius, 0, uc, nullptr) != LPM_OK) return lpm_fail(nullptr, LPM_E_ARG, "unknown lattice");
     const int64_t n = t->n;
     const int32_t *bp = t->bond_ptr, *bs = t->bond_split, *p2 = t->bond_p2;
-    // ---- host validation of the topology (every index a kernel will dereference)
-    if (bp[0] != 0) return lpm_fail(nullptr, LPM_E_TOPO, "bond_ptr[0] must be 0");
     int maxb = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        if (bp[i + 1] < bp[i] || bs[i] < bp[i] || bs[i] > bp[i + 1])
-            return lpm_fail(nullptr, LPM_E_TOPO, "bond_ptr / bond_split not monotone at particle " + std::to_string(i));
-        maxb = std::max(maxb, bp[i + 1] - bp[i]);
-    }
-    if (maxb > LPM_MAXB) return lpm_fail(nullptr, LPM_E_LIMIT, "a particle has more than 32 bonds");
-    const int64_t nbond = bp[n];
-    std::vector<int32_t> opp(nbond, -1);
-    for (int64_t i = 0; i < n; ++i) {
-        for (int b = bp[i]; b < bp[i + 1]; ++b) {
-            const int64_t j = p2[b];
-            if (j < 0 || j >= n || j == i) return lpm_fail(nullptr, LPM_E_TOPO, "bond partner out of range or self bond");
-            const bool l0 = b < bs[i];
-            for (int b2 = (l0 ? bp[i] : bs[i]); b2 < b; ++b2)
-                if (p2[b2] == j) return lpm_fail(nullptr, LPM_E_TOPO, "duplicate bond in one layer");
-            const int lo = l0 ? bp[j] : bs[j], hi = l0 ? bs[j] : bp[j + 1];
-            for (int q = lo; q < hi; ++q)
-                if (p2[q] == i) { opp[b] = q; break; }
-            if (opp[b] < 0)
-                return lpm_fail(nullptr, LPM_E_TOPO, "bond " + std::to_string(i) + "->" + std::to_string(j) + " has no reverse bond");
+    int64_t nbond = 0;
+    std::vector<int32_t> opp;
+    if (!search) {
+        // ---- host validation of the topology (every index a kernel will dereference)
+        if (bp[0] != 0) return lpm_fail(nullptr, LPM_E_TOPO, "bond_ptr[0] must be 0");
+        for (int64_t i = 0; i < n; ++i) {
+            if (bp[i + 1] < bp[i] || bs[i] < bp[i] || bs[i] > bp[i + 1])
+                return lpm_fail(nullptr, LPM_E_TOPO, "bond_ptr / bond_split not monotone at particle " + std::to_string(i));
+            maxb = std::max(maxb, bp[i + 1] - bp[i]);
+        }
+        if (maxb > LPM_MAXB) return lpm_fail(nullptr, LPM_E_LIMIT, "a particle has more than 32 bonds");
+        nbond = bp[n];
+        opp.assign(nbond, -1);
+        for (int64_t i = 0; i < n; ++i) {
+            for (int b = bp[i]; b < bp[i + 1]; ++b) {
+                const int64_t j = p2[b];
+                if (j < 0 || j >= n || j == i) return lpm_fail(nullptr, LPM_E_TOPO, "bond partner out of range or self bond");
+                const bool l0 = b < bs[i];
+                for (int b2 = (l0 ? bp[i] : bs[i]); b2 < b; ++b2)
+                    if (p2[b2] == j) return lpm_fail(nullptr, LPM_E_TOPO, "duplicate bond in one layer");
+                const int lo = l0 ? bp[j] : bs[j], hi = l0 ? bs[j] : bp[j + 1];
+                for (int q = lo; q < hi; ++q)
+                    if (p2[q] == i) { opp[b] = q; break; }
+                if (opp[b] < 0)
+                    return lpm_fail(nullptr, LPM_E_TOPO,
+                                    "bond " + std::to_string(i) + "->" + std::to_string(j) + " has no reverse bond");
+            }
         }
     }
     lpm_ctx *c = new lpm_ctx();
@@ -609,19 +616,6 @@ int lpm_create(const lpm_topology *t, lpm_ctx **out) {
     c->cutoff[1] = uc[6];
     c->volume = uc[7];
     c->n = n;
-    c->nbond = nbond;
-    c->maxbond = maxb;
-    for (int64_t p0 = 0; p0 < n; p0 += 32)
-        c->gp_bonds = std::max<int>(c->gp_bonds, bp[std::min<int64_t>(n, p0 + 32)] - bp[p0]);
-    // 2-hop records of a row: for each bond i->k, the bonds of k in the same layer
-    for (int64_t i = 0; i < n; ++i) {
-        int hops = 0;
-        for (int b = bp[i]; b < bp[i + 1]; ++b) {
-            const int k = p2[b];
-            hops += (b < bs[i]) ? bs[k] - bp[k] : bp[k + 1] - bs[k];
-        }
-        c->asm_hops = std::max(c->asm_hops, hops);
-    }
     c->vpad = c->dim == 3 ? 4 : 2;
     c->precision = 0;
     c->cg_tol = 1e-6;
@@ -635,17 +629,68 @@ int lpm_create(const lpm_topology *t, lpm_ctx **out) {
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { c->err = hipGetErrorString(e); return fail(LPM_E_HIP); }
     int rc = 0;
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
+    };
+    const double *x = t->xyz;
 #define AL(ptr, cnt) if ((rc = dalloc(c, &c->ptr, (cnt)))) return fail(rc)
-    AL(bptr, n + 1); AL(bsplit, n); AL(p2, nbond); AL(opp, nbond); AL(nbarr, n);
+    // ---- per-particle arrays
     AL(xyz, 3 * n); AL(xyz0, 3 * n); AL(xyz_last, 3 * n); AL(Pin, 3 * n); AL(Pex, 3 * n); AL(stress, 6 * n);
     AL(damage, n); AL(damage_last, n); AL(damage_visual, n); AL(Ddot_local, n); AL(Ddot_nonlocal, n); AL(nonlocal_L, n);
     AL(sv, LPM_NSV * n); AL(sv_last, LPM_NSV * n); AL(mat, LPM_NMAT * n);
     AL(dLe_total, 2 * n); AL(TdLe_total, 2 * n); AL(cs_sum, 6 * n); AL(type, n); AL(cons, 3 * n);
-    AL(dis0, nbond); AL(dis, nbond); AL(dis_last, nbond); AL(dLe, nbond); AL(dLp, nbond); AL(dLp_last, nbond);
-    AL(bstrain, nbond); AL(Kn, nbond); AL(Tv, nbond); AL(cs, 3 * nbond); AL(bforce, nbond); AL(bforce_last, nbond);
-    AL(bdamage, nbond); AL(bdamage_last, nbond);
     AL(residual, c->dim * n); AL(disp, c->dim * n); AL(diag64, c->dim * n);
     AL(scal, SC_COUNT); AL(counter, 8); AL(iflag, 8);
+    if (!(up(c->xyz, x, sizeof(double) * 3 * n) && up(c->xyz0, x, sizeof(double) * 3 * n) &&
+          up(c->xyz_last, x, sizeof(double) * 3 * n))) {
+        c->err = "position upload failed";
+        return fail(LPM_E_HIP);
+    }
+    // ---- bonds: Bond(p1, p2, layer, dis) (bond.h:61-73), reverse bonds, 2-hop bookkeeping
+    if (search) {
+        if ((rc = device_topology(c, x))) return fail(rc);
+        nbond = c->nbond;
+    } else {
+        c->nbond = nbond;
+        c->maxbond = maxb;
+        for (int64_t p0 = 0; p0 < n; p0 += 32)
+            c->gp_bonds = std::max<int>(c->gp_bonds, bp[std::min<int64_t>(n, p0 + 32)] - bp[p0]);
+        // 2-hop records of a row: for each bond i->k, the bonds of k in the same layer
+        for (int64_t i = 0; i < n; ++i) {
+            int hops = 0;
+            for (int b = bp[i]; b < bp[i + 1]; ++b) {
+                const int k = p2[b];
+                hops += (b < bs[i]) ? bs[k] - bp[k] : bp[k + 1] - bs[k];
+            }
+            c->asm_hops = std::max(c->asm_hops, hops);
+        }
+        AL(bptr, n + 1); AL(bsplit, n); AL(p2, nbond); AL(opp, nbond); AL(nbarr, n);
+        AL(dis0, nbond); AL(dis, nbond); AL(dis_last, nbond); AL(cs, 3 * nbond);
+        std::vector<double> dis0(nbond), cs(3 * nbond);
+        std::vector<int32_t> nb(n);
+        for (int64_t i = 0; i < n; ++i) {
+            nb[i] = bp[i + 1] - bp[i];
+            for (int b = bp[i]; b < bp[i + 1]; ++b) {
+                const double *a = &x[3 * i], *q = &x[3 * (int64_t)p2[b]];
+                const double d = std::sqrt(std::pow((a[0] - q[0]), 2) + std::pow((a[1] - q[1]), 2) + std::pow((a[2] - q[2]), 2));
+                dis0[b] = d;
+                cs[3 * b] = (a[0] - q[0]) / d;
+                cs[3 * b + 1] = (a[1] - q[1]) / d;
+                cs[3 * b + 2] = (a[2] - q[2]) / d;
+            }
+        }
+        const bool ok = up(c->bptr, bp, sizeof(int32_t) * (n + 1)) && up(c->bsplit, bs, sizeof(int32_t) * n) &&
+                        up(c->p2, p2, sizeof(int32_t) * nbond) && up(c->opp, opp.data(), sizeof(int32_t) * nbond) &&
+                        up(c->nbarr, nb.data(), sizeof(int32_t) * n) && up(c->dis0, dis0.data(), sizeof(double) * nbond) &&
+                        up(c->dis, dis0.data(), sizeof(double) * nbond) &&
+                        up(c->dis_last, dis0.data(), sizeof(double) * nbond) &&
+                        up(c->cs, cs.data(), sizeof(double) * 3 * nbond) && hipStreamSynchronize(c->stream) == hipSuccess;
+        if (!ok) { c->err = "topology upload failed"; return fail(LPM_E_HIP); }
+    }
+    // ---- per-bond state
+    AL(dLe, nbond); AL(dLp, nbond); AL(dLp_last, nbond);
+    AL(bstrain, nbond); AL(Kn, nbond); AL(Tv, nbond); AL(bforce, nbond); AL(bforce_last, nbond);
+    AL(bdamage, nbond); AL(bdamage_last, nbond);
 #undef AL
     c->partial_cap = std::max({blocks_for(n * c->dim, 256), blocks_for(n, 4), blocks_for(n, 256), 1});
     // CG: (||r||^2, gamma) partials in two parity slots + the SpMV's delta partials
@@ -655,39 +700,17 @@ int lpm_create(const lpm_topology *t, lpm_ctx **out) {
         c->err = "hipHostMalloc failed";
         return fail(LPM_E_NOMEM);
     }
-    // ---- initial bond geometry: Bond(p1, p2, layer, dis) (bond.h:61-73)
-    std::vector<double> dis0(nbond), cs(3 * nbond), sv(LPM_NSV * n, 0.0), mat(LPM_NMAT * n, 0.0);
-    std::vector<int32_t> nb(n);
-    const double *x = t->xyz;
-    for (int64_t i = 0; i < n; ++i) {
-        nb[i] = bp[i + 1] - bp[i];
-        for (int b = bp[i]; b < bp[i + 1]; ++b) {
-            const double *a = &x[3 * i], *q = &x[3 * (int64_t)p2[b]];
-            const double d = std::sqrt(std::pow((a[0] - q[0]), 2) + std::pow((a[1] - q[1]), 2) + std::pow((a[2] - q[2]), 2));
-            dis0[b] = d;
-            cs[3 * b] = (a[0] - q[0]) / d;
-            cs[3 * b + 1] = (a[1] - q[1]) / d;
-            cs[3 * b + 2] = (a[2] - q[2]) / d;
-        }
-    }
+    std::vector<double> sv(LPM_NSV * n, 0.0), mat(LPM_NMAT * n, 0.0);
     if (c->model == LPM_FATIGUE_HCF)  // particle_fatigue_hcf.h:24-25, 229-236
         for (int64_t i = 0; i < n; ++i) {
             sv[LPM_NSV * i + 1] = (double)INT_MAX;
             mat[LPM_NMAT * i + 5] = 1.0;
             mat[LPM_NMAT * i + 6] = 1.108;
         }
-    auto up = [&](void *dst, const void *src, size_t bytes) {
-        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
-    };
-    bool ok = up(c->bptr, bp, sizeof(int32_t) * (n + 1)) && up(c->bsplit, bs, sizeof(int32_t) * n) &&
-              up(c->p2, p2, sizeof(int32_t) * nbond) && up(c->opp, opp.data(), sizeof(int32_t) * nbond) &&
-              up(c->nbarr, nb.data(), sizeof(int32_t) * n) && up(c->xyz, x, sizeof(double) * 3 * n) &&
-              up(c->xyz0, x, sizeof(double) * 3 * n) && up(c->xyz_last, x, sizeof(double) * 3 * n) &&
-              up(c->dis0, dis0.data(), sizeof(double) * nbond) && up(c->dis, dis0.data(), sizeof(double) * nbond) &&
-              up(c->dis_last, dis0.data(), sizeof(double) * nbond) && up(c->cs, cs.data(), sizeof(double) * 3 * nbond) &&
-              up(c->sv, sv.data(), sizeof(double) * LPM_NSV * n) && up(c->sv_last, sv.data(), sizeof(double) * LPM_NSV * n) &&
-              up(c->mat, mat.data(), sizeof(double) * LPM_NMAT * n);
-    if (!ok) { c->err = "topology upload failed"; return fail(LPM_E_HIP); }
+    const bool ok = up(c->sv, sv.data(), sizeof(double) * LPM_NSV * n) &&
+                    up(c->sv_last, sv.data(), sizeof(double) * LPM_NSV * n) &&
+                    up(c->mat, mat.data(), sizeof(double) * LPM_NMAT * n);
+    if (!ok) { c->err = "state upload failed"; return fail(LPM_E_HIP); }
     if ((rc = launch_pattern(c))) return fail(rc);
     if (hipStreamSynchronize(c->stream) != hipSuccess) { c->err = "sync failed"; return fail(LPM_E_HIP); }
     *out = c;

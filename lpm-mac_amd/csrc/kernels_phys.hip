@@ -15,6 +15,8 @@
 //              (slices of 64 rows, lane-major; see sell_off in reduce.h)
 #include <algorithm>
 #include <climits>
+#include <cmath>
+#include <vector>
 
 #include "lpm_internal.h"
 #include "reduce.h"
@@ -863,6 +865,145 @@ __global__ __launch_bounds__(256) void k_nonlocal_search(int64_t n, double ratio
     if (act && !nlidx) counts[i] = cnt;
 }
 
+
+// ---------------------------------------------------------------- bond search on the device
+// Assembly::createBonds (assembly.h:212-234) tests every pair; here particles are binned on a
+// grid whose cell edge exceeds the bond cutoff, and each particle examines the 27 (9 in 2D)
+// surrounding cells. The reference's test and distance expression (Particle::distanceTo,
+// particle.h:276-280) decide acceptance and layer, and each particle's lists are kept sorted by
+// partner id, the order of the reference's all-pairs loop, so the result is bit-identical to it.
+
+__global__ __launch_bounds__(256) void k_cell_count(int64_t n, const double *__restrict__ xyz, double lo0, double lo1,
+                                                     double lo2, double h, int64_t nc0, int64_t nc1, int64_t nc2,
+                                                     int32_t *__restrict__ pcell, int32_t *__restrict__ count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double lo[3] = {lo0, lo1, lo2};
+    const int64_t nc[3] = {nc0, nc1, nc2};
+    int64_t c[3];
+    for (int k = 0; k < 3; ++k) {
+        c[k] = (int64_t)floor((xyz[3 * i + k] - lo[k]) / h);
+        c[k] = c[k] < 0 ? 0 : (c[k] > nc[k] - 1 ? nc[k] - 1 : c[k]);
+    }
+    const int64_t cell = (c[2] * nc1 + c[1]) * nc0 + c[0];
+    pcell[i] = (int32_t)cell;
+    atomicAdd(&count[cell], 1);
+}
+
+// members of each cell (their order inside a cell does not matter: every list is sorted below)
+__global__ __launch_bounds__(256) void k_cell_fill(int64_t n, const int32_t *__restrict__ pcell,
+                                                    const int32_t *__restrict__ head, int32_t *__restrict__ fill,
+                                                    int32_t *__restrict__ items) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t cell = pcell[i];
+    items[head[cell] + atomicAdd(&fill[cell], 1)] = (int32_t)i;
+}
+
+// insert j into the ascending row[0..m) and return m + 1
+__device__ __forceinline__ int sorted_insert(int32_t *row, int m, int32_t j) {
+    int pos = m;
+    while (pos > 0 && row[pos - 1] > j) {
+        row[pos] = row[pos - 1];
+        --pos;
+    }
+    row[pos] = j;
+    return m + 1;
+}
+
+// One thread per particle: accepted partners are insertion-sorted by id into its layer-0 and
+// layer-1 rows of tmp0 / tmp1 (LPM_MAXB entries each); flag[0] = 1 if a particle has more than
+// LPM_MAXB bonds (the rows are then incomplete and the caller fails).
+__global__ __launch_bounds__(256) void k_bond_search(int64_t n, const double *__restrict__ xyz,
+                                                      const int32_t *__restrict__ pcell, int64_t nc0, int64_t nc1,
+                                                      int64_t nc2, const int32_t *__restrict__ head,
+                                                      const int32_t *__restrict__ items, double c0, double c1,
+                                                      int32_t *__restrict__ tmp0, int32_t *__restrict__ tmp1,
+                                                      int32_t *__restrict__ cnt0, int32_t *__restrict__ cnt1,
+                                                      int32_t *__restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t cell = pcell[i];
+    const int64_t cx = cell % nc0, cy = (cell / nc0) % nc1, cz = cell / (nc0 * nc1);
+    const double ax = xyz[3 * i], ay = xyz[3 * i + 1], az = xyz[3 * i + 2];
+    int32_t *r0 = tmp0 + i * LPM_MAXB, *r1 = tmp1 + i * LPM_MAXB;
+    int n0 = 0, n1 = 0;
+    bool over = false;
+    for (int64_t dz = -1; dz <= 1; ++dz)
+        for (int64_t dy = -1; dy <= 1; ++dy)
+            for (int64_t dx = -1; dx <= 1; ++dx) {
+                const int64_t x = cx + dx, y = cy + dy, z = cz + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= nc0 || y >= nc1 || z >= nc2) continue;
+                const int64_t cc = (z * nc1 + y) * nc0 + x;
+                for (int t = head[cc]; t < head[cc + 1]; ++t) {
+                    const int32_t j = items[t];
+                    if (j == (int32_t)i) continue;
+                    const double ex = ax - xyz[3 * (int64_t)j], ey = ay - xyz[3 * (int64_t)j + 1],
+                                 ez = az - xyz[3 * (int64_t)j + 2];
+                    const double d = sqrt(ex * ex + ey * ey + ez * ez);
+                    if (!(d < c1)) continue;
+                    if (n0 + n1 >= LPM_MAXB) {
+                        over = true;
+                        continue;
+                    }
+                    if (d < c0) n0 = sorted_insert(r0, n0, j);
+                    else n1 = sorted_insert(r1, n1, j);
+                }
+            }
+    cnt0[i] = n0;
+    cnt1[i] = n1;
+    if (over) flag[0] = 1;
+}
+
+__global__ __launch_bounds__(256) void k_bond_compact(int64_t n, const int32_t *__restrict__ bptr,
+                                                       const int32_t *__restrict__ bsplit, const int32_t *__restrict__ tmp0,
+                                                       const int32_t *__restrict__ tmp1, int32_t *__restrict__ p2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b0 = bptr[i], bm = bsplit[i], b1 = bptr[i + 1];
+    for (int k = 0; k < bm - b0; ++k) p2[b0 + k] = tmp0[i * LPM_MAXB + k];
+    for (int k = 0; k < b1 - bm; ++k) p2[bm + k] = tmp1[i * LPM_MAXB + k];
+}
+
+// Per particle: bond count, Bond(p1, p2, layer, dis) initial geometry (bond.h:61-73), the reverse
+// bond (found in the partner's list of the same layer, which exists since the acceptance test is
+// symmetric) and the number of 2-hop records of the assembly (max over particles in hops_max).
+__global__ __launch_bounds__(256) void k_bond_init(int64_t n, const int32_t *__restrict__ bptr,
+                                                    const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2,
+                                                    const double *__restrict__ xyz, int32_t *__restrict__ nb,
+                                                    int32_t *__restrict__ opp, double *__restrict__ dis0,
+                                                    double *__restrict__ dis, double *__restrict__ dis_last,
+                                                    double *__restrict__ cs, int32_t *__restrict__ hops_max,
+                                                    int32_t *__restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b0 = bptr[i], bm = bsplit[i], b1 = bptr[i + 1];
+    nb[i] = b1 - b0;
+    int hops = 0;
+    for (int b = b0; b < b1; ++b) {
+        const int64_t j = p2[b];
+        const double ex = xyz[3 * i] - xyz[3 * j], ey = xyz[3 * i + 1] - xyz[3 * j + 1], ez = xyz[3 * i + 2] - xyz[3 * j + 2];
+        const double d = sqrt(ex * ex + ey * ey + ez * ez);
+        dis0[b] = d;
+        dis[b] = d;
+        dis_last[b] = d;
+        cs[3 * b] = ex / d;
+        cs[3 * b + 1] = ey / d;
+        cs[3 * b + 2] = ez / d;
+        const bool l0 = b < bm;
+        const int lo = l0 ? bptr[j] : bsplit[j], hi = l0 ? bsplit[j] : bptr[j + 1];
+        int o = -1;
+        for (int q = lo; q < hi; ++q)
+            if (p2[q] == (int32_t)i) {
+                o = q;
+                break;
+            }
+        opp[b] = o;
+        if (o < 0) flag[1] = 1;
+        hops += hi - lo;
+    }
+    atomicMax(hops_max, hops);
+}
 }  // namespace
 
 // =================================================================== launchers
@@ -1040,6 +1181,101 @@ int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any) {
     rc = check_launch(c, "k_fatigue_damage");
     if (rc) return rc;
     return read_flag(c, any);
+}
+
+// Device bond search and initial bond state for lpm_create (positions already in c->xyz).
+// Sets c->nbond, c->maxbond, c->gp_bonds, c->asm_hops and allocates the topology arrays.
+int device_topology(lpm_ctx *c, const double *xyz_host) {
+    const int64_t n = c->n;
+    const double c1 = 1.01 * c->cutoff[1], c0 = 1.01 * c->cutoff[0];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], xyz_host[3 * i + k]);
+            hi[k] = std::max(hi[k], xyz_host[3 * i + k]);
+        }
+    const double h = c1 * 1.0001;  // a cell edge above the cutoff: partners lie in adjacent cells
+    int64_t nc[3];
+    for (int k = 0; k < 3; ++k) nc[k] = std::max<int64_t>(1, (int64_t)std::floor((hi[k] - lo[k]) / h) + 1);
+    const int64_t ncell = nc[0] * nc[1] * nc[2];
+    if (ncell >= 0x7fffffffLL) return lpm_fail(c, LPM_E_LIMIT, "bond search grid too large");
+    int32_t *pcell = nullptr, *count = nullptr, *head = nullptr, *items = nullptr, *tmp0 = nullptr, *tmp1 = nullptr,
+            *cnt = nullptr, *flag = nullptr;
+    auto cleanup = [&]() {
+        void *ps[] = {pcell, count, head, items, tmp0, tmp1, cnt, flag};
+        for (void *q : ps)
+            if (q) hipFree(q);
+    };
+    auto al = [&](int32_t **p, int64_t cntv) { return hipMalloc(p, sizeof(int32_t) * std::max<int64_t>(cntv, 1)) == hipSuccess; };
+    if (!al(&pcell, n) || !al(&count, ncell) || !al(&head, ncell + 1) || !al(&items, n) || !al(&tmp0, n * LPM_MAXB) ||
+        !al(&tmp1, n * LPM_MAXB) || !al(&cnt, 2 * n) || !al(&flag, 4)) {
+        cleanup();
+        return lpm_fail(c, LPM_E_NOMEM, "bond search scratch");
+    }
+    int rc = LPM_OK;
+    std::vector<int32_t> h_count(ncell), h_head(ncell + 1, 0), h_cnt(2 * n);
+    hipMemsetAsync(count, 0, sizeof(int32_t) * ncell, c->stream);
+    hipMemsetAsync(flag, 0, sizeof(int32_t) * 4, c->stream);
+    hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const double *)c->xyz, lo[0],
+                       lo[1], lo[2], h, nc[0], nc[1], nc[2], pcell, count);
+    if ((rc = check_launch(c, "k_cell_count"))) { cleanup(); return rc; }
+    hipMemcpyAsync(h_count.data(), count, sizeof(int32_t) * ncell, hipMemcpyDeviceToHost, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { cleanup(); return lpm_fail(c, LPM_E_HIP, "bond search sync"); }
+    for (int64_t q = 0; q < ncell; ++q) h_head[q + 1] = h_head[q] + h_count[q];
+    hipMemcpyAsync(head, h_head.data(), sizeof(int32_t) * (ncell + 1), hipMemcpyHostToDevice, c->stream);
+    hipMemsetAsync(count, 0, sizeof(int32_t) * ncell, c->stream);
+    hipLaunchKernelGGL(k_cell_fill, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const int32_t *)pcell,
+                       (const int32_t *)head, count, items);
+    if ((rc = check_launch(c, "k_cell_fill"))) { cleanup(); return rc; }
+    hipLaunchKernelGGL(k_bond_search, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const double *)c->xyz,
+                       (const int32_t *)pcell, nc[0], nc[1], nc[2], (const int32_t *)head, (const int32_t *)items, c0, c1,
+                       tmp0, tmp1, cnt, cnt + n, flag);
+    if ((rc = check_launch(c, "k_bond_search"))) { cleanup(); return rc; }
+    int32_t h_flag[4] = {0, 0, 0, 0};
+    hipMemcpyAsync(h_cnt.data(), cnt, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(h_flag, flag, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { cleanup(); return lpm_fail(c, LPM_E_HIP, "bond search sync"); }
+    if (h_flag[0]) { cleanup(); return lpm_fail(c, LPM_E_LIMIT, "a particle has more than 32 bonds"); }
+    std::vector<int32_t> bp(n + 1, 0), bs(n);
+    int64_t tot = 0;
+    int maxb = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        bp[i] = (int32_t)tot;
+        bs[i] = (int32_t)(tot + h_cnt[i]);
+        tot += h_cnt[i] + h_cnt[n + i];
+        maxb = std::max(maxb, h_cnt[i] + h_cnt[n + i]);
+        if (tot > 0x7fffffffLL) { cleanup(); return lpm_fail(c, LPM_E_LIMIT, "more than 2^31 bonds"); }
+    }
+    bp[n] = (int32_t)tot;
+    c->nbond = tot;
+    c->maxbond = maxb;
+    c->gp_bonds = 0;
+    for (int64_t p0 = 0; p0 < n; p0 += 32)
+        c->gp_bonds = std::max<int>(c->gp_bonds, bp[std::min<int64_t>(n, p0 + 32)] - bp[p0]);
+#define ALT(ptr, cntv)                                                                                          \
+    if (hipMalloc(&c->ptr, sizeof(*c->ptr) * std::max<int64_t>((cntv), 1)) != hipSuccess) {                     \
+        cleanup();                                                                                               \
+        return lpm_fail(c, LPM_E_NOMEM, "topology arrays");                                                      \
+    }
+    ALT(bptr, n + 1); ALT(bsplit, n); ALT(p2, tot); ALT(opp, tot); ALT(nbarr, n);
+    ALT(dis0, tot); ALT(dis, tot); ALT(dis_last, tot); ALT(cs, 3 * tot);
+#undef ALT
+    hipMemcpyAsync(c->bptr, bp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(c->bsplit, bs.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream);
+    hipLaunchKernelGGL(k_bond_compact, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const int32_t *)c->bptr,
+                       (const int32_t *)c->bsplit, (const int32_t *)tmp0, (const int32_t *)tmp1, c->p2);
+    if ((rc = check_launch(c, "k_bond_compact"))) { cleanup(); return rc; }
+    hipMemsetAsync(flag, 0, sizeof(int32_t) * 4, c->stream);
+    hipLaunchKernelGGL(k_bond_init, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const int32_t *)c->bptr,
+                       (const int32_t *)c->bsplit, (const int32_t *)c->p2, (const double *)c->xyz, c->nbarr, c->opp,
+                       c->dis0, c->dis, c->dis_last, c->cs, flag + 2, flag);
+    if ((rc = check_launch(c, "k_bond_init"))) { cleanup(); return rc; }
+    hipMemcpyAsync(h_flag, flag, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { cleanup(); return lpm_fail(c, LPM_E_HIP, "bond init sync"); }
+    cleanup();
+    if (h_flag[1]) return lpm_fail(c, LPM_E_TOPO, "a bond has no reverse bond");
+    c->asm_hops = h_flag[2];
+    return LPM_OK;
 }
 
 int launch_pattern(lpm_ctx *c) {

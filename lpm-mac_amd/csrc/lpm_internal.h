@@ -182,6 +182,8 @@ int launch_nonlocal_ddot(lpm_ctx *c, int undamaged);
 int launch_static_damage(lpm_ctx *c, int undamaged, int *any);
 int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any);
 int launch_pattern(lpm_ctx *c);
+// bond search, reverse bonds and initial bond geometry on the device (lpm_create without bonds)
+int device_topology(lpm_ctx *c, const double *xyz_host);
 int launch_tslot(lpm_ctx *c);
 int launch_nonlocal_search(lpm_ctx *c, double ratio);
 int launch_residual_spec(lpm_ctx *c, std::vector<double> &r2, std::vector<double> &R2);

@@ -179,13 +179,19 @@ class Context:
     """One particle system on one GPU (an Assembly + Solver of the reference)."""
 
     def __init__(self, lattice, radius, model, xyz, bonds=None, device=0):
+        """bonds: (bond_ptr, bond_split, bond_p2) lists, None for the host search (lpm_build_bonds),
+        or "device" for the search inside lpm_create on the GPU."""
         L = lib()
         self.xyz0 = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
-        if bonds is None:
-            bonds = build_bonds(lattice, radius, self.xyz0)
-        self.bp, self.bs, self.p2 = (np.ascontiguousarray(b, dtype=np.int32) for b in bonds)
-        t = Topology(lattice, radius, model, len(self.xyz0), _dp(self.xyz0), _ip(self.bp), _ip(self.bs),
-                     _ip(self.p2), device)
+        if isinstance(bonds, str):
+            if bonds != "device":
+                raise ValueError("bonds must be None, 'device' or (bond_ptr, bond_split, bond_p2)")
+            t = Topology(lattice, radius, model, len(self.xyz0), _dp(self.xyz0), None, None, None, device)
+        else:
+            if bonds is None:
+                bonds = build_bonds(lattice, radius, self.xyz0)
+            self._bonds = tuple(np.ascontiguousarray(b, dtype=np.int32) for b in bonds)
+            t = Topology(lattice, radius, model, len(self.xyz0), _dp(self.xyz0), *(_ip(b) for b in self._bonds), device)
         h = _P()
         rc = L.lpm_create(ctypes.byref(t), ctypes.byref(h))
         if rc != 0:

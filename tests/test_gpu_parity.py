@@ -54,6 +54,45 @@ def test_topology_bitwise(case):
     np.testing.assert_array_equal(p.g.get("cs"), p.o.getd("cs"))
 
 
+@pytest.mark.parametrize("case", ["sc", "hex", "square", "fcc_rotated", "jittered", "dogbone"])
+def test_device_bond_search_bitwise(case):
+    """Bonds searched on the device inside lpm_create (cell grid, the reference's distance test,
+    lists sorted by partner id) equal the host search, itself pinned to the all-pairs loop: bond
+    lists, layer split, reverse bonds, bond counts, the 2-hop pattern and the initial lengths and
+    direction cosines, bit for bit."""
+    say("[gpu] device bond search", case)
+    rng = np.random.default_rng(3)
+    if case == "sc":
+        xyz, _ = sc_cube(20)
+        lat, r = G.SC3D, 0.5
+    elif case == "hex":
+        xyz, _ = hex_plate(60, 50)
+        lat, r = G.HEXAGON2D, 0.5
+    elif case == "square":
+        xyz = G.generate_lattice(G.SQUARE2D, [0, 40, 0, 30, 0, 1], 0.5)
+        lat, r = G.SQUARE2D, 0.5
+    elif case == "fcc_rotated":
+        xyz = G.generate_lattice(G.FCC3D, [0, 8, 0, 8, 0, 8], 0.3, 0, (0.3, 0.7, 1.1))
+        lat, r = G.FCC3D, 0.3
+    elif case == "jittered":
+        xyz, _ = sc_cube(16)
+        xyz = xyz + 0.05 * rng.standard_normal(xyz.shape)
+        lat, r = G.SC3D, 0.5
+    else:
+        x, _ = hex_plate(80, 160)
+        y = (x[:, 1] - x[:, 1].min()) / np.ptp(x[:, 1])
+        w = np.where(np.abs(y - 0.5) <= 0.15, 0.5, np.where(np.abs(y - 0.5) >= 0.25, 1.0,
+                                                             0.5 + 0.5 * (np.abs(y - 0.5) - 0.15) / 0.10))
+        xyz = np.ascontiguousarray(x[np.abs(x[:, 0] - 0.5 * np.ptp(x[:, 0])) <= 0.5 * w * np.ptp(x[:, 0]) + 1e-9])
+        lat, r = G.HEXAGON2D, 0.5
+    host = G.Context(lat, r, G.ELASTIC, xyz)
+    dev = G.Context(lat, r, G.ELASTIC, xyz, bonds="device")
+    say("   particles", len(xyz), "bonds", host.size("nbonds"), dev.size("nbonds"))
+    for f in ("bond_ptr", "bond_split", "bond_p2", "opp", "nb", "conn_ptr", "conn_idx", "dis_initial", "cs"):
+        np.testing.assert_array_equal(dev.get(f), host.get(f), err_msg=f)
+    assert host.size("nbonds") > 0
+
+
 # ----------------------------------------------------------------- geometry, force, residual (FP64, bitwise)
 @pytest.mark.parametrize("model", [O.ELASTIC, O.ELASTIC_DAMAGE])
 def test_force_state_bitwise(model):
