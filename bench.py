@@ -89,6 +89,7 @@ def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0, 
     ls = ctx.load_step(disp=[(bottom, "x", 0.0), (bottom, "y", 0.0), (bottom, "z", 0.0), (top, "z", dz)])
     ctx.update_geometry()
     ctx.update_force_state()
+    ctx.box = box
     return ctx, ls, xyz, time.time() - t0
 
 
@@ -248,6 +249,44 @@ def main():
                  "cg_iters_per_step": ctx.stat("cg_iters") / K},
                 (ctx.stat("newton_iters") / K, ctx.stat("cg_iters") / K, ctx.stat("assemblies") / K))
 
+    # Per-step output (Assembly::writeDump, about 50 MB of text per step here; excluded from the
+    # headline): 3 steps each appending a frame from the asynchronous writer, against one frame
+    # written with nothing to overlap it.
+    dump = None
+    if not args.no_cpu_baseline:
+        import tempfile
+
+        path = os.path.join(tempfile.gettempdir(), f"lpm_bench_{os.getpid()}.dump")
+        try:
+            reset_state(ctx, xyz)
+            hip_sync(ctx)
+            t3 = time.perf_counter()
+            for k in range(3):
+                if not ctx.solve_step_static(ls, -1):
+                    raise RuntimeError("dump run did not converge")
+            hip_sync(ctx)
+            t3 = time.perf_counter() - t3
+            reset_state(ctx, xyz)
+            hip_sync(ctx)
+            td = time.perf_counter()
+            for k in range(3):
+                if not ctx.solve_step_static(ls, -1):
+                    raise RuntimeError("dump run did not converge")
+                ctx.dump_async(path, k + 1, ctx.box)
+            ctx.dump_wait()
+            td = time.perf_counter() - td
+            frame_bytes = os.path.getsize(path) / 3
+            t1 = time.perf_counter()
+            ctx.dump_async(path, 4, ctx.box)
+            ctx.dump_wait()
+            t1 = time.perf_counter() - t1
+            dump = {"load_steps_per_s_with_a_frame_each_step": 3 / td, "load_steps_per_s_same_steps_without": 3 / t3,
+                    "frame_bytes": frame_bytes,
+                    "frame_alone_ms": 1e3 * t1, "steps": 3}
+        finally:
+            if os.path.exists(path):
+                os.remove(path)
+
     # The same K steps again: without the predictor (the reference's Newton start), and with the
     # reference's algorithm (Jacobi, no predictor), whose iteration counts drive the CPU
     # projection; both rates are reported beside the headline.
@@ -288,6 +327,7 @@ def main():
             "final_residual_over_reaction": nr / nR if nR > 0 else None,
             "parallelism": "single GPU",
             "setup_seconds": t_setup,
+            "async_dump": dump,
             "same_steps_without_predictor": nopred,
             "same_steps_reference_algorithm_jacobi_no_predictor": jac,
         },

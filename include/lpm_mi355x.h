@@ -255,6 +255,20 @@ int lpm_solve_problem_cyclic(lpm_ctx *ctx, int32_t load_type, int32_t n_interval
                              int64_t n_spectrum, const int32_t *groups, int32_t n_groups, int32_t time_mode,
                              double tau, int32_t undamaged_type, int32_t max_cycles_run, int32_t *cycles);
 
+/* ------------------------------------------------------------------ output
+ * Assembly::writeDump (assembly.h:309-335) without stopping the solver: the fields the dump
+ * prints (positions, displacements, stress, damage, damage_visual, state_var[0], type) are
+ * copied on the device in stream order, copied to pinned host memory on a side stream, and a
+ * writer thread formats and appends the frame to `path` (text byte-identical to the
+ * reference's fprintf, box[6] as Assembly::box). Returns at once; the context may step on.
+ * One write is in flight per context: a second call first waits for the previous one.
+ * lpm_dump_wait joins the write in flight and returns its status (no reference counterpart). */
+int lpm_dump_async(lpm_ctx *ctx, const char *path, int32_t step, const double *box);
+int lpm_dump_wait(lpm_ctx *ctx);
+/* The dump writer's number formatting, exposed for testing: x[0..n) as printf("%.4e") would
+ * print them, one per line, NUL-terminated, into out[cap] (LPM_E_LIMIT if it does not fit). */
+int lpm_format_e4(const double *x, int64_t n, char *out, int64_t cap);
+
 /* ------------------------------------------------------------------ tangent and adjoint
  * (no reference counterpart: sensitivities of the load-step result, frozen damage / broken
  * bonds / plastic bond lengths, for parameter calibration)

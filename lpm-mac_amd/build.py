@@ -12,8 +12,9 @@ ARCH = os.environ.get("LPM_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"),
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
-# kernels whose FP64 results must follow the reference's operation order: no FMA contraction
-EXACT = {"kernels_phys.hip": ["-ffp-contract=off"], "ctx.hip": ["-ffp-contract=off"]}
+# per-file flags: no FMA contraction where FP64 results must follow the reference's operation
+# order; host OpenMP for the dump writer's parallel formatting
+EXTRA = {"kernels_phys.hip": ["-ffp-contract=off"], "ctx.hip": ["-ffp-contract=off"], "io.hip": ["-fopenmp"]}
 
 
 def run(cmd):
@@ -27,9 +28,9 @@ def run(cmd):
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     objs = []
-    for src in ("kernels_phys.hip", "kernels_cg.hip", "kernels_coarse.hip", "kernels_adj.hip", "ctx.hip"):
+    for src in ("kernels_phys.hip", "kernels_cg.hip", "kernels_coarse.hip", "kernels_adj.hip", "ctx.hip", "io.hip"):
         o = os.path.join(OBJ, src + ".o")
-        run(["hipcc", "-c", *HIP_FLAGS, *EXACT.get(src, []), "-o", o, os.path.join(CSRC, src)])
+        run(["hipcc", "-c", *HIP_FLAGS, *EXTRA.get(src, []), "-o", o, os.path.join(CSRC, src)])
         objs.append(o)
     o = os.path.join(OBJ, "topology.o")
     run(["g++", "-c", "-O2", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),

@@ -730,6 +730,7 @@ int lpm_destroy(lpm_ctx *c) {
                     c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part, c->xyz_prev};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    dump_free(c);
     coarse_free(c);
     for (auto &g : c->groups)
         if (g.ids) hipFree(g.ids);

@@ -111,6 +111,9 @@ struct lpm_ctx {
     double *xyz_prev = nullptr;
     uint8_t *dofmask = nullptr;       // per particle: bit k set when DoF k is eliminated
     int64_t coarse_version = -1;
+    // Asynchronous dump writer (io.hip): device snapshot -> pinned host copy on a side stream ->
+    // formatted and appended by a writer thread; one write in flight per context
+    struct DumpState *dump = nullptr;
     // Specimen ensemble: independent particle systems (contiguous id ranges, no bond between
     // them) solved in the same launches, each with its own Newton test.
     int nspec = 0;                    // 0: one system
@@ -182,6 +185,8 @@ int launch_nonlocal_ddot(lpm_ctx *c, int undamaged);
 int launch_static_damage(lpm_ctx *c, int undamaged, int *any);
 int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any);
 int launch_pattern(lpm_ctx *c);
+// joins a dump write still in flight and frees the writer's buffers (lpm_destroy)
+void dump_free(lpm_ctx *c);
 // bond search, reverse bonds and initial bond geometry on the device (lpm_create without bonds)
 int device_topology(lpm_ctx *c, const double *xyz_host);
 int launch_tslot(lpm_ctx *c);

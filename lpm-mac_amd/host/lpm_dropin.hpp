@@ -109,11 +109,13 @@ struct Options {
     int diag10_literal = 0;    // reference: 1 (derivative.h:38)
     int rollback_pex = 0;      // reference: 0
     int precision = 0;         // 0 mixed FP32/FP64, 1 FP64
-    int precond = 0;           // 0 Jacobi, 1 block Jacobi
+    int precond = 0;           // 0 Jacobi, 1 block Jacobi, 2 Jacobi + rigid-body coarse space, 3 block Jacobi + coarse
     int elmap_literal = 1;     // reference: 1 (column-major read of el_mapping)
     double cg_tol = 1e-6;      // reference: 2.22e-16 (FP64)
     double cg_maxit = -1;      // reference: 2*ndof
     int cg_forcing = 0;        // 1: inexact-Newton CG tolerance per Newton iteration (reference: 0)
+    int predictor = 0;         // 1: static steps start from the previous converged increment (reference: 0)
+    int async_dump = 1;        // 1: writeDump from a device snapshot on a writer thread (same text; reference: inline)
     int verbose = 1;           // print like the reference
     int device = 0;
 };
@@ -648,6 +650,15 @@ class Assembly {
                     pt->stress[5], pt->damage, pt->damage_visual, pt->state_var[0]);
         fclose(f);
     }
+    // writeDump without stopping the solver: the device snapshot is formatted and appended by a
+    // writer thread (lpm_dump_async); waitDump() joins it
+    void writeDumpAsync(const std::string &dumpFile, int step) {
+        ensureDevice();
+        lpm::check(lpm_dump_async(dev->ctx, dumpFile.c_str(), step, box.data()), dev->ctx, "writeDump");
+    }
+    void waitDump() {
+        if (dev && dev->ctx) lpm::check(lpm_dump_wait(dev->ctx), dev->ctx, "writeDump");
+    }
     // Assembly::readDump (assembly.h:337-386): id type x y z
     void readDump(const std::string &dumpFile, UnitCell &cell) {
         FILE *f = fopen(dumpFile.c_str(), "r+");
@@ -737,6 +748,7 @@ class Assembly {
         lpm_set_mode(c, "cg_tol", o.cg_tol);
         lpm_set_mode(c, "cg_maxit", o.cg_maxit);
         lpm_set_mode(c, "cg_forcing", o.cg_forcing);
+        lpm_set_mode(c, "predictor", o.predictor);
         lpm_set_mode(c, "verbose", o.verbose);
     }
 
@@ -1017,8 +1029,12 @@ class SolverStatic : public Solver<nlayer> {
                     lpm_download(this->ctx(), "Pex", pex.data(), (int64_t)pex.size());
                 }
                 if (solveProblemStep(load[i], dt)) {
-                    this->ass.pull(false);
-                    this->ass.writeDump(this->dumpFile, i + start_index);
+                    if (lpm::options().async_dump) {
+                        this->ass.writeDumpAsync(this->dumpFile, i + start_index);
+                    } else {
+                        this->ass.pull(false);
+                        this->ass.writeDump(this->dumpFile, i + start_index);
+                    }
                     printf("Loading step %d has finished, spent %f seconds\n\nData output ...\n\n", i + start_index,
                            omp_get_wtime() - t1);
                     break;
@@ -1031,6 +1047,7 @@ class SolverStatic : public Solver<nlayer> {
                 printf("Step-%d not converging\n\n", i + start_index);
             }
         }
+        this->ass.waitDump();
         this->ass.pull(true);
         start_index += n_step;
     }
@@ -1096,9 +1113,14 @@ class SolverFatigue : public Solver<nlayer> {
             }
             lpm::check(lpm_update_state_var(this->ctx()), this->ctx(), "updateStateVar");
             lpm::check(lpm_store_state(this->ctx()), this->ctx(), "storeStateVar");
-            this->ass.pull(false);
-            this->ass.writeDump(this->dumpFile, i);
+            if (lpm::options().async_dump) {
+                this->ass.writeDumpAsync(this->dumpFile, i);
+            } else {
+                this->ass.pull(false);
+                this->ass.writeDump(this->dumpFile, i);
+            }
         }
+        this->ass.waitDump();
         this->ass.pull(true);
         start_index += n_step;
     }

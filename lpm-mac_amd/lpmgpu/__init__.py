@@ -89,6 +89,9 @@ _SIGS = {
                                                 _D, ctypes.c_int32, ctypes.c_int32, _IP]),
     "lpm_adjoint_reaction": (_I, [_P, ctypes.c_int32, ctypes.c_int32, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
     "lpm_stream": (_P, [_P]),
+    "lpm_dump_async": (_I, [_P, _S, ctypes.c_int32, _DP]),
+    "lpm_dump_wait": (_I, [_P]),
+    "lpm_format_e4": (_I, [_DP, _L, ctypes.c_char_p, _L]),
 }
 
 _lib = None
@@ -157,6 +160,15 @@ def unit_cell(lattice, radius, literal=1):
     m = np.zeros(9)
     _check(lib().lpm_unit_cell(lattice, radius, literal, _dp(out), _dp(m)))
     return out, m.reshape(3, 3)
+
+
+def format_e4(x):
+    """The dump writer's "%.4e" formatting of x (host code of the library)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    cap = 48 * len(x) + 64
+    buf = ctypes.create_string_buffer(cap)
+    _check(lib().lpm_format_e4(_dp(x), len(x), buf, cap))
+    return buf.value.decode().split("\n")[:-1]
 
 
 def ce_from_e(plane_stress, E, mu):
@@ -470,6 +482,14 @@ class Context:
         self._chk(lib().lpm_adjoint_reaction(self.h, int(group), ax, gk.ctypes.data_as(_DP), gt.ctypes.data_as(_DP),
                                              ctypes.byref(F), ctypes.byref(it), ctypes.byref(rr)))
         return F.value, gk, gt, it.value, rr.value
+
+    def dump_async(self, path, step, box):
+        """Append a writeDump frame for the current state to `path` from a writer thread."""
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        self._chk(lib().lpm_dump_async(self.h, str(path).encode(), int(step), _dp(b)))
+
+    def dump_wait(self):
+        self._chk(lib().lpm_dump_wait(self.h))
 
     def copy_bench(self, nbytes, reps):
         ms = _D()

@@ -73,3 +73,36 @@ def test_bad_topology_is_rejected_without_launching():
         G.Context(G.SC3D, 0.5, G.ELASTIC, xyz, bonds=(bp, bs, p2))
     with pytest.raises(G.LPMError, match="lattice"):
         G.Context(17, 0.5, G.ELASTIC, xyz, bonds=(bp, bs, p2))
+
+
+def test_dump_number_formatting_matches_printf():
+    """The dump writer's "%.4e" (an exact fast path with a printf fallback near ties) against the
+    C library's own snprintf, on values of every magnitude, exact ties, values one ulp either side
+    of ties and decade boundaries, signed zeros, infinities and NaN."""
+    import ctypes
+    import ctypes.util
+
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.snprintf.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_double]
+
+    def printf_e4(v):
+        buf = ctypes.create_string_buffer(64)
+        libc.snprintf(buf, 64, b"%.4e", float(v))
+        return buf.value.decode()
+
+    rng = np.random.default_rng(0)
+    vals = [0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -1.0, 5e-324, 1.7976931348623157e308]
+    vals += list(rng.standard_normal(20000) * 10.0 ** rng.integers(-30, 30, 20000))
+    vals += list(rng.uniform(-1, 1, 20000))
+    for e in range(-25, 26):
+        for k in (10000, 12345, 99999, 50000, 99995, 10005):
+            for frac in (0.0, 0.5):
+                t = (k + frac) * 10.0 ** (e - 4)
+                vals += [t, np.nextafter(t, np.inf), np.nextafter(t, -np.inf), -t]
+        p = 10.0 ** e
+        vals += [p, np.nextafter(p, 0), np.nextafter(p, np.inf), 9.99995 * p, 9.99994999 * p, 9.999950001 * p]
+    vals = np.array(vals, dtype=np.float64)
+    got = G.format_e4(vals)
+    want = [printf_e4(v) for v in vals]
+    bad = [(v, g, w) for v, g, w in zip(vals, got, want) if g != w]
+    assert not bad, bad[:10]
