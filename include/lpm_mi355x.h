@@ -103,6 +103,33 @@ typedef struct {
  * geometry, also on the device. The device search bonds every pair within the cutoff, so
  * ensembles whose specimens overlap in space pass their own lists. */
 int lpm_create(const lpm_topology *topo, lpm_ctx **out);
+
+/* ------------------------------------------------------------------ geometry preprocessing
+ * The reference's geometry tools (geometry/geo1_CT_configuration.cpp, geo2_dogbone_config2d.cpp)
+ * cut holes and notches out of a lattice plate: isValid (:137-171) keeps the points in no cut,
+ * and searchNeighbor (:79-134) keeps a bond only if the midpoints of its 10 equal sub-segments
+ * are all valid. A cut (z ignored):
+ *   LPM_CUT_CIRCLE         (x-cx)^2 + (y-cy)^2 <= size^2                (inCircle, :45-49)
+ *   LPM_CUT_NOTCH          |y-ty| <= size, between the lines of slope -+1/sqrt(3) through the
+ *                          tip on the side x > tx                        (inNotch, :52-62)
+ *   LPM_CUT_REVERSE_NOTCH  the same opening to x < tx                   (inReverseNotch, :64-74) */
+enum { LPM_CUT_CIRCLE = 0, LPM_CUT_NOTCH = 1, LPM_CUT_REVERSE_NOTCH = 2 };
+typedef struct {
+    int32_t kind;
+    double x, y;  /* centre or notch tip */
+    double size;  /* radius or notch half width */
+} lpm_cut;
+/* keep[i] = 1 if particle i lies in no cut (the tools' particle filter) */
+int lpm_cut_particles(const double *xyz, int64_t n, const lpm_cut *cuts, int32_t ncut, uint8_t *keep);
+/* lpm_create with the device bond search (topo->bond_ptr must be NULL) dropping every bond the
+ * tools' searchNeighbor drops; the kept lists are in the same order. */
+int lpm_create_cut(const lpm_topology *topo, const lpm_cut *cuts, int32_t ncut, lpm_ctx **out);
+/* benchmark/AM_Ti64/generate_dump.ipynb createPlate: one particle per pixel equal to 255 of a
+ * binary image (row-major, nrows x ncols, row 0 at the top), pixel edge `pixel`:
+ * x = pixel/2 + pixel*col, y = nrows*pixel - (pixel/2 + pixel*row), z = 0, in row-major order.
+ * Pass xyz == NULL to get the count in *n first. */
+int lpm_image_particles(const uint8_t *img, int64_t nrows, int64_t ncols, double pixel, double *xyz, int64_t cap,
+                        int64_t *n);
 int lpm_destroy(lpm_ctx *ctx);
 const char *lpm_last_error(const lpm_ctx *ctx); /* ctx may be NULL: last error of lpm_create */
 /* sizes: "n","dim","nbonds","nconn","nnl","ndof","knnz","maxconn","maxbond","nl_ptr",

@@ -562,11 +562,16 @@ extern "C" {
 
 const char *lpm_last_error(const lpm_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
 
-int lpm_create(const lpm_topology *t, lpm_ctx **out) {
+static int create(const lpm_topology *t, const lpm_cut *cuts, int32_t ncut, lpm_ctx **out) {
     if (!t || !out) return lpm_fail(nullptr, LPM_E_ARG, "null argument");
     *out = nullptr;
     // bond_ptr == NULL: search the bonds on the device; else the caller's lists, validated here
     const bool search = t->bond_ptr == nullptr;
+    if (ncut < 0 || (ncut > 0 && (!cuts || !search)))
+        return lpm_fail(nullptr, LPM_E_ARG, "cuts apply to the device bond search (bond_ptr NULL)");
+    for (int32_t k = 0; k < ncut; ++k)
+        if (cuts[k].kind < LPM_CUT_CIRCLE || cuts[k].kind > LPM_CUT_REVERSE_NOTCH)
+            return lpm_fail(nullptr, LPM_E_ARG, "unknown cut kind");
     if (t->n <= 0 || !t->xyz || (!search && (!t->bond_split || (!t->bond_p2 && t->bond_ptr[t->n] > 0))))
         return lpm_fail(nullptr, LPM_E_ARG, "empty or incomplete topology");
     if (t->model < 0 || t->model > 3) return lpm_fail(nullptr, LPM_E_ARG, "unknown particle model");
@@ -648,7 +653,7 @@ int lpm_create(const lpm_topology *t, lpm_ctx **out) {
     }
     // ---- bonds: Bond(p1, p2, layer, dis) (bond.h:61-73), reverse bonds, 2-hop bookkeeping
     if (search) {
-        if ((rc = device_topology(c, x))) return fail(rc);
+        if ((rc = device_topology(c, x, cuts, ncut))) return fail(rc);
         nbond = c->nbond;
     } else {
         c->nbond = nbond;
@@ -715,6 +720,12 @@ int lpm_create(const lpm_topology *t, lpm_ctx **out) {
     if (hipStreamSynchronize(c->stream) != hipSuccess) { c->err = "sync failed"; return fail(LPM_E_HIP); }
     *out = c;
     return LPM_OK;
+}
+
+int lpm_create(const lpm_topology *t, lpm_ctx **out) { return create(t, nullptr, 0, out); }
+
+int lpm_create_cut(const lpm_topology *t, const lpm_cut *cuts, int32_t ncut, lpm_ctx **out) {
+    return create(t, cuts, ncut, out);
 }
 
 int lpm_destroy(lpm_ctx *c) {

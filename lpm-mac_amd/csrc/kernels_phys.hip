@@ -900,6 +900,43 @@ __global__ __launch_bounds__(256) void k_cell_fill(int64_t n, const int32_t *__r
     items[head[cell] + atomicAdd(&fill[cell], 1)] = (int32_t)i;
 }
 
+// Cut regions of the reference's geometry tools (geometry/geo1_CT_configuration.cpp:45-74,
+// isValid at :137-171; the dogbone tools repeat them): a point is valid if it lies in no cut.
+// inCircle: (x-cx)^2 + (y-cy)^2 <= r^2. inNotch: |y - ty| <= w and the point lies between the
+// lines through the tip of slope -+1/sqrt(3) on the side x > tx; the reverse notch opens to x < tx.
+// z is ignored. Written in the tools' operation order (file compiled without FMA contraction).
+__host__ __device__ inline bool cut_point_valid(double px, double py, const lpm_cut *cuts, int ncut) {
+    const double s3 = 1. / sqrt(3.);
+    for (int k = 0; k < ncut; ++k) {
+        const lpm_cut &c = cuts[k];
+        if (c.kind == LPM_CUT_CIRCLE) {
+            const double dx = px - c.x, dy = py - c.y;
+            if (dx * dx + dy * dy <= c.size * c.size) return false;
+        } else if (fabs(py - c.y) <= c.size) {
+            const double up = py - c.y;
+            if (c.kind == LPM_CUT_NOTCH) {
+                if (up >= s3 * (c.x - px) && up <= s3 * (-c.x + px)) return false;
+            } else if (up <= s3 * (c.x - px) && up >= s3 * (-c.x + px)) {
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
+// searchNeighbor's bond test (geometry/geo1_CT_configuration.cpp:95-108): the bond i->j is cut
+// if the midpoint of any of its 10 equal sub-segments is not valid.
+__host__ __device__ inline bool cut_bond_valid(const double *a, const double *b, const lpm_cut *cuts, int ncut) {
+    const int n_interval = 10;
+    const double dx = (b[0] - a[0]) / n_interval, dy = (b[1] - a[1]) / n_interval;
+    bool valid = true;
+    for (int k = 0; k < n_interval; ++k) {
+        const double x1 = a[0] + k * dx, y1 = a[1] + k * dy, x2 = a[0] + (k + 1) * dx, y2 = a[1] + (k + 1) * dy;
+        valid = cut_point_valid((x1 + x2) / 2.0, (y1 + y2) / 2.0, cuts, ncut) && valid;
+    }
+    return valid;
+}
+
 // insert j into the ascending row[0..m) and return m + 1
 __device__ __forceinline__ int sorted_insert(int32_t *row, int m, int32_t j) {
     int pos = m;
@@ -918,6 +955,7 @@ __global__ __launch_bounds__(256) void k_bond_search(int64_t n, const double *__
                                                       const int32_t *__restrict__ pcell, int64_t nc0, int64_t nc1,
                                                       int64_t nc2, const int32_t *__restrict__ head,
                                                       const int32_t *__restrict__ items, double c0, double c1,
+                                                      const lpm_cut *__restrict__ cuts, int ncut,
                                                       int32_t *__restrict__ tmp0, int32_t *__restrict__ tmp1,
                                                       int32_t *__restrict__ cnt0, int32_t *__restrict__ cnt1,
                                                       int32_t *__restrict__ flag) {
@@ -942,6 +980,7 @@ __global__ __launch_bounds__(256) void k_bond_search(int64_t n, const double *__
                                  ez = az - xyz[3 * (int64_t)j + 2];
                     const double d = sqrt(ex * ex + ey * ey + ez * ez);
                     if (!(d < c1)) continue;
+                    if (ncut > 0 && !cut_bond_valid(&xyz[3 * i], &xyz[3 * (int64_t)j], cuts, ncut)) continue;
                     if (n0 + n1 >= LPM_MAXB) {
                         over = true;
                         continue;
@@ -1185,7 +1224,7 @@ int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any) {
 
 // Device bond search and initial bond state for lpm_create (positions already in c->xyz).
 // Sets c->nbond, c->maxbond, c->gp_bonds, c->asm_hops and allocates the topology arrays.
-int device_topology(lpm_ctx *c, const double *xyz_host) {
+int device_topology(lpm_ctx *c, const double *xyz_host, const lpm_cut *cuts_host, int ncut) {
     const int64_t n = c->n;
     const double c1 = 1.01 * c->cutoff[1], c0 = 1.01 * c->cutoff[0];
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -1201,8 +1240,9 @@ int device_topology(lpm_ctx *c, const double *xyz_host) {
     if (ncell >= 0x7fffffffLL) return lpm_fail(c, LPM_E_LIMIT, "bond search grid too large");
     int32_t *pcell = nullptr, *count = nullptr, *head = nullptr, *items = nullptr, *tmp0 = nullptr, *tmp1 = nullptr,
             *cnt = nullptr, *flag = nullptr;
+    lpm_cut *dcuts = nullptr;
     auto cleanup = [&]() {
-        void *ps[] = {pcell, count, head, items, tmp0, tmp1, cnt, flag};
+        void *ps[] = {pcell, count, head, items, tmp0, tmp1, cnt, flag, dcuts};
         for (void *q : ps)
             if (q) hipFree(q);
     };
@@ -1211,6 +1251,10 @@ int device_topology(lpm_ctx *c, const double *xyz_host) {
         !al(&tmp1, n * LPM_MAXB) || !al(&cnt, 2 * n) || !al(&flag, 4)) {
         cleanup();
         return lpm_fail(c, LPM_E_NOMEM, "bond search scratch");
+    }
+    if (ncut > 0) {
+        if (hipMalloc(&dcuts, sizeof(lpm_cut) * ncut) != hipSuccess) { cleanup(); return lpm_fail(c, LPM_E_NOMEM, "cuts"); }
+        hipMemcpyAsync(dcuts, cuts_host, sizeof(lpm_cut) * ncut, hipMemcpyHostToDevice, c->stream);
     }
     int rc = LPM_OK;
     std::vector<int32_t> h_count(ncell), h_head(ncell + 1, 0), h_cnt(2 * n);
@@ -1229,7 +1273,7 @@ int device_topology(lpm_ctx *c, const double *xyz_host) {
     if ((rc = check_launch(c, "k_cell_fill"))) { cleanup(); return rc; }
     hipLaunchKernelGGL(k_bond_search, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const double *)c->xyz,
                        (const int32_t *)pcell, nc[0], nc[1], nc[2], (const int32_t *)head, (const int32_t *)items, c0, c1,
-                       tmp0, tmp1, cnt, cnt + n, flag);
+                       (const lpm_cut *)dcuts, ncut, tmp0, tmp1, cnt, cnt + n, flag);
     if ((rc = check_launch(c, "k_bond_search"))) { cleanup(); return rc; }
     int32_t h_flag[4] = {0, 0, 0, 0};
     hipMemcpyAsync(h_cnt.data(), cnt, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost, c->stream);
@@ -1374,4 +1418,12 @@ int launch_nonlocal_search(lpm_ctx *c, double ratio) {
     LPM_HIP(c, hipStreamSynchronize(c->stream));
     hipFree(counts);
     return rc;
+}
+
+extern "C" int lpm_cut_particles(const double *xyz, int64_t n, const lpm_cut *cuts, int32_t ncut, uint8_t *keep) {
+    if (n < 0 || (n > 0 && (!xyz || !keep)) || ncut < 0 || (ncut > 0 && !cuts)) return LPM_E_ARG;
+    for (int32_t k = 0; k < ncut; ++k)
+        if (cuts[k].kind < LPM_CUT_CIRCLE || cuts[k].kind > LPM_CUT_REVERSE_NOTCH) return LPM_E_ARG;
+    for (int64_t i = 0; i < n; ++i) keep[i] = cut_point_valid(xyz[3 * i], xyz[3 * i + 1], cuts, ncut) ? 1 : 0;
+    return LPM_OK;
 }

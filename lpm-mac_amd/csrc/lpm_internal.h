@@ -188,7 +188,7 @@ int launch_pattern(lpm_ctx *c);
 // joins a dump write still in flight and frees the writer's buffers (lpm_destroy)
 void dump_free(lpm_ctx *c);
 // bond search, reverse bonds and initial bond geometry on the device (lpm_create without bonds)
-int device_topology(lpm_ctx *c, const double *xyz_host);
+int device_topology(lpm_ctx *c, const double *xyz_host, const lpm_cut *cuts, int ncut);
 int launch_tslot(lpm_ctx *c);
 int launch_nonlocal_search(lpm_ctx *c, double ratio);
 int launch_residual_spec(lpm_ctx *c, std::vector<double> &r2, std::vector<double> &R2);

@@ -309,6 +309,31 @@ int lpm_build_bonds(int lattice, double radius, int64_t n, const double *xyz, in
     return LPM_OK;
 }
 
+// benchmark/AM_Ti64/generate_dump.ipynb, createPlate: a particle at the centre of every pixel
+// equal to 255, rows from the top of the image down, in row-major order
+int lpm_image_particles(const uint8_t *img, int64_t nrows, int64_t ncols, double pixel, double *xyz, int64_t cap,
+                        int64_t *n) {
+    if (!img || nrows < 0 || ncols < 0 || !(pixel > 0) || !n) return LPM_E_ARG;
+    const double radius = pixel / 2;
+    const double size_y = nrows * pixel;
+    int64_t m = 0;
+    for (int64_t j = 0; j < nrows; ++j) {
+        const double y = size_y - (radius + pixel * j);
+        for (int64_t i = 0; i < ncols; ++i) {
+            if (img[j * ncols + i] != 255) continue;
+            if (xyz) {
+                if (m >= cap) return LPM_E_LIMIT;
+                xyz[3 * m] = radius + pixel * i;
+                xyz[3 * m + 1] = y;
+                xyz[3 * m + 2] = 0.0;
+            }
+            ++m;
+        }
+    }
+    *n = m;
+    return LPM_OK;
+}
+
 int lpm_ce_from_e(int plane_stress, double E, double mu, double *Ce) {
     if (!Ce) return LPM_E_ARG;
     // particle_elastic.h:77-88

@@ -28,6 +28,20 @@ class Topology(ctypes.Structure):
                 ("bond_ptr", _IP), ("bond_split", _IP), ("bond_p2", _IP), ("device", ctypes.c_int32)]
 
 
+class Cut(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("x", _D), ("y", _D), ("size", _D)]
+
+
+CUT_CIRCLE, CUT_NOTCH, CUT_REVERSE_NOTCH = 0, 1, 2
+
+
+def _cut_array(cuts):
+    arr = (Cut * max(len(cuts), 1))()
+    for k, (kind, x, y, size) in enumerate(cuts):
+        arr[k] = Cut(int(kind), float(x), float(y), float(size))
+    return arr
+
+
 class DispBC(ctypes.Structure):
     _fields_ = [("group", ctypes.c_int32), ("mode", ctypes.c_int32), ("flag", ctypes.c_int32), ("step", _D)]
 
@@ -92,6 +106,9 @@ _SIGS = {
     "lpm_dump_async": (_I, [_P, _S, ctypes.c_int32, _DP]),
     "lpm_dump_wait": (_I, [_P]),
     "lpm_format_e4": (_I, [_DP, _L, ctypes.c_char_p, _L]),
+    "lpm_create_cut": (_I, [ctypes.POINTER(Topology), ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "lpm_cut_particles": (_I, [_DP, _L, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint8)]),
+    "lpm_image_particles": (_I, [ctypes.POINTER(ctypes.c_uint8), _L, _L, _D, _DP, _L, ctypes.POINTER(_L)]),
 }
 
 _lib = None
@@ -162,6 +179,27 @@ def unit_cell(lattice, radius, literal=1):
     return out, m.reshape(3, 3)
 
 
+def cut_particles(xyz, cuts):
+    """1 for the particles in no cut (the geometry tools' isValid), else 0."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    keep = np.zeros(len(xyz), dtype=np.uint8)
+    arr = _cut_array(cuts)
+    _check(lib().lpm_cut_particles(_dp(xyz), len(xyz), ctypes.cast(arr, ctypes.c_void_p), len(cuts),
+                                   keep.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+    return keep
+
+
+def image_particles(img, pixel):
+    """One particle per pixel equal to 255 (generate_dump.ipynb createPlate); img is nrows x ncols uint8."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    n = _L()
+    ip = img.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    _check(lib().lpm_image_particles(ip, img.shape[0], img.shape[1], pixel, None, 0, ctypes.byref(n)))
+    out = np.zeros((n.value, 3))
+    _check(lib().lpm_image_particles(ip, img.shape[0], img.shape[1], pixel, _dp(out), n.value, ctypes.byref(n)))
+    return out
+
+
 def format_e4(x):
     """The dump writer's "%.4e" formatting of x (host code of the library)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
@@ -190,9 +228,10 @@ _INT_FIELDS = {"type", "disp_constraint", "nb", "bond_ptr", "bond_split", "bond_
 class Context:
     """One particle system on one GPU (an Assembly + Solver of the reference)."""
 
-    def __init__(self, lattice, radius, model, xyz, bonds=None, device=0):
+    def __init__(self, lattice, radius, model, xyz, bonds=None, device=0, cuts=None):
         """bonds: (bond_ptr, bond_split, bond_p2) lists, None for the host search (lpm_build_bonds),
-        or "device" for the search inside lpm_create on the GPU."""
+        or "device" for the search inside lpm_create on the GPU; cuts (with bonds="device"):
+        [(kind, x, y, size)] holes and notches whose bonds the search drops (lpm_create_cut)."""
         L = lib()
         self.xyz0 = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         if isinstance(bonds, str):
@@ -205,7 +244,13 @@ class Context:
             self._bonds = tuple(np.ascontiguousarray(b, dtype=np.int32) for b in bonds)
             t = Topology(lattice, radius, model, len(self.xyz0), _dp(self.xyz0), *(_ip(b) for b in self._bonds), device)
         h = _P()
-        rc = L.lpm_create(ctypes.byref(t), ctypes.byref(h))
+        if cuts:
+            if bonds != "device":
+                raise ValueError("cuts apply to the device bond search (bonds='device')")
+            arr = _cut_array(cuts)
+            rc = L.lpm_create_cut(ctypes.byref(t), ctypes.cast(arr, ctypes.c_void_p), len(cuts), ctypes.byref(h))
+        else:
+            rc = L.lpm_create(ctypes.byref(t), ctypes.byref(h))
         if rc != 0:
             raise LPMError(f"lpm_create failed ({rc}): {L.lpm_last_error(None).decode()}")
         self.h = h

@@ -308,6 +308,40 @@ static void create_bonds(Sys &s) {
     }
 }
 
+// ---- geometry tools (geometry/geo1_CT_configuration.cpp; the dogbone tools repeat these)
+// inCircle (:45-49)
+static bool in_circle(const double *pt, double cx, double cy, double r) {
+    double dist_sq = std::pow(pt[0] - cx, 2.0) + std::pow(pt[1] - cy, 2.0);
+    return dist_sq <= r * r;
+}
+// inNotch (:52-62)
+static bool in_notch(const double *pt, double tx, double ty, double hf_width) {
+    if (std::abs(pt[1] - ty) <= hf_width) {
+        if ((pt[1] - ty) >= 1. / std::sqrt(3.) * (tx - pt[0]) && (pt[1] - ty) <= 1. / std::sqrt(3.) * (-tx + pt[0]))
+            return true;
+    }
+    return false;
+}
+// inReverseNotch (:64-74)
+static bool in_reverse_notch(const double *pt, double tx, double ty, double hf_width) {
+    if (std::abs(pt[1] - ty) <= hf_width) {
+        if ((pt[1] - ty) <= 1. / std::sqrt(3.) * (tx - pt[0]) && (pt[1] - ty) >= 1. / std::sqrt(3.) * (-tx + pt[0]))
+            return true;
+    }
+    return false;
+}
+// isValid (:137-171) with the tool's hard-coded cuts as a list: kind 0 circle (x, y, r),
+// 1 notch (tip x, tip y, half width), 2 reverse notch
+static bool is_valid(const double *pt, int ncut, const int *kind, const double *par) {
+    for (int k = 0; k < ncut; ++k) {
+        const double *q = &par[3 * k];
+        if (kind[k] == 0 && in_circle(pt, q[0], q[1], q[2])) return false;
+        if (kind[k] == 1 && in_notch(pt, q[0], q[1], q[2])) return false;
+        if (kind[k] == 2 && in_reverse_notch(pt, q[0], q[1], q[2])) return false;
+    }
+    return true;
+}
+
 // Assembly::createBonds restated literally (all pairs): used to pin the cell-grid version.
 static void create_bonds_bruteforce(Sys &s) {
     const int n = s.N;
@@ -1524,6 +1558,52 @@ void ora_set_hcf(void *p, int id, double L, int ps, double E, double mu, double 
 }
 
 void ora_search_nonlocal(void *p, double ratio) { search_nonlocal(*(Sys *)p, ratio); }
+
+// isValid over particles (the tools' filter, geometry/geo1_CT_configuration.cpp:194-196)
+void ora_cut_particles(int n, const double *xyz, int ncut, const int *kind, const double *par, unsigned char *keep) {
+    for (int i = 0; i < n; ++i) keep[i] = is_valid(&xyz[3 * i], ncut, kind, par) ? 1 : 0;
+}
+// searchNeighbor (geometry/geo1_CT_configuration.cpp:79-134): all pairs, bonds within the cutoff
+// whose 10 sub-segment midpoints are all valid, listed per particle i in increasing j as
+// (layer, i, j). Returns the count; fills the arrays if cap allows.
+int ora_search_neighbor_cut(int lattice, double radius, int n, const double *xyz, int ncut, const int *kind,
+                            const double *par, int cap, int *layer, int *p1, int *p2) {
+    const Cell cell = make_cell(lattice, radius);
+    std::vector<std::vector<int>> nbr(n), lay(n);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) {
+            const double *a = &xyz[3 * i], *b = &xyz[3 * j];
+            double dis = std::sqrt(std::pow((b[0] - a[0]), 2) + std::pow((b[1] - a[1]), 2) + std::pow((b[2] - a[2]), 2));
+            if ((dis < 1.01 * cell.cutoff[1]) && (j != i)) {
+                bool valid = true;
+                int n_interval = 10;
+                double dx = (b[0] - a[0]) / n_interval;
+                double dy = (b[1] - a[1]) / n_interval;
+                double dz = (b[2] - a[2]) / n_interval;
+                for (int k = 0; k < n_interval; ++k) {
+                    double pt1[3] = {a[0] + k * dx, a[1] + k * dy, a[2] + k * dz};
+                    double pt2[3] = {a[0] + (k + 1) * dx, a[1] + (k + 1) * dy, a[2] + (k + 1) * dz};
+                    double mid[3] = {(pt1[0] + pt2[0]) / 2.0, (pt1[1] + pt2[1]) / 2.0, (pt1[2] + pt2[2]) / 2.0};
+                    valid = is_valid(mid, ncut, kind, par) && valid;
+                }
+                if (valid) {
+                    nbr[i].push_back(j);
+                    lay[i].push_back(dis < 1.01 * cell.cutoff[0] ? 0 : 1);
+                }
+            }
+        }
+    }
+    int m = 0;
+    for (int i = 0; i < n; ++i)
+        for (size_t k = 0; k < nbr[i].size(); ++k, ++m)
+            if (m < cap) {
+                layer[m] = lay[i][k];
+                p1[m] = i;
+                p2[m] = nbr[i][k];
+            }
+    return m;
+}
 void ora_update_geometry(void *p) { update_geometry(*(Sys *)p); }
 void ora_update_force_state(void *p) { update_force_state(*(Sys *)p); }
 void ora_update_rr(void *p) { update_rr(*(Sys *)p); }

@@ -58,6 +58,8 @@ def lib():
             "ora_set_j2_damage_threshold": (None, [_P, _I, _D]),
             "ora_set_hcf": (None, [_P, _I, _D, _I, _D, _D, _D, _D, _D, _D, _D]),
             "ora_search_nonlocal": (None, [_P, _D]),
+            "ora_cut_particles": (None, [_I, _DP, _I, _IP, _DP, ctypes.POINTER(ctypes.c_ubyte)]),
+            "ora_search_neighbor_cut": (_I, [_I, _D, _I, _DP, _I, _IP, _DP, _I, _IP, _IP, _IP]),
             "ora_update_geometry": (None, [_P]),
             "ora_update_force_state": (None, [_P]),
             "ora_update_rr": (None, [_P]),
@@ -113,6 +115,36 @@ def generate(lattice, box, radius, eulerflag=0, angles=(0.0, 0.0, 0.0)):
     out = np.zeros((n, 3))
     L.ora_generate(lattice, _dp(box), radius, eulerflag, _dp(ang), _dp(out), n)
     return out
+
+
+def _cuts(cuts):
+    """cuts: [(kind, x, y, size)] with kind 0 circle, 1 notch, 2 reverse notch."""
+    kind = np.ascontiguousarray([c[0] for c in cuts], dtype=np.int32)
+    par = np.ascontiguousarray([c[1:4] for c in cuts], dtype=np.float64).reshape(-1)
+    return kind, par
+
+
+def cut_particles(xyz, cuts):
+    """The geometry tools' isValid over particles (1 = kept)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    kind, par = _cuts(cuts)
+    keep = np.zeros(len(xyz), dtype=np.uint8)
+    lib().ora_cut_particles(len(xyz), _dp(xyz), len(kind), _ip(kind), _dp(par),
+                            keep.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+    return keep
+
+
+def search_neighbor_cut(lattice, radius, xyz, cuts):
+    """The geometry tools' searchNeighbor (all pairs): (layer, p1, p2) in the tools' order."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    kind, par = _cuts(cuts)
+    z = np.zeros(1, dtype=np.int32)
+    m = lib().ora_search_neighbor_cut(lattice, radius, len(xyz), _dp(xyz), len(kind), _ip(kind), _dp(par), 0,
+                                      _ip(z), _ip(z), _ip(z))
+    lay, p1, p2 = (np.zeros(m, dtype=np.int32) for _ in range(3))
+    lib().ora_search_neighbor_cut(lattice, radius, len(xyz), _dp(xyz), len(kind), _ip(kind), _dp(par), m,
+                                  _ip(lay), _ip(p1), _ip(p2))
+    return lay, p1, p2
 
 
 def rmatrix(flag, angles):

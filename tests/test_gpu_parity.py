@@ -93,6 +93,47 @@ def test_device_bond_search_bitwise(case):
     assert host.size("nbonds") > 0
 
 
+@pytest.mark.parametrize("case", ["ct_square", "hex_notch", "image"])
+def test_device_cut_search_matches_geometry_tool(case):
+    """lpm_create_cut (device search that drops the bonds the reference's geometry tools drop) on
+    specimens with holes and notches, after the tools' particle filter: bond lists, layer split
+    and order equal the oracle's all-pairs searchNeighbor (geometry/geo1_CT_configuration.cpp)."""
+    say("[gpu] device cut search", case)
+    import os
+
+    if case == "ct_square":
+        xyz = G.generate_lattice(G.SQUARE2D, [0, 40, 0, 40, 0, 1], 0.5)
+        lat, r = G.SQUARE2D, 0.5
+        cuts = [(G.CUT_CIRCLE, 30.0, 10.2, 4.0), (G.CUT_CIRCLE, 30.0, 30.2, 4.0), (G.CUT_NOTCH, 18.3, 20.5, 0.3)]
+    elif case == "hex_notch":
+        xyz, _ = hex_plate(50, 40)
+        lat, r = G.HEXAGON2D, 0.5
+        cuts = [(G.CUT_CIRCLE, 12.0, 17.0, 3.5), (G.CUT_REVERSE_NOTCH, 35.2, 17.75, 0.35)]
+    else:
+        img = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "G6a_XZ_128.npy"))
+        pixel = 0.0061
+        xyz = G.image_particles(img, pixel)
+        lat, r = G.SQUARE2D, pixel / 2
+        cuts = [(G.CUT_CIRCLE, 64 * pixel, 30 * pixel, 12 * pixel), (G.CUT_NOTCH, 40 * pixel, 64 * pixel, 0.3 * pixel)]
+    keep = G.cut_particles(xyz, cuts)
+    np.testing.assert_array_equal(keep, O.cut_particles(xyz, cuts))
+    xyz = np.ascontiguousarray(xyz[keep == 1])
+    g = G.Context(lat, r, G.ELASTIC, xyz, bonds="device", cuts=cuts)
+    lay, p1, p2 = O.search_neighbor_cut(lat, r, xyz, cuts)
+    full = O.search_neighbor_cut(lat, r, xyz, [])
+    say("   particles", len(xyz), "bonds", len(lay), "cut", len(full[0]) - len(lay))
+    assert len(lay) < len(full[0])
+    n = len(xyz)
+    bp = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(bp, p1 + 1, 1)
+    bp = np.cumsum(bp)
+    order = np.lexsort((np.arange(len(p1)), lay, p1))  # per particle: layer 0 then 1, list order kept
+    bs = bp[:-1] + np.bincount(p1[lay == 0], minlength=n)
+    np.testing.assert_array_equal(g.get("bond_ptr"), bp)
+    np.testing.assert_array_equal(g.get("bond_split"), bs)
+    np.testing.assert_array_equal(g.get("bond_p2"), p2[order])
+
+
 # ----------------------------------------------------------------- geometry, force, residual (FP64, bitwise)
 @pytest.mark.parametrize("model", [O.ELASTIC, O.ELASTIC_DAMAGE])
 def test_force_state_bitwise(model):

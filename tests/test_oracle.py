@@ -599,3 +599,72 @@ def test_j2_return_mapping_textbook_cases():
     b = np.arange(bptr[0], bptr[1])
     exp = dis[b] * dl_u * (cs[b, 0] ** 2 - 0.5 * cs[b, 1] ** 2 - 0.5 * cs[b, 2] ** 2)
     np.testing.assert_allclose(s.getd("dLp")[b] - dlp0[b], exp, rtol=1e-12, atol=1e-18)
+
+
+# ----------------------------------------------------------------- geometry tools (holes, notches, images)
+def test_geometry_cut_shapes_closed_form():
+    """The tools' inCircle / inNotch / inReverseNotch on points whose answer the geometry fixes:
+    circle boundary and just outside; the notch is the wedge of half-angle 30 degrees opening to
+    +x from the tip, clipped to |y - ty| <= half width; the reverse notch opens to -x."""
+    s3 = 1 / math.sqrt(3)
+    pts = np.array([[3.0, 0.0, 0], [3.0 + 1e-9, 0.0, 0], [0.0, 0.0, 0],            # circle r=3 at origin
+                    [11.0, 5.0, 0], [9.0, 5.0, 0], [11.0, 5.0 + 0.9 * s3, 0],         # notch tip (10, 5), hw 2
+                    [11.0, 5.0 + 1.1 * s3, 0], [20.0, 7.5, 0], [20.0, 6.5, 0],
+                    [29.0, 5.0, 0], [31.0, 5.0, 0]])                                   # reverse notch tip (30, 5)
+    removed_circle = O.cut_particles(pts, [(0, 0.0, 0.0, 3.0)]) == 0
+    assert list(removed_circle[:3]) == [True, False, True]
+    removed_notch = O.cut_particles(pts, [(1, 10.0, 5.0, 2.0)]) == 0
+    # inside the wedge and the band; behind the tip; inside the wedge; outside the wedge; outside the band; inside
+    assert list(removed_notch[3:9]) == [True, False, True, False, False, True]
+    removed_rev = O.cut_particles(pts, [(2, 30.0, 5.0, 2.0)]) == 0
+    assert list(removed_rev[9:11]) == [True, False]
+
+
+def _seg_point_distance(a, b, c):
+    ab, ac = b - a, c - a
+    t = np.clip(np.dot(ac, ab) / np.dot(ab, ab), 0.0, 1.0)
+    return np.linalg.norm(a + t * ab - c)
+
+
+def test_search_neighbor_cut_against_exact_geometry():
+    """searchNeighbor of the tools: with no cut it equals createBonds (same bonds, same per-layer
+    order); with a hole, a bond is dropped when its segment passes well inside the circle and
+    kept when it stays outside (the 10-midpoint rule decides only grazing segments, skipped)."""
+    xyz = O.generate(O.SQUARE2D, [0, 20, 0, 20, 0, 1], 0.5)
+    lay, p1, p2 = O.search_neighbor_cut(O.SQUARE2D, 0.5, xyz, [])
+    s = O.System(O.SQUARE2D, 0.5, O.ELASTIC, xyz, bruteforce=True)
+    bp, bsp, q2 = s.geti("bptr"), s.geti("bsplit"), s.geti("p2")
+    for i in range(len(xyz)):
+        sel = p1 == i
+        np.testing.assert_array_equal(p2[sel][lay[sel] == 0], q2[bp[i]:bsp[i]])
+        np.testing.assert_array_equal(p2[sel][lay[sel] == 1], q2[bsp[i]:bp[i + 1]])
+    c, r = np.array([10.3, 9.7, 0.0]), 4.2
+    kept = xyz  # the bond rule alone (the tools also drop the particles inside, see above)
+    lay, p1, p2 = O.search_neighbor_cut(O.SQUARE2D, 0.5, kept, [(0, c[0], c[1], r)])
+    present = set(zip(p1.tolist(), p2.tolist()))
+    full = O.search_neighbor_cut(O.SQUARE2D, 0.5, kept, [])
+    checked = 0
+    for i, j in zip(full[1].tolist(), full[2].tolist()):
+        d = _seg_point_distance(kept[i], kept[j], c)
+        if d < r - 0.2:
+            assert (i, j) not in present
+            checked += 1
+        elif d > r:
+            assert (i, j) in present
+            checked += 1
+    assert checked > 0.9 * len(full[0]) and len(present) < len(full[0])
+
+
+def test_image_particles_from_reference_image():
+    """createPlate of the image-to-dump notebook on the reference's G6a_XZ_128 image: one particle
+    per white pixel, at the pixel centre with rows counted from the top, in row-major order."""
+    import lpmgpu as G
+
+    img = np.load(os.path.join(HERE, "golden", "G6a_XZ_128.npy"))
+    pixel = 0.0061
+    xyz = G.image_particles(img, pixel)
+    rows, cols = np.nonzero(img == 255)
+    assert len(xyz) == len(rows) == 14749
+    np.testing.assert_allclose(xyz[:, 0], (cols + 0.5) * pixel, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(xyz[:, 1], (img.shape[0] - rows - 0.5) * pixel, rtol=0, atol=1e-15)
+    assert np.all(xyz[:, 2] == 0)
