@@ -321,7 +321,9 @@ def main():
             "spmv_store": "half (symmetric)" if spmv_sym_active else "full",
             "spmv_launch": {"slots_unrolled": int(ctx.get_mode("spmv_unr")),
                             "wavefronts_per_workgroup": int(ctx.get_mode("spmv_wpb")),
-                            "xcd_remap": int(ctx.get_mode("spmv_remap")), "tuned": int(ctx.get_mode("spmv_tune"))},
+                            "xcd_remap": int(ctx.get_mode("spmv_remap")), "tuned": int(ctx.get_mode("spmv_tune")),
+                            "wavefronts_per_slice": int(ctx.get_mode("spmv_coop")) or 1,
+                            "slices_per_workgroup": int(ctx.get_mode("spmv_spw")) if ctx.get_mode("spmv_coop") else None},
             "solves_on_half_store": sym_solves, "cg_solves": cg_solves,
             "newton_iters_per_step": counts[0], "cg_iters_per_step": counts[1],
             "final_residual_over_reaction": nr / nR if nR > 0 else None,

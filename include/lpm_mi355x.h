@@ -173,7 +173,10 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *  "spmv_tune"        1: on the first solve of each matrix kind time every SpMV launch
  *                        configuration and keep the fastest | 0: use the values below
  *  "spmv_unr" (1,2,4), "spmv_wpb" (wavefronts per workgroup 1,2,4,8), "spmv_remap" (0/1:
- *                        XCD-contiguous workgroup order) -- the SpMV launch configuration
+ *                        XCD-contiguous workgroup order), "spmv_coop" (0: one wavefront per
+ *                        64-row slice | 8, 16: that many wavefronts share each slice),
+ *                        "spmv_spw" (consecutive slices per cooperative workgroup) -- the SpMV
+ *                        launch configuration
  *  "max_NR_iter", "tol_NR"  Newton limits (Solver ctor arguments, solver.h:39-50)
  *  "verbose"          print per-iteration norms like the reference
  */

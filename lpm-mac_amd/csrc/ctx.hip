@@ -809,6 +809,12 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     } else if (w == "spmv_wpb") {
         if (v != 1 && v != 2 && v != 4 && v != 8) return lpm_fail(c, LPM_E_ARG, "spmv_wpb must be 1, 2, 4 or 8");
         c->spmv_wpb = (int)v;
+    } else if (w == "spmv_coop") {
+        if (v != 0 && v != 8 && v != 16) return lpm_fail(c, LPM_E_ARG, "spmv_coop must be 0, 8 or 16");
+        c->spmv_coop = (int)v;
+    } else if (w == "spmv_spw") {
+        if (v < 1 || v > 64) return lpm_fail(c, LPM_E_ARG, "spmv_spw must be 1..64");
+        c->spmv_spw = (int)v;
     } else if (w == "spmv_remap") c->spmv_remap = (int)v ? 1 : 0;
     else if (w == "spmv_sym") {
         c->spmv_sym = (int)v ? 1 : 0;
@@ -842,6 +848,8 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "spmv_unr") return c->spmv_unr;
     if (w == "spmv_wpb") return c->spmv_wpb;
     if (w == "spmv_remap") return c->spmv_remap;
+    if (w == "spmv_coop") return c->spmv_coop;
+    if (w == "spmv_spw") return c->spmv_spw;
     return NAN;
 }
 

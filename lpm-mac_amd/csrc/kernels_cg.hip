@@ -153,6 +153,83 @@ __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const i
     block_partial<1>(v, partial, cap);
 }
 
+// Cooperative variant: the NWS wavefronts of a workgroup work through SPW consecutive slices
+// together, wavefront v taking slots v, v + NWS, ... of each slice, so a workgroup streams one
+// contiguous range of the matrix (a few thousand concurrent streams of one slice per wavefront
+// leave the DRAM pages less well used). The partial row sums meet in LDS and are added in
+// wavefront order, so the result does not depend on scheduling.
+template <typename T, int D, int VP, bool FUSED, int UNR, int NWS>
+__global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslice, int spw,
+                                                         const int64_t *__restrict__ sp, const int32_t *__restrict__ cptr,
+                                                         const int32_t *__restrict__ scol, const T *__restrict__ aval,
+                                                         const Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ w,
+                                                         double *partial, int cap, const double *scal, int remap) {
+    if (FUSED && scal[SC_DONE] != 0.0) return;
+    constexpr int EB = D * D;
+    __shared__ T red[NWS][D][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t grp = xcd_block(remap);
+    double delta = 0.0;
+    for (int t = 0; t < spw; ++t) {
+        const int64_t k = grp * spw + t;
+        if (k >= nslice) break;  // uniform over the workgroup
+        const int64_t row = k * 64 + lane;
+        const int len = row < n ? cptr[row + 1] - cptr[row] : 0;
+        const int64_t base = sp[k];
+        const int W = (int)((sp[k + 1] - base) >> 6);
+        const int32_t *ci = scol + base + lane;
+        const T *av = aval + EB * base + lane;
+        T acc[D];
+        for (int r = 0; r < D; ++r) acc[r] = (T)0;
+        for (int s0 = wv; s0 < W; s0 += NWS * UNR) {
+            int j[UNR];
+            Vec<T, VP> pj[UNR];
+            T e[UNR][EB];
+#pragma unroll
+            for (int q = 0; q < UNR; ++q) {
+                const int s = s0 + q * NWS;
+                j[q] = (s < len) ? ci[(int64_t)s * 64] : -1;
+            }
+#pragma unroll
+            for (int q = 0; q < UNR; ++q) {
+                const int s = s0 + q * NWS;
+                if (j[q] >= 0) {
+                    pj[q] = u[j[q]];
+#pragma unroll
+                    for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = av[((int64_t)s * EB + e2) * 64];
+                } else {
+                    for (int t2 = 0; t2 < VP; ++t2) pj[q].v[t2] = (T)0;
+                    for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = (T)0;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < UNR; ++q)
+                for (int r = 0; r < D; ++r)
+                    for (int c = 0; c < D; ++c) acc[r] += e[q][r * D + c] * pj[q].v[c];
+        }
+        for (int r = 0; r < D; ++r) red[wv][r][lane] = acc[r];
+        __syncthreads();
+        if (wv == 0 && row < n) {
+            Vec<T, VP> yv;
+            for (int t2 = 0; t2 < VP; ++t2) yv.v[t2] = (T)0;
+            for (int r = 0; r < D; ++r) {
+                T y = (T)0;
+                for (int q = 0; q < NWS; ++q) y += red[q][r][lane];
+                yv.v[r] = y;
+            }
+            w[row] = yv;
+            if (FUSED) {
+                const Vec<T, VP> ui = u[row];
+                for (int r = 0; r < D; ++r) delta += (double)ui.v[r] * (double)yv.v[r];
+            }
+        }
+        __syncthreads();  // red is reused by the next slice
+    }
+    if (!FUSED) return;
+    double v[1] = {delta};
+    block_partial<1>(v, partial, cap);
+}
+
 // ---------------------------------------------------------------- symmetric half store
 // When the stored matrix is symmetric (the analytic tangent of an undamaged or uniformly
 // damaged lattice with the diagonal filled symmetrically, Dirichlet rows and columns
@@ -718,12 +795,29 @@ int launch_precond_setup(lpm_ctx *c) {
     return c->dim == 3 ? precond_t<double, 3, 4>(c) : precond_t<double, 2, 2>(c);
 }
 
-static int spmv_grid(const lpm_ctx *c) { return blocks_for(c->nslice, c->spmv_wpb); }
+// workgroups of the SpMV launch (= partials the fused SpMV leaves)
+static int spmv_grid(const lpm_ctx *c) {
+    if (c->spmv_coop && !c->sym_ok) return blocks_for(c->nslice, c->spmv_spw);
+    return blocks_for(c->nslice, c->spmv_wpb);
+}
+
+template <typename T, int D, int VP, bool FUSED, int UNR, int NWS>
+static void spmv_coop_go(lpm_ctx *c, const void *in, void *out, double *part) {
+    typedef Vec<T, VP> V;
+    hipLaunchKernelGGL((k_spmv_coop<T, D, VP, FUSED, UNR, NWS>), dim3(spmv_grid(c)), dim3(64 * NWS), 0, c->stream, c->n,
+                       c->nslice, c->spmv_spw, c->sp, c->cptr, c->scol, (const T *)c->aval, (const V *)in, (V *)out, part,
+                       c->partial_cap, (const double *)c->scal, c->spmv_remap);
+}
 
 template <typename T, int D, int VP, bool FUSED, int UNR>
 static void spmv_go(lpm_ctx *c, const void *in, void *out, double *part) {
     typedef Vec<T, VP> V;
     const int g = spmv_grid(c), tb = 64 * c->spmv_wpb;
+    if (c->spmv_coop && !c->sym_ok) {
+        if (c->spmv_coop == 16) spmv_coop_go<T, D, VP, FUSED, (UNR > 1 ? 2 : 1), 16>(c, in, out, part);
+        else spmv_coop_go<T, D, VP, FUSED, (UNR > 1 ? 2 : 1), 8>(c, in, out, part);
+        return;
+    }
     if (c->sym_ok)
         hipLaunchKernelGGL((k_spmv_sym<T, D, VP, FUSED, UNR>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice, c->cptr,
                            c->dslot, c->sp_u, c->ucol, (const T *)c->uval, c->sp_l, c->lcol, c->lpos, (const V *)in, (V *)out,
@@ -748,7 +842,7 @@ static int spmv_launch(lpm_ctx *c, bool fused, void *in, void *out) {
     default:
         fused ? spmv_go<T, D, VP, true, 2>(c, in, out, part) : spmv_go<T, D, VP, false, 2>(c, in, out, part);
     }
-    return check_launch(c, c->sym_ok ? "k_spmv_sym" : "k_spmv");
+    return check_launch(c, c->sym_ok ? "k_spmv_sym" : (c->spmv_coop ? "k_spmv_coop" : "k_spmv"));
 }
 
 // Launch-configuration tuner: times the fused SpMV the CG runs (v_p -> v_w, delta partials;
@@ -757,20 +851,35 @@ template <typename T, int D, int VP>
 static int spmv_tune_t(lpm_ctx *c) {
     // 4 or 8 wavefronts per workgroup: the fused SpMV leaves one partial per workgroup, which
     // every workgroup of the next vector kernel sums, so fewer workgroups keep that cheap
-    static const int unrs[] = {1, 2, 4}, wpbs[] = {4, 8}, remaps[] = {0, 1};
+    // per-wavefront slices: unroll 1, 2, 4 x 4 or 8 wavefronts per workgroup; cooperative
+    // slices (full store only): 8 or 16 wavefronts per slice x unroll 1, 2 x 2 or 4 slices per
+    // workgroup; each with and without the XCD remap
+    struct Cfg {
+        int coop, unr, wpb, spw;
+    };
+    std::vector<Cfg> cfgs;
+    for (int un : {1, 2, 4})
+        for (int wp : {4, 8}) cfgs.push_back({0, un, wp, 1});
+    if (!c->sym_ok)
+        for (int nw : {8, 16})
+            for (int un : {1, 2})
+                for (int sw : {2, 4}) cfgs.push_back({nw, un, c->spmv_wpb, sw});
     hipEvent_t a, b;
     LPM_HIP(c, timing_event(&a));
     LPM_HIP(c, timing_event(&b));
     double best = 1e300;
-    int bu = 2, bw = 4, br = 1;
+    Cfg bc{0, 2, 4, 1};
+    int br = 1;
     const int reps = 6;
     // the fused kernel returns at once while the stop flag of the last solve is still set
     LPM_HIP(c, hipMemsetAsync(c->scal + SC_DONE, 0, sizeof(double), c->stream));
-    for (int un : unrs)
-        for (int wp : wpbs)
-            for (int rm : remaps) {
-                c->spmv_unr = un;
-                c->spmv_wpb = wp;
+    for (const Cfg &cf : cfgs)
+        for (int rm : {0, 1}) {
+            {
+                c->spmv_coop = cf.coop;
+                c->spmv_unr = cf.unr;
+                c->spmv_wpb = cf.wpb;
+                c->spmv_spw = cf.spw;
                 c->spmv_remap = rm;
                 int rc = spmv_launch<T, D, VP>(c, true, c->v_p, c->v_w);
                 if (rc) return rc;
@@ -783,15 +892,17 @@ static int spmv_tune_t(lpm_ctx *c) {
                 LPM_HIP(c, hipEventElapsedTime(&ms, a, b));
                 if (ms < best) {
                     best = ms;
-                    bu = un;
-                    bw = wp;
+                    bc = cf;
                     br = rm;
                 }
             }
+        }
     hipEventDestroy(a);
     hipEventDestroy(b);
-    c->spmv_unr = bu;
-    c->spmv_wpb = bw;
+    c->spmv_coop = bc.coop;
+    c->spmv_unr = bc.unr;
+    c->spmv_wpb = bc.wpb;
+    c->spmv_spw = bc.spw;
     c->spmv_remap = br;
     c->tune_best_ms = best / reps;
     return LPM_OK;

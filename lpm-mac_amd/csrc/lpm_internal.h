@@ -79,6 +79,7 @@ struct lpm_ctx {
     // with spmv_tune the fastest of all configurations is measured on the first solve of each
     // matrix kind (store, precision) and kept (the pattern never changes)
     int spmv_unr = 2, spmv_wpb = 4, spmv_remap = 1, spmv_tune = 1, tune_key = -1;
+    int spmv_coop = 0, spmv_spw = 1;  // cooperative slices: wavefronts per slice (0: one wavefront each), slices per workgroup
     double tune_best_ms = 0;  // mean time of the fused SpMV in the configuration the tuner kept
     // Two-level preconditioner (precond == 2): Jacobi plus an additive coarse correction on
     // rigid-body modes of cubic aggregates of particles, M^-1 = D^-1 + P Ac^-1 P^T, Ac = P^T K P.
