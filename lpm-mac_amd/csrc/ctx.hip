@@ -251,6 +251,7 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
         nr[s] = std::sqrt(r2[s]);
         tolm[s] = std::max(nr[s], std::sqrt(R2[s]));
         active[s] = nr[s] > c->tol_NR * tolm[s];
+        if (!std::isfinite(nr[s]) || !std::isfinite(tolm[s])) failed[s] = 1, active[s] = 0;  // see newton_iteration
     }
     for (;;) {
         int nact = 0;
@@ -297,6 +298,7 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
         double wrel = 0.0;
         for (int s = 0; s < S; ++s) {
             nr[s] = std::sqrt(r2[s]);
+            if (active[s] && !std::isfinite(nr[s])) failed[s] = 1, active[s] = 0;
             if (active[s] && !(nr[s] > c->tol_NR * tolm[s])) active[s] = 0;
             if (active[s]) {
                 ++nleft;
@@ -332,6 +334,13 @@ int newton_iteration(lpm_ctx *c, int *n_iter) {
         printf("|  Norm of residual is %.5e, norm of reaction is %.5e, tolerance criterion is based on %s force\n",
                norm_residual, std::sqrt(R2), norm_residual > std::sqrt(R2) ? "residual" : "reaction");
     int ni = 0;
+    // A residual that is no longer finite (a solve that diverged once bonds broke through the
+    // specimen) fails the iteration; the reference's test `norm > tol * mult` is false for NaN
+    // and would report convergence.
+    if (!std::isfinite(norm_residual) || !std::isfinite(tol_multiplier)) {
+        *n_iter = c->max_NR_iter;
+        return LPM_OK;
+    }
     while (norm_residual > c->tol_NR * tol_multiplier) {
         if (++ni > c->max_NR_iter) {
             *n_iter = c->max_NR_iter;
@@ -354,6 +363,10 @@ int newton_iteration(lpm_ctx *c, int *n_iter) {
         c->st_newton++;
         if (c->verbose)
             printf("|  |  Norm of residual is %.3e, residual ratio is %.3e\n", norm_residual, norm_residual / tol_multiplier);
+        if (!std::isfinite(norm_residual)) {
+            *n_iter = c->max_NR_iter;
+            return LPM_OK;
+        }
     }
     *n_iter = ni;
     return LPM_OK;
@@ -462,6 +475,7 @@ int solve_problem(lpm_ctx *c, const lpm_load_step *steps, int32_t n_in, int unda
     std::vector<double> pex;
     int n_step = (int)load.size();
     for (int i = 0; i < n_step; i++) {
+        int halvings = 0;
         for (;;) {
             if (c->rollback_pex) {
                 pex.resize(3 * c->n);
@@ -473,6 +487,8 @@ int solve_problem(lpm_ctx *c, const lpm_load_step *steps, int32_t n_in, int unda
             if (fatigue) RC(solve_step_fatigue(c, &v, &ok));
             else RC(solve_step_static(c, &v, undamaged, &ok));
             if (ok) break;
+            if (++halvings > 30)  // the reference halves without end
+                return lpm_fail(c, LPM_E_STATE, "load step " + std::to_string(i) + " does not converge after 30 halvings");
             RC(reset_state(c, true));
             if (c->rollback_pex)
                 LPM_HIP(c, hipMemcpyAsync(c->Pex, pex.data(), sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));

@@ -1021,7 +1021,9 @@ class SolverStatic : public Solver<nlayer> {
         double dt = 1;
         std::vector<double> pex;
         for (int i = 0; i < n_step; i++) {
-            for (;;) {
+            for (int halvings = 0;; ++halvings) {
+                if (halvings > 30)  // the reference halves without end
+                    throw std::runtime_error("load step " + std::to_string(i + start_index) + " does not converge after 30 halvings");
                 printf("Loading step-%d, iteration starts:\n", i + start_index);
                 double t1 = omp_get_wtime();
                 if (lpm::options().rollback_pex) {
@@ -1102,7 +1104,9 @@ class SolverFatigue : public Solver<nlayer> {
         this->ass.push();
         int n_step = (int)load.size();
         for (int i = 0; i < n_step; i++) {
-            for (;;) {
+            for (int halvings = 0;; ++halvings) {
+                if (halvings > 30)  // the reference halves without end
+                    throw std::runtime_error("load step " + std::to_string(i + start_index) + " does not converge after 30 halvings");
                 printf("Loading step-%d, iteration starts:\n", i + start_index);
                 if (solveProblemStep(load[i])) break;
                 lpm::check(lpm_reset_state(this->ctx(), 1), this->ctx(), "resetStateVar");

@@ -30,13 +30,10 @@ def say(*a):
 
 
 def _csr_to_list(bp, bs, p2):
-    lay, p1, pp = [], [], []
-    for i in range(len(bp) - 1):
-        for b in range(bp[i], bp[i + 1]):
-            lay.append(0 if b < bs[i] else 1)
-            p1.append(i)
-            pp.append(p2[b])
-    return np.array(lay), np.array(p1), np.array(pp)
+    bp, bs = np.asarray(bp, dtype=np.int64), np.asarray(bs, dtype=np.int64)
+    p1 = np.repeat(np.arange(len(bp) - 1), np.diff(bp))
+    lay = (np.arange(bp[-1]) >= bs[p1]).astype(np.int64)
+    return lay, p1, np.asarray(p2)
 
 
 @pytest.mark.parametrize("precond,precision", [(0, 1), (2, 1), (2, 0)])
@@ -87,10 +84,10 @@ def _cracked_plate(n=40):
     lay, p1, pp = lay[keep], p1[keep], pp[keep]
     # back to CSR (same order: per particle, layer 0 then layer 1)
     n_ = len(xyz)
-    bp2 = np.zeros(n_ + 1, dtype=np.int32)
+    bp2 = np.zeros(n_ + 1, dtype=np.int64)
     np.add.at(bp2, p1 + 1, 1)
     bp2 = np.cumsum(bp2).astype(np.int32)
-    bs2 = np.array([bp2[i] + np.sum((p1 == i) & (lay == 0)) for i in range(n_)], dtype=np.int32)
+    bs2 = (bp2[:-1] + np.bincount(p1[lay == 0], minlength=n_)).astype(np.int32)
     return xyz, box, (lay, p1, pp), (bp2, bs2, pp.astype(np.int32))
 
 

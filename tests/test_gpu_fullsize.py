@@ -1,0 +1,218 @@
+"""Benchmark configurations 3 and 4 at their full sizes, checked through properties that do not
+depend on size (the oracle would take hours there), plus the same physics at a size the oracle
+solves.
+
+Config 3: FCC single crystal of 40^3 cubic cells (256,009 particles in the rotated box), random
+lattice rotation (seed 7), J2 plasticity (E = 200 GPa, nu = 0.3, sigma_y = 300 MPa, linear
+hardening H = 2 GPa, no damage), uniaxial tension to 2% in 50 steps.
+Config 4: 2D hex plate of 400 x 400 (160,000 particles) with an edge crack of a quarter of the
+width at mid-height, bond breaking at a critical stretch of 1e-3, mode-I opening over 40 steps;
+then the adjoint d(top reaction)/dKn at the final state (broken-bond set frozen) against
+central differences of the same solver."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from scenarios import G, Pair, hex_plate, rel
+from test_gpu_configs import _cracked_plate
+
+pytestmark = pytest.mark.gpu
+
+E, NU, SY, H = 200e3, 0.3, 300.0, 2e3
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _fcc(ncell):
+    r = 0.5 / math.sqrt(2)  # cubic cell edge 2 sqrt(2) r = 1
+    rng = np.random.default_rng(7)
+    ang = tuple(rng.uniform(0, 2 * math.pi, 3))
+    box = [0, ncell, 0, ncell, 0, ncell]
+    xyz = G.generate_lattice(G.FCC3D, box, r, 0, ang)
+    z = xyz[:, 2]
+    lo, hi = np.nonzero(z < z.min() + 0.4)[0], np.nonzero(z > z.max() - 0.4)[0]
+    return xyz, box, r, lo, hi, z.max() - z.min()
+
+
+def test_config3_reduced_fcc_j2_parity():
+    """Config 3's material and loading on 4^3 cells (258 particles), 10 steps to 2% strain: FP64
+    GPU against the FP64 oracle, displacements and equivalent plastic strain at every step."""
+    say("[gpu] config 3 reduced (FCC 4^3 cells, J2)")
+    xyz, box, r, lo, hi, L = _fcc(4)
+    p = Pair(O.FCC3D, r, O.J2, xyz, box)
+    p.set_j2(0.0, False, E, NU, SY, 0.0, H, 0.0, 1.0, 1.0)
+    disp = [(lo, "x", 0.0), (lo, "y", 0.0), (lo, "z", 0.0), (hi, "z", 0.02 * L / 10)]
+    p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=80, cg_transpose=0, diag10_literal=0)
+    p.g.mode(dirichlet_static=1, cg_tol=1e-12, tol_NR=1e-11, max_NR_iter=80, cg_maxit=20000, precision=1,
+             cg_transpose=0, diag10_literal=0)
+    for s in (p.o, p.g):
+        s.update_geometry()
+        s.update_force_state()
+    for k in range(10):
+        ls_o, ls_g = p.load(disp=disp)
+        assert p.o.solve_step_static(ls_o, -1)
+        assert p.g.solve_step_static(ls_g, -1)
+        u_o, u_g = p.o.getd("xyz") - xyz.ravel(), p.g.get("xyz") - xyz.ravel()
+        a_o = p.o.getd("state_var").reshape(-1, 7)[:, 0]
+        a_g = p.g.get("state_var").reshape(-1, 7)[:, 0]
+        assert rel(u_g, u_o) < 1e-8
+        np.testing.assert_allclose(a_g, a_o, rtol=1e-6, atol=1e-9 * max(a_o.max(), 1e-12))
+    say("   final disp rel err", rel(u_g, u_o), "alpha max", a_o.max())
+    assert a_o.max() > 0
+
+
+def test_config3_full_size_fcc_j2():
+    """256,009 particles, 50 steps to 2%, headline solver settings (FP32 two-level CG, forcing,
+    predictor, Newton 1e-10): every step converges to a residual below 1e-8 of the reaction; the
+    response is linear (equal increments) before yield; the load increment halves between
+    sigma_y/E and 2.5 sigma_y/E of strain; after yield the tangent is within a factor of two of
+    the uniaxial hardening ratio H/(E+H); at least 90% of the middle half has yielded."""
+    say("[gpu] config 3 full size")
+    xyz, box, r, lo, hi, L = _fcc(40)
+    assert len(xyz) == 256009
+    g = G.Context(G.FCC3D, r, G.J2, xyz, bonds="device")
+    g.set_j2(0.0, False, E, NU, SY, 0.0, H, 0.0, 1.0, damage_threshold=1.0, elmap_literal=0)
+    g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=2, cg_tol=1e-3, cg_forcing=1,
+           tol_NR=1e-10, max_NR_iter=50, cg_maxit=20000, predictor=1)
+    steps = 50
+    ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", 0.0), (lo, "z", 0.0), (hi, "z", 0.02 * L / steps)])
+    g.update_geometry()
+    g.update_force_state()
+    F = []
+    for k in range(steps):
+        assert g.solve_step_static(ls, -1), k
+        nr, nR = g.update_rr()
+        assert nr <= 1e-8 * nR, (k, nr / nR)
+        F.append(g.get("Pin").reshape(-1, 3)[hi, 2].sum())
+    dF = np.diff(np.r_[0.0, F])
+    strain = 0.02 * (np.arange(steps) + 1) / steps
+    say("   increments", [f"{v:.3e}" for v in dF[:7]], "... last", f"{dF[-1]:.3e}")
+    assert np.ptp(dF[:3]) < 0.01 * dF[0]
+    k_half = int(np.argmax(dF < 0.5 * dF[0]))
+    assert SY / E <= strain[k_half] <= 2.5 * SY / E
+    ratio = dF[-10:].mean() / dF[0]
+    say("   yield at strain", strain[k_half], "hardening ratio", ratio, "vs H/(E+H)", H / (E + H))
+    assert 0.5 * H / (E + H) < ratio < 2 * H / (E + H)
+    alpha = g.get("state_var").reshape(-1, 7)[:, 0]
+    z = xyz[:, 2]
+    bulk = (z > z.min() + 0.25 * L) & (z < z.max() - 0.25 * L)
+    yielded = float(np.mean(alpha[bulk] > 0))
+    say("   fraction of the middle half (in z) that yielded", yielded)
+    assert yielded > 0.9  # particles on the lateral surfaces, with fewer bonds, may stay elastic
+
+
+def _frozen_reaction(xyz, csr, bd, lo, hi, total, scale_kn0=1.0):
+    """One-shot FP64 solve of the cracked plate from rest with the broken-bond set frozen and
+    layer-0 Kn scaled: the top reaction."""
+    g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=csr)
+    g.set_elastic(0.0, True, 69e3, 0.3, 1e9, elmap_literal=0)
+    g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=1, precond=2, cg_tol=1e-12, tol_NR=5e-9,
+           max_NR_iter=60, cg_maxit=50000)
+    g.set("bdamage", bd)
+    bp, bs = g.get("bond_ptr"), g.get("bond_split")
+    own = np.repeat(np.arange(len(xyz)), np.diff(bp))
+    layer = np.arange(len(bd)) >= bs[own]  # True for layer-1 bonds
+    kn = g.get("Kn")
+    kn[~layer] *= scale_kn0
+    g.set("Kn", kn)
+    g.update_geometry()
+    g.update_force_state()
+    ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", -total), (hi, "x", 0.0), (hi, "y", total)])
+    assert g.solve_step_static(ls, -1)
+    return g.get("Pin").reshape(-1, 3)[hi, 1].sum(), ~layer
+
+
+def test_config4_full_size_breaking_and_adjoint():
+    """160,000 particles, 40 mode-I steps (opening 6e-4 per edge per step, Newton 1e-6 as config 1,
+    headline FP32 settings): every step converges; bonds break ahead of the initial crack tip within
+    three lattice spacings of the crack plane, each together with its reverse bond; then, at the final state solved tightly in FP64
+    with the broken set frozen, the adjoint dF/dKn summed over layer 0 equals the central
+    difference (step 1e-3) of two FP64 one-shot solves within 1e-5, and the full gradient satisfies
+    Euler's identity for a function homogeneous of degree one in (Kn, Tv) within 1e-9."""
+    say("[gpu] config 4 full size")
+    xyz, box, bl, csr = _cracked_plate(400)
+    assert len(xyz) == 160000
+    inc, steps = 0.0006, 40
+    g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=csr)
+    g.set_elastic(0.0, True, 69e3, 0.3, 1e-3, elmap_literal=0)
+    g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=2, cg_tol=1e-3, cg_forcing=1,
+           tol_NR=1e-6, max_NR_iter=50, cg_maxit=50000, predictor=1)
+    y = xyz[:, 1]
+    lo, hi = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
+    ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", -inc), (hi, "x", 0.0), (hi, "y", inc)])
+    g.update_geometry()
+    g.update_force_state()
+    for k in range(steps):
+        assert g.solve_step_static(ls, -1), k
+    bd = g.get("bdamage")
+    broken = np.nonzero(bd > 0.5)[0]
+    opp = g.get("opp")
+    say("   broken bonds", len(broken))
+    assert len(broken) > 0
+    np.testing.assert_array_equal(bd[opp], bd)
+    p1 = np.repeat(np.arange(len(xyz)), np.diff(g.get("bond_ptr")))
+    p2 = g.get("bond_p2")
+    y0 = 0.5 * (y.min() + y.max()) + 0.25
+    a = xyz[:, 0].min() + 0.25 * np.ptp(xyz[:, 0])
+    mid = 0.5 * (xyz[p1[broken]] + xyz[p2[broken]])
+    say("   broken-bond midpoints: |y - crack plane| max", np.abs(mid[:, 1] - y0).max(), "x min", mid[:, 0].min(),
+        "initial tip", a)
+    assert np.all(np.abs(mid[:, 1] - y0) < 3.0)  # mode I: the crack grows along its plane ...
+    assert np.all(mid[:, 0] > a - 1.0)  # ... from the initial tip
+    # final state solved tightly in FP64 with the broken set frozen, then the adjoint
+    g.set_elastic(0.0, True, 69e3, 0.3, 1e9, elmap_literal=0)
+    g.set("bdamage", bd)
+    # FP64 cancellation in positions of up to 400 at strains near 1e-4 puts the residual floor of
+    # this plate at 2.2e-9 of the reaction norm: Newton to 5e-9
+    g.mode(precision=1, cg_tol=1e-12, tol_NR=5e-9, predictor=0)
+    still = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", 0.0), (hi, "x", 0.0), (hi, "y", 0.0)])
+    assert g.solve_step_static(still, -1)
+    F, gKn, gTv, nref, rres = g.adjoint_reaction(g.group(hi), "y")
+    say("   F", F, "refinements", nref, "rel residual", rres)
+    assert rres < 1e-9
+    # F is homogeneous of degree one in (Kn, Tv) at fixed boundary displacements, so the full
+    # gradient satisfies Euler's identity: sum(gKn Kn) + sum(gTv Tv) = F
+    euler = np.sum(gKn * g.get("Kn")) + np.sum(gTv * g.get("Tv"))
+    say("   Euler identity rel err", abs(euler - F) / abs(F))
+    assert abs(euler - F) <= 1e-9 * abs(F)
+    h = 1e-3  # truncation ~h^2 = 1e-6; solve noise (~1e-9 of F) / h = 1e-6
+    total = inc * steps
+    Fp, l0 = _frozen_reaction(xyz, csr, bd, lo, hi, total, 1 + h)
+    Fm, _ = _frozen_reaction(xyz, csr, bd, lo, hi, total, 1 - h)
+    F0, _ = _frozen_reaction(xyz, csr, bd, lo, hi, total, 1.0)
+    assert F == pytest.approx(F0, rel=1e-7)
+    fd = (Fp - Fm) / (2 * h)
+    adj = np.sum(gKn[l0] * g.get("Kn")[l0])
+    say(f"   dF/d(Kn layer 0 scale): adjoint {adj:.10e} fd {fd:.10e} rel {abs(adj - fd) / abs(fd):.2e}")
+    assert abs(adj - fd) <= 1e-5 * abs(fd)
+
+
+def test_non_finite_residual_reports_failure():
+    """A residual that is not finite fails the step (the reference's test `norm > tol * mult` is
+    false for NaN and would report convergence), and the load-step driver gives up with an error
+    after 30 halvings instead of halving without end. The condition is forced from the host: one
+    bond stiffness set to NaN on a 40 x 40 plate."""
+    say("[gpu] non-finite residual")
+    xyz, box, bl, csr = _cracked_plate(40)
+    g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=csr)
+    g.set_elastic(0.0, True, 69e3, 0.3, 1e-3, elmap_literal=0)
+    g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=1, precond=1, cg_tol=1e-12, tol_NR=1e-6,
+           max_NR_iter=60, cg_maxit=20000)
+    y = xyz[:, 1]
+    lo, hi = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
+    ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", -0.0004), (hi, "x", 0.0), (hi, "y", 0.0004)])
+    g.update_geometry()
+    g.update_force_state()
+    assert g.solve_step_static(ls, -1)
+    kn = g.get("Kn")
+    kn[len(kn) // 2] = np.nan
+    g.set("Kn", kn)
+    assert not g.solve_step_static(ls, -1)
+    with pytest.raises(G.LPMError, match="30 halvings"):
+        g.solve_problem_static([ls])
