@@ -306,15 +306,44 @@ int lpm_format_e4(const double *x, int64_t n, char *out, int64_t cap);
  * updateForceState (particle.h:163-215, bond.h:30-39, particle_elastic.h:38-50) at the current
  * state: jv = (dPin/dx) v, both dim*n host arrays in DoF order. */
 int lpm_force_jvp(lpm_ctx *ctx, const double *v, double *jv);
+/* The transposed product y = (dPin/dx)^T w of the same Jacobian (not symmetric when the bonds
+ * of a particle's layer carry different damage, or a bond is broken and its reverse is not). */
+int lpm_force_vjp(lpm_ctx *ctx, const double *w, double *y);
 /* Adjoint sensitivity of F = sum over the particles of `group` of Pin[axis] (the group's
  * reaction, Solver::updateRR, solver.h:108-128) at the current converged state, with respect
- * to every bond's Kn and Tv (nbonds doubles each; either may be NULL). One adjoint solve
- * J_ff lambda = (dF/du)_free on the exact force Jacobian J (symmetric), by iterative
- * refinement whose inner solves run the CG on the assembled, Dirichlet-eliminated K; then
- * dF/dKn_b = (w_i - w_j).cs_b dLe_b (1 - d_b), dF/dTv_b = (w_i - w_j).cs_b dLe_total(i,l_b) (1 - d_b)
- * with w = e_group,axis - lambda. Reassembles K at the current state. */
+ * to every bond's Kn and Tv (nbonds doubles each; either may be NULL). One transposed solve
+ * J_ff^T lambda = (J^T e)_free on the exact force Jacobian J, by iterative refinement whose
+ * residuals use the exact product J^T (lpm_force_vjp) and whose inner solves run the CG on
+ * the assembled, Dirichlet-eliminated K; then, with w = e_group,axis - lambda and
+ * t_b = (cs_b.w_i + cs_opp(b).w_j)/2 (= (w_i - w_j).cs_b / 2 for an unbroken pair),
+ * dF/dKn_b = 2 t_b dLe_b (1 - d_b), dF/dTv_b = 2 t_b dLe_total(i,l_b) (1 - d_b).
+ * Reassembles K at the current state. */
 int lpm_adjoint_reaction(lpm_ctx *ctx, int32_t group, int32_t axis, double *dF_dKn, double *dF_dTv, double *F,
                          int32_t *refinements, double *rel_residual);
+/* Adjoint sensitivity of a linear objective of the current converged state,
+ *   G = sum_d w_pin[d] Pin[d] + sum_d w_x[d] x[d]
+ * (dim*n host arrays in DoF order, d = dim*id + axis; either may be NULL): reaction sums (w_pin
+ * on constrained DoFs) and displacement fields (w_x; x - x0 has the same derivative as x).
+ * Returns dG/dKn, dG/dTv and dG/dbdamage per bond (nbonds doubles each, any may be NULL) and G.
+ * One transposed solve J_ff^T lambda = (J^T w_pin + w_x)_free as in lpm_adjoint_reaction, then
+ * with w = w_pin - lambda: dG/dKn_b and dG/dTv_b as there, and
+ * dG/dbdamage_b = -2 t_b (Kn_b dLe_b + Tv_b dLe_total(i,l_b)) (zero on broken bonds). */
+int lpm_adjoint_linear(lpm_ctx *ctx, const double *w_pin, const double *w_x, double *dG_dKn, double *dG_dTv,
+                       double *dG_dbdamage, double *G, int32_t *refinements, double *rel_residual);
+
+/* ------------------------------------------------------------------ load-step checkpoints
+ * (no reference counterpart: the reference keeps only the last converged copy,
+ * storeStateVar / resetStateVar, particle.h:108-148, assembly.h:140-159)
+ * Save the state a converged step leaves in device slot `slot` (0 .. LPM_MAX_CHECKPOINTS-1):
+ * positions, external loads, constraint flags, damage, state variables and the bond state
+ * (dis, dLp, bforce, bdamage), each with its last-converged copy. Restore puts it back,
+ * recomputes geometry and forces, and leaves the matrix to be reassembled by the next step
+ * or adjoint. Slots are allocated on first save and kept until released (slot -1: all) or
+ * lpm_destroy. Used to run an adjoint at each step of a load path after the forward pass. */
+#define LPM_MAX_CHECKPOINTS 4096
+int lpm_checkpoint_save(lpm_ctx *ctx, int32_t slot);
+int lpm_checkpoint_restore(lpm_ctx *ctx, int32_t slot);
+int lpm_checkpoint_release(lpm_ctx *ctx, int32_t slot);
 
 /* ------------------------------------------------------------------ specimen ensembles
  * (no reference counterpart: the reference runs one specimen per process, particle.h:17)

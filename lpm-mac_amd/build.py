@@ -44,5 +44,29 @@ def build(verbose=False):
     return LIB
 
 
+def build_autograd(verbose=False):
+    """The torch autograd harness (host/lpm_autograd.cpp) as the Python module lpm_autograd next to
+    the library, linked to it through its C ABI. Host code only (g++ against torch's headers)."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+
+    tlib = os.path.join(os.path.dirname(torch.__file__), "lib")
+    out = os.path.join(HERE, "lpm_autograd" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out + ".tmp"
+    cmd = ["g++", "-shared", "-fPIC", "-O2", "-std=c++17", f"-D_GLIBCXX_USE_CXX11_ABI={int(torch._C._GLIBCXX_USE_CXX11_ABI)}",
+           "-DTORCH_API_INCLUDE_EXTENSION_H", "-DTORCH_EXTENSION_NAME=lpm_autograd",
+           *[f"-I{p}" for p in ce.include_paths()], "-I", sysconfig.get_paths()["include"], "-I", os.path.join(ROOT, "include"),
+           "-o", tmp, os.path.join(HERE, "host", "lpm_autograd.cpp"), f"-L{HERE}", "-llpm_mi355x", f"-L{tlib}", "-lc10",
+           "-ltorch", "-ltorch_cpu", "-ltorch_python", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{tlib}"]
+    run(cmd)
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_autograd(verbose=True)

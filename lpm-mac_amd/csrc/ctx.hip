@@ -759,6 +759,8 @@ int lpm_destroy(lpm_ctx *c) {
         if (p) hipFree(p);
     dump_free(c);
     coarse_free(c);
+    for (void *p : c->ckpt)
+        if (p) hipFree(p);
     for (auto &g : c->groups)
         if (g.ids) hipFree(g.ids);
     if (c->h_scal) hipHostFree(c->h_scal);
@@ -1206,6 +1208,22 @@ int lpm_force_jvp(lpm_ctx *c, const double *v, double *jv) {
     return rc;
 }
 
+int lpm_force_vjp(lpm_ctx *c, const double *w, double *y) {
+    if (!c || !w || !y) return LPM_E_ARG;
+    const int64_t nd = c->n * c->dim;
+    double *d = nullptr;
+    LPM_HIP(c, hipMalloc(&d, sizeof(double) * 2 * std::max<int64_t>(nd, 1)));
+    int rc = LPM_OK;
+    if (hipMemcpyAsync(d, w, sizeof(double) * nd, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "vjp upload");
+    if (!rc) rc = force_vjp(c, d, d + nd);
+    if (!rc && hipMemcpyAsync(y, d + nd, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "vjp download");
+    hipStreamSynchronize(c->stream);
+    hipFree(d);
+    return rc;
+}
+
 int lpm_adjoint_reaction(lpm_ctx *c, int32_t group, int32_t axis, double *dF_dKn, double *dF_dTv, double *F,
                          int32_t *refinements, double *rel_residual) {
     if (!c) return LPM_E_ARG;
@@ -1216,6 +1234,91 @@ int lpm_adjoint_reaction(lpm_ctx *c, int32_t group, int32_t axis, double *dF_dKn
     RC(adjoint_reaction(c, group, axis, dF_dKn, dF_dTv, F, &it, &rr));
     if (refinements) *refinements = it;
     if (rel_residual) *rel_residual = rr;
+    return LPM_OK;
+}
+
+int lpm_adjoint_linear(lpm_ctx *c, const double *w_pin, const double *w_x, double *dG_dKn, double *dG_dTv,
+                       double *dG_dbdamage, double *G, int32_t *refinements, double *rel_residual) {
+    if (!c) return LPM_E_ARG;
+    if (!w_pin && !w_x) return lpm_fail(c, LPM_E_ARG, "no objective weights");
+    const int64_t nd = c->n * c->dim;
+    double *d = nullptr;
+    LPM_HIP(c, hipMalloc(&d, sizeof(double) * 2 * std::max<int64_t>(nd, 1)));
+    int rc = LPM_OK;
+    if (w_pin && hipMemcpyAsync(d, w_pin, sizeof(double) * nd, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "adjoint weights upload");
+    if (!rc && w_x && hipMemcpyAsync(d + nd, w_x, sizeof(double) * nd, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "adjoint weights upload");
+    int it = 0;
+    double rr = 0;
+    if (!rc) rc = adjoint_linear(c, w_pin ? d : nullptr, w_x ? d + nd : nullptr, dG_dKn, dG_dTv, dG_dbdamage, G, &it, &rr);
+    hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (rc) return rc;
+    if (refinements) *refinements = it;
+    if (rel_residual) *rel_residual = rr;
+    return LPM_OK;
+}
+
+// The arrays a checkpoint holds, in buffer order: {pointer, doubles} (cons, int32, is stored
+// after them, padded to whole doubles).
+static std::vector<std::pair<double *, int64_t>> ckpt_fields(lpm_ctx *c) {
+    const int64_t n = c->n, nb = c->nbond;
+    return {{c->xyz, 3 * n},       {c->xyz_last, 3 * n}, {c->Pex, 3 * n},           {c->damage, n},
+            {c->damage_last, n},   {c->sv, LPM_NSV * n}, {c->sv_last, LPM_NSV * n}, {c->dis, nb},
+            {c->dis_last, nb},     {c->dLp, nb},         {c->dLp_last, nb},         {c->bforce, nb},
+            {c->bforce_last, nb},  {c->bdamage, nb},     {c->bdamage_last, nb}};
+}
+
+static int checkpoint_copy(lpm_ctx *c, int32_t slot, bool save) {
+    if (slot < 0 || slot >= LPM_MAX_CHECKPOINTS) return lpm_fail(c, LPM_E_ARG, "checkpoint slot out of range");
+    const auto f = ckpt_fields(c);
+    int64_t tot = 0;
+    for (auto &x : f) tot += x.second;
+    const int64_t ncons = (3 * c->n + 1) / 2;  // int32 flags in doubles
+    if ((int64_t)c->ckpt.size() <= slot) c->ckpt.resize(slot + 1, nullptr);
+    if (!save && !c->ckpt[slot]) return lpm_fail(c, LPM_E_STATE, "checkpoint slot " + std::to_string(slot) + " is empty");
+    if (!c->ckpt[slot]) LPM_HIP(c, hipMalloc(&c->ckpt[slot], sizeof(double) * (tot + ncons)));
+    double *buf = (double *)c->ckpt[slot];
+    int64_t off = 0;
+    for (auto &x : f) {
+        if (x.second > 0) {
+            if (save) LPM_HIP(c, hipMemcpyAsync(buf + off, x.first, sizeof(double) * x.second, hipMemcpyDeviceToDevice, c->stream));
+            else LPM_HIP(c, hipMemcpyAsync(x.first, buf + off, sizeof(double) * x.second, hipMemcpyDeviceToDevice, c->stream));
+        }
+        off += x.second;
+    }
+    if (save) LPM_HIP(c, hipMemcpyAsync(buf + off, c->cons, sizeof(int32_t) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    else LPM_HIP(c, hipMemcpyAsync(c->cons, buf + off, sizeof(int32_t) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    if (!save) {
+        c->pred_valid = false;  // the increment before the restored state is not known
+        c->k_ready = false;
+        c->k_version++;         // constraint flags changed: the coarse space is rebuilt with K
+        RC(geometry_force_state(c));
+    }
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    return LPM_OK;
+}
+
+int lpm_checkpoint_save(lpm_ctx *c, int32_t slot) {
+    if (!c) return LPM_E_ARG;
+    return checkpoint_copy(c, slot, true);
+}
+
+int lpm_checkpoint_restore(lpm_ctx *c, int32_t slot) {
+    if (!c) return LPM_E_ARG;
+    return checkpoint_copy(c, slot, false);
+}
+
+int lpm_checkpoint_release(lpm_ctx *c, int32_t slot) {
+    if (!c) return LPM_E_ARG;
+    if (slot < -1 || slot >= LPM_MAX_CHECKPOINTS) return lpm_fail(c, LPM_E_ARG, "checkpoint slot out of range");
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int32_t s = 0; s < (int32_t)c->ckpt.size(); ++s)
+        if ((slot == -1 || s == slot) && c->ckpt[s]) {
+            hipFree(c->ckpt[s]);
+            c->ckpt[s] = nullptr;
+        }
     return LPM_OK;
 }
 

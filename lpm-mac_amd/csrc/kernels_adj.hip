@@ -1,5 +1,6 @@
 // Tangent (Jacobian-vector product) of the internal force and the adjoint sensitivity of a
-// reaction sum with respect to the bond stiffness parameters Kn and Tv.
+// linear objective of the converged state, G = a.Pin + b.x (reaction sums and displacement
+// fields), with respect to the bond parameters Kn, Tv and the bond damage.
 //
 // The internal force Pin(x) is the one of Assembly::updateGeometry + updateForceState
 // (particle.h:163-215, bond.h:30-39, particle_elastic.h:38-50): with d_b = x_i - x_j,
@@ -12,8 +13,8 @@
 //   d f_b = (2 Kn_b d dis_b + 2 Tv_b sum_{layer} d dis) (1 - bdamage_b),
 //   d Pin_i = sum_b [ d cs_b (f_b + f_opp)/2 + cs_b (d f_b + d f_opp)/2 ].
 // The analytic stiffness of the reference (derivative.h) drops small terms (derivative.h:5),
-// so it differs from this Jacobian by O(strain); it serves as the preconditioner of the
-// adjoint solve, and the exact Jacobian above gives its residual.
+// so it differs from this Jacobian by O(strain); it serves as the inner operator of the
+// adjoint solve, whose residual comes from the exact transposed product J^T w (k_vjp_*).
 #include <algorithm>
 #include <vector>
 
@@ -69,6 +70,75 @@ __global__ __launch_bounds__(256) void k_jvp_particle(int64_t n, int dim, const 
     for (int k = 0; k < dim; ++k) jv[(int64_t)dim * i + k] = P[k];
 }
 
+// Transposed product y = J^T w of the same Jacobian. J is not symmetric once the bonds of a
+// particle's layer carry different damage (the dilatation term of f_b is scaled by that bond's
+// own 1 - d_b), or when a bond is broken and its reverse is not, so the adjoint needs J^T
+// itself. Transposing bond by bond, with o = opp(b), i the owner of b and j its partner
+// (cs, dis and the weights a vanish on a broken bond):
+//   t_b = (cs_b . w_i + cs_o . w_j) / 2,
+//   T(i, l) = sum over the layer-l bonds b of i of 2 Tv_b t_b (1 - d_b),
+//   c_b = 2 Kn_b t_b (1 - d_b) + T(i, l_b)   (0 for a broken bond),
+//   a_b = (f_b + f_o) / (2 dis_b),
+//   y_i = sum over the bonds b of i of
+//         a_b (I - cs_b cs_b^T) w_i - a_o (I - cs_o cs_o^T) w_j + c_b cs_b - c_o cs_o.
+// pass 1, one thread per particle: c_b of each of its bonds
+__global__ __launch_bounds__(256) void k_vjp_bond(int64_t n, int dim, const int32_t *__restrict__ bptr,
+                                                   const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2,
+                                                   const int32_t *__restrict__ opp, const double *__restrict__ cs,
+                                                   const double *__restrict__ Kn, const double *__restrict__ Tv,
+                                                   const double *__restrict__ bdamage, const double *__restrict__ w,
+                                                   double *__restrict__ cb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b0 = bptr[i], bm = bsplit[i], b1 = bptr[i + 1];
+    double wi[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k) wi[k] = w[(int64_t)dim * i + k];
+    double T[2] = {0.0, 0.0};
+    for (int b = b0; b < b1; ++b) {
+        const int64_t j = p2[b];
+        const int o = opp[b];
+        double t = 0.0;
+        for (int k = 0; k < dim; ++k) t += cs[3 * b + k] * wi[k] + cs[3 * (int64_t)o + k] * w[(int64_t)dim * j + k];
+        t *= 0.5;
+        cb[b] = t;  // t_b for now
+        T[b < bm ? 0 : 1] += 2.0 * Tv[b] * t * (1.0 - bdamage[b]);
+    }
+    for (int b = b0; b < b1; ++b) {
+        const bool broken = fabs(bdamage[b] - 1.0) < LPM_EPS;
+        cb[b] = broken ? 0.0 : 2.0 * Kn[b] * cb[b] * (1.0 - bdamage[b]) + T[b < bm ? 0 : 1];
+    }
+}
+
+// pass 2, one thread per particle: y_i
+__global__ __launch_bounds__(256) void k_vjp_particle(int64_t n, int dim, const int32_t *__restrict__ bptr,
+                                                       const int32_t *__restrict__ p2, const int32_t *__restrict__ opp,
+                                                       const double *__restrict__ dis, const double *__restrict__ cs,
+                                                       const double *__restrict__ bforce, const double *__restrict__ bdamage,
+                                                       const double *__restrict__ cb, const double *__restrict__ w,
+                                                       double *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double wi[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k) wi[k] = w[(int64_t)dim * i + k];
+    double Y[3] = {0, 0, 0};
+    for (int b = bptr[i]; b < bptr[i + 1]; ++b) {
+        const int64_t j = p2[b];
+        const int o = opp[b];
+        double wj[3] = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) wj[k] = w[(int64_t)dim * j + k];
+        const double cb_[3] = {cs[3 * b], cs[3 * b + 1], cs[3 * b + 2]};
+        const double co[3] = {cs[3 * (int64_t)o], cs[3 * (int64_t)o + 1], cs[3 * (int64_t)o + 2]};
+        const double f = 0.5 * (bforce[b] + bforce[o]);
+        const double ab = fabs(bdamage[b] - 1.0) < LPM_EPS ? 0.0 : f / dis[b];
+        const double ao = fabs(bdamage[o] - 1.0) < LPM_EPS ? 0.0 : f / dis[o];
+        const double pb = cb_[0] * wi[0] + cb_[1] * wi[1] + cb_[2] * wi[2];
+        const double po = co[0] * wj[0] + co[1] * wj[1] + co[2] * wj[2];
+        for (int k = 0; k < 3; ++k)
+            Y[k] += ab * (wi[k] - cb_[k] * pb) - ao * (wj[k] - co[k] * po) + cb[b] * cb_[k] - cb[o] * co[k];
+    }
+    for (int k = 0; k < dim; ++k) y[(int64_t)dim * i + k] = Y[k];
+}
+
 // Zero the entries of DoFs that the Dirichlet elimination removes (constrained, or of a fully
 // damaged particle; stiffness.h:272-293).
 __global__ __launch_bounds__(256) void k_mask_free(int64_t nd, int dim, const int32_t *__restrict__ cons,
@@ -108,26 +178,41 @@ __global__ __launch_bounds__(256) void k_add_x(int64_t n, const T *__restrict__ 
     for (int k = 0; k < D; ++k) lam[D * i + k] += (double)x[VP * i + k];
 }
 
-// per-bond sensitivities with w = e - lambda:
-//   dF/dKn_b = (w_i - w_j) . cs_b  dLe_b (1 - bdamage_b)
-//   dF/dTv_b = (w_i - w_j) . cs_b  dLe_total(i, l_b) (1 - bdamage_b)
+// r = r + v (the displacement part of the adjoint right-hand side)
+__global__ __launch_bounds__(256) void k_axpy1(int64_t nd, const double *__restrict__ v, double *__restrict__ r) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < nd) r[d] += v[d];
+}
+
+// per-bond sensitivities with w = e - lambda: f_b enters Pin_i as cs_b f_b / 2 and Pin_j as
+// cs_o f_b / 2 (o = opp(b), the reverse bond, cs_o = -cs_b unless one of the two is broken), so
+// with t_b = (cs_b . w_i + cs_o . w_j) / 2:
+//   dG/dKn_b = 2 t_b dLe_b (1 - bdamage_b)
+//   dG/dTv_b = 2 t_b dLe_total(i, l_b) (1 - bdamage_b)
+//   dG/dbdamage_b = -2 t_b (Kn_b dLe_b + Tv_b dLe_total(i, l_b))
+// (a broken bond has dLe = 0, so all three vanish there)
 __global__ __launch_bounds__(256) void k_param_grad(int64_t n, int dim, const int32_t *__restrict__ bptr,
                                                      const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2,
+                                                     const int32_t *__restrict__ opp,
                                                      const double *__restrict__ cs, const double *__restrict__ dLe,
                                                      const double *__restrict__ dLe_total, const double *__restrict__ bdamage,
+                                                     const double *__restrict__ Kn, const double *__restrict__ Tv,
                                                      const double *__restrict__ e, const double *__restrict__ lam,
-                                                     double *__restrict__ gKn, double *__restrict__ gTv) {
+                                                     double *__restrict__ gKn, double *__restrict__ gTv,
+                                                     double *__restrict__ gBd) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double wi[3] = {0, 0, 0};
     for (int k = 0; k < dim; ++k) wi[k] = e[(int64_t)dim * i + k] - lam[(int64_t)dim * i + k];
     for (int b = bptr[i]; b < bptr[i + 1]; ++b) {
-        const int64_t j = p2[b];
-        double s = 0.0;
-        for (int k = 0; k < dim; ++k) s += (wi[k] - (e[(int64_t)dim * j + k] - lam[(int64_t)dim * j + k])) * cs[3 * b + k];
-        const double w = 1.0 - bdamage[b];
+        const int64_t j = p2[b], o = opp[b];
+        double s = 0.0;  // 2 t_b
+        for (int k = 0; k < dim; ++k)
+            s += wi[k] * cs[3 * b + k] + (e[(int64_t)dim * j + k] - lam[(int64_t)dim * j + k]) * cs[3 * o + k];
+        const double w = 1.0 - bdamage[b], lt = dLe_total[2 * i + (b < bsplit[i] ? 0 : 1)];
         gKn[b] = s * dLe[b] * w;
-        gTv[b] = s * dLe_total[2 * i + (b < bsplit[i] ? 0 : 1)] * w;
+        gTv[b] = s * lt * w;
+        gBd[b] = -s * (Kn[b] * dLe[b] + Tv[b] * lt);
     }
 }
 
@@ -162,6 +247,24 @@ int force_jvp(lpm_ctx *c, const double *v, double *jv) {
     return rc;
 }
 
+// y = (dPin/dx)^T w at the current geometry and forces (both dim*n, device)
+int force_vjp(lpm_ctx *c, const double *w, double *y) {
+    const int g = blocks_for(c->n, 256);
+    double *cb = nullptr;
+    LPM_HIP(c, hipMalloc(&cb, sizeof(double) * std::max<int64_t>(c->nbond, 1)));
+    hipLaunchKernelGGL(k_vjp_bond, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->bptr, c->bsplit, c->p2, c->opp, c->cs,
+                       c->Kn, c->Tv, c->bdamage, w, cb);
+    int rc = check(c, "k_vjp_bond");
+    if (!rc) {
+        hipLaunchKernelGGL(k_vjp_particle, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->bptr, c->p2, c->opp, c->dis, c->cs,
+                           c->bforce, c->bdamage, cb, w, y);
+        rc = check(c, "k_vjp_particle");
+    }
+    hipStreamSynchronize(c->stream);
+    hipFree(cb);
+    return rc;
+}
+
 template <typename T, int D, int VP>
 static int add_x_t(lpm_ctx *c, double *lam) {
     hipLaunchKernelGGL((k_add_x<T, D, VP>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, (const T *)c->v_x, lam);
@@ -173,11 +276,12 @@ static int add_x(lpm_ctx *c, double *lam) {
     return c->dim == 3 ? add_x_t<double, 3, 4>(c, lam) : add_x_t<double, 2, 2>(c, lam);
 }
 
-// Adjoint of F = sum over the group's particles of Pin[axis] at the current state.
-int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, double *F, int *refinements,
-                     double *rel_residual) {
+// Adjoint of the linear objective G = a.Pin + b.x at the current state (a, b: dim*n device
+// arrays in DoF order; either may be null). On return gKn, gTv, gBd (host, nbond each, any may
+// be null) hold dG/dKn, dG/dTv and dG/dbdamage, and G its value.
+int adjoint_linear(lpm_ctx *c, const double *a_dev, const double *b_dev, double *gKn, double *gTv, double *gBd, double *G,
+                   int *refinements, double *rel_residual) {
     const int64_t n = c->n, nd = n * c->dim;
-    const Group &grp = c->groups[gid];
     int rc = geometry_force_state(c);
     if (rc) return rc;
     // K with constrained DoFs eliminated: the preconditioner of the exact-Jacobian solve
@@ -199,20 +303,24 @@ int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, do
     if (gd > c->partial_cap) return done(lpm_fail(c, LPM_E_LIMIT, "adjoint grid exceeds partial buffer"));
     if (hipMemsetAsync(buf, 0, sizeof(double) * 5 * nd, c->stream) != hipSuccess)
         return done(lpm_fail(c, LPM_E_HIP, "adjoint clear"));
-    if (grp.n > 0)
-        hipLaunchKernelGGL(k_indicator, dim3(blocks_for(grp.n, 256)), dim3(256), 0, c->stream, grp.n, grp.ids, c->dim, axis, e);
-    if ((rc = check(c, "k_indicator"))) return done(rc);
-    // g = (dF/du) restricted to the free DoFs = (J^T e)_free = (J e)_free (J is symmetric)
-    if ((rc = force_jvp(c, e, g))) return done(rc);
+    if (a_dev && hipMemcpyAsync(e, a_dev, sizeof(double) * nd, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        return done(lpm_fail(c, LPM_E_HIP, "adjoint weights"));
+    // g = (dG/dx) restricted to the free DoFs = (J^T a + b)_free
+    if (a_dev && (rc = force_vjp(c, e, g))) return done(rc);
+    if (b_dev) {
+        hipLaunchKernelGGL(k_axpy1, dim3(gd), dim3(256), 0, c->stream, nd, b_dev, g);
+        if ((rc = check(c, "k_axpy1"))) return done(rc);
+    }
     hipLaunchKernelGGL(k_mask_free, dim3(gd), dim3(256), 0, c->stream, nd, c->dim, c->cons, c->damage_visual, g);
     if ((rc = check(c, "k_mask_free"))) return done(rc);
-    // J_ff lambda = g by iterative refinement: r = g - J_ff lambda (exact, FP64), K_ff d = r (CG), lambda += d
+    // J_ff^T lambda = g by iterative refinement: r = g - J_ff^T lambda (exact, FP64), K_ff d = r (CG
+    // on the assembled stiffness, which approximates J), lambda += d
     double g2 = -1.0, r2 = 0.0;
     int it = 0;
     const int max_ref = 40;
     c->cg_tol = std::max(saved_tol, c->precision == 0 ? 1e-6 : 1e-12);
     for (;; ++it) {
-        if ((rc = force_jvp(c, lam, jv))) return done(rc);
+        if ((rc = force_vjp(c, lam, jv))) return done(rc);
         hipLaunchKernelGGL(k_mask_free, dim3(gd), dim3(256), 0, c->stream, nd, c->dim, c->cons, c->damage_visual, jv);
         hipLaunchKernelGGL(k_sub_norm, dim3(gd), dim3(256), 0, c->stream, nd, g, jv, r, c->partial, c->partial_cap);
         if ((rc = check(c, "k_sub_norm"))) return done(rc);
@@ -234,32 +342,57 @@ int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, do
     }
     if (refinements) *refinements = it;
     if (rel_residual) *rel_residual = g2 > 0 ? sqrt(r2 / g2) : 0.0;
-    // per-bond sensitivities, written straight into the caller's host arrays via a device buffer
+    // per-bond sensitivities, copied into the caller's host arrays
+    const int64_t nb = std::max<int64_t>(c->nbond, 1);
     double *gb = nullptr;
-    if (hipMalloc(&gb, sizeof(double) * 2 * std::max<int64_t>(c->nbond, 1)) != hipSuccess)
-        return done(lpm_fail(c, LPM_E_NOMEM, "adjoint output"));
+    if (hipMalloc(&gb, sizeof(double) * 3 * nb) != hipSuccess) return done(lpm_fail(c, LPM_E_NOMEM, "adjoint output"));
     hipLaunchKernelGGL(k_param_grad, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->dim, c->bptr, c->bsplit, c->p2,
-                       c->cs, c->dLe, c->dLe_total, c->bdamage, e, lam, gb, gb + c->nbond);
+                       c->opp, c->cs, c->dLe, c->dLe_total, c->bdamage, c->Kn, c->Tv, e, lam, gb, gb + nb, gb + 2 * nb);
     rc = check(c, "k_param_grad");
-    if (!rc && gKn) rc = hipMemcpyAsync(gKn, gb, sizeof(double) * c->nbond, hipMemcpyDeviceToHost, c->stream) == hipSuccess
-                             ? LPM_OK : lpm_fail(c, LPM_E_HIP, "adjoint copy");
-    if (!rc && gTv) rc = hipMemcpyAsync(gTv, gb + c->nbond, sizeof(double) * c->nbond, hipMemcpyDeviceToHost, c->stream) == hipSuccess
-                             ? LPM_OK : lpm_fail(c, LPM_E_HIP, "adjoint copy");
+    double *outs[3] = {gKn, gTv, gBd};
+    for (int k = 0; k < 3 && !rc; ++k)
+        if (outs[k] && hipMemcpyAsync(outs[k], gb + k * nb, sizeof(double) * c->nbond, hipMemcpyDeviceToHost, c->stream) !=
+                           hipSuccess)
+            rc = lpm_fail(c, LPM_E_HIP, "adjoint copy");
     hipStreamSynchronize(c->stream);
     hipFree(gb);
     if (rc) return done(rc);
-    if (F) {
-        // F from the current Pin (the reaction the step reports, solver.h:108-128)
-        std::vector<double> pin(3 * n);
-        std::vector<int32_t> ids(grp.n);
+    if (G) {
+        // G from the current Pin and positions, summed in DoF order
+        std::vector<double> pin(3 * n), x(3 * n), wa(a_dev ? nd : 0), wb(b_dev ? nd : 0);
         LPM_HIP(c, hipMemcpyAsync(pin.data(), c->Pin, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
-        if (grp.n) LPM_HIP(c, hipMemcpyAsync(ids.data(), grp.ids, sizeof(int32_t) * grp.n, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipMemcpyAsync(x.data(), c->xyz, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+        if (a_dev) LPM_HIP(c, hipMemcpyAsync(wa.data(), a_dev, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream));
+        if (b_dev) LPM_HIP(c, hipMemcpyAsync(wb.data(), b_dev, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream));
         LPM_HIP(c, hipStreamSynchronize(c->stream));
-        std::sort(ids.begin(), ids.end());
-        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
         double s = 0;
-        for (int32_t id : ids) s += pin[3 * (int64_t)id + axis];
-        *F = s;
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < c->dim; ++k) {
+                const int64_t d = (int64_t)c->dim * i + k;
+                if (a_dev && wa[d] != 0.0) s += wa[d] * pin[3 * i + k];
+                if (b_dev && wb[d] != 0.0) s += wb[d] * x[3 * i + k];
+            }
+        *G = s;
     }
     return done(LPM_OK);
+}
+
+// Adjoint of F = sum over the group's particles of Pin[axis] at the current state: the linear
+// objective with a = the indicator of (group particle, axis), each particle counted once.
+int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, double *F, int *refinements,
+                     double *rel_residual) {
+    const int64_t nd = c->n * c->dim;
+    const Group &grp = c->groups[gid];
+    double *a = nullptr;
+    LPM_HIP(c, hipMalloc(&a, sizeof(double) * std::max<int64_t>(nd, 1)));
+    int rc = LPM_OK;
+    if (hipMemsetAsync(a, 0, sizeof(double) * nd, c->stream) != hipSuccess) rc = lpm_fail(c, LPM_E_HIP, "adjoint clear");
+    if (!rc && grp.n > 0) {
+        hipLaunchKernelGGL(k_indicator, dim3(blocks_for(grp.n, 256)), dim3(256), 0, c->stream, grp.n, grp.ids, c->dim, axis, a);
+        rc = check(c, "k_indicator");
+    }
+    if (!rc) rc = adjoint_linear(c, a, nullptr, gKn, gTv, nullptr, F, refinements, rel_residual);
+    hipStreamSynchronize(c->stream);
+    hipFree(a);
+    return rc;
 }

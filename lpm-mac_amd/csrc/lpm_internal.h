@@ -115,6 +115,10 @@ struct lpm_ctx {
     // Asynchronous dump writer (io.hip): device snapshot -> pinned host copy on a side stream ->
     // formatted and appended by a writer thread; one write in flight per context
     struct DumpState *dump = nullptr;
+    // Load-step checkpoints (lpm_checkpoint_save / restore): one device buffer per slot with the
+    // state a converged step leaves (positions, loads, constraint flags, particle and bond state
+    // and their last-converged copies); null where the slot is empty
+    std::vector<void *> ckpt;
     // Specimen ensemble: independent particle systems (contiguous id ranges, no bond between
     // them) solved in the same launches, each with its own Newton test.
     int nspec = 0;                    // 0: one system
@@ -218,8 +222,11 @@ int calc_stiffness(lpm_ctx *c);
 int update_stiffness_disp_bc(lpm_ctx *c);
 // ---- tangent and adjoint (kernels_adj.hip)
 int force_jvp(lpm_ctx *c, const double *v, double *jv);
+int force_vjp(lpm_ctx *c, const double *w, double *y);
 int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, double *F, int *refinements,
                      double *rel_residual);
+int adjoint_linear(lpm_ctx *c, const double *a_dev, const double *b_dev, double *gKn, double *gTv, double *gBd, double *G,
+                   int *refinements, double *rel_residual);
 
 // helpers
 static inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -54,6 +54,10 @@ class LoadStepC(ctypes.Structure):
     _fields_ = [("n_disp", ctypes.c_int32), ("disp", ctypes.POINTER(DispBC)), ("n_force", ctypes.c_int32),
                 ("force", ctypes.POINTER(ForceBC))]
 
+    def address(self):
+        """The lpm_load_step* as an integer (for C++ callers such as lpm_autograd)."""
+        return ctypes.addressof(self)
+
 
 _SIGS = {
     "lpm_generate_lattice": (_I, [_I, _DP, _D, _I, _DP, _DP, _L, _LP]),
@@ -97,11 +101,16 @@ _SIGS = {
     "lpm_spmv_bench": (_I, [_P, ctypes.c_int32, _DP]),
     "lpm_copy_bench": (_I, [_P, ctypes.c_int64, ctypes.c_int32, _DP]),
     "lpm_force_jvp": (_I, [_P, _DP, _DP]),
+    "lpm_force_vjp": (_I, [_P, _DP, _DP]),
     "lpm_set_specimens": (_I, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "lpm_specimen_stat": (_I, [_P, _S, _DP]),
     "lpm_solve_problem_cyclic_specimens": (_I, [_P, ctypes.c_int32, ctypes.c_int32, _DP, _L, _IP, _DP, ctypes.c_int32,
                                                 _D, ctypes.c_int32, ctypes.c_int32, _IP]),
     "lpm_adjoint_reaction": (_I, [_P, ctypes.c_int32, ctypes.c_int32, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
+    "lpm_adjoint_linear": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
+    "lpm_checkpoint_save": (_I, [_P, ctypes.c_int32]),
+    "lpm_checkpoint_restore": (_I, [_P, ctypes.c_int32]),
+    "lpm_checkpoint_release": (_I, [_P, ctypes.c_int32]),
     "lpm_stream": (_P, [_P]),
     "lpm_dump_async": (_I, [_P, _S, ctypes.c_int32, _DP]),
     "lpm_dump_wait": (_I, [_P]),
@@ -269,6 +278,11 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    @property
+    def handle(self):
+        """The lpm_ctx* as an integer (for C++ callers such as lpm_autograd)."""
+        return self.h.value
 
     def _chk(self, rc):
         if rc != 0:
@@ -504,6 +518,39 @@ class Context:
     def specimen_stat(self, name):
         out = np.zeros(self._nspec)
         self._chk(lib().lpm_specimen_stat(self.h, name.encode(), out.ctypes.data_as(_DP)))
+        return out
+
+    def adjoint_linear(self, w_pin=None, w_x=None):
+        """dG/dKn, dG/dTv, dG/dbdamage per bond for G = w_pin . Pin + w_x . x (DoF-order weights,
+        either may be None). Returns (G, dKn, dTv, dbdamage, refinements, relative residual)."""
+        nd = self.n * self.size("dim")
+        nb = self.size("nbonds")
+        w = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (w_pin, w_x)]
+        for a in w:
+            assert a is None or a.size == nd
+        gk, gt, gd = np.zeros(nb), np.zeros(nb), np.zeros(nb)
+        G, rr, it = _D(), _D(), ctypes.c_int32()
+        ptr = [None if a is None else a.ctypes.data_as(_DP) for a in w]
+        self._chk(lib().lpm_adjoint_linear(self.h, ptr[0], ptr[1], gk.ctypes.data_as(_DP), gt.ctypes.data_as(_DP),
+                                           gd.ctypes.data_as(_DP), ctypes.byref(G), ctypes.byref(it), ctypes.byref(rr)))
+        return G.value, gk, gt, gd, it.value, rr.value
+
+    def checkpoint_save(self, slot):
+        self._chk(lib().lpm_checkpoint_save(self.h, int(slot)))
+
+    def checkpoint_restore(self, slot):
+        self._chk(lib().lpm_checkpoint_restore(self.h, int(slot)))
+
+    def checkpoint_release(self, slot=-1):
+        self._chk(lib().lpm_checkpoint_release(self.h, int(slot)))
+
+    def force_vjp(self, w):
+        """Transposed product (dPin/dx)^T w, DoF order."""
+        nd = self.n * self.size("dim")
+        w = np.ascontiguousarray(w, dtype=np.float64).ravel()
+        assert w.size == nd
+        out = np.zeros(nd)
+        self._chk(lib().lpm_force_vjp(self.h, w.ctypes.data_as(_DP), out.ctypes.data_as(_DP)))
         return out
 
     def force_jvp(self, v):
