@@ -32,10 +32,11 @@ HBM_PEAK = 8.0e12
 # --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
 #   bench.py --steps 1 --warmup 0 --no-cpu-baseline --pmc-calibrate
 # Each reading is scaled by what the same counter reports for k_copy_calib, which moves a known
-# 256 MiB in and 256 MiB out: FETCH_SIZE reads 131083.2 KiB for it (x1.9998), WRITE_SIZE
-# 262144.0 KiB (x1.0000). Mean over the fused CG SpMV launches that did work: FETCH_SIZE
-# 315397.1 KiB -> 645.9 MB; WRITE_SIZE 4122 KiB (the 3 MB output vector, padded to 4 entries).
-MEASURED_TRAFFIC = {(64, "k_spmv (CG SpMV, full store)"): 315397.1 * 1024 * 262144.0 / 131083.2}
+# 256 MiB in and 256 MiB out: FETCH_SIZE reads 131083.3 KiB for it (x1.9998), WRITE_SIZE
+# 262144.0 KiB (x1.0000). Mean over the fused CG SpMV launches that did work (the configuration
+# the tuner kept in that run): FETCH_SIZE 317832.7 KiB -> 650.9 MB; WRITE_SIZE 4121.8 KiB -> 4.2 MB
+# (the 3 MB output vector, padded to 4 entries). The tuner's other variants fetch 640-653 MB.
+MEASURED_TRAFFIC = {(64, "k_spmv (CG SpMV, full store)"): 317832.7 * 1024 * 262144.0 / 131083.3 + 4121.8 * 1024}
 
 
 def sc_workload(n):
