@@ -291,9 +291,13 @@ def main():
     # The same K steps again: without the predictor (the reference's Newton start), and with the
     # reference's algorithm (Jacobi, no predictor), whose iteration counts drive the CPU
     # projection; both rates are reported beside the headline.
-    jac = nopred = None
+    jac = nopred = graph = None
     jac_counts = counts
     if not args.no_cpu_baseline:
+        # the same steps with each CG batch replayed from a captured HIP graph (per-SpMV timing
+        # is off in that mode, so the headline and its roofline come from direct launches)
+        graph, _ = rerun(cg_graph=1)
+        ctx.mode(cg_graph=0)
         if args.predictor:
             nopred, _ = rerun(predictor=0)
         if args.precond != 0 or args.predictor:
@@ -331,7 +335,7 @@ def main():
             "parallelism": "single GPU",
             "setup_seconds": t_setup,
             "async_dump": dump,
-            "same_steps_without_predictor": nopred,
+            "same_steps_with_cg_graph": graph, "same_steps_without_predictor": nopred,
             "same_steps_reference_algorithm_jacobi_no_predictor": jac,
         },
         "roofline": {
@@ -404,10 +408,32 @@ def ensemble_bench(args):
     t0 = time.perf_counter()
     for _ in range(args.steps):
         if not ctx.solve_step_static(ls, -1):
-            raise RuntimeError("ensemble step did not converge")
+            bad = np.nonzero(ctx.specimen_stat("converged") != 1)[0]
+            raise RuntimeError(f"ensemble step did not converge: specimens {bad.tolist()}, Newton iterations "
+                               f"{ctx.specimen_stat('newton_iters')[bad].tolist()}")
     hip_sync(ctx)
     t_batch = time.perf_counter() - t0
     newton = ctx.specimen_stat("newton_iters")
+    # the next steps of the same load path with each CG batch replayed from a captured graph
+    ctx.mode(cg_graph=1)
+    ctx.solve_step_static(ls, -1)  # capture
+    hip_sync(ctx)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if not ctx.solve_step_static(ls, -1):
+            raise RuntimeError("ensemble step did not converge")
+    hip_sync(ctx)
+    t_graph = time.perf_counter() - t0
+    # and direct launches again over the next steps, so that the graph's rate sits between two
+    # direct runs of neighbouring steps
+    ctx.mode(cg_graph=0)
+    hip_sync(ctx)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if not ctx.solve_step_static(ls, -1):
+            raise RuntimeError("ensemble step did not converge")
+    hip_sync(ctx)
+    t_direct2 = time.perf_counter() - t0
     del ctx
     # the same specimens, one context each, back to back
     ctxs = []
@@ -428,6 +454,18 @@ def ensemble_bench(args):
     for c1, _ in ctxs:
         hip_sync(c1)
     t_seq = time.perf_counter() - t0
+    for c1, l1 in ctxs:
+        c1.mode(cg_graph=1)
+        c1.solve_step_static(l1, -1)
+        hip_sync(c1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for c1, l1 in ctxs:
+            if not c1.solve_step_static(l1, -1):
+                raise RuntimeError("single-specimen step did not converge")
+    for c1, _ in ctxs:
+        hip_sync(c1)
+    t_seq_graph = time.perf_counter() - t0
     total = int(first[-1])
     out = {
         "metric": "specimen load steps/s, 64 uneven hex specimens in one context",
@@ -445,7 +483,10 @@ def ensemble_bench(args):
         "config": {"workload": f"{S} hex plates, {total} particles, sizes {int(sizes.min())}-{int(sizes.max())}",
                    "precond": args.precond, "newton_iters_per_step_max": float(newton.max()),
                    "one_context_per_specimen_specimen_steps_per_s": S * args.steps / t_seq,
-                   "speedup_vs_one_context_per_specimen": t_seq / t_batch},
+                   "speedup_vs_one_context_per_specimen": t_seq / t_batch,
+                   "with_cg_graph": {"specimen_steps_per_s": S * args.steps / t_graph,
+                                     "direct_launches_next_steps_specimen_steps_per_s": S * args.steps / t_direct2,
+                                     "one_context_per_specimen_specimen_steps_per_s": S * args.steps / t_seq_graph}},
     }
     print(json.dumps(out), flush=True)
 
@@ -523,24 +564,31 @@ def fatigue_ensemble_bench(args):
         p2.append(b_p2 + first[-1])
         first.append(first[-1] + len(x))
     union = tuple(np.concatenate(a).astype(np.int32) for a in (bp, bs, p2))
-    ctx, groups, mat = make(parts, first[:-1], union, nspec=True)
-    for s in range(S):
-        mat[first[s]:first[s + 1], 0] = A[s]
-    ctx.set("mat", mat)
-    setup(ctx)
-    disp = []
-    for (top, bottom), a in zip(groups, amp):
-        disp += [(top, "x", 0.0), (top, "y", 0.1 * a), (top, "z", 0.0), (bottom, "x", 0.0), (bottom, "y", 0.0),
-                 (bottom, "z", 0.0)]
-    hip_sync(ctx)
-    t0 = time.perf_counter()
-    ctx.solve_problem_fatigue_static([ctx.load_step(disp=disp)])
-    n_done = ctx.solve_problem_cyclic_specimens(G.LOAD_DOGBONE_DISP, 1, spec, groups, amp, tau, 4)
-    hip_sync(ctx)
-    t_batch = time.perf_counter() - t0
-    dmax = float(ctx.get("damage").max())
-    steps = ctx.stat("cg_solves")
-    del ctx
+
+    def run_union(graph):
+        ctx, groups, mat = make(parts, first[:-1], union, nspec=True)
+        for s in range(S):
+            mat[first[s]:first[s + 1], 0] = A[s]
+        ctx.set("mat", mat)
+        setup(ctx)
+        ctx.mode(cg_graph=graph)
+        disp = []
+        for (top, bottom), a in zip(groups, amp):
+            disp += [(top, "x", 0.0), (top, "y", 0.1 * a), (top, "z", 0.0), (bottom, "x", 0.0), (bottom, "y", 0.0),
+                     (bottom, "z", 0.0)]
+        hip_sync(ctx)
+        t0 = time.perf_counter()
+        ctx.solve_problem_fatigue_static([ctx.load_step(disp=disp)])
+        n_done = ctx.solve_problem_cyclic_specimens(G.LOAD_DOGBONE_DISP, 1, spec, groups, amp, tau, 4)
+        hip_sync(ctx)
+        t = time.perf_counter() - t0
+        res = (t, n_done, float(ctx.get("damage").max()), ctx.stat("cg_solves"))
+        ctx.close()
+        return res
+
+    t_batch, n_done, dmax, steps = run_union(0)
+    # the same run with each CG batch replayed from a captured HIP graph
+    t_graph, n_graph, _, _ = run_union(1)
     # the same specimens, one context each, back to back
     t_seq = 0.0
     for s, x in enumerate(parts):
@@ -575,7 +623,8 @@ def fatigue_ensemble_bench(args):
                                f"R=0.1, {cycles} cycles, tau={tau}",
                    "precond": args.precond, "fatigue_steps": steps, "max_damage": dmax,
                    "one_context_per_specimen_specimen_cycles_per_s": S * n_done / t_seq,
-                   "speedup_vs_one_context_per_specimen": t_seq / t_batch},
+                   "speedup_vs_one_context_per_specimen": t_seq / t_batch,
+                   "with_cg_graph": {"specimen_cycles_per_s": S * n_graph / t_graph}},
     }
     print(json.dumps(out), flush=True)
 

@@ -162,6 +162,10 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        and by uploading positions.
  *  "coarse_graph"     1: replay the coarse inversion's launches from a HIP graph captured once |
  *                        0 (default): launch them directly
+ *  "cg_graph"         1: capture one batch of CG iterations (cg_batch iterations: vector updates,
+ *                        coarse correction, SpMV) into a HIP graph and replay it for every batch
+ *                        of every solve; re-captured when sizes, launch configuration or buffers
+ *                        change. Per-SpMV timing is off in this mode | 0 (default): launch directly
  *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 and 3 (0: smallest of 8, 9,
  *                        10, ... giving at most 768 coarse DoFs); get_mode("ncoarse") reads the
  *                        coarse dimension once built

@@ -247,12 +247,14 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
     std::vector<double> tolm(S), nr(S), w(S);
     std::vector<char> active(S), failed(S, 0);
     c->spec_newton.assign(S, 0);
+    const bool ref_start = (int)c->tolm_ref.size() == S;
     for (int s = 0; s < S; ++s) {
         nr[s] = std::sqrt(r2[s]);
-        tolm[s] = std::max(nr[s], std::sqrt(R2[s]));
+        tolm[s] = ref_start ? c->tolm_ref[s] : std::max(nr[s], std::sqrt(R2[s]));
         active[s] = nr[s] > c->tol_NR * tolm[s];
         if (!std::isfinite(nr[s]) || !std::isfinite(tolm[s])) failed[s] = 1, active[s] = 0;  // see newton_iteration
     }
+    c->tolm_ref.clear();
     for (;;) {
         int nact = 0;
         double eta = 0.1;
@@ -329,7 +331,9 @@ int newton_iteration(lpm_ctx *c, int *n_iter) {
     RC(geometry_force_state(c));
     RC(launch_residual(c, &r2, &R2));
     double norm_residual = std::sqrt(r2);
-    const double tol_multiplier = std::max(norm_residual, std::sqrt(R2));
+    // the reference's multiplier: at its own start, or at the reference's start before a prediction
+    const double tol_multiplier = c->tolm_ref.size() == 1 ? c->tolm_ref[0] : std::max(norm_residual, std::sqrt(R2));
+    c->tolm_ref.clear();
     if (c->verbose)
         printf("|  Norm of residual is %.5e, norm of reaction is %.5e, tolerance criterion is based on %s force\n",
                norm_residual, std::sqrt(R2), norm_residual > std::sqrt(R2) ? "residual" : "reaction");
@@ -402,11 +406,26 @@ int reset_state(lpm_ctx *c, bool reset_xyz) {
 // unloading): after the step's boundary conditions, move the free DoFs by the previous converged
 // step's increment; Newton then corrects from there to the same test
 int predict(lpm_ctx *c) {
+    c->tolm_ref.clear();
     if (!c->predictor) return LPM_OK;
     const int64_t n3 = 3 * c->n;
     if (!c->xyz_prev) {
         LPM_HIP(c, hipMalloc(&c->xyz_prev, sizeof(double) * n3));
         c->pred_valid = false;
+    }
+    if (c->pred_valid) {
+        // Newton's test keeps the reference's multiplier: the norms at the reference's start
+        RC(geometry_force_state(c));
+        if (c->nspec > 0) {
+            std::vector<double> r2, R2;
+            RC(launch_residual_spec(c, r2, R2));
+            c->tolm_ref.resize(c->nspec);
+            for (int s = 0; s < c->nspec; ++s) c->tolm_ref[s] = std::max(std::sqrt(r2[s]), std::sqrt(R2[s]));
+        } else {
+            double r2 = 0, R2 = 0;
+            RC(launch_residual(c, &r2, &R2));
+            c->tolm_ref.assign(1, std::max(std::sqrt(r2), std::sqrt(R2)));
+        }
     }
     hipLaunchKernelGGL(k_predict, dim3(blocks_for(n3, 256)), dim3(256), 0, c->stream, n3, (const int32_t *)c->cons,
                        (const double *)c->xyz_last, c->xyz_prev, c->xyz, c->pred_valid ? 1 : 0);
@@ -761,6 +780,7 @@ int lpm_destroy(lpm_ctx *c) {
     coarse_free(c);
     for (void *p : c->ckpt)
         if (p) hipFree(p);
+    if (c->cg_exec) hipGraphExecDestroy(c->cg_exec);
     for (auto &g : c->groups)
         if (g.ids) hipFree(g.ids);
     if (c->h_scal) hipHostFree(c->h_scal);
@@ -814,6 +834,7 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
     else if (w == "cg_forcing") c->cg_forcing = (int)v ? 1 : 0;
     else if (w == "coarse_graph") c->coarse_graph = (int)v ? 1 : 0;
+    else if (w == "cg_graph") c->cg_graph = (int)v ? 1 : 0;
     else if (w == "predictor") {
         c->predictor = (int)v ? 1 : 0;
         c->pred_valid = false;
@@ -860,6 +881,7 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "spmv_tune") return c->spmv_tune;
     if (w == "cg_forcing") return c->cg_forcing;
     if (w == "coarse_graph") return c->coarse_graph;
+    if (w == "cg_graph") return c->cg_graph;
     if (w == "predictor") return c->predictor;
     if (w == "coarse_cells") return c->coarse_cells;
     if (w == "ncoarse") return c->ncoarse;

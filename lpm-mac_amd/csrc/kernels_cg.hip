@@ -61,9 +61,17 @@ template <typename T, int D, int VP>
 __global__ __launch_bounds__(256) void k_cg_init(int64_t n, const double *__restrict__ b64, const T *__restrict__ invd,
                                                   int block_pre, Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
                                                   Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
-                                                  Vec<T, VP> *__restrict__ s, double *partial, int cap, double *scal) {
-    // the prologue SpMV that follows checks the flag; clear what the previous solve left
-    if (blockIdx.x == 0 && threadIdx.x == 0) scal[SC_DONE] = 0.0;
+                                                  Vec<T, VP> *__restrict__ s, double *partial, int cap, double *scal,
+                                                  double tol, double maxit) {
+    // the prologue SpMV that follows checks the flag; clear what the previous solve left and
+    // set the solve's constants, which the iteration kernels read from here (so that a batch
+    // of them can be captured once and replayed for every solve)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[SC_DONE] = 0.0;
+        scal[SC_IT0] = 0.0;
+        scal[SC_TOL] = tol;
+        scal[SC_MAXIT] = maxit;
+    }
     double v[2] = {0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         Vec<T, VP> rv, zv, zero;
@@ -516,7 +524,6 @@ __device__ void cg_scalar_step(const double *prr, const double *pg, const double
             if (it == 0) {
                 scal[SC_RHS2] = rr;
                 scal[SC_THR] = thr;
-                scal[SC_MAXIT] = maxit;
             }
             scal[SC_RR] = rr;
             scal[SC_ITER] = it;
@@ -533,9 +540,11 @@ __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__
                                                  Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
                                                  Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
                                                  Vec<T, VP> *__restrict__ s, const Vec<T, VP> *__restrict__ w,
-                                                 double *partial, int cap, int np_vec, int np_spmv, double *scal, int it,
-                                                 double tol, double maxit) {
+                                                 double *partial, int cap, int np_vec, int np_spmv, double *scal, int k) {
     if (scal[SC_DONE] != 0.0) return;
+    // iteration k of the batch that starts at scal[SC_IT0]
+    const int it = (int)scal[SC_IT0] + k;
+    const double tol = scal[SC_TOL], maxit = scal[SC_MAXIT];
     __shared__ double coef[3];  // alpha, beta, done
     const double *pv = partial + (int64_t)(it & 1) * 2 * cap;
     cg_scalar_step(pv, pv + cap, partial + (int64_t)4 * cap, np_vec, np_spmv, scal, it, tol, maxit, coef);
@@ -571,9 +580,10 @@ template <typename T, int D, int VP>
 __global__ __launch_bounds__(256) void k_cg_vec_a(int64_t n, Vec<T, VP> *__restrict__ x, Vec<T, VP> *__restrict__ r,
                                                    const Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
                                                    Vec<T, VP> *__restrict__ s, const Vec<T, VP> *__restrict__ w,
-                                                   double *partial, int cap, int np_vec, int np_spmv, double *scal, int it,
-                                                   double tol, double maxit) {
+                                                   double *partial, int cap, int np_vec, int np_spmv, double *scal, int k) {
     if (scal[SC_DONE] != 0.0) return;
+    const int it = (int)scal[SC_IT0] + k;
+    const double tol = scal[SC_TOL], maxit = scal[SC_MAXIT];
     __shared__ double coef[3];
     cg_scalar_step(partial + (int64_t)(it & 1) * 2 * cap, partial + cap, partial + (int64_t)4 * cap, np_vec, np_spmv, scal,
                    it, tol, maxit, coef);
@@ -625,6 +635,11 @@ __global__ __launch_bounds__(256) void k_cg_vec_b(int64_t n, const T *__restrict
         for (int k = 0; k < D; ++k) v[0] += (double)rv.v[k] * (double)zv.v[k];
     }
     block_partial<1>(v, partial + cap, cap);
+}
+
+// end of a batch of CG iterations: the next batch starts `batch` iterations later
+__global__ __launch_bounds__(64) void k_cg_advance(double *scal, int batch) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[SC_IT0] += (double)batch;
 }
 
 // ---------------------------------------------------------------- one-block finalize
@@ -1004,6 +1019,45 @@ int sym_prepare(lpm_ctx *c) {
     return LPM_OK;
 }
 
+// Captures one batch of CG iterations into c->cg_exec unless the graph captured before is
+// still valid: the key lists every value the batch's launches take (sizes, grids, the tuned
+// SpMV configuration, the half-store switch and every buffer), so any change re-captures.
+template <typename F>
+static int cg_graph_prepare(lpm_ctx *c, int gv, int gs, int batch, F issue) {
+    const std::vector<uintptr_t> key = {
+        (uintptr_t)c->n, (uintptr_t)c->dim, (uintptr_t)c->precision, (uintptr_t)c->precond, (uintptr_t)gv, (uintptr_t)gs,
+        (uintptr_t)batch, (uintptr_t)c->nslice, (uintptr_t)c->spmv_unr, (uintptr_t)c->spmv_wpb, (uintptr_t)c->spmv_coop,
+        (uintptr_t)c->spmv_spw, (uintptr_t)c->spmv_remap, (uintptr_t)c->sym_ok, (uintptr_t)c->partial_cap,
+        (uintptr_t)c->sp, (uintptr_t)c->cptr, (uintptr_t)c->scol, (uintptr_t)c->aval, (uintptr_t)c->dslot, (uintptr_t)c->sp_u,
+        (uintptr_t)c->ucol, (uintptr_t)c->uval, (uintptr_t)c->sp_l, (uintptr_t)c->lcol, (uintptr_t)c->lpos,
+        (uintptr_t)c->v_x, (uintptr_t)c->v_r, (uintptr_t)c->v_u, (uintptr_t)c->v_p, (uintptr_t)c->v_s, (uintptr_t)c->v_w,
+        (uintptr_t)c->invdiag, (uintptr_t)c->partial, (uintptr_t)c->scal, (uintptr_t)c->agg_of, (uintptr_t)c->agg_d,
+        (uintptr_t)c->agg_ptr, (uintptr_t)c->agg_idx, (uintptr_t)c->dofmask, (uintptr_t)c->cz, (uintptr_t)c->cy,
+        (uintptr_t)c->Ainv, (uintptr_t)c->nagg, (uintptr_t)c->coarse_S, (uintptr_t)c->nmode, (uintptr_t)c->cgrp_cb,
+        (uintptr_t)c->cgrp_off, (uintptr_t)c->ncgrp, (uintptr_t)c->cgrp_nbmax};
+    if (c->cg_exec && key == c->cg_key) return LPM_OK;
+    if (c->cg_exec) {
+        hipGraphExecDestroy(c->cg_exec);
+        c->cg_exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    LPM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = issue();
+    const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+    if (rc || ce != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return rc ? rc : lpm_fail(c, LPM_E_HIP, std::string("CG batch capture: ") + hipGetErrorString(ce));
+    }
+    const hipError_t ie = hipGraphInstantiate(&c->cg_exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        c->cg_exec = nullptr;
+        return lpm_fail(c, LPM_E_HIP, std::string("CG batch graph: ") + hipGetErrorString(ie));
+    }
+    c->cg_key = key;
+    return LPM_OK;
+}
+
 template <typename T, int D, int VP>
 static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     typedef Vec<T, VP> V;
@@ -1014,7 +1068,7 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     const int64_t maxit = c->cg_maxit < 0 ? 2 * n * c->dim : c->cg_maxit;
     hipLaunchKernelGGL((k_cg_init<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, c->residual, (const T *)c->invdiag,
                        block_jacobi(c), (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s, c->partial,
-                       c->partial_cap, c->scal);
+                       c->partial_cap, c->scal, c->cg_tol, (double)maxit);
     int rc = check_launch(c, "k_cg_init");
     if (rc) return rc;
     const bool coarse = two_level(c);
@@ -1034,35 +1088,46 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     // the device flag turns the rest of a batch into no-ops once the solve has stopped
     const int batch = ((c->cg_batch > 0 ? c->cg_batch : 16) + 1) & ~1;
     // timing: events around the SpMV of every `timing`-th iteration only (an event record costs
-    // a few microseconds of idle GPU time, so timing every launch would slow the solve)
+    // a few microseconds of idle GPU time, so timing every launch would slow the solve); not in
+    // graph mode, whose batches are replayed whole
+    const bool graph = c->cg_graph != 0;
     std::vector<hipEvent_t> evs;
-    if (c->timing) {
+    if (c->timing && !graph) {
         evs.resize(2 * batch);
         for (auto &e : evs) LPM_HIP(c, timing_event(&e));
     }
-    auto timed = [&](int64_t it) { return c->timing > 0 && it % c->timing == 0; };
+    auto timed = [&](int64_t it) { return !graph && c->timing > 0 && it % c->timing == 0; };
     int64_t it0 = 0;
-    for (;;) {
-        for (int k = 0; k < batch; ++k) {
+    // one batch: iteration k of the batch starting at scal[SC_IT0], then the batch base moves on
+    auto issue = [&](int64_t base) -> int {
+        int r = LPM_OK;
+        for (int k = 0; k < batch && !r; ++k) {
             if (coarse) {
                 hipLaunchKernelGGL((k_cg_vec_a<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (V *)c->v_x, (V *)c->v_r,
                                    (const V *)c->v_u, (V *)c->v_p, (V *)c->v_s, (const V *)c->v_w, c->partial, c->partial_cap,
-                                   gv, gs, c->scal, (int)(it0 + k), c->cg_tol, (double)maxit);
-                rc = check_launch(c, "k_cg_vec_a");
-                if (!rc) rc = coarse_u();
+                                   gv, gs, c->scal, k);
+                r = check_launch(c, "k_cg_vec_a");
+                if (!r) r = coarse_u();
             } else {
                 hipLaunchKernelGGL((k_cg_vec<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (const T *)c->invdiag,
                                    block_jacobi(c), (V *)c->v_x, (V *)c->v_r, (V *)c->v_u, (V *)c->v_p, (V *)c->v_s,
-                                   (const V *)c->v_w, c->partial, c->partial_cap, gv, gs, c->scal, (int)(it0 + k), c->cg_tol,
-                                   (double)maxit);
-                rc = check_launch(c, "k_cg_vec");
+                                   (const V *)c->v_w, c->partial, c->partial_cap, gv, gs, c->scal, k);
+                r = check_launch(c, "k_cg_vec");
             }
-            if (rc) return rc;
-            if (timed(it0 + k)) LPM_HIP(c, hipEventRecord(evs[2 * k], c->stream));
-            rc = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
-            if (rc) return rc;
-            if (timed(it0 + k)) LPM_HIP(c, hipEventRecord(evs[2 * k + 1], c->stream));
+            if (r) break;
+            if (timed(base + k) && hipEventRecord(evs[2 * k], c->stream) != hipSuccess) r = lpm_fail(c, LPM_E_HIP, "event");
+            if (!r) r = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
+            if (!r && timed(base + k) && hipEventRecord(evs[2 * k + 1], c->stream) != hipSuccess)
+                r = lpm_fail(c, LPM_E_HIP, "event");
         }
+        if (r) return r;
+        hipLaunchKernelGGL(k_cg_advance, dim3(1), dim3(64), 0, c->stream, c->scal, batch);
+        return check_launch(c, "k_cg_advance");
+    };
+    if (graph && (rc = cg_graph_prepare(c, gv, gs, batch, [&]() { return issue(0); }))) return rc;
+    for (;;) {
+        if (graph) LPM_HIP(c, hipGraphLaunch(c->cg_exec, c->stream));
+        else if ((rc = issue(it0))) return rc;
         LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
         LPM_HIP(c, hipStreamSynchronize(c->stream));
         // SpMVs of the batch that did work: one per update (iterations it0 .. ITER-1 updated)

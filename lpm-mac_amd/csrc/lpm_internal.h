@@ -31,6 +31,8 @@ enum {
     SC_FLAG,       // any-flag (bond broken / damage changed) as a double 0/1
     SC_GAMMA_P,    // gamma = (r, M^-1 r) of iterations with parity 0 / 1 (two slots)
     SC_ALPHA_P = SC_GAMMA_P + 2,  // alpha of iterations with parity 0 / 1 (two slots)
+    SC_IT0 = SC_ALPHA_P + 2,      // first iteration of the current batch of CG iterations
+    SC_TOL,                       // relative tolerance of the current solve
     SC_COUNT = 32
 };
 
@@ -104,12 +106,21 @@ struct lpm_ctx {
     double *cz = nullptr, *cy = nullptr;  // restriction partials (nagg * coarse_S * nmode) / correction
     int coarse_S = 1;                 // restriction workgroups per aggregate
     hipGraphExec_t gj_graph = nullptr;  // the inversion's launches, captured once (coarse_graph = 1)
+    // CG batch graph (mode "cg_graph"): one batch of CG iterations captured once and replayed;
+    // cg_key lists every value its launches captured (grids, launch configuration, buffers)
+    int cg_graph = 0;
+    hipGraphExec_t cg_exec = nullptr;
+    std::vector<uintptr_t> cg_key;
     int coarse_graph = 0;             // 1: replay the inversion from a HIP graph, 0: launch directly
     // Load-step predictor (mode "predictor"): free DoFs start each step from the converged
     // position plus the previous step's converged increment
     int predictor = 0;
     bool pred_valid = false;          // xyz_prev holds the converged positions of the step before
     double *xyz_prev = nullptr;
+    // Newton tolerance multipliers max(||r0||, ||R0||) evaluated at the reference's start (the
+    // converged positions with the step's boundary conditions, before the prediction), one per
+    // specimen (one for a single system); used by the next Newton call only, then cleared
+    std::vector<double> tolm_ref;
     uint8_t *dofmask = nullptr;       // per particle: bit k set when DoF k is eliminated
     int64_t coarse_version = -1;
     // Asynchronous dump writer (io.hip): device snapshot -> pinned host copy on a side stream ->
