@@ -196,3 +196,42 @@ def test_batched_adjoint_matches_single_specimen_adjoints():
         say(f"   specimen {s}: F {F1:.10e} dKn rel err {ek:.2e} dTv rel err {et:.2e}")
         assert ek < 1e-8 and et < 1e-8
     assert F == pytest.approx(Fs, rel=1e-10)
+
+
+def test_ensemble_predictor_reaches_the_same_equilibrium():
+    """Load-step predictor on a specimen ensemble with small strain increments (1e-4 to 3e-4 of
+    each specimen's height per step, as in the ensemble benchmark): every specimen converges to
+    Newton 1e-10, whose multiplier is taken at the reference's start, and ends within 1e-8 of the
+    run without the predictor."""
+    say("[gpu] ensemble with the load-step predictor")
+    sizes, strains = [(40, 46), (70, 80), (90, 104)], [1e-4, 3e-4, 2e-4]
+    parts, cases = [], []
+    for (nx, ny), e in zip(sizes, strains):
+        x, box = hex_plate(nx, ny)
+        parts.append((x, G.build_bonds(G.HEXAGON2D, 0.5, x)))
+        cases.append((x, e * (x[:, 1].max() - x[:, 1].min())))
+    xyz, bonds, first = _union(parts)
+    out = []
+    for pred in (0, 1):
+        g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=bonds)
+        g.set_specimens(first)
+        g.set_elastic(0.0, True, 110e3, 0.34, 1.0, elmap_literal=0)
+        g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=2, cg_tol=1e-3, cg_forcing=1,
+               tol_NR=1e-10, max_NR_iter=30, cg_maxit=50000, predictor=pred)
+        disp = []
+        for s, (x, pull) in enumerate(cases):
+            ids = np.arange(first[s], first[s + 1])
+            y = x[:, 1]
+            lo, hi = ids[y < y.min() + 1e-6], ids[y > y.max() - 1e-6]
+            disp += [(lo, "x", 0.0), (lo, "y", 0.0), (hi, "y", pull)]
+        ls = g.load_step(disp=disp)
+        g.update_geometry()
+        g.update_force_state()
+        for k in range(5):
+            assert g.solve_step_static(ls, -1), (pred, k, g.specimen_stat("newton_iters"))
+        say("   predictor", pred, "Newton per specimen (last step)", g.specimen_stat("newton_iters"))
+        out.append(g.get("xyz").reshape(-1, 3) - xyz)
+    for s in range(len(cases)):
+        err = rel(out[1][first[s]:first[s + 1]], out[0][first[s]:first[s + 1]])
+        say(f"   specimen {s}: displacement rel diff {err:.2e}")
+        assert err < 1e-8
