@@ -40,10 +40,12 @@ __global__ __launch_bounds__(256) void k_dofmask(int64_t n, int dim, const int32
 
 // Stage 1, one thread per row i: for each aggregate h among the columns of row i (in the
 // row's aggregate list), V = P_i^T (sum over slots with agg(j) = h, in slot order, of K_ij P_j).
-// With P_j = [I | R(d_j)], K_ij P_j = [K_ij | K_ij R(d_j)].
+// With P_j = [I | R(d_j)], K_ij P_j = [K_ij | K_ij R(d_j)]. Column indices come from the SELL
+// copy (scol: slot c of row i at sp[i/64] + 64 c + i%64), which the lanes of a wavefront (the
+// 64 rows of a slice) read coalesced, where the CSR list would put every lane on its own line.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_coarse_rows(int64_t n, const int32_t *__restrict__ cptr,
-                                                      const int32_t *__restrict__ cidx, const int64_t *__restrict__ sp,
+                                                      const int32_t *__restrict__ scol, const int64_t *__restrict__ sp,
                                                       const T *__restrict__ aval, const int32_t *__restrict__ agg_of,
                                                       const double *__restrict__ agg_d, const uint8_t *__restrict__ mask,
                                                       const int32_t *__restrict__ ra_ptr, const int32_t *__restrict__ ra_id,
@@ -51,7 +53,8 @@ __global__ __launch_bounds__(256) void k_coarse_rows(int64_t n, const int32_t *_
     constexpr int EB = D * D, M = D == 3 ? 6 : 3;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int c0 = cptr[i], nc = cptr[i + 1] - c0;
+    const int nc = cptr[i + 1] - cptr[i];
+    const int32_t *ci = scol + sp[i >> 6] + (i & 63);
     double Pi[3][6];
     rigid_modes<D>(&agg_d[3 * i], mask[i], Pi);
     for (int f = ra_ptr[i]; f < ra_ptr[i + 1]; ++f) {
@@ -60,7 +63,7 @@ __global__ __launch_bounds__(256) void k_coarse_rows(int64_t n, const int32_t *_
         for (int r = 0; r < D; ++r)
             for (int m = 0; m < M; ++m) W[r][m] = 0.0;
         for (int c = 0; c < nc; ++c) {
-            const int64_t j = cidx[c0 + c];
+            const int64_t j = ci[(int64_t)c * 64];
             if (agg_of[j] != h) continue;
             double Pj[3][6];
             rigid_modes<D>(&agg_d[3 * j], mask[j], Pj);
@@ -554,7 +557,7 @@ static int coarse_setup(lpm_ctx *c) {
 template <typename T, int D>
 static int coarse_rows_t(lpm_ctx *c) {
     hipLaunchKernelGGL((k_coarse_rows<T, D>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr,
-                       c->cidx, c->sp, (const T *)c->aval, c->agg_of, c->agg_d, c->dofmask, c->ra_ptr, c->ra_id, c->cvbuf);
+                       c->scol, c->sp, (const T *)c->aval, c->agg_of, c->agg_d, c->dofmask, c->ra_ptr, c->ra_id, c->cvbuf);
     return ck(c, "k_coarse_rows");
 }
 
