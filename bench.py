@@ -141,6 +141,33 @@ def cpu_baseline(n, gpu_counts, max_threads):
     t_cg = (time.perf_counter() - t0) / n_cg_sample
     newton_per_step, cg_per_step, asm_per_step = gpu_counts  # of the GPU run with the same (Jacobi) solver
     t_step = asm_per_step * t_asm + (newton_per_step + asm_per_step) * t_force + cg_per_step * t_cg
+    # the reference-faithful variant: assembly and forces serial, as the reference's loops are;
+    # only the CG's sparse products threaded (Eigen parallelises them when built with OpenMP)
+    ref = None
+    try:
+        import ctypes
+
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        try:
+            s.apply_bc(ls)
+            t0 = time.perf_counter()
+            s.calc_stiffness()
+            s.update_stiffness_disp_bc()
+            t_asm1 = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            s.update_geometry()
+            s.update_force_state()
+            s.update_rr()
+            t_force1 = time.perf_counter() - t0
+        finally:
+            gomp.omp_set_num_threads(max_threads)
+        t_step1 = asm_per_step * t_asm1 + (newton_per_step + asm_per_step) * t_force1 + cg_per_step * t_cg
+        ref = {"value": 1.0 / t_step1, "unit": "load_steps/s", "cores": max_threads,
+               "sample": (f"same projection with the assembly ({t_asm1:.2f} s) and the geometry+force+residual pass "
+                          f"({t_force1:.3f} s) on one thread, the CG iterations on {max_threads}")}
+    except OSError:
+        pass
     return {
         "value": 1.0 / t_step,
         "unit": "load_steps/s",
@@ -150,6 +177,7 @@ def cpu_baseline(n, gpu_counts, max_threads):
                    f"({t_force:.3f} s), {n_cg_sample} Jacobi-CG iterations ({t_cg * 1e3:.1f} ms each) timed; "
                    f"step projected with the per-step counts of the GPU Jacobi run: {asm_per_step:.2f} assemblies, "
                    f"{newton_per_step:.2f} Newton iterations, {cg_per_step:.0f} CG iterations"),
+        "reference_faithful_threading": ref,
     }
 
 
@@ -177,6 +205,9 @@ def main():
     ap.add_argument("--fatigue", action="store_true", help="with --ensemble: HCF dogbones through fatigue cycles")
     ap.add_argument("--timing-every", type=int, default=16,
                     help="bracket the CG SpMV of every k-th iteration with HIP events")
+    ap.add_argument("--config", type=int, default=2,
+                    help="2: the headline (SC 64^3); 3: FCC 40^3 J2 crystal; 4: cracked 400 x 400 plate with breaking "
+                         "and the adjoint (one JSON line each)")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="run the 256 MiB calibration copy (k_copy_calib) only 3 times, for rocprofv3 --pmc passes "
                          "whose FETCH_SIZE / WRITE_SIZE readings it scales")
@@ -189,6 +220,12 @@ def main():
     import lpmgpu as G
 
     G.lib()
+    if args.config == 3:
+        config3_bench(args)
+        return
+    if args.config == 4:
+        config4_bench(args)
+        return
     if args.ensemble > 0:
         if args.fatigue:
             fatigue_ensemble_bench(args)
@@ -626,6 +663,118 @@ def fatigue_ensemble_bench(args):
                    "speedup_vs_one_context_per_specimen": t_seq / t_batch,
                    "with_cg_graph": {"specimen_cycles_per_s": S * n_graph / t_graph}},
     }
+    print(json.dumps(out), flush=True)
+
+
+def _line(metric, value, unit, steps, t, config, dtype="fp32-mixed (FP32 K/SpMV/CG, FP64 residual/positions/state)"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": 0,
+            "ms_per_step": 1e3 * t / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def config3_bench(args):
+    """FCC single crystal of 40^3 cubic cells (256,009 particles in the rotated box; rotation
+    angles uniform in [0, 2 pi) from numpy's default_rng(7)), J2 plasticity (E = 200 GPa,
+    nu = 0.3, sigma_y = 300 MPa, H = 2 GPa, no damage), uniaxial tension to 2% in 50 load steps
+    with the headline solver settings; all 50 steps are timed, the device setup apart."""
+    import lpmgpu as G
+
+    r = 0.5 / math.sqrt(2)
+    rng = np.random.default_rng(7)
+    ang = tuple(rng.uniform(0, 2 * math.pi, 3))
+    xyz = G.generate_lattice(G.FCC3D, [0, 40, 0, 40, 0, 40], r, 0, ang)
+    z = xyz[:, 2]
+    lo, hi = np.nonzero(z < z.min() + 0.4)[0], np.nonzero(z > z.max() - 0.4)[0]
+    L = z.max() - z.min()
+    t0 = time.perf_counter()
+    g = G.Context(G.FCC3D, r, G.J2, xyz, bonds="device")
+    g.set_j2(0.0, False, 200e3, 0.3, 300.0, 0.0, 2e3, 0.0, 1.0, damage_threshold=1.0, elmap_literal=0)
+    g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
+           cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=50, cg_maxit=20000, predictor=args.predictor)
+    steps = 50
+    ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", 0.0), (lo, "z", 0.0), (hi, "z", 0.02 * L / steps)])
+    g.update_geometry()
+    g.update_force_state()
+    hip_sync(g)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        if not g.solve_step_static(ls, -1):
+            raise RuntimeError(f"load step {k} did not converge")
+    hip_sync(g)
+    t = time.perf_counter() - t0
+    alpha = g.get("state_var").reshape(-1, 7)[:, 0]
+    out = _line("load steps/s, FCC 40^3 cells J2 single crystal to 2% strain", steps / t, "load_steps/s", steps, t,
+                {"workload": f"FCC 40^3 cells, {len(xyz)} particles, J2 (E 200 GPa, sigma_y 300 MPa, H 2 GPa), "
+                             f"uniaxial tension 2% in {steps} steps",
+                 "bonds": g.size("nbonds"), "newton_iters_per_step": g.stat("newton_iters") / steps,
+                 "cg_iters_per_step": g.stat("cg_iters") / steps, "newton_tol": args.tol_nr,
+                 "predictor": args.predictor, "precond": args.precond, "setup_seconds": t_setup,
+                 "yielded_fraction": float(np.mean(alpha > 0))})
+    print(json.dumps(out), flush=True)
+
+
+def config4_bench(args):
+    """Hex plate of 400 x 400 (160,000 particles) with an edge crack of a quarter of the width at
+    mid-height (bonds crossing the crack line removed), bond breaking at a critical stretch of
+    1e-3, mode-I opening of 6e-4 per edge per step over 40 steps (Newton 1e-6, the headline FP32
+    solver settings), then the adjoint d(top reaction)/d(Kn, Tv) at the final state with the broken
+    set frozen (an FP64 polish step and one transposed solve). Both parts are timed."""
+    import lpmgpu as G
+
+    box = [0.0, 400.0, 0.0, 0.8660254037844386 * 400, 0.0, 1.0]
+    xyz = G.generate_lattice(G.HEXAGON2D, box, 0.5)
+    bp, bs, p2 = (np.asarray(a) for a in G.build_bonds(G.HEXAGON2D, 0.5, xyz))
+    n = len(xyz)
+    p1 = np.repeat(np.arange(n), np.diff(bp))
+    lay = (np.arange(len(p2)) >= bs[p1]).astype(np.int32)
+    y0 = 0.5 * (xyz[:, 1].min() + xyz[:, 1].max()) + 0.25
+    a = xyz[:, 0].min() + 0.25 * np.ptp(xyz[:, 0])
+    ya, yb = xyz[p1, 1], xyz[p2, 1]
+    cross = (ya - y0) * (yb - y0) < 0
+    tt = np.where(cross, (y0 - ya) / np.where(cross, yb - ya, 1.0), 0.0)
+    keep = ~(cross & (xyz[p1, 0] + tt * (xyz[p2, 0] - xyz[p1, 0]) < a))
+    lay, p1, p2 = lay[keep], p1[keep], p2[keep]
+    bp2 = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(bp2, p1 + 1, 1)
+    bp2 = np.cumsum(bp2).astype(np.int32)
+    bs2 = (bp2[:-1] + np.bincount(p1[lay == 0], minlength=n)).astype(np.int32)
+    g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=(bp2, bs2, p2.astype(np.int32)))
+    g.set_elastic(0.0, True, 69e3, 0.3, 1e-3, elmap_literal=0)
+    g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
+           cg_forcing=args.cg_forcing, tol_NR=1e-6, max_NR_iter=50, cg_maxit=50000, predictor=args.predictor)
+    y = xyz[:, 1]
+    lo, hi = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
+    inc, steps = 0.0006, 40
+    ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", -inc), (hi, "x", 0.0), (hi, "y", inc)])
+    g.update_geometry()
+    g.update_force_state()
+    hip_sync(g)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        if not g.solve_step_static(ls, -1):
+            raise RuntimeError(f"load step {k} did not converge")
+    hip_sync(g)
+    t = time.perf_counter() - t0
+    bd = g.get("bdamage")
+    newton, cg = g.stat("newton_iters") / steps, g.stat("cg_iters") / steps
+    t0 = time.perf_counter()
+    g.set_elastic(0.0, True, 69e3, 0.3, 1e9, elmap_literal=0)
+    g.set("bdamage", bd)
+    g.mode(precision=1, cg_tol=1e-12, tol_NR=5e-9, predictor=0)
+    still = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", 0.0), (hi, "x", 0.0), (hi, "y", 0.0)])
+    if not g.solve_step_static(still, -1):
+        raise RuntimeError("FP64 polish step did not converge")
+    F, gKn, gTv, nref, rres = g.adjoint_reaction(g.group(hi), "y")
+    t_adj = time.perf_counter() - t0
+    out = _line("load steps/s, 400 x 400 hex plate with an edge crack, bond breaking, mode I", steps / t, "load_steps/s",
+                steps, t,
+                {"workload": f"hex 400 x 400, {n} particles, edge crack 0.25 W, critical stretch 1e-3, "
+                             f"opening {inc} per edge per step, {steps} steps",
+                 "bonds": g.size("nbonds"), "broken_bonds": int((bd > 0.5).sum()), "newton_iters_per_step": newton,
+                 "cg_iters_per_step": cg, "newton_tol": 1e-6, "predictor": args.predictor, "precond": args.precond,
+                 "adjoint": {"seconds_polish_and_adjoint": t_adj, "refinements": nref, "relative_residual": rres,
+                             "top_reaction": F}})
     print(json.dumps(out), flush=True)
 
 

@@ -781,6 +781,7 @@ int lpm_destroy(lpm_ctx *c) {
     for (void *p : c->ckpt)
         if (p) hipFree(p);
     if (c->cg_exec) hipGraphExecDestroy(c->cg_exec);
+    if (c->adj_scr) hipFree(c->adj_scr);
     for (auto &g : c->groups)
         if (g.ids) hipFree(g.ids);
     if (c->h_scal) hipHostFree(c->h_scal);

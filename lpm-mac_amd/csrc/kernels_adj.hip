@@ -224,26 +224,29 @@ static int check(lpm_ctx *c, const char *what) {
     return LPM_OK;
 }
 
+// the context's per-bond scratch (4 doubles per bond; kernels on the context stream use it in
+// turn, so one buffer serves every product)
+static int adj_scratch(lpm_ctx *c, double **p) {
+    if (!c->adj_scr) LPM_HIP(c, hipMalloc(&c->adj_scr, sizeof(double) * 4 * std::max<int64_t>(c->nbond, 1)));
+    *p = c->adj_scr;
+    return LPM_OK;
+}
+
 // jv = dPin/dx . v at the current geometry and forces (both dim*n, device)
 int force_jvp(lpm_ctx *c, const double *v, double *jv) {
     const int g = blocks_for(c->n, 256);
-    double *dcs = nullptr, *dfb = nullptr;
-    LPM_HIP(c, hipMalloc(&dcs, sizeof(double) * 3 * std::max<int64_t>(c->nbond, 1)));
-    if (hipMalloc(&dfb, sizeof(double) * std::max<int64_t>(c->nbond, 1)) != hipSuccess) {
-        hipFree(dcs);
-        return lpm_fail(c, LPM_E_NOMEM, "jvp scratch");
-    }
+    double *dcs = nullptr;
+    int rc = adj_scratch(c, &dcs);
+    if (rc) return rc;
+    double *dfb = dcs + 3 * std::max<int64_t>(c->nbond, 1);
     hipLaunchKernelGGL(k_jvp_bond, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->bptr, c->bsplit, c->p2, c->dis, c->cs,
                        c->Kn, c->Tv, c->bdamage, v, dcs, dfb);
-    int rc = check(c, "k_jvp_bond");
+    rc = check(c, "k_jvp_bond");
     if (!rc) {
         hipLaunchKernelGGL(k_jvp_particle, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->bptr, c->opp, c->cs, c->bforce,
                            dcs, dfb, jv);
         rc = check(c, "k_jvp_particle");
     }
-    hipStreamSynchronize(c->stream);
-    hipFree(dcs);
-    hipFree(dfb);
     return rc;
 }
 
@@ -251,17 +254,16 @@ int force_jvp(lpm_ctx *c, const double *v, double *jv) {
 int force_vjp(lpm_ctx *c, const double *w, double *y) {
     const int g = blocks_for(c->n, 256);
     double *cb = nullptr;
-    LPM_HIP(c, hipMalloc(&cb, sizeof(double) * std::max<int64_t>(c->nbond, 1)));
+    int rc = adj_scratch(c, &cb);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_vjp_bond, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->bptr, c->bsplit, c->p2, c->opp, c->cs,
                        c->Kn, c->Tv, c->bdamage, w, cb);
-    int rc = check(c, "k_vjp_bond");
+    rc = check(c, "k_vjp_bond");
     if (!rc) {
         hipLaunchKernelGGL(k_vjp_particle, dim3(g), dim3(256), 0, c->stream, c->n, c->dim, c->bptr, c->p2, c->opp, c->dis, c->cs,
                            c->bforce, c->bdamage, cb, w, y);
         rc = check(c, "k_vjp_particle");
     }
-    hipStreamSynchronize(c->stream);
-    hipFree(cb);
     return rc;
 }
 

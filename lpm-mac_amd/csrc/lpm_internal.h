@@ -130,6 +130,8 @@ struct lpm_ctx {
     // state a converged step leaves (positions, loads, constraint flags, particle and bond state
     // and their last-converged copies); null where the slot is empty
     std::vector<void *> ckpt;
+    // per-bond scratch of the force JVP / VJP (4 doubles per bond), allocated on first use
+    double *adj_scr = nullptr;
     // Specimen ensemble: independent particle systems (contiguous id ranges, no bond between
     // them) solved in the same launches, each with its own Newton test.
     int nspec = 0;                    // 0: one system
