@@ -36,7 +36,13 @@ HBM_PEAK = 8.0e12
 # 262144.0 KiB (x1.0000). Mean over the fused CG SpMV launches that did work (the configuration
 # the tuner kept in that run): FETCH_SIZE 317832.7 KiB -> 650.9 MB; WRITE_SIZE 4121.8 KiB -> 4.2 MB
 # (the 3 MB output vector, padded to 4 entries). The tuner's other variants fetch 640-653 MB.
-MEASURED_TRAFFIC = {(64, "k_spmv (CG SpMV, full store)"): 317832.7 * 1024 * 262144.0 / 131083.3 + 4121.8 * 1024}
+# With int16 column offsets (the default where they fit, as at 64^3), the configuration the
+# tuner kept in both passes: FETCH_SIZE 296578.8 KiB -> 607.3 MB; WRITE_SIZE 4114.8 KiB -> 4.2 MB
+# (calibration 131083.4 KiB fetched for the 256 MiB copy).
+MEASURED_TRAFFIC = {
+    (64, "k_spmv (CG SpMV, full store)"): 317832.7 * 1024 * 262144.0 / 131083.3 + 4121.8 * 1024,
+    (64, "k_spmv (CG SpMV, full store, int16 column offsets)"): 296578.8 * 1024 * 262144.0 / 131083.4 + 4114.8 * 1024,
+}
 
 
 def sc_workload(n):
@@ -57,9 +63,10 @@ def sc_workload(n):
 def spmv_bytes(ctx):
     """Algorithmic bytes of one FP32 3x3-block SpMV and the kernel that ran it.
 
-    Full store: every block's 9 values + its column index, row pointers, one read of the
-    input vector and one write of the output (4 B per DoF; the 16 B per particle padding of
-    the device vectors is not counted).
+    Full store: every block's 9 values + its column index (4 B, or 2 B when the columns are
+    stored as int16 offsets from the row), row pointers, one read of the input vector and one
+    write of the output (4 B per DoF; the 16 B per particle padding of the device vectors is
+    not counted).
     Half store (symmetric matrix): the upper blocks (j >= i) with their column indices, for
     each lower block a column index and the position of its transpose, row pointers and
     diagonal slots, and the same vector traffic. A lower block's values are the transposed
@@ -70,7 +77,9 @@ def spmv_bytes(ctx):
     if ctx.stat("spmv_sym_active"):
         nup, nlo = (nblk + n) // 2, (nblk - n) // 2
         return 40 * nup + 8 * nlo + 4 * (n + 1) + 4 * n + vec, "k_spmv_sym (CG SpMV, half store)"
-    return 40 * nblk + 4 * (n + 1) + vec, "k_spmv (CG SpMV, full store)"
+    ib = int(ctx.get_mode("spmv_index_bytes"))
+    name = "k_spmv (CG SpMV, full store)" if ib == 4 else "k_spmv (CG SpMV, full store, int16 column offsets)"
+    return (36 + ib) * nblk + 4 * (n + 1) + vec, name
 
 
 def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0, predictor=0):

@@ -772,7 +772,7 @@ int lpm_destroy(lpm_ctx *c) {
                     c->cs_sum, c->type, c->cons, c->dis0, c->dis, c->dis_last, c->dLe, c->dLp, c->dLp_last, c->bstrain, c->Kn,
                     c->Tv, c->cs, c->bforce, c->bforce_last, c->bdamage, c->bdamage_last, c->residual, c->disp, c->diag64,
                     c->aval, c->invdiag, c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w, c->scal, c->partial, c->counter, c->iflag, c->sp, c->scol,
-                    c->dslot, c->sp_u, c->sp_l, c->ucol, c->lcol, c->lpos, c->uval,
+                    c->dslot, c->sp_u, c->sp_l, c->ucol, c->lcol, c->lpos, c->uval, c->scol16,
                     c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part, c->xyz_prev};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -856,6 +856,7 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
         if (v < 1 || v > 64) return lpm_fail(c, LPM_E_ARG, "spmv_spw must be 1..64");
         c->spmv_spw = (int)v;
     } else if (w == "spmv_remap") c->spmv_remap = (int)v ? 1 : 0;
+    else if (w == "spmv_idx16") c->spmv_idx16 = (int)v ? 1 : 0;
     else if (w == "spmv_sym") {
         c->spmv_sym = (int)v ? 1 : 0;
         c->sym_version = -1;
@@ -889,6 +890,8 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "spmv_unr") return c->spmv_unr;
     if (w == "spmv_wpb") return c->spmv_wpb;
     if (w == "spmv_remap") return c->spmv_remap;
+    if (w == "spmv_idx16") return c->spmv_idx16;
+    if (w == "spmv_index_bytes") return (c->scol16 && c->spmv_idx16) ? 2 : 4;
     if (w == "spmv_coop") return c->spmv_coop;
     if (w == "spmv_spw") return c->spmv_spw;
     return NAN;

@@ -92,11 +92,19 @@ __global__ __launch_bounds__(256) void k_cg_init(int64_t n, const double *__rest
 }
 
 // ---------------------------------------------------------------- SpMV  w = A u  (+ delta = (w, u))
+// Column of a SELL slot: int32 column ids, or int16 offsets j - i (2 bytes less per block, used
+// when every offset of the pattern fits).
+template <typename IX>
+__device__ __forceinline__ int col_of(IX v, int64_t row) {
+    if constexpr (sizeof(IX) == 2) return (int)row + (int)v;
+    else return (int)v;
+}
+
 // One wavefront per 64-row slice; lane l owns row 64k+l and walks its slots. UNR slots are
 // loaded before they are consumed so that each lane has several independent loads in flight.
-template <typename T, int D, int VP, bool FUSED, int UNR>
+template <typename T, int D, int VP, bool FUSED, int UNR, typename IX>
 __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const int64_t *__restrict__ sp,
-                                               const int32_t *__restrict__ cptr, const int32_t *__restrict__ scol,
+                                               const int32_t *__restrict__ cptr, const IX *__restrict__ scol,
                                                const T *__restrict__ aval, const Vec<T, VP> *__restrict__ u,
                                                Vec<T, VP> *__restrict__ w, double *partial, int cap, const double *scal,
                                                int remap) {
@@ -110,7 +118,7 @@ __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const i
         const int len = row < n ? cptr[row + 1] - cptr[row] : 0;
         const int64_t base = sp[k];
         const int W = (int)((sp[k + 1] - base) >> 6);
-        const int32_t *ci = scol + base + lane;
+        const IX *ci = scol + base + lane;
         const T *av = aval + EB * base + lane;
         T acc[D];
         for (int r = 0; r < D; ++r) acc[r] = (T)0;
@@ -120,7 +128,7 @@ __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const i
             Vec<T, VP> pj[UNR];
             T e[UNR][EB];
 #pragma unroll
-            for (int q = 0; q < UNR; ++q) j[q] = (s + q < len) ? ci[(int64_t)(s + q) * 64] : -1;
+            for (int q = 0; q < UNR; ++q) j[q] = (s + q < len) ? col_of(ci[(int64_t)(s + q) * 64], row) : -1;
 #pragma unroll
             for (int q = 0; q < UNR; ++q) {
                 if (j[q] >= 0) {
@@ -139,7 +147,7 @@ __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const i
         }
         for (; s < W; ++s) {
             if (s < len) {
-                const int j = ci[(int64_t)s * 64];
+                const int j = col_of(ci[(int64_t)s * 64], row);
                 const Vec<T, VP> pj = u[j];
                 for (int r = 0; r < D; ++r)
                     for (int c = 0; c < D; ++c) acc[r] += av[((int64_t)s * EB + r * D + c) * 64] * pj.v[c];
@@ -166,10 +174,10 @@ __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const i
 // contiguous range of the matrix (a few thousand concurrent streams of one slice per wavefront
 // leave the DRAM pages less well used). The partial row sums meet in LDS and are added in
 // wavefront order, so the result does not depend on scheduling.
-template <typename T, int D, int VP, bool FUSED, int UNR, int NWS>
+template <typename T, int D, int VP, bool FUSED, int UNR, int NWS, typename IX>
 __global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslice, int spw,
                                                          const int64_t *__restrict__ sp, const int32_t *__restrict__ cptr,
-                                                         const int32_t *__restrict__ scol, const T *__restrict__ aval,
+                                                         const IX *__restrict__ scol, const T *__restrict__ aval,
                                                          const Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ w,
                                                          double *partial, int cap, const double *scal, int remap) {
     if (FUSED && scal[SC_DONE] != 0.0) return;
@@ -185,7 +193,7 @@ __global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslic
         const int len = row < n ? cptr[row + 1] - cptr[row] : 0;
         const int64_t base = sp[k];
         const int W = (int)((sp[k + 1] - base) >> 6);
-        const int32_t *ci = scol + base + lane;
+        const IX *ci = scol + base + lane;
         const T *av = aval + EB * base + lane;
         T acc[D];
         for (int r = 0; r < D; ++r) acc[r] = (T)0;
@@ -196,7 +204,7 @@ __global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslic
 #pragma unroll
             for (int q = 0; q < UNR; ++q) {
                 const int s = s0 + q * NWS;
-                j[q] = (s < len) ? ci[(int64_t)s * 64] : -1;
+                j[q] = (s < len) ? col_of(ci[(int64_t)s * 64], row) : -1;
             }
 #pragma unroll
             for (int q = 0; q < UNR; ++q) {
@@ -811,6 +819,9 @@ int launch_precond_setup(lpm_ctx *c) {
 }
 
 // workgroups of the SpMV launch (= partials the fused SpMV leaves)
+// int16 column offsets: built (every offset fits) and not switched off
+static bool spmv_idx16(const lpm_ctx *c) { return c->scol16 && c->spmv_idx16; }
+
 static int spmv_grid(const lpm_ctx *c) {
     if (c->spmv_coop && !c->sym_ok) return blocks_for(c->nslice, c->spmv_spw);
     return blocks_for(c->nslice, c->spmv_wpb);
@@ -819,9 +830,14 @@ static int spmv_grid(const lpm_ctx *c) {
 template <typename T, int D, int VP, bool FUSED, int UNR, int NWS>
 static void spmv_coop_go(lpm_ctx *c, const void *in, void *out, double *part) {
     typedef Vec<T, VP> V;
-    hipLaunchKernelGGL((k_spmv_coop<T, D, VP, FUSED, UNR, NWS>), dim3(spmv_grid(c)), dim3(64 * NWS), 0, c->stream, c->n,
-                       c->nslice, c->spmv_spw, c->sp, c->cptr, c->scol, (const T *)c->aval, (const V *)in, (V *)out, part,
-                       c->partial_cap, (const double *)c->scal, c->spmv_remap);
+    if (spmv_idx16(c))
+        hipLaunchKernelGGL((k_spmv_coop<T, D, VP, FUSED, UNR, NWS, int16_t>), dim3(spmv_grid(c)), dim3(64 * NWS), 0, c->stream,
+                           c->n, c->nslice, c->spmv_spw, c->sp, c->cptr, (const int16_t *)c->scol16, (const T *)c->aval,
+                           (const V *)in, (V *)out, part, c->partial_cap, (const double *)c->scal, c->spmv_remap);
+    else
+        hipLaunchKernelGGL((k_spmv_coop<T, D, VP, FUSED, UNR, NWS, int32_t>), dim3(spmv_grid(c)), dim3(64 * NWS), 0, c->stream,
+                           c->n, c->nslice, c->spmv_spw, c->sp, c->cptr, c->scol, (const T *)c->aval, (const V *)in, (V *)out,
+                           part, c->partial_cap, (const double *)c->scal, c->spmv_remap);
 }
 
 template <typename T, int D, int VP, bool FUSED, int UNR>
@@ -837,9 +853,13 @@ static void spmv_go(lpm_ctx *c, const void *in, void *out, double *part) {
         hipLaunchKernelGGL((k_spmv_sym<T, D, VP, FUSED, UNR>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice, c->cptr,
                            c->dslot, c->sp_u, c->ucol, (const T *)c->uval, c->sp_l, c->lcol, c->lpos, (const V *)in, (V *)out,
                            part, c->partial_cap, (const double *)c->scal, c->spmv_remap);
+    else if (spmv_idx16(c))
+        hipLaunchKernelGGL((k_spmv<T, D, VP, FUSED, UNR, int16_t>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice, c->sp,
+                           c->cptr, (const int16_t *)c->scol16, (const T *)c->aval, (const V *)in, (V *)out, part,
+                           c->partial_cap, (const double *)c->scal, c->spmv_remap);
     else
-        hipLaunchKernelGGL((k_spmv<T, D, VP, FUSED, UNR>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice, c->sp, c->cptr,
-                           c->scol, (const T *)c->aval, (const V *)in, (V *)out, part, c->partial_cap,
+        hipLaunchKernelGGL((k_spmv<T, D, VP, FUSED, UNR, int32_t>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice, c->sp,
+                           c->cptr, c->scol, (const T *)c->aval, (const V *)in, (V *)out, part, c->partial_cap,
                            (const double *)c->scal, c->spmv_remap);
 }
 
@@ -1028,7 +1048,7 @@ static int cg_graph_prepare(lpm_ctx *c, int gv, int gs, int batch, F issue) {
         (uintptr_t)c->n, (uintptr_t)c->dim, (uintptr_t)c->precision, (uintptr_t)c->precond, (uintptr_t)gv, (uintptr_t)gs,
         (uintptr_t)batch, (uintptr_t)c->nslice, (uintptr_t)c->spmv_unr, (uintptr_t)c->spmv_wpb, (uintptr_t)c->spmv_coop,
         (uintptr_t)c->spmv_spw, (uintptr_t)c->spmv_remap, (uintptr_t)c->sym_ok, (uintptr_t)c->partial_cap,
-        (uintptr_t)c->sp, (uintptr_t)c->cptr, (uintptr_t)c->scol, (uintptr_t)c->aval, (uintptr_t)c->dslot, (uintptr_t)c->sp_u,
+        (uintptr_t)c->sp, (uintptr_t)c->cptr, (uintptr_t)c->scol, (uintptr_t)c->scol16, (uintptr_t)c->spmv_idx16, (uintptr_t)c->aval, (uintptr_t)c->dslot, (uintptr_t)c->sp_u,
         (uintptr_t)c->ucol, (uintptr_t)c->uval, (uintptr_t)c->sp_l, (uintptr_t)c->lcol, (uintptr_t)c->lpos,
         (uintptr_t)c->v_x, (uintptr_t)c->v_r, (uintptr_t)c->v_u, (uintptr_t)c->v_p, (uintptr_t)c->v_s, (uintptr_t)c->v_w,
         (uintptr_t)c->invdiag, (uintptr_t)c->partial, (uintptr_t)c->scal, (uintptr_t)c->agg_of, (uintptr_t)c->agg_d,

@@ -797,6 +797,26 @@ __global__ __launch_bounds__(256) void k_sell_cols(int64_t n, const int32_t *__r
     for (int c = 0; c < W; ++c) scol[sp[k] + (int64_t)c * 64 + l] = c < nc ? cidx[c0 + c] : (int32_t)i;
 }
 
+// The SELL columns as int16 offsets j - i (padding slots 0: the row itself); *bad is set when an
+// offset does not fit, and the int16 copy is then not used.
+__global__ __launch_bounds__(256) void k_sell_cols16(int64_t n, const int32_t *__restrict__ cptr,
+                                                      const int32_t *__restrict__ cidx, const int64_t *__restrict__ sp,
+                                                      int16_t *__restrict__ scol16, int32_t *__restrict__ bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k = i >> 6, l = i & 63;
+    const int W = (int)((sp[k + 1] - sp[k]) >> 6);
+    const int c0 = cptr[i], nc = cptr[i + 1] - c0;
+    for (int c = 0; c < W; ++c) {
+        int d = c < nc ? cidx[c0 + c] - (int)i : 0;
+        if (d < -32768 || d > 32767) {
+            *bad = 1;
+            d = 0;
+        }
+        scol16[sp[k] + (int64_t)c * 64 + l] = (int16_t)d;
+    }
+}
+
 // position of block (j, i) in block row j, for the transposed store
 __global__ __launch_bounds__(256) void k_tslot(int64_t nconn, int64_t n, const int32_t *__restrict__ cptr,
                                                 const int32_t *__restrict__ cidx, const int32_t *__restrict__ rowof,
@@ -1368,8 +1388,22 @@ int launch_pattern(lpm_ctx *c) {
     LPM_HIP(c, hipMemcpyAsync(c->sp, sp.data(), sizeof(int64_t) * (c->nslice + 1), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_sell_cols, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->scol);
     rc = check_launch(c, "k_sell_cols");
+    if (rc) return rc;
+    // int16 offsets for the SpMV when every column lies within 32k rows of its row
+    LPM_HIP(c, hipMalloc(&c->scol16, sizeof(int16_t) * (c->nslots > 0 ? c->nslots : 1)));
+    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_sell_cols16, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp,
+                       c->scol16, c->iflag);
+    rc = check_launch(c, "k_sell_cols16");
+    if (rc) return rc;
+    int32_t bad = 0;
+    LPM_HIP(c, hipMemcpyAsync(&bad, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     LPM_HIP(c, hipStreamSynchronize(c->stream));
-    return rc;
+    if (bad) {
+        hipFree(c->scol16);
+        c->scol16 = nullptr;
+    }
+    return LPM_OK;
 }
 
 int launch_tslot(lpm_ctx *c) {

@@ -64,6 +64,8 @@ struct lpm_ctx {
     // SELL-64 copy of the pattern used by the CG matrix (see reduce.h)
     int64_t *sp = nullptr;
     int32_t *scol = nullptr;
+    int16_t *scol16 = nullptr;        // the same columns as offsets j - i when all fit in int16 (else null)
+    int spmv_idx16 = 1;               // mode: the SpMV reads scol16 when it exists
     int64_t nslice = 0, nslots = 0;
     // Symmetric half of the CG matrix, used by the SpMV when the stored matrix is symmetric:
     // upper SELL-64 (blocks j >= i of row i, the diagonal first) holding the values, and a
