@@ -181,6 +181,7 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        64-row slice | 8, 16: that many wavefronts share each slice),
  *                        "spmv_spw" (consecutive slices per cooperative workgroup) -- the SpMV
  *                        launch configuration
+ *  "cg_vec_wg"        most workgroups of the CG vector kernels (default 1024; grid-stride beyond)
  *  "spmv_idx16"       1 (default): the SpMV reads its column indices as int16 offsets j - i when
  *                        every offset of the pattern fits (get_mode("spmv_index_bytes") = 2) |
  *                        0: int32 column ids (4)

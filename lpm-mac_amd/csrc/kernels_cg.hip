@@ -1083,7 +1083,7 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     typedef Vec<T, VP> V;
     const int64_t n = c->n;
     // vector kernels: grid-stride over at most 1024 workgroups (4 per CU; few partials to sum)
-    const int gv = std::max(1, std::min(blocks_for(n, 256), 1024)), gs = spmv_grid(c);
+    const int gv = std::max(1, std::min(blocks_for(n, 256), c->cg_vec_wg)), gs = spmv_grid(c);
     if (gv > c->partial_cap || gs > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "CG grid exceeds partial buffer");
     const int64_t maxit = c->cg_maxit < 0 ? 2 * n * c->dim : c->cg_maxit;
     hipLaunchKernelGGL((k_cg_init<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, c->residual, (const T *)c->invdiag,
