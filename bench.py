@@ -222,7 +222,7 @@ def main():
                          "whose FETCH_SIZE / WRITE_SIZE readings it scales")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding of this path is a design, not an implementation)", file=sys.stderr)
         sys.exit(2)
 
