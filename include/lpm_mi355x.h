@@ -339,6 +339,20 @@ int lpm_adjoint_reaction(lpm_ctx *ctx, int32_t group, int32_t axis, double *dF_d
 int lpm_adjoint_linear(lpm_ctx *ctx, const double *w_pin, const double *w_x, double *dG_dKn, double *dG_dTv,
                        double *dG_dbdamage, double *G, int32_t *refinements, double *rel_residual);
 
+/* Sensitivity of the same linear objective, at the current equilibrium, to each particle's HCF
+ * parameters A, B, d (mat rows 0-2; n doubles each, any may be NULL) through the last fatigue
+ * damage update (lpm_update_fatigue_damage: SolverFatigue::updateFatigueDamage,
+ * solver_fatigue.h:75-126, with ParticleFatigueHCF::updateParticleStateVariables and
+ * updateBondsForce, particle_fatigue_hcf.h:65-80,127-145):
+ *   D'_i = D_i + dt sum_j w_ij A_j s0_j^B_j (s2 - s1)_j^d_j,  bd_b = max(bd_b, max(D'_i, D'_j)),
+ * with the energies, the step dt and the earlier history held fixed: one transposed solve gives
+ * dG/dbdamage, which is carried back through the maxima (std::max order), the nonlocal weights
+ * w_ij at the update's positions, and the rate law. Particles whose damage was clamped to 1 and
+ * the excluded type contribute nothing. Needs the HCF model, an update since the context was
+ * created, and one nonlocal length (symmetric neighbour lists). */
+int lpm_adjoint_fatigue_params(lpm_ctx *ctx, const double *w_pin, const double *w_x, double *dG_dA, double *dG_dB,
+                               double *dG_dd, double *G, int32_t *refinements, double *rel_residual);
+
 /* ------------------------------------------------------------------ load-step checkpoints
  * (no reference counterpart: the reference keeps only the last converged copy,
  * storeStateVar / resetStateVar, particle.h:108-148, assembly.h:140-159)

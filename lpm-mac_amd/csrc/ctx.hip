@@ -782,6 +782,8 @@ int lpm_destroy(lpm_ctx *c) {
         if (p) hipFree(p);
     if (c->cg_exec) hipGraphExecDestroy(c->cg_exec);
     if (c->adj_scr) hipFree(c->adj_scr);
+    if (c->fat_xyz) hipFree(c->fat_xyz);
+    if (c->fat_range) hipFree(c->fat_range);
     for (auto &g : c->groups)
         if (g.ids) hipFree(g.ids);
     if (c->h_scal) hipHostFree(c->h_scal);
@@ -1280,6 +1282,38 @@ int lpm_adjoint_linear(lpm_ctx *c, const double *w_pin, const double *w_x, doubl
     int it = 0;
     double rr = 0;
     if (!rc) rc = adjoint_linear(c, w_pin ? d : nullptr, w_x ? d + nd : nullptr, dG_dKn, dG_dTv, dG_dbdamage, G, &it, &rr);
+    hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (rc) return rc;
+    if (refinements) *refinements = it;
+    if (rel_residual) *rel_residual = rr;
+    return LPM_OK;
+}
+
+int lpm_adjoint_fatigue_params(lpm_ctx *c, const double *w_pin, const double *w_x, double *dG_dA, double *dG_dB,
+                               double *dG_dd, double *G, int32_t *refinements, double *rel_residual) {
+    if (!c) return LPM_E_ARG;
+    if (!w_pin && !w_x) return lpm_fail(c, LPM_E_ARG, "no objective weights");
+    if (c->nonlocal_L && c->n > 0) {
+        // the gather over each particle's own nonlocal list needs symmetric lists: one length
+        std::vector<double> L(c->n);
+        LPM_HIP(c, hipMemcpyAsync(L.data(), c->nonlocal_L, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+        for (double v : L)
+            if (v != L[0]) return lpm_fail(c, LPM_E_STATE, "fatigue-parameter sensitivity needs one nonlocal length");
+    }
+    const int64_t nd = c->n * c->dim;
+    double *d = nullptr;
+    LPM_HIP(c, hipMalloc(&d, sizeof(double) * 2 * std::max<int64_t>(nd, 1)));
+    int rc = LPM_OK;
+    if (w_pin && hipMemcpyAsync(d, w_pin, sizeof(double) * nd, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "adjoint weights upload");
+    if (!rc && w_x && hipMemcpyAsync(d + nd, w_x, sizeof(double) * nd, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "adjoint weights upload");
+    int it = 0;
+    double rr = 0;
+    if (!rc)
+        rc = adjoint_fatigue_params(c, w_pin ? d : nullptr, w_x ? d + nd : nullptr, dG_dA, dG_dB, dG_dd, G, &it, &rr);
     hipStreamSynchronize(c->stream);
     hipFree(d);
     if (rc) return rc;

@@ -216,6 +216,95 @@ __global__ __launch_bounds__(256) void k_param_grad(int64_t n, int dim, const in
     }
 }
 
+// ---- sensitivity of the next equilibrium to the HCF law's parameters through one fatigue update
+// (SolverFatigue::updateFatigueDamage, solver_fatigue.h:75-126; ParticleFatigueHCF,
+// particle_fatigue_hcf.h:127-160): D'_i = D_i + dt * Ddot_nl,i, Ddot_nl,i = sum_j w_ij Ddot_j with
+// w_ij = phi_ij V_m,j / sum_k phi_ik V_m,k, Ddot_j = A_j s0_j^B_j (s2 - s1)_j^d_j, and the bond
+// damage of the next force evaluation bd_b = max(bd_b, max(D_i, D_j)) (std::max order, as
+// updateBondsForce, particle_fatigue_hcf.h:65-80). The energies s, the update's step dt and the
+// earlier history are held fixed.
+
+// dG/dD'_i: the bond damages that D'_i sets. Bond b = (i -> j) takes D_i when D_i >= D_j, its
+// reverse (j -> i) when D_i > D_j, each only where the maximum did not stay at the old bond
+// damage (bd_b = D_i). Zero where D'_i was clamped to 1 and for the excluded type.
+__global__ __launch_bounds__(256) void k_fat_gd(int64_t n, int undamaged, const int32_t *__restrict__ type,
+                                                 const int32_t *__restrict__ bptr, const int32_t *__restrict__ p2,
+                                                 const int32_t *__restrict__ opp, const double *__restrict__ damage,
+                                                 const double *__restrict__ bdamage, const double *__restrict__ gbd,
+                                                 double *__restrict__ gD) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double Di = damage[i];
+    double s = 0.0;
+    if (type[i] != undamaged && Di != 1.0) {
+        for (int b = bptr[i]; b < bptr[i + 1]; ++b) {
+            const double Dj = damage[p2[b]];
+            const int o = opp[b];
+            if (Di >= Dj && bdamage[b] == Di) s += gbd[b];
+            if (Di > Dj && bdamage[o] == Di) s += gbd[o];
+        }
+    }
+    gD[i] = s;
+}
+
+// the nonlocal weights' normaliser sum_k phi_ik V_m,k at the update's positions
+__global__ __launch_bounds__(256) void k_fat_norm(int64_t n, int undamaged, double volume, int nneighbors,
+                                                   const int32_t *__restrict__ type, const int32_t *__restrict__ nlptr,
+                                                   const int32_t *__restrict__ nlidx, const int32_t *__restrict__ bptr,
+                                                   const double *__restrict__ xyz, const double *__restrict__ L,
+                                                   double *__restrict__ norm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double A = 0.0;
+    if (type[i] != undamaged) {
+        for (int q = nlptr[i]; q < nlptr[i + 1]; ++q) {
+            const int64_t j = nlidx[q];
+            if (type[j] == undamaged) continue;
+            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1], dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+            const double u = 1 / (1 + pow(sqrt(dx * dx + dy * dy + dz * dz) / L[i], 8.0));
+            A += u * u * volume * (bptr[j + 1] - bptr[j]) / nneighbors;
+        }
+    }
+    norm[i] = A;
+}
+
+// per particle j: S_j = sum over i with j in NL(i) of dG/dD'_i w_ij (gathered over NL(j), the
+// lists being symmetric), then dG/dA_j = dt S_j s0^B r^d, dG/dB_j = dt S_j Ddot_j ln s0,
+// dG/dd_j = dt S_j Ddot_j ln r (r = s2 - s1 before the update)
+__global__ __launch_bounds__(256) void k_fat_params(int64_t n, int undamaged, double volume, int nneighbors, double dt,
+                                                     const int32_t *__restrict__ type, const int32_t *__restrict__ nlptr,
+                                                     const int32_t *__restrict__ nlidx, const int32_t *__restrict__ bptr,
+                                                     const double *__restrict__ xyz, const double *__restrict__ L,
+                                                     const double *__restrict__ norm, const double *__restrict__ gD,
+                                                     const double *__restrict__ mat, const double *__restrict__ sv,
+                                                     const double *__restrict__ range, double *__restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double gA = 0.0, gB = 0.0, gd = 0.0;
+    if (type[j] != undamaged) {
+        const double Vj = volume * (bptr[j + 1] - bptr[j]) / nneighbors;
+        double S = 0.0;
+        for (int q = nlptr[j]; q < nlptr[j + 1]; ++q) {
+            const int64_t i = nlidx[q];
+            if (type[i] == undamaged || norm[i] == 0.0) continue;
+            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1], dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+            const double u = 1 / (1 + pow(sqrt(dx * dx + dy * dy + dz * dz) / L[i], 8.0));
+            S += gD[i] * u * u * Vj / norm[i];
+        }
+        const double A = mat[LPM_NMAT * j], B = mat[LPM_NMAT * j + 1], d = mat[LPM_NMAT * j + 2];
+        const double s0 = sv[LPM_NSV * j], r = range[j];
+        if (r > 0) {
+            const double base = pow(s0, B) * pow(r, d);
+            gA = dt * S * base;
+            gB = s0 > 0 ? dt * S * A * base * log(s0) : 0.0;
+            gd = dt * S * A * base * log(r);
+        }
+    }
+    out[j] = gA;
+    out[n + j] = gB;
+    out[2 * n + j] = gd;
+}
+
 }  // namespace
 
 static int check(lpm_ctx *c, const char *what) {
@@ -282,7 +371,7 @@ static int add_x(lpm_ctx *c, double *lam) {
 // arrays in DoF order; either may be null). On return gKn, gTv, gBd (host, nbond each, any may
 // be null) hold dG/dKn, dG/dTv and dG/dbdamage, and G its value.
 int adjoint_linear(lpm_ctx *c, const double *a_dev, const double *b_dev, double *gKn, double *gTv, double *gBd, double *G,
-                   int *refinements, double *rel_residual) {
+                   int *refinements, double *rel_residual, double *gBd_dev) {
     const int64_t n = c->n, nd = n * c->dim;
     int rc = geometry_force_state(c);
     if (rc) return rc;
@@ -351,6 +440,9 @@ int adjoint_linear(lpm_ctx *c, const double *a_dev, const double *b_dev, double 
     hipLaunchKernelGGL(k_param_grad, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->dim, c->bptr, c->bsplit, c->p2,
                        c->opp, c->cs, c->dLe, c->dLe_total, c->bdamage, c->Kn, c->Tv, e, lam, gb, gb + nb, gb + 2 * nb);
     rc = check(c, "k_param_grad");
+    if (!rc && gBd_dev &&
+        hipMemcpyAsync(gBd_dev, gb + 2 * nb, sizeof(double) * c->nbond, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "adjoint copy");
     double *outs[3] = {gKn, gTv, gBd};
     for (int k = 0; k < 3 && !rc; ++k)
         if (outs[k] && hipMemcpyAsync(outs[k], gb + k * nb, sizeof(double) * c->nbond, hipMemcpyDeviceToHost, c->stream) !=
@@ -396,5 +488,43 @@ int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, do
     if (!rc) rc = adjoint_linear(c, a, nullptr, gKn, gTv, nullptr, F, refinements, rel_residual);
     hipStreamSynchronize(c->stream);
     hipFree(a);
+    return rc;
+}
+
+// dG/dA, dG/dB, dG/dd per particle (host arrays, n each, any may be null) for the linear objective
+// G = a.Pin + b.x at the current equilibrium, through the last fatigue damage update
+int adjoint_fatigue_params(lpm_ctx *c, const double *a_dev, const double *b_dev, double *gA, double *gB, double *gd,
+                           double *G, int *refinements, double *rel_residual) {
+    if (c->model != 3) return lpm_fail(c, LPM_E_STATE, "fatigue-parameter sensitivity needs the HCF model");
+    if (!c->fat_valid) return lpm_fail(c, LPM_E_STATE, "no fatigue damage update to differentiate");
+    if (!c->nlptr) return lpm_fail(c, LPM_E_STATE, "no nonlocal neighbour lists");
+    const int64_t n = c->n, nb = std::max<int64_t>(c->nbond, 1);
+    double *buf = nullptr;  // gbd (nbond), gD (n), norm (n), out (3n)
+    LPM_HIP(c, hipMalloc(&buf, sizeof(double) * (nb + 5 * std::max<int64_t>(n, 1))));
+    double *gbd = buf, *gD = buf + nb, *norm = gD + n, *out = norm + n;
+    int rc = adjoint_linear(c, a_dev, b_dev, nullptr, nullptr, nullptr, G, refinements, rel_residual, gbd);
+    const int g = blocks_for(n, 256);
+    if (!rc) {
+        hipLaunchKernelGGL(k_fat_gd, dim3(g), dim3(256), 0, c->stream, n, c->fat_undamaged, c->type, c->bptr, c->p2, c->opp,
+                           c->damage, c->bdamage, gbd, gD);
+        rc = check(c, "k_fat_gd");
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(k_fat_norm, dim3(g), dim3(256), 0, c->stream, n, c->fat_undamaged, c->volume, c->nneighbors,
+                           c->type, c->nlptr, c->nlidx, c->bptr, c->fat_xyz, c->nonlocal_L, norm);
+        rc = check(c, "k_fat_norm");
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(k_fat_params, dim3(g), dim3(256), 0, c->stream, n, c->fat_undamaged, c->volume, c->nneighbors,
+                           c->fat_dt, c->type, c->nlptr, c->nlidx, c->bptr, c->fat_xyz, c->nonlocal_L, norm, gD, c->mat,
+                           c->sv, c->fat_range, out);
+        rc = check(c, "k_fat_params");
+    }
+    double *outs[3] = {gA, gB, gd};
+    for (int k = 0; k < 3 && !rc; ++k)
+        if (outs[k] && hipMemcpyAsync(outs[k], out + k * n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = lpm_fail(c, LPM_E_HIP, "fatigue sensitivity copy");
+    hipStreamSynchronize(c->stream);
+    hipFree(buf);
     return rc;
 }

@@ -704,6 +704,14 @@ __global__ __launch_bounds__(256) void k_fatigue_damage(int64_t n, int model, in
     set_flag(any, flag);
 }
 
+// energy range s2 - s1 of each particle before a fatigue update resets it (0 when not positive)
+__global__ __launch_bounds__(256) void k_energy_range(int64_t n, const double *__restrict__ sv, double *__restrict__ range) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d = sv[LPM_NSV * i + 2] - sv[LPM_NSV * i + 1];
+    range[i] = d > 0 ? d : 0.0;
+}
+
 // ---------------------------------------------------------------- pattern (2-hop conns)
 // Assembly::updateConnections (assembly.h:236-266): sorted unique {bond partners and their
 // same-layer partners}, self included. One workgroup per particle: candidates are gathered in
@@ -1234,6 +1242,17 @@ int launch_static_damage(lpm_ctx *c, int undamaged, int *any) {
 int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any) {
     int rc = launch_nonlocal_ddot(c, undamaged);
     if (rc) return rc;
+    if (c->model == HCF) {
+        // what the fatigue-parameter sensitivity needs of this update
+        if (!c->fat_xyz) LPM_HIP(c, hipMalloc(&c->fat_xyz, sizeof(double) * 3 * c->n));
+        if (!c->fat_range) LPM_HIP(c, hipMalloc(&c->fat_range, sizeof(double) * c->n));
+        LPM_HIP(c, hipMemcpyAsync(c->fat_xyz, c->xyz, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(k_energy_range, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->sv, c->fat_range);
+        if ((rc = check_launch(c, "k_energy_range"))) return rc;
+        c->fat_dt = dNdt;
+        c->fat_undamaged = undamaged;
+        c->fat_valid = true;
+    }
     LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
     hipLaunchKernelGGL(k_fatigue_damage, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->model, undamaged,
                        dNdt, c->type, c->mat, c->Ddot_nonlocal, c->damage_last, c->damage, c->sv, c->iflag);

@@ -135,6 +135,13 @@ struct lpm_ctx {
     std::vector<void *> ckpt;
     // per-bond scratch of the force JVP / VJP (4 doubles per bond), allocated on first use
     double *adj_scr = nullptr;
+    // the last fatigue damage update (SolverFatigue::updateFatigueDamage): positions its nonlocal
+    // weights used and each particle's energy range s2 - s1 (the update resets both), its cycle
+    // step and excluded type, for the fatigue-parameter sensitivity
+    double *fat_xyz = nullptr, *fat_range = nullptr;
+    double fat_dt = 0.0;
+    int fat_undamaged = -1;
+    bool fat_valid = false;
     // Specimen ensemble: independent particle systems (contiguous id ranges, no bond between
     // them) solved in the same launches, each with its own Newton test.
     int nspec = 0;                    // 0: one system
@@ -242,7 +249,9 @@ int force_vjp(lpm_ctx *c, const double *w, double *y);
 int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, double *F, int *refinements,
                      double *rel_residual);
 int adjoint_linear(lpm_ctx *c, const double *a_dev, const double *b_dev, double *gKn, double *gTv, double *gBd, double *G,
-                   int *refinements, double *rel_residual);
+                   int *refinements, double *rel_residual, double *gBd_dev = nullptr);
+int adjoint_fatigue_params(lpm_ctx *c, const double *a_dev, const double *b_dev, double *gA, double *gB, double *gd,
+                           double *G, int *refinements, double *rel_residual);
 
 // helpers
 static inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -108,6 +108,7 @@ _SIGS = {
                                                 _D, ctypes.c_int32, ctypes.c_int32, _IP]),
     "lpm_adjoint_reaction": (_I, [_P, ctypes.c_int32, ctypes.c_int32, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
     "lpm_adjoint_linear": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
+    "lpm_adjoint_fatigue_params": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
     "lpm_checkpoint_save": (_I, [_P, ctypes.c_int32]),
     "lpm_checkpoint_restore": (_I, [_P, ctypes.c_int32]),
     "lpm_checkpoint_release": (_I, [_P, ctypes.c_int32]),
@@ -534,6 +535,22 @@ class Context:
         self._chk(lib().lpm_adjoint_linear(self.h, ptr[0], ptr[1], gk.ctypes.data_as(_DP), gt.ctypes.data_as(_DP),
                                            gd.ctypes.data_as(_DP), ctypes.byref(G), ctypes.byref(it), ctypes.byref(rr)))
         return G.value, gk, gt, gd, it.value, rr.value
+
+    def adjoint_fatigue_params(self, w_pin=None, w_x=None):
+        """dG/dA, dG/dB, dG/dd per particle (HCF law) through the last fatigue damage update, for
+        G = w_pin . Pin + w_x . x at the current equilibrium. Returns (G, dA, dB, dd, refinements,
+        relative residual)."""
+        nd = self.n * self.size("dim")
+        w = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (w_pin, w_x)]
+        for a in w:
+            assert a is None or a.size == nd
+        ga, gb, gd = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        G, rr, it = _D(), _D(), ctypes.c_int32()
+        ptr = [None if a is None else a.ctypes.data_as(_DP) for a in w]
+        self._chk(lib().lpm_adjoint_fatigue_params(self.h, ptr[0], ptr[1], ga.ctypes.data_as(_DP), gb.ctypes.data_as(_DP),
+                                                   gd.ctypes.data_as(_DP), ctypes.byref(G), ctypes.byref(it),
+                                                   ctypes.byref(rr)))
+        return G.value, ga, gb, gd, it.value, rr.value
 
     def checkpoint_save(self, slot):
         self._chk(lib().lpm_checkpoint_save(self.h, int(slot)))
