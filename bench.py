@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--config", type=int, default=2,
                     help="2: the headline (SC 64^3); 3: FCC 40^3 J2 crystal; 4: cracked 400 x 400 plate with breaking "
                          "and the adjoint (one JSON line each)")
+    ap.add_argument("--cg-graph", type=int, default=0,
+                    help="with --config 3|4: 1 replays each batch of CG iterations from a captured HIP graph")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="run the 256 MiB calibration copy (k_copy_calib) only 3 times, for rocprofv3 --pmc passes "
                          "whose FETCH_SIZE / WRITE_SIZE readings it scales")
@@ -699,7 +701,8 @@ def config3_bench(args):
     g = G.Context(G.FCC3D, r, G.J2, xyz, bonds="device")
     g.set_j2(0.0, False, 200e3, 0.3, 300.0, 0.0, 2e3, 0.0, 1.0, damage_threshold=1.0, elmap_literal=0)
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
-           cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=50, cg_maxit=20000, predictor=args.predictor)
+           cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=50, cg_maxit=20000, predictor=args.predictor,
+           cg_graph=args.cg_graph)
     steps = 50
     ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", 0.0), (lo, "z", 0.0), (hi, "z", 0.02 * L / steps)])
     g.update_geometry()
@@ -718,7 +721,7 @@ def config3_bench(args):
                              f"uniaxial tension 2% in {steps} steps",
                  "bonds": g.size("nbonds"), "newton_iters_per_step": g.stat("newton_iters") / steps,
                  "cg_iters_per_step": g.stat("cg_iters") / steps, "newton_tol": args.tol_nr,
-                 "predictor": args.predictor, "precond": args.precond, "setup_seconds": t_setup,
+                 "predictor": args.predictor, "precond": args.precond, "cg_graph": args.cg_graph, "setup_seconds": t_setup,
                  "yielded_fraction": float(np.mean(alpha > 0))})
     print(json.dumps(out), flush=True)
 
@@ -751,7 +754,8 @@ def config4_bench(args):
     g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=(bp2, bs2, p2.astype(np.int32)))
     g.set_elastic(0.0, True, 69e3, 0.3, 1e-3, elmap_literal=0)
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
-           cg_forcing=args.cg_forcing, tol_NR=1e-6, max_NR_iter=50, cg_maxit=50000, predictor=args.predictor)
+           cg_forcing=args.cg_forcing, tol_NR=1e-6, max_NR_iter=50, cg_maxit=50000, predictor=args.predictor,
+           cg_graph=args.cg_graph)
     y = xyz[:, 1]
     lo, hi = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
     inc, steps = 0.0006, 40
@@ -782,6 +786,7 @@ def config4_bench(args):
                              f"opening {inc} per edge per step, {steps} steps",
                  "bonds": g.size("nbonds"), "broken_bonds": int((bd > 0.5).sum()), "newton_iters_per_step": newton,
                  "cg_iters_per_step": cg, "newton_tol": 1e-6, "predictor": args.predictor, "precond": args.precond,
+                 "cg_graph": args.cg_graph,
                  "adjoint": {"seconds_polish_and_adjoint": t_adj, "refinements": nref, "relative_residual": rres,
                              "top_reaction": F}})
     print(json.dumps(out), flush=True)
