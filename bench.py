@@ -755,7 +755,7 @@ def config4_bench(args):
     g.set_elastic(0.0, True, 69e3, 0.3, 1e-3, elmap_literal=0)
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
            cg_forcing=args.cg_forcing, tol_NR=1e-6, max_NR_iter=50, cg_maxit=50000, predictor=args.predictor,
-           cg_graph=args.cg_graph)
+           cg_graph=args.cg_graph, coarse_cells=args.coarse_cells)
     y = xyz[:, 1]
     lo, hi = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
     inc, steps = 0.0006, 40
@@ -786,7 +786,7 @@ def config4_bench(args):
                              f"opening {inc} per edge per step, {steps} steps",
                  "bonds": g.size("nbonds"), "broken_bonds": int((bd > 0.5).sum()), "newton_iters_per_step": newton,
                  "cg_iters_per_step": cg, "newton_tol": 1e-6, "predictor": args.predictor, "precond": args.precond,
-                 "cg_graph": args.cg_graph,
+                 "cg_graph": args.cg_graph, "coarse_dofs": int(g.get_mode("ncoarse")),
                  "adjoint": {"seconds_polish_and_adjoint": t_adj, "refinements": nref, "relative_residual": rres,
                              "top_reaction": F}})
     print(json.dumps(out), flush=True)

@@ -392,6 +392,14 @@ static int coarse_setup(lpm_ctx *c) {
     const int D = c->dim, M = D == 3 ? 6 : 3;
     std::vector<double> x(3 * n);
     std::vector<int32_t> cp(n + 1), ci(c->nconn);
+    // bonds, to split cells into the pieces that intact bonds connect
+    std::vector<int32_t> hbp(n + 1), hp2(std::max<int64_t>(c->nbond, 1));
+    std::vector<double> hbd(std::max<int64_t>(c->nbond, 1));
+    LPM_HIP(c, hipMemcpyAsync(hbp.data(), c->bptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    if (c->nbond > 0) {
+        LPM_HIP(c, hipMemcpyAsync(hp2.data(), c->p2, sizeof(int32_t) * c->nbond, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipMemcpyAsync(hbd.data(), c->bdamage, sizeof(double) * c->nbond, hipMemcpyDeviceToHost, c->stream));
+    }
     LPM_HIP(c, hipMemcpyAsync(x.data(), c->xyz0, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
     LPM_HIP(c, hipMemcpyAsync(cp.data(), c->cptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
     LPM_HIP(c, hipMemcpyAsync(ci.data(), c->cidx, sizeof(int32_t) * c->nconn, hipMemcpyDeviceToHost, c->stream));
@@ -403,7 +411,17 @@ static int coarse_setup(lpm_ctx *c) {
     // target) equal cells: a remainder cell one or two lattice rows thick would make an aggregate
     // whose modes are dependent once its prescribed DoFs are masked. The target edge is the set
     // value, or, per group, the smallest of 8, 9, 10, ... lattice spacings that keeps the group's
-    // block at most 768 DoFs (the inversion's cost grows with its cube).
+    // block at most 768 DoFs (the inversion's cost grows with its cube). A cell that a crack or
+    // hole cuts is split into the pieces its intact bonds connect: the two faces of a crack move
+    // apart, which one set of rigid-body modes per cell cannot follow.
+    std::vector<int32_t> uf(n);
+    auto find = [&](int32_t a) {
+        while (uf[a] != a) {
+            uf[a] = uf[uf[a]];
+            a = uf[a];
+        }
+        return a;
+    };
     const int64_t nsp = std::max(c->nspec, 1);
     std::vector<int64_t> first(nsp + 1, 0);
     if (c->nspec > 0)
@@ -430,8 +448,8 @@ static int coarse_setup(lpm_ctx *c) {
                 dims[k] = std::max<int64_t>(1, (int64_t)std::llround((hi[k] - lo[k]) / target));
                 edge[k] = (hi[k] - lo[k]) / (double)dims[k];
             }
-            std::vector<int32_t> id(dims[0] * dims[1] * dims[2], -1);
-            ns = 0;
+            // cell of each particle, then the connected pieces of each cell (union-find over the
+            // intact bonds whose two ends share a cell), numbered in order of first member
             for (int64_t i = first[s]; i < first[s + 1]; ++i) {
                 int64_t key = 0;
                 for (int k = D - 1; k >= 0; --k) {
@@ -439,8 +457,22 @@ static int coarse_setup(lpm_ctx *c) {
                         edge[k] > 0.0 ? std::min<int64_t>(dims[k] - 1, (int64_t)((x[3 * i + k] - lo[k]) / edge[k])) : 0;
                     key = key * dims[k] + ck;
                 }
-                if (id[key] < 0) id[key] = ns++;
-                agg[i] = nagg + id[key];
+                agg[i] = (int32_t)key;
+                uf[i] = (int32_t)i;
+            }
+            for (int64_t i = first[s]; i < first[s + 1]; ++i)
+                for (int b = hbp[i]; b < hbp[i + 1]; ++b) {
+                    const int32_t j = hp2[b];
+                    if (j < first[s] || j >= first[s + 1] || agg[j] != agg[i] || std::fabs(hbd[b] - 1.0) < LPM_EPS) continue;
+                    const int32_t ra = find((int32_t)i), rb = find(j);
+                    if (ra != rb) uf[std::max(ra, rb)] = std::min(ra, rb);
+                }
+            std::vector<int32_t> id(first[s + 1] - first[s], -1);
+            ns = 0;
+            for (int64_t i = first[s]; i < first[s + 1]; ++i) {
+                const int64_t r = find((int32_t)i) - first[s];
+                if (id[r] < 0) id[r] = ns++;
+                agg[i] = nagg + id[r];
             }
             if (c->coarse_cells > 0 || (int64_t)ns * M <= 768) break;
         }
