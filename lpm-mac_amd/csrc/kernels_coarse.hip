@@ -450,20 +450,29 @@ static int coarse_setup(lpm_ctx *c) {
             }
             // cell of each particle, then the connected pieces of each cell (union-find over the
             // intact bonds whose two ends share a cell), numbered in order of first member
+            const int64_t ncells = dims[0] * dims[1] * dims[2];
+            std::vector<int64_t> key(first[s + 1] - first[s]);
+            std::vector<char> used(ncells, 0);
+            int64_t ncell = 0;
             for (int64_t i = first[s]; i < first[s + 1]; ++i) {
-                int64_t key = 0;
+                int64_t kk = 0;
                 for (int k = D - 1; k >= 0; --k) {
                     const int64_t ck =
                         edge[k] > 0.0 ? std::min<int64_t>(dims[k] - 1, (int64_t)((x[3 * i + k] - lo[k]) / edge[k])) : 0;
-                    key = key * dims[k] + ck;
+                    kk = kk * dims[k] + ck;
                 }
-                agg[i] = (int32_t)key;
-                uf[i] = (int32_t)i;
+                key[i - first[s]] = kk;
+                if (!used[kk]) used[kk] = 1, ++ncell;
             }
+            // splitting only adds aggregates: a size whose plain cells exceed the budget is skipped
+            if (c->coarse_cells <= 0 && ncell * M > 768 && ncells > 1) continue;
+            for (int64_t i = first[s]; i < first[s + 1]; ++i) uf[i] = (int32_t)i;
             for (int64_t i = first[s]; i < first[s + 1]; ++i)
                 for (int b = hbp[i]; b < hbp[i + 1]; ++b) {
                     const int32_t j = hp2[b];
-                    if (j < first[s] || j >= first[s + 1] || agg[j] != agg[i] || std::fabs(hbd[b] - 1.0) < LPM_EPS) continue;
+                    if (j < first[s] || j >= first[s + 1] || key[j - first[s]] != key[i - first[s]] ||
+                        std::fabs(hbd[b] - 1.0) < LPM_EPS)
+                        continue;
                     const int32_t ra = find((int32_t)i), rb = find(j);
                     if (ra != rb) uf[std::max(ra, rb)] = std::min(ra, rb);
                 }
@@ -474,6 +483,17 @@ static int coarse_setup(lpm_ctx *c) {
                 if (id[r] < 0) id[r] = ns++;
                 agg[i] = nagg + id[r];
             }
+            if ((int64_t)ns * M > 1024) {
+                // more pieces than one block holds (a shattered specimen): plain cells at this size
+                std::vector<int32_t> cid(ncells, -1);
+                ns = 0;
+                for (int64_t i = first[s]; i < first[s + 1]; ++i) {
+                    const int64_t kk = key[i - first[s]];
+                    if (cid[kk] < 0) cid[kk] = ns++;
+                    agg[i] = nagg + cid[kk];
+                }
+            }
+            if (ncells == 1) break;
             if (c->coarse_cells > 0 || (int64_t)ns * M <= 768) break;
         }
         if ((int64_t)ns * M > 1024)
