@@ -82,7 +82,7 @@ def spmv_bytes(ctx):
     return (36 + ib) * nblk + 4 * (n + 1) + vec, name
 
 
-def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0, predictor=0):
+def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0, predictor=0, coarse_reuse=0):
     import lpmgpu as G
 
     xyz, box, r, L = sc_workload(n)
@@ -91,7 +91,7 @@ def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0, 
     ctx.set_elastic(0.0, False, 200e3, 0.3, 1.0, elmap_literal=0)
     ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, cg_tol=cg_tol, tol_NR=tol_nr,
              max_NR_iter=30, cg_maxit=20000, precond=precond, spmv_sym=spmv_sym, cg_forcing=cg_forcing,
-             coarse_cells=coarse_cells, predictor=predictor)
+             coarse_cells=coarse_cells, predictor=predictor, coarse_reuse=coarse_reuse)
     z = xyz[:, 2]
     bottom = np.nonzero(z < z.min() + 1e-6)[0]
     top = np.nonzero(z > z.max() - 1e-6)[0]
@@ -207,6 +207,9 @@ def main():
     ap.add_argument("--predictor", type=int, default=1,
                     help="1: start each static step from the previous step's converged increment (Newton runs to the "
                          "same test); 0: the reference's start")
+    ap.add_argument("--coarse-reuse", type=int, default=1,
+                    help="1: keep the two-level coarse space across assemblies while it still works (rebuilt when the "
+                         "eliminated DoFs change or the CG slows down); 0: rebuild it with every assembly")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--ensemble", type=int, default=0,
@@ -246,7 +249,7 @@ def main():
         ensemble_bench(args)
         return
     ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond, args.spmv_sym, args.cg_forcing,
-                                      args.coarse_cells, args.predictor)
+                                      args.coarse_cells, args.predictor, args.coarse_reuse)
     if args.verbose:
         print(f"setup {t_setup:.2f} s, n={ctx.n} bonds={ctx.size('nbonds')} blocks={ctx.size('nconn')}", flush=True)
     # warmup steps, then back to the initial state so that the timed steps are steps 1..K
@@ -279,6 +282,7 @@ def main():
     achieved = B / (spmv_ms * 1e-3) / 1e9
     counts = (ctx.stat("newton_iters") / K, ctx.stat("cg_iters") / K, ctx.stat("assemblies") / K)
     sym_solves, cg_solves = ctx.stat("sym_solves"), ctx.stat("cg_solves")
+    coarse_builds = ctx.stat("coarse_builds") / K
     spmv_sym_active = ctx.stat("spmv_sym_active")
     ncoarse = ctx.get_mode("ncoarse")
     # equilibrium check of the last step (size-independent): residual and reaction balance
@@ -339,15 +343,19 @@ def main():
     # The same K steps again: without the predictor (the reference's Newton start), and with the
     # reference's algorithm (Jacobi, no predictor), whose iteration counts drive the CPU
     # projection; both rates are reported beside the headline.
-    jac = nopred = graph = None
+    jac = nopred = graph = noreuse = None
     jac_counts = counts
     if not args.no_cpu_baseline:
         # the same steps with each CG batch replayed from a captured HIP graph (per-SpMV timing
         # is off in that mode, so the headline and its roofline come from direct launches)
         graph, _ = rerun(cg_graph=1)
         ctx.mode(cg_graph=0)
+        if args.coarse_reuse and args.precond in (2, 3):
+            # the coarse space rebuilt with every assembly
+            noreuse, _ = rerun(coarse_reuse=0)
         if args.predictor:
-            nopred, _ = rerun(predictor=0)
+            # neither the predictor nor coarse reuse: the two-level solver from the reference's start
+            nopred, _ = rerun(predictor=0, coarse_reuse=0)
         if args.precond != 0 or args.predictor:
             jac, jac_counts = rerun(precond=0, predictor=0)
     out = {
@@ -370,7 +378,7 @@ def main():
             "predictor": "previous step's converged increment" if args.predictor else "none (the reference's start)",
             "precond": {0: "jacobi", 1: "block-jacobi", 2: "two-level (jacobi + rigid-body aggregate coarse space)",
                         3: "two-level (3x3 block jacobi + rigid-body aggregate coarse space)"}[args.precond],
-            "coarse_dofs": int(ncoarse),
+            "coarse_dofs": int(ncoarse), "coarse_reuse": args.coarse_reuse, "coarse_builds_per_step": coarse_builds,
             "spmv_store": "half (symmetric)" if spmv_sym_active else "full",
             "spmv_launch": {"slots_unrolled": int(ctx.get_mode("spmv_unr")),
                             "wavefronts_per_workgroup": int(ctx.get_mode("spmv_wpb")),
@@ -383,7 +391,8 @@ def main():
             "parallelism": "single GPU",
             "setup_seconds": t_setup,
             "async_dump": dump,
-            "same_steps_with_cg_graph": graph, "same_steps_without_predictor": nopred,
+            "same_steps_with_cg_graph": graph, "same_steps_without_coarse_reuse": noreuse,
+            "same_steps_without_predictor_or_coarse_reuse": nopred,
             "same_steps_reference_algorithm_jacobi_no_predictor": jac,
         },
         "roofline": {
@@ -702,7 +711,7 @@ def config3_bench(args):
     g.set_j2(0.0, False, 200e3, 0.3, 300.0, 0.0, 2e3, 0.0, 1.0, damage_threshold=1.0, elmap_literal=0)
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
            cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=50, cg_maxit=20000, predictor=args.predictor,
-           cg_graph=args.cg_graph)
+           cg_graph=args.cg_graph, coarse_reuse=args.coarse_reuse)
     steps = 50
     ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", 0.0), (lo, "z", 0.0), (hi, "z", 0.02 * L / steps)])
     g.update_geometry()
@@ -721,7 +730,9 @@ def config3_bench(args):
                              f"uniaxial tension 2% in {steps} steps",
                  "bonds": g.size("nbonds"), "newton_iters_per_step": g.stat("newton_iters") / steps,
                  "cg_iters_per_step": g.stat("cg_iters") / steps, "newton_tol": args.tol_nr,
-                 "predictor": args.predictor, "precond": args.precond, "cg_graph": args.cg_graph, "setup_seconds": t_setup,
+                 "predictor": args.predictor, "precond": args.precond, "cg_graph": args.cg_graph,
+                 "coarse_reuse": args.coarse_reuse, "coarse_builds_per_step": g.stat("coarse_builds") / steps,
+                 "setup_seconds": t_setup,
                  "yielded_fraction": float(np.mean(alpha > 0))})
     print(json.dumps(out), flush=True)
 
@@ -755,7 +766,7 @@ def config4_bench(args):
     g.set_elastic(0.0, True, 69e3, 0.3, 1e-3, elmap_literal=0)
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
            cg_forcing=args.cg_forcing, tol_NR=1e-6, max_NR_iter=50, cg_maxit=50000, predictor=args.predictor,
-           cg_graph=args.cg_graph, coarse_cells=args.coarse_cells)
+           cg_graph=args.cg_graph, coarse_cells=args.coarse_cells, coarse_reuse=args.coarse_reuse)
     y = xyz[:, 1]
     lo, hi = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
     inc, steps = 0.0006, 40
@@ -786,7 +797,7 @@ def config4_bench(args):
                              f"opening {inc} per edge per step, {steps} steps",
                  "bonds": g.size("nbonds"), "broken_bonds": int((bd > 0.5).sum()), "newton_iters_per_step": newton,
                  "cg_iters_per_step": cg, "newton_tol": 1e-6, "predictor": args.predictor, "precond": args.precond,
-                 "cg_graph": args.cg_graph, "coarse_dofs": int(g.get_mode("ncoarse")),
+                 "cg_graph": args.cg_graph, "coarse_dofs": int(g.get_mode("ncoarse")), "coarse_reuse": args.coarse_reuse,
                  "adjoint": {"seconds_polish_and_adjoint": t_adj, "refinements": nref, "relative_residual": rres,
                              "top_reaction": F}})
     print(json.dumps(out), flush=True)

@@ -162,6 +162,11 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        and by uploading positions.
  *  "coarse_graph"     1: replay the coarse inversion's launches from a HIP graph captured once |
  *                        0 (default): launch them directly
+ *  "coarse_reuse"     1: keep the two-level coarse space (Ac and its inverse) across assemblies
+ *                        while the eliminated DoFs are unchanged and the CG needs at most 1.25x
+ *                        (+4) the iterations of the first solve after the last build; rebuilt
+ *                        otherwise (stat "coarse_builds" counts builds) | 0 (default): rebuilt with
+ *                        every assembly. K itself is reassembled either way.
  *  "cg_graph"         1: capture one batch of CG iterations (cg_batch iterations: vector updates,
  *                        coarse correction, SpMV) into a HIP graph and replay it for every batch
  *                        of every solve; re-captured when sizes, launch configuration or buffers

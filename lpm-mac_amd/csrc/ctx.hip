@@ -839,6 +839,7 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "cg_forcing") c->cg_forcing = (int)v ? 1 : 0;
     else if (w == "coarse_graph") c->coarse_graph = (int)v ? 1 : 0;
     else if (w == "cg_graph") c->cg_graph = (int)v ? 1 : 0;
+    else if (w == "coarse_reuse") c->coarse_reuse = (int)v ? 1 : 0;
     else if (w == "predictor") {
         c->predictor = (int)v ? 1 : 0;
         c->pred_valid = false;
@@ -888,6 +889,7 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "cg_forcing") return c->cg_forcing;
     if (w == "coarse_graph") return c->coarse_graph;
     if (w == "cg_graph") return c->cg_graph;
+    if (w == "coarse_reuse") return c->coarse_reuse;
     if (w == "predictor") return c->predictor;
     if (w == "coarse_cells") return c->coarse_cells;
     if (w == "ncoarse") return c->ncoarse;
@@ -1189,6 +1191,7 @@ double lpm_stat(const lpm_ctx *c, const char *name) {
     const std::string w(name);
     if (w == "newton_iters") return (double)c->st_newton;
     if (w == "cg_iters") return (double)c->st_cg_iters;
+    if (w == "coarse_builds") return (double)c->st_coarse_builds;
     if (w == "cg_solves") return (double)c->st_cg_solves;
     if (w == "assemblies") return (double)c->st_asm;
     if (w == "spmv_launches") return (double)c->st_spmv;
@@ -1207,6 +1210,7 @@ int lpm_reset_stats(lpm_ctx *c) {
     if (!c) return LPM_E_ARG;
     c->st_newton = c->st_cg_iters = c->st_cg_solves = c->st_asm = c->st_spmv = c->st_spmv_timed = 0;
     c->st_sym_solves = c->st_sym_rejects = 0;
+    c->st_coarse_builds = 0;
     c->st_spmv_ms = 0;
     return LPM_OK;
 }

@@ -1187,8 +1187,16 @@ int cg_solve(lpm_ctx *c, int *iters, double *error, int *success) {
     if ((rc = spmv_tune(c))) return rc;
     if (c->sym_ok) c->st_sym_solves++;
     if (c->precision == 0)
-        return c->dim == 3 ? cg_t<float, 3, 4>(c, iters, error, success) : cg_t<float, 2, 2>(c, iters, error, success);
-    return c->dim == 3 ? cg_t<double, 3, 4>(c, iters, error, success) : cg_t<double, 2, 2>(c, iters, error, success);
+        rc = c->dim == 3 ? cg_t<float, 3, 4>(c, iters, error, success) : cg_t<float, 2, 2>(c, iters, error, success);
+    else
+        rc = c->dim == 3 ? cg_t<double, 3, 4>(c, iters, error, success) : cg_t<double, 2, 2>(c, iters, error, success);
+    if (!rc && two_level(c) && c->coarse_reuse) {
+        // a kept coarse space is rebuilt before the next solve once the CG needs clearly more
+        // iterations than the first solve after the last build
+        if (c->coarse_ref_iters < 0) c->coarse_ref_iters = *iters;
+        else if (*iters > 1.25 * c->coarse_ref_iters + 4) c->coarse_stale = true;
+    }
+    return rc;
 }
 
 // x_i *= w[specimen of i] (undoes the per-specimen right-hand-side scaling of an ensemble solve)

@@ -368,6 +368,13 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(int M, int S, const int32_t
     if (lane == 0) cy[base + r] = s;
 }
 
+// flag = 1 where the DoF masks differ
+__global__ __launch_bounds__(256) void k_mask_diff(int64_t n, const uint8_t *__restrict__ a, const uint8_t *__restrict__ b,
+                                                    int32_t *flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && a[i] != b[i]) *flag = 1;
+}
+
 }  // namespace
 
 static int ck(lpm_ctx *c, const char *what) {
@@ -380,7 +387,7 @@ void coarse_free(lpm_ctx *c) {
     if (c->gj_graph) hipGraphExecDestroy(c->gj_graph);
     void *ptrs[] = {c->agg_of, c->agg_ptr, c->agg_idx, c->agg_d, c->ra_ptr, c->ra_id,  c->cq_ptr,
                     c->cq_f,   c->cq_gh,   c->cvbuf,   c->Ac,    c->cz,     c->cy,     c->dofmask,
-                    c->cgrp_cb, c->cgrp_off, c->agg_grp};
+                    c->cgrp_cb, c->cgrp_off, c->agg_grp, c->dofmask_prev};
     for (void *p : ptrs)
         if (p) hipFree(p);
 }
@@ -623,6 +630,20 @@ int coarse_prepare(lpm_ctx *c) {
     hipLaunchKernelGGL(k_dofmask, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->dim, c->cons, c->damage_visual,
                        c->dofmask);
     if ((rc = ck(c, "k_dofmask"))) return rc;
+    if (c->coarse_reuse && c->coarse_built && !c->coarse_stale) {
+        // keep the coarse space of the last build if the eliminated DoFs are the same
+        LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+        hipLaunchKernelGGL(k_mask_diff, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const uint8_t *)c->dofmask,
+                           (const uint8_t *)c->dofmask_prev, c->iflag);
+        if ((rc = ck(c, "k_mask_diff"))) return rc;
+        int32_t changed = 0;
+        LPM_HIP(c, hipMemcpyAsync(&changed, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+        if (!changed) {
+            c->coarse_version = c->k_version;
+            return LPM_OK;
+        }
+    }
     if (c->precision == 0) rc = c->dim == 3 ? coarse_rows_t<float, 3>(c) : coarse_rows_t<float, 2>(c);
     else rc = c->dim == 3 ? coarse_rows_t<double, 3>(c) : coarse_rows_t<double, 2>(c);
     if (rc) return rc;
@@ -695,6 +716,14 @@ int coarse_prepare(lpm_ctx *c) {
     }
     c->Ainv = (nsteps & 1) ? A1 : A0;
     c->coarse_version = c->k_version;
+    c->st_coarse_builds++;
+    c->coarse_built = true;
+    c->coarse_stale = false;
+    c->coarse_ref_iters = -1;
+    if (c->coarse_reuse) {
+        if (!c->dofmask_prev) LPM_HIP(c, hipMalloc(&c->dofmask_prev, std::max<int64_t>(n, 1)));
+        LPM_HIP(c, hipMemcpyAsync(c->dofmask_prev, c->dofmask, n, hipMemcpyDeviceToDevice, c->stream));
+    }
     return LPM_OK;
 }
 

@@ -126,6 +126,14 @@ struct lpm_ctx {
     std::vector<double> tolm_ref;
     uint8_t *dofmask = nullptr;       // per particle: bit k set when DoF k is eliminated
     int64_t coarse_version = -1;
+    // mode "coarse_reuse": keep the coarse space and Ac^-1 across assemblies while the eliminated
+    // DoFs stay the same and the CG iteration count stays within 1.25x (+4) of the first solve
+    // after the last build
+    int coarse_reuse = 0;
+    bool coarse_built = false, coarse_stale = false;
+    int coarse_ref_iters = -1;
+    int64_t st_coarse_builds = 0;
+    uint8_t *dofmask_prev = nullptr;
     // Asynchronous dump writer (io.hip): device snapshot -> pinned host copy on a side stream ->
     // formatted and appended by a writer thread; one write in flight per context
     struct DumpState *dump = nullptr;
