@@ -115,6 +115,7 @@ struct Options {
     double cg_maxit = -1;      // reference: 2*ndof
     int cg_forcing = 0;        // 1: inexact-Newton CG tolerance per Newton iteration (reference: 0)
     int predictor = 0;         // 1: static steps start from the previous converged increment (reference: 0)
+    int coarse_reuse = 0;      // 1: keep the two-level coarse space across assemblies while it still works
     int async_dump = 1;        // 1: writeDump from a device snapshot on a writer thread (same text; reference: inline)
     int verbose = 1;           // print like the reference
     int device = 0;
@@ -749,6 +750,7 @@ class Assembly {
         lpm_set_mode(c, "cg_maxit", o.cg_maxit);
         lpm_set_mode(c, "cg_forcing", o.cg_forcing);
         lpm_set_mode(c, "predictor", o.predictor);
+        lpm_set_mode(c, "coarse_reuse", o.coarse_reuse);
         lpm_set_mode(c, "verbose", o.verbose);
     }
 
