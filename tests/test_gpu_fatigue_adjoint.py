@@ -107,3 +107,83 @@ def test_fatigue_parameter_sensitivity_matches_central_differences():
         err = abs(adj - fd) / abs(fd)
         say(f"   {name}: adjoint {adj:.10e} fd {fd:.10e} rel {err:.2e}")
         assert err < 1e-5
+
+
+def test_batched_fatigue_parameter_sensitivity_matches_single_specimens():
+    """Three HCF specimens of different sizes, fatigue parameters and force amplitudes in one
+    context, through the same loading, unloading, fatigue update and reloading sequence. One
+    sensitivity call with the objective summed over every specimen's top edge gives each
+    specimen's dG_s/d(A, B, d): no bond or nonlocal neighbour joins two specimens. Each slice must
+    match the sensitivity of the specimen's own context (checked against oracle finite
+    differences above)."""
+    from test_gpu_ensemble import _union
+
+    say("[gpu] batched HCF parameter sensitivity, 3 specimens")
+    cases = [((10, 16), (A0, B0, D0), FMAX), ((14, 12), (8e-5, 2.019, 0.5), 500.0), ((8, 20), (3e-5, 2.1, 0.45), 300.0)]
+    per, parts = [], []
+    for (nx, ny), abd, fmax in cases:
+        x, box = hex_plate(nx, ny)
+        parts.append((x, G.build_bonds(G.HEXAGON2D, 0.5, x)))
+        left = np.nonzero(x[:, 0] < x[:, 0].min() + 0.5)[0]
+        bottom, top = faces(x, 1)
+        t = np.zeros(len(x), dtype=np.int32)
+        t[top] = 4
+        t[bottom] = 4
+        per.append((x, abd, fmax, left, bottom, top, t))
+
+    def run(ctx, specs, abds):
+        ctx.set("type", np.concatenate([p[6] for p in specs]))
+        ctx.set_hcf(4.0, True, 110e3, 0.34, A0, B0, D0, elmap_literal=0)
+        mat = ctx.get("mat").reshape(-1, 10)
+        off = 0
+        for p, abd in zip(specs, abds):
+            mat[off:off + len(p[0]), 0:3] = abd
+            off += len(p[0])
+        ctx.set("mat", mat)
+        ctx.search_nonlocal(1.5)
+        ctx.mode(cg_tol=1e-13, tol_NR=1e-10, max_NR_iter=60, cg_maxit=20000, precision=1, cg_transpose=0,
+                 diag10_literal=0)
+        ctx.update_geometry()
+        ctx.update_force_state()
+        w = np.zeros(2 * off)
+
+        def step(scale):
+            disp, force, o = [], [], 0
+            for x, abd, fmax, left, bottom, top, t in specs:
+                disp += [(left + o, "x", 0.0), (bottom + o, "y", 0.0)]
+                force += [(top + o, 0.0, scale * fmax, 0.0)]
+                o += len(x)
+            return ctx.load_step(disp=disp, force=force)
+
+        o = 0
+        for p in specs:
+            w[2 * (p[5] + o) + 1] = 1.0
+            o += len(p[0])
+        ls = [step(1.0), step(-0.9), step(0.9)]
+        assert ctx.solve_step_fatigue(ls[0])
+        ctx.update_state_var()
+        assert ctx.solve_step_fatigue(ls[1])
+        ctx.update_state_var()
+        ctx.update_fatigue_damage(DT, 4)
+        assert ctx.solve_step_fatigue(ls[2])
+        return ctx.adjoint_fatigue_params(w_x=w)
+
+    xyz, bonds, first = _union(parts)
+    g = G.Context(G.HEXAGON2D, 0.5, G.HCF, xyz, bonds=bonds)
+    g.set_specimens(first)
+    Gv, gA, gB, gd, nref, rres = run(g, per, [p[1] for p in per])
+    say(f"   ensemble G {Gv:.12e} refinements {nref} rel residual {rres:.1e}")
+    assert rres < 1e-9
+    Gs = 0.0
+    for s, p in enumerate(per):
+        c1 = G.Context(G.HEXAGON2D, 0.5, G.HCF, p[0], bonds="device")
+        G1, a1, b1, d1, _, r1 = run(c1, [p], [p[1]])
+        assert r1 < 1e-9
+        Gs += G1
+        sl = slice(first[s], first[s + 1])
+        errs = [np.linalg.norm(u[sl] - v) / np.linalg.norm(v) for u, v in ((gA, a1), (gB, b1), (gd, d1))]
+        say(f"   specimen {s} ({len(p[0])} particles): G {G1:.10e} rel err A {errs[0]:.1e} B {errs[1]:.1e} "
+            f"d {errs[2]:.1e}")
+        assert np.linalg.norm(a1) > 0
+        assert max(errs) < 1e-7
+    assert Gv == pytest.approx(Gs, rel=1e-10)
