@@ -242,8 +242,12 @@ def main():
         return
     if args.ensemble > 0:
         if args.fatigue:
-            fatigue_ensemble_bench(args)
-            return
+            # switched off: a run of this mode under rocprofv3 --kernel-trace ended in a host
+            # segmentation fault inside a HIP runtime call made by the CG (cg_t<float, 2, 2>, from
+            # lpm_solve_problem_cyclic_specimens), and its cause has not been found
+            print("bench.py --ensemble N --fatigue is switched off: a profiled run of it crashed with a host "
+                  "segmentation fault inside a HIP runtime call from the CG, cause not yet found", file=sys.stderr)
+            sys.exit(3)
         if args.steps == 20:
             args.steps = 5
         ensemble_bench(args)
