@@ -210,6 +210,9 @@ def main():
     ap.add_argument("--coarse-reuse", type=int, default=1,
                     help="1: keep the two-level coarse space across assemblies while it still works (rebuilt when the "
                          "eliminated DoFs change or the CG slows down); 0: rebuild it with every assembly")
+    ap.add_argument("--cg-batch-predict", type=int, default=0,
+                    help="1: a CG solve's first batch covers the previous solve's iteration count (one host round "
+                         "trip for solves of similar length); 0: fixed batches of 16 iterations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--ensemble", type=int, default=0,
@@ -254,6 +257,7 @@ def main():
         return
     ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond, args.spmv_sym, args.cg_forcing,
                                       args.coarse_cells, args.predictor, args.coarse_reuse)
+    ctx.mode(cg_batch_predict=args.cg_batch_predict)
     if args.verbose:
         print(f"setup {t_setup:.2f} s, n={ctx.n} bonds={ctx.size('nbonds')} blocks={ctx.size('nconn')}", flush=True)
     # warmup steps, then back to the initial state so that the timed steps are steps 1..K
@@ -347,13 +351,17 @@ def main():
     # The same K steps again: without the predictor (the reference's Newton start), and with the
     # reference's algorithm (Jacobi, no predictor), whose iteration counts drive the CPU
     # projection; both rates are reported beside the headline.
-    jac = nopred = graph = noreuse = None
+    jac = nopred = graph = noreuse = nobp = None
     jac_counts = counts
     if not args.no_cpu_baseline:
         # the same steps with each CG batch replayed from a captured HIP graph (per-SpMV timing
         # is off in that mode, so the headline and its roofline come from direct launches)
         graph, _ = rerun(cg_graph=1)
         ctx.mode(cg_graph=0)
+        if args.cg_batch_predict:
+            # fixed batches of 16 CG iterations, the default
+            nobp, _ = rerun(cg_batch_predict=0)
+            ctx.mode(cg_batch_predict=1)
         if args.coarse_reuse and args.precond in (2, 3):
             # the coarse space rebuilt with every assembly
             noreuse, _ = rerun(coarse_reuse=0)
@@ -382,7 +390,7 @@ def main():
             "predictor": "previous step's converged increment" if args.predictor else "none (the reference's start)",
             "precond": {0: "jacobi", 1: "block-jacobi", 2: "two-level (jacobi + rigid-body aggregate coarse space)",
                         3: "two-level (3x3 block jacobi + rigid-body aggregate coarse space)"}[args.precond],
-            "coarse_dofs": int(ncoarse), "coarse_reuse": args.coarse_reuse, "coarse_builds_per_step": coarse_builds,
+            "coarse_dofs": int(ncoarse), "coarse_reuse": args.coarse_reuse, "cg_batch_predict": args.cg_batch_predict, "coarse_builds_per_step": coarse_builds,
             "spmv_store": "half (symmetric)" if spmv_sym_active else "full",
             "spmv_launch": {"slots_unrolled": int(ctx.get_mode("spmv_unr")),
                             "wavefronts_per_workgroup": int(ctx.get_mode("spmv_wpb")),
@@ -395,7 +403,8 @@ def main():
             "parallelism": "single GPU",
             "setup_seconds": t_setup,
             "async_dump": dump,
-            "same_steps_with_cg_graph": graph, "same_steps_without_coarse_reuse": noreuse,
+            "same_steps_with_cg_graph": graph, "same_steps_without_cg_batch_prediction": nobp,
+            "same_steps_without_coarse_reuse": noreuse,
             "same_steps_without_predictor_or_coarse_reuse": nopred,
             "same_steps_reference_algorithm_jacobi_no_predictor": jac,
         },

@@ -171,6 +171,10 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        coarse correction, SpMV) into a HIP graph and replay it for every batch
  *                        of every solve; re-captured when sizes, launch configuration or buffers
  *                        change. Per-SpMV timing is off in this mode | 0 (default): launch directly
+ *  "cg_batch_predict" 1: without graph replay, a solve's first batch covers the previous solve's
+ *                        updates plus two (at most 512), so solves of similar length stop after
+ *                        one host round trip; the iterate is the same | 0 (default): batches of
+ *                        cg_batch iterations
  *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 and 3 (0: smallest of 8, 9,
  *                        10, ... giving at most 768 coarse DoFs); get_mode("ncoarse") reads the
  *                        coarse dimension once built

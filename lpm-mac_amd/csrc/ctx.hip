@@ -835,6 +835,7 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "tol_NR") c->tol_NR = v;
     else if (w == "verbose") c->verbose = (int)v;
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
+    else if (w == "cg_batch_predict") c->cg_batch_predict = (int)v ? 1 : 0;
     else if (w == "cg_vec_wg") c->cg_vec_wg = std::max(1, (int)v);
     else if (w == "cg_forcing") c->cg_forcing = (int)v ? 1 : 0;
     else if (w == "coarse_graph") c->coarse_graph = (int)v ? 1 : 0;
@@ -883,6 +884,7 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "tol_NR") return c->tol_NR;
     if (w == "verbose") return c->verbose;
     if (w == "cg_batch") return c->cg_batch;
+    if (w == "cg_batch_predict") return c->cg_batch_predict;
     if (w == "cg_vec_wg") return c->cg_vec_wg;
     if (w == "spmv_sym") return c->spmv_sym;
     if (w == "spmv_tune") return c->spmv_tune;

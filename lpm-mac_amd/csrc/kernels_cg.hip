@@ -1107,21 +1107,34 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     // batches of `batch` iterations (an even number, so that a batch starts on parity 0);
     // the device flag turns the rest of a batch into no-ops once the solve has stopped
     const int batch = ((c->cg_batch > 0 ? c->cg_batch : 16) + 1) & ~1;
+    const bool graph = c->cg_graph != 0;
+    // without graph replay, the first batch can cover the iterations the previous solve needed
+    // (its updates, the iteration that detects convergence and one more, rounded up to even):
+    // solves of similar length then stop in one batch, with one host round trip and few no-op
+    // iterations behind the converging one
+    int first = batch;
+    if (!graph && c->cg_batch_predict && c->cg_last_updates > 0)
+        first = (int)((std::min<int64_t>(c->cg_last_updates + 2, 512) + 1) & ~(int64_t)1);
     // timing: events around the SpMV of every `timing`-th iteration only (an event record costs
     // a few microseconds of idle GPU time, so timing every launch would slow the solve); not in
-    // graph mode, whose batches are replayed whole
-    const bool graph = c->cg_graph != 0;
-    std::vector<hipEvent_t> evs;
+    // graph mode, whose batches are replayed whole. One pair per timed iteration of a batch.
+    struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events() {
+            for (auto &x : e) hipEventDestroy(x);
+        }
+    } evs;
     if (c->timing && !graph) {
-        evs.resize(2 * batch);
-        for (auto &e : evs) LPM_HIP(c, timing_event(&e));
+        evs.e.resize(2 * (std::max(first, batch) / c->timing + 2));
+        for (auto &e : evs.e) LPM_HIP(c, timing_event(&e));
     }
     auto timed = [&](int64_t it) { return !graph && c->timing > 0 && it % c->timing == 0; };
     int64_t it0 = 0;
-    // one batch: iteration k of the batch starting at scal[SC_IT0], then the batch base moves on
-    auto issue = [&](int64_t base) -> int {
-        int r = LPM_OK;
-        for (int k = 0; k < batch && !r; ++k) {
+    // one batch of nb iterations: iteration k of the batch starting at scal[SC_IT0], then the
+    // batch base moves on
+    auto issue = [&](int64_t base, int nb) -> int {
+        int r = LPM_OK, te = 0;
+        for (int k = 0; k < nb && !r; ++k) {
             if (coarse) {
                 hipLaunchKernelGGL((k_cg_vec_a<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, (V *)c->v_x, (V *)c->v_r,
                                    (const V *)c->v_u, (V *)c->v_p, (V *)c->v_s, (const V *)c->v_w, c->partial, c->partial_cap,
@@ -1135,43 +1148,47 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
                 r = check_launch(c, "k_cg_vec");
             }
             if (r) break;
-            if (timed(base + k) && hipEventRecord(evs[2 * k], c->stream) != hipSuccess) r = lpm_fail(c, LPM_E_HIP, "event");
+            const bool tk = timed(base + k);
+            if (tk && hipEventRecord(evs.e[2 * te], c->stream) != hipSuccess) r = lpm_fail(c, LPM_E_HIP, "event");
             if (!r) r = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
-            if (!r && timed(base + k) && hipEventRecord(evs[2 * k + 1], c->stream) != hipSuccess)
-                r = lpm_fail(c, LPM_E_HIP, "event");
+            if (!r && tk && hipEventRecord(evs.e[2 * te + 1], c->stream) != hipSuccess) r = lpm_fail(c, LPM_E_HIP, "event");
+            if (tk) ++te;
         }
         if (r) return r;
-        hipLaunchKernelGGL(k_cg_advance, dim3(1), dim3(64), 0, c->stream, c->scal, batch);
+        hipLaunchKernelGGL(k_cg_advance, dim3(1), dim3(64), 0, c->stream, c->scal, nb);
         return check_launch(c, "k_cg_advance");
     };
-    if (graph && (rc = cg_graph_prepare(c, gv, gs, batch, [&]() { return issue(0); }))) return rc;
-    for (;;) {
+    if (graph && (rc = cg_graph_prepare(c, gv, gs, batch, [&]() { return issue(0, batch); }))) return rc;
+    for (int nb = first;; nb = batch) {
         if (graph) LPM_HIP(c, hipGraphLaunch(c->cg_exec, c->stream));
-        else if ((rc = issue(it0))) return rc;
+        else if ((rc = issue(it0, nb))) return rc;
         LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
         LPM_HIP(c, hipStreamSynchronize(c->stream));
         // SpMVs of the batch that did work: one per update (iterations it0 .. ITER-1 updated)
         const bool done = c->h_scal[SC_DONE] != 0.0;
-        int real = done ? (int)(c->h_scal[SC_ITER] - (double)it0) : batch;
-        real = real < 0 ? 0 : (real > batch ? batch : real);
+        int real = done ? (int)(c->h_scal[SC_ITER] - (double)it0) : nb;
+        real = real < 0 ? 0 : (real > nb ? nb : real);
         c->st_spmv += real;
-        for (int k = 0; k < real; ++k) {
+        for (int k = 0, te = 0; k < nb; ++k) {
             if (!timed(it0 + k)) continue;
-            float ms = 0;
-            LPM_HIP(c, hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
-            c->st_spmv_ms += ms;
-            c->st_spmv_timed++;
+            if (k < real) {
+                float ms = 0;
+                LPM_HIP(c, hipEventElapsedTime(&ms, evs.e[2 * te], evs.e[2 * te + 1]));
+                c->st_spmv_ms += ms;
+                c->st_spmv_timed++;
+            }
+            ++te;
         }
         if (done) break;
-        it0 += batch;
+        it0 += nb;
         if (it0 > maxit + 2 * batch) break;  // the device cap stops earlier; this only guards the loop
     }
-    for (auto &e : evs) hipEventDestroy(e);
     c->st_spmv += 1;  // the prologue SpMV
     const double rhs2 = c->h_scal[SC_RHS2];
     const double done = c->h_scal[SC_DONE];
     // Eigen reports the updates done before the converging one (its loop breaks before i++)
     const int64_t upd = (int64_t)c->h_scal[SC_ITER];
+    c->cg_last_updates = upd;
     *iters = (int)((done == 1.0 && upd > 0) ? upd - 1 : upd);
     *error = (done == 3.0) ? 0.0 : sqrt(c->h_scal[SC_RR] / rhs2);
     *success = (done == 3.0) || (*error <= c->cg_tol);

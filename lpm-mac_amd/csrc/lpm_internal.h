@@ -55,6 +55,10 @@ struct lpm_ctx {
     // modes
     int diag10_literal = 1, cg_transpose = 1, dirichlet_static = 0, rollback_pex = 0;
     int precision = 0, precond = 0, verbose = 0, max_NR_iter = 30, cg_batch = 16;
+    // mode "cg_batch_predict": without graph replay, a solve's first batch is as long as the
+    // previous solve's update count plus two (cg_last_updates, -1 before the first solve)
+    int cg_batch_predict = 0;
+    int64_t cg_last_updates = -1;
     int cg_vec_wg = 1024;  // most workgroups of the CG vector kernels (grid-stride beyond)
     double cg_tol = 2.220446049250313e-16, tol_NR = 1e-6;
     int cg_forcing = 0;  // 1: per-Newton-iteration CG tolerance from the remaining reduction
