@@ -1239,7 +1239,7 @@ int lpm_force_jvp(lpm_ctx *c, const double *v, double *jv) {
     if (!rc) rc = force_jvp(c, d, d + nd);
     if (!rc && hipMemcpyAsync(jv, d + nd, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
         rc = lpm_fail(c, LPM_E_HIP, "jvp download");
-    hipStreamSynchronize(c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
     hipFree(d);
     return rc;
 }
@@ -1255,7 +1255,7 @@ int lpm_force_vjp(lpm_ctx *c, const double *w, double *y) {
     if (!rc) rc = force_vjp(c, d, d + nd);
     if (!rc && hipMemcpyAsync(y, d + nd, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
         rc = lpm_fail(c, LPM_E_HIP, "vjp download");
-    hipStreamSynchronize(c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
     hipFree(d);
     return rc;
 }
@@ -1288,7 +1288,7 @@ int lpm_adjoint_linear(lpm_ctx *c, const double *w_pin, const double *w_x, doubl
     int it = 0;
     double rr = 0;
     if (!rc) rc = adjoint_linear(c, w_pin ? d : nullptr, w_x ? d + nd : nullptr, dG_dKn, dG_dTv, dG_dbdamage, G, &it, &rr);
-    hipStreamSynchronize(c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
     hipFree(d);
     if (rc) return rc;
     if (refinements) *refinements = it;
@@ -1320,7 +1320,7 @@ int lpm_adjoint_fatigue_params(lpm_ctx *c, const double *w_pin, const double *w_
     double rr = 0;
     if (!rc)
         rc = adjoint_fatigue_params(c, w_pin ? d : nullptr, w_x ? d + nd : nullptr, dG_dA, dG_dB, dG_dd, G, &it, &rr);
-    hipStreamSynchronize(c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
     hipFree(d);
     if (rc) return rc;
     if (refinements) *refinements = it;

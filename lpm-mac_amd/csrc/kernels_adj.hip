@@ -386,7 +386,7 @@ int adjoint_linear(lpm_ctx *c, const double *a_dev, const double *b_dev, double 
     auto done = [&](int code) {
         c->residual = saved_res;
         c->cg_tol = saved_tol;
-        hipStreamSynchronize(c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess && !code) code = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
         hipFree(buf);
         return code;
     };
@@ -448,7 +448,7 @@ int adjoint_linear(lpm_ctx *c, const double *a_dev, const double *b_dev, double 
         if (outs[k] && hipMemcpyAsync(outs[k], gb + k * nb, sizeof(double) * c->nbond, hipMemcpyDeviceToHost, c->stream) !=
                            hipSuccess)
             rc = lpm_fail(c, LPM_E_HIP, "adjoint copy");
-    hipStreamSynchronize(c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
     hipFree(gb);
     if (rc) return done(rc);
     if (G) {
@@ -486,7 +486,7 @@ int adjoint_reaction(lpm_ctx *c, int gid, int axis, double *gKn, double *gTv, do
         rc = check(c, "k_indicator");
     }
     if (!rc) rc = adjoint_linear(c, a, nullptr, gKn, gTv, nullptr, F, refinements, rel_residual);
-    hipStreamSynchronize(c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
     hipFree(a);
     return rc;
 }
@@ -524,7 +524,7 @@ int adjoint_fatigue_params(lpm_ctx *c, const double *a_dev, const double *b_dev,
     for (int k = 0; k < 3 && !rc; ++k)
         if (outs[k] && hipMemcpyAsync(outs[k], out + k * n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
             rc = lpm_fail(c, LPM_E_HIP, "fatigue sensitivity copy");
-    hipStreamSynchronize(c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
     hipFree(buf);
     return rc;
 }

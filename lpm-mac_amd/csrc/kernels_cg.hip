@@ -1313,7 +1313,7 @@ int copy_bench(lpm_ctx *c, int64_t bytes, int reps, double *ms) {
     }
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
-    hipStreamSynchronize(c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
     hipFree(a);
     hipFree(b);
     return rc;
