@@ -1,6 +1,7 @@
 // A driver written against the reference API (cf. reference examples/ex1_isoelasticity_2d_hex.cpp):
 // 2D hexagonal plate, bottom edge clamped, top edge pulled in y over several load steps.
-// Build: see examples/Makefile. Run: ./ex1_hex_plate [nx ny steps]
+// Build: see examples/Makefile. Run: ./ex1_hex_plate [nx ny steps [tol [precision]]]
+//   (precision 1: FP64 matrix and CG, relative CG tolerance 1e-13)
 #include "lpm.h"
 #include "particle.h"
 #include "assembly.h"
@@ -20,6 +21,10 @@ int main(int argc, char **argv) {
     // Read el_mapping row-major (its documented meaning). The reference's column-major read
     // makes this lattice's tangent stiffness indefinite after the first step and CG diverges.
     lpm::options().elmap_literal = 0;
+    if (argc > 5 && atoi(argv[5]) == 1) {
+        lpm::options().precision = 1;
+        lpm::options().cg_tol = 1e-13;
+    }
 
     int eulerflag = 0;
     double angles[] = {PI / 180.0 * 0.0, PI / 180.0 * 0.0, PI / 180.0 * 0.0};
@@ -33,7 +38,7 @@ int main(int argc, char **argv) {
     bool is_plane_stress = false;
     double E0 = 69e3, mu0 = 0.3, critical_bstrain = 1;
     int max_iter = 30, start_index = 0;
-    double tol_iter = 1e-10;
+    double tol_iter = argc > 4 ? atof(argv[4]) : 1e-10;
     int undamaged_pt_type = -1;
     std::string dumpFile{"ex1_hex_plate.dump"};
 

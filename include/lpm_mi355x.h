@@ -70,11 +70,11 @@ int lpm_rotation_matrix(int eulerflag, const double *angles, double *R);
 int lpm_build_bonds(int lattice, double radius, int64_t n, const double *xyz, int32_t *bond_ptr,
                     int32_t *bond_split, int32_t *bond_p2, int64_t cap, int64_t *nbonds);
 
-/* Unit-cell constants; replaces UnitCell (include/unit_cell.h:125-225).
+/* Unit-cell constants; replaces UnitCell (include/unit_cell.h:7-127).
  * out[0]=dim out[1]=nneighbors out[2]=nneighbors1 out[3]=nneighbors2 out[4]=nneighbors_AFEM
  * out[5]=cutoff0 out[6]=cutoff1 out[7]=particle_volume; map[9] = el_mapping (row r, col c at
  * map[3r+c]) as the reference reads it when literal != 0 (column-major Eigen::Map of the
- * row-major source array, unit_cell.h:147) or as written when literal == 0. */
+ * row-major source array, unit_cell.h:49) or as written when literal == 0. */
 int lpm_unit_cell(int lattice, double radius, int literal, double *out, double *map);
 
 /* ------------------------------------------------------------------ context
@@ -143,6 +143,17 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *  "dirichlet_static" 0: static step does not eliminate constrained DoFs (solver_static.h:113)
  *                      | 1: eliminate (as the fatigue step does, solver_fatigue.h:288)
  *  "rollback_pex"     0: Pex kept on a step restart (assembly.h:150-159) | 1: restored
+ *  "stiffness_fd"     0: analytic blocks (StiffnessMode::Analytical, stiffness.h:67-101) |
+ *                        1: forward-difference blocks (StiffnessMode::FiniteDifference,
+ *                        stiffness.h:103-154, particle.h:183-195): the members of one class of a
+ *                        colouring (no two in one conns list) are moved by EPS*radius along one
+ *                        axis together, every bond is re-evaluated, and each row takes
+ *                        (Pin - Pin_base)/EPS/radius into the block of its moved partner; then
+ *                        damage, state variables, bdamage and dLp return to their last converged
+ *                        copies as the reference's restore leaves them. Positions are restored
+ *                        exactly (the reference's (x + h) - h can leave them one ulp off).
+ *                        LPM_E_ARG for J2Plasticity, whose reference FD pass carries the whole
+ *                        return-mapping increment. get_mode("fd_colors") = classes once built.
  *  "precision"        0: FP32 K, SpMV and CG (FP64 residual, positions, state) | 1: FP64 everywhere
  *  "cg_tol"           relative CG tolerance (Eigen default 2.22e-16; FP32 runs use ~1e-5)
  *  "cg_maxit"         CG iteration cap (-1: 2*ndof, Eigen default)

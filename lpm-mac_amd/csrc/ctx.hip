@@ -200,7 +200,8 @@ int geometry_force_state(lpm_ctx *c) {
 int calc_stiffness(lpm_ctx *c) {
     RC(cg_alloc(c));
     if (c->cg_transpose) RC(launch_tslot(c));
-    RC(launch_assemble(c));
+    if (c->stiffness_fd) RC(launch_assemble_fd(c));
+    else RC(launch_assemble(c));
     c->k_ready = true;
     c->k_version++;
     c->st_asm++;
@@ -778,6 +779,7 @@ int lpm_destroy(lpm_ctx *c) {
         if (p) hipFree(p);
     dump_free(c);
     coarse_free(c);
+    fd_free(c);
     for (void *p : c->ckpt)
         if (p) hipFree(p);
     if (c->cg_exec) hipGraphExecDestroy(c->cg_exec);
@@ -862,7 +864,11 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
         c->spmv_spw = (int)v;
     } else if (w == "spmv_remap") c->spmv_remap = (int)v ? 1 : 0;
     else if (w == "spmv_idx16") c->spmv_idx16 = (int)v ? 1 : 0;
-    else if (w == "spmv_sym") {
+    else if (w == "stiffness_fd") {
+        if ((int)v && c->model == LPM_J2PLASTICITY)
+            return lpm_fail(c, LPM_E_ARG, "finite-difference stiffness is not provided for J2Plasticity");
+        c->stiffness_fd = (int)v ? 1 : 0;
+    } else if (w == "spmv_sym") {
         c->spmv_sym = (int)v ? 1 : 0;
         c->sym_version = -1;
     } else return lpm_fail(c, LPM_E_ARG, "unknown mode " + w);
@@ -877,6 +883,8 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "dirichlet_static") return c->dirichlet_static;
     if (w == "rollback_pex") return c->rollback_pex;
     if (w == "precision") return c->precision;
+    if (w == "stiffness_fd") return c->stiffness_fd;
+    if (w == "fd_colors") return c->fd_ncolor;
     if (w == "cg_tol") return c->cg_tol;
     if (w == "cg_maxit") return (double)c->cg_maxit;
     if (w == "precond") return c->precond;

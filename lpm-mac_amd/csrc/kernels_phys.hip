@@ -1205,6 +1205,175 @@ int launch_dirichlet(lpm_ctx *c) {
     return check_launch(c, "k_dirichlet");
 }
 
+// ---------------------------------------------------------------- finite-difference stiffness
+// Stiffness::localStiffnessFD (stiffness.h:103-154): column s of block (i, j) is
+// (Pin_i(x + h e_js) - Pin_i(x)) / EPS / radius with h = EPS * radius, every bond's geometry and
+// force evaluated at the perturbed positions. Pin_i depends only on the positions of conns(i)
+// (1-hop partners and same-layer 2-hop partners: bond geometry, the layer sums and the reverse
+// bonds' forces), so the members of one colour class -- no two of them in one conns list -- are
+// perturbed together, and one pass per class and axis gives, for every row, column s of the one
+// block whose partner moved. Positions are restored exactly after each pass.
+__global__ __launch_bounds__(256) void k_fd_move(int64_t m, const int32_t *__restrict__ ids, int s, double h,
+                                                  const double *__restrict__ x0, double *__restrict__ xyz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int64_t j = ids[t];
+    xyz[3 * j + s] = x0[3 * j + s] + h;  // xyz_temp[r] += EPS * radius (stiffness.h:120)
+}
+
+__global__ __launch_bounds__(256) void k_fd_restore(int64_t m, const int32_t *__restrict__ ids, int s,
+                                                     const double *__restrict__ x0, double *__restrict__ xyz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int64_t j = ids[t];
+    xyz[3 * j + s] = x0[3 * j + s];
+}
+
+// Row i: the slot whose partner belongs to class `col` receives column s of its block,
+// K[r][s] = (Pin_i[r] - P0_i[r]) / EPS / radius (stiffness.h:134-135, 147-150), stored like
+// k_assemble stores the analytic block (in the transposed position when `transpose`).
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_fd_column(int64_t n, int col, int s, int transpose, double radius,
+                                                    const int32_t *__restrict__ cptr, const int32_t *__restrict__ cidx,
+                                                    const int32_t *__restrict__ color, const int32_t *__restrict__ tslot,
+                                                    const int64_t *__restrict__ sp, const double *__restrict__ Pin,
+                                                    const double *__restrict__ P0, T *__restrict__ aval,
+                                                    double *__restrict__ diag64) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c0 = cptr[i], nci = cptr[i + 1] - c0;
+    int slot = -1;
+    for (int q = 0; q < nci; ++q)
+        if (color[cidx[c0 + q]] == col) {
+            slot = q;
+            break;
+        }
+    if (slot < 0) return;
+    const int64_t j = cidx[c0 + slot];
+    double K[3];
+    for (int r = 0; r < D; ++r) K[r] = (Pin[3 * i + r] - P0[3 * i + r]) / LPM_EPS / radius;
+    if (!transpose) {
+        for (int r = 0; r < D; ++r) aval[sell_off(sp, i, slot, D * D, r * D + s)] = (T)K[r];
+    } else {
+        const int ts = tslot[c0 + slot];
+        for (int r = 0; r < D; ++r) aval[sell_off(sp, j, ts, D * D, s * D + r)] = (T)K[r];
+    }
+    if (j == i) diag64[(int64_t)D * i + s] = K[s];
+}
+
+void fd_free(lpm_ctx *c) {
+    for (void *p : {(void *)c->fd_color, (void *)c->fd_ids, (void *)c->fd_x0, (void *)c->fd_P0})
+        if (p) hipFree(p);
+    c->fd_color = c->fd_ids = nullptr;
+    c->fd_x0 = c->fd_P0 = nullptr;
+    c->fd_ptr.clear();
+    c->fd_ncolor = 0;
+}
+
+// Greedy colouring in id order: particle k takes the smallest class not used by any particle
+// that shares a conns list with it (the members of conns(i) for every i in conns(k); conns is
+// symmetric). Built once: the pattern never changes.
+static int fd_prepare(lpm_ctx *c) {
+    if (c->fd_color) return LPM_OK;
+    const int64_t n = c->n;
+    std::vector<int32_t> cp(n + 1), ci(c->nconn);
+    LPM_HIP(c, hipMemcpyAsync(cp.data(), c->cptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(ci.data(), c->cidx, sizeof(int32_t) * c->nconn, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<int32_t> color(n, -1);
+    std::vector<int64_t> seen;  // seen[class] == k: class used next to particle k
+    int ncol = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        for (int32_t a = cp[k]; a < cp[k + 1]; ++a) {
+            const int64_t i = ci[a];
+            for (int32_t b = cp[i]; b < cp[i + 1]; ++b) {
+                const int32_t q = color[ci[b]];
+                if (q >= 0) seen[q] = k;
+            }
+        }
+        int32_t q = 0;
+        while (q < ncol && seen[q] == k) ++q;
+        if (q == ncol) {
+            ++ncol;
+            seen.push_back(-1);
+        }
+        color[k] = q;
+    }
+    std::vector<int64_t> ptr(ncol + 1, 0);
+    for (int64_t k = 0; k < n; ++k) ++ptr[color[k] + 1];
+    for (int q = 0; q < ncol; ++q) ptr[q + 1] += ptr[q];
+    std::vector<int32_t> ids(n);
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t k = 0; k < n; ++k) ids[fill[color[k]]++] = (int32_t)k;
+    LPM_HIP(c, hipMalloc(&c->fd_color, sizeof(int32_t) * n));
+    LPM_HIP(c, hipMalloc(&c->fd_ids, sizeof(int32_t) * n));
+    LPM_HIP(c, hipMalloc(&c->fd_x0, sizeof(double) * 3 * n));
+    LPM_HIP(c, hipMalloc(&c->fd_P0, sizeof(double) * 3 * n));
+    LPM_HIP(c, hipMemcpyAsync(c->fd_color, color.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->fd_ids, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+    c->fd_ptr = ptr;
+    c->fd_ncolor = ncol;
+    return LPM_OK;
+}
+
+int launch_assemble_fd(lpm_ctx *c) {
+    // ParticleJ2Plasticity's FD pass runs the return mapping (particle_j2plasticity.h:78-140) on
+    // the stress of the previous force evaluation while the bond geometry is perturbed, so its
+    // quotient carries the whole plastic increment; not provided.
+    if (c->model == LPM_J2PLASTICITY)
+        return lpm_fail(c, LPM_E_ARG, "finite-difference stiffness is not provided for J2Plasticity");
+    if (c->cg_transpose && !c->tslot) return lpm_fail(c, LPM_E_STATE, "transpose map missing");
+    int rc = fd_prepare(c);
+    if (rc) return rc;
+    const int64_t n = c->n;
+    const size_t n3 = sizeof(double) * 3 * n;
+    LPM_HIP(c, hipMemcpyAsync(c->fd_x0, c->xyz, n3, hipMemcpyDeviceToDevice, c->stream));
+    // base forces at the current positions (Pin_temp, stiffness.h:114-116)
+    rc = launch_geometry_force(c, true);
+    if (!rc) rc = launch_particle_force(c);
+    if (rc) return rc;
+    LPM_HIP(c, hipMemcpyAsync(c->fd_P0, c->Pin, n3, hipMemcpyDeviceToDevice, c->stream));
+    const double h = LPM_EPS * c->radius;
+    for (int q = 0; q < c->fd_ncolor; ++q) {
+        const int64_t m = c->fd_ptr[q + 1] - c->fd_ptr[q];
+        const int32_t *ids = c->fd_ids + c->fd_ptr[q];
+        for (int s = 0; s < c->dim; ++s) {
+            hipLaunchKernelGGL(k_fd_move, dim3(blocks_for(m, 256)), dim3(256), 0, c->stream, m, ids, s, h, c->fd_x0, c->xyz);
+            if ((rc = check_launch(c, "k_fd_move"))) return rc;
+            // updateBondsGeometry, updateBondsForce and updateParticleForce (stiffness.h:122-132);
+            // the state update between them changes no force of these models
+            if ((rc = launch_geometry_force(c, true))) return rc;
+            if ((rc = launch_particle_force(c))) return rc;
+#define FD_COL(T, D)                                                                                                   \
+    hipLaunchKernelGGL((k_fd_column<T, D>), dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, q, s, c->cg_transpose, \
+                       c->radius, c->cptr, c->cidx, c->fd_color, c->tslot, c->sp, c->Pin, c->fd_P0, (T *)c->aval, c->diag64)
+            if (c->precision == 0) {
+                if (c->dim == 3) FD_COL(float, 3);
+                else FD_COL(float, 2);
+            } else {
+                if (c->dim == 3) FD_COL(double, 3);
+                else FD_COL(double, 2);
+            }
+#undef FD_COL
+            if ((rc = check_launch(c, "k_fd_column"))) return rc;
+            hipLaunchKernelGGL(k_fd_restore, dim3(blocks_for(m, 256)), dim3(256), 0, c->stream, m, ids, s, c->fd_x0, c->xyz);
+            if ((rc = check_launch(c, "k_fd_restore"))) return rc;
+        }
+    }
+    // resumeParticleState (particle.h:183-195) leaves every particle it visits -- all of them --
+    // with damage, state variables and bond state reset to the last converged copies and the
+    // bond forces re-evaluated
+    const size_t D = sizeof(double), nb = c->nbond;
+    LPM_HIP(c, hipMemcpyAsync(c->damage, c->damage_last, n * D, hipMemcpyDeviceToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->sv, c->sv_last, LPM_NSV * n * D, hipMemcpyDeviceToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->bdamage, c->bdamage_last, nb * D, hipMemcpyDeviceToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->dLp, c->dLp_last, nb * D, hipMemcpyDeviceToDevice, c->stream));
+    rc = launch_geometry_force(c, true);
+    if (!rc) rc = launch_particle_force(c);
+    return rc;
+}
+
 int launch_state_var(lpm_ctx *c) {
     if (c->model == 0) return LPM_OK;  // ParticleElastic has no state update (particle.h:62)
     hipLaunchKernelGGL(k_state_var, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->model, c->volume,

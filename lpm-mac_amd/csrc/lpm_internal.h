@@ -193,6 +193,15 @@ struct lpm_ctx {
     double *h_scal = nullptr;    // pinned host mirror of scal
     int32_t *h_iflag = nullptr;
     bool k_ready = false;
+    // Finite-difference stiffness (mode "stiffness_fd", StiffnessMode::FiniteDifference): a
+    // colouring in which no two particles of a class share a conns list (so one perturbed pass
+    // per class and axis gives one block column per row), the classes' members, and the saved
+    // positions and base forces of the current assembly; built on the first FD assembly
+    int stiffness_fd = 0;
+    int fd_ncolor = 0;
+    int32_t *fd_color = nullptr, *fd_ids = nullptr;  // class of each particle / members by class
+    std::vector<int64_t> fd_ptr;                     // class offsets into fd_ids (host)
+    double *fd_x0 = nullptr, *fd_P0 = nullptr;
     // groups
     std::vector<Group> groups;
     // statistics
@@ -219,6 +228,8 @@ int launch_bond_force(lpm_ctx *c);
 int launch_particle_force(lpm_ctx *c);
 int launch_residual(lpm_ctx *c, double *nr2, double *nR2);
 int launch_assemble(lpm_ctx *c);
+int launch_assemble_fd(lpm_ctx *c);  // finite-difference blocks (mode "stiffness_fd")
+void fd_free(lpm_ctx *c);
 int launch_dirichlet(lpm_ctx *c);
 int launch_state_var(lpm_ctx *c);
 int launch_nonlocal_ddot(lpm_ctx *c, int undamaged);

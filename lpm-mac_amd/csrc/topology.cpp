@@ -23,7 +23,7 @@ struct CellConst {
     double data[9];  // el_mapping source array, in the order it is written in unit_cell.h
 };
 
-// UnitCell::setSquare/setHexagon/setSimpleCubic/setFCC/setBCC (unit_cell.h:142-225)
+// UnitCell::setSquare/setHexagon/setSimpleCubic/setFCC/setBCC (unit_cell.h:44-127)
 bool unit_cell(int lattice, double r, CellConst &c) {
     std::memset(&c, 0, sizeof c);
     if (lattice == LPM_SQUARE2D) {

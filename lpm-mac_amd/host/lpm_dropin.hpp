@@ -148,7 +148,7 @@ class UnitCell {
     int nneighbors_AFEM{0}, nneighbors_AFEM1{0}, nneighbors_AFEM2{0};
     double radius{0};
     double particle_volume{0};
-    lpm::Mat3 el_mapping;  // as the reference reads it (column-major Map, unit_cell.h:147)
+    lpm::Mat3 el_mapping;  // as the reference reads it (column-major Map, unit_cell.h:49)
     std::vector<double> neighbor_cutoff;
 
     UnitCell(LatticeType p_lattice, double p_radius) : lattice(p_lattice), radius(p_radius) {
@@ -162,7 +162,7 @@ class UnitCell {
         neighbor_cutoff = {out[5], out[6]};
         particle_volume = out[7];
         for (int k = 0; k < 9; ++k) el_mapping.a[k] = map[k];
-        // nneighbors_AFEM1/2 (unit_cell.h:154-155,170-171,188-189,205-206,222-223)
+        // nneighbors_AFEM1/2 (unit_cell.h:56-57,72-73,90-91,107-108,124-125)
         switch (lattice) {
             case LatticeType::Square2D: nneighbors_AFEM1 = 12; nneighbors_AFEM2 = 12; break;
             case LatticeType::Hexagon2D: nneighbors_AFEM1 = 18; nneighbors_AFEM2 = 18; break;
@@ -389,7 +389,7 @@ class ParticleJ2Plasticity : public Particle<nlayer> {
     }
 };
 
-// ParticleFatigueHCF (particle_fatigue_hcf.h:219-209)
+// ParticleFatigueHCF (particle_fatigue_hcf.h:162-209)
 template <int nlayer>
 class ParticleFatigueHCF : public Particle<nlayer> {
    public:
@@ -882,12 +882,14 @@ class Stiffness {
    public:
     StiffnessMode mode;
     Assembly<nlayer> *ass{nullptr};
-    Stiffness(std::vector<Particle<nlayer> *> &, StiffnessMode m) : mode(m) {
-        if (m == StiffnessMode::FiniteDifference)
-            fprintf(stderr, "StiffnessMode::FiniteDifference is not implemented on the GPU; the analytical blocks are used\n");
-    }
+    Stiffness(std::vector<Particle<nlayer> *> &, StiffnessMode m) : mode(m) {}
     void reset(std::vector<Particle<nlayer> *> &) {}
-    void calcStiffness(std::vector<Particle<nlayer> *> &) { lpm::check(lpm_calc_stiffness(ass->dev->ctx), ass->dev->ctx, "calcStiffness"); }
+    // localStiffness picks ANA or FD by mode (stiffness.h:50-56); FD is refused for J2Plasticity
+    void calcStiffness(std::vector<Particle<nlayer> *> &) {
+        lpm::check(lpm_set_mode(ass->dev->ctx, "stiffness_fd", mode == StiffnessMode::FiniteDifference ? 1 : 0),
+                   ass->dev->ctx, "StiffnessMode");
+        lpm::check(lpm_calc_stiffness(ass->dev->ctx), ass->dev->ctx, "calcStiffness");
+    }
     void updateStiffnessDispBC(std::vector<Particle<nlayer> *> &) {
         lpm::check(lpm_update_stiffness_disp_bc(ass->dev->ctx), ass->dev->ctx, "updateStiffnessDispBC");
     }
@@ -929,6 +931,8 @@ class Solver {
         stiffness.ass = &ass;
         lpm_set_mode(ass.dev->ctx, "max_NR_iter", max_NR_iter);
         lpm_set_mode(ass.dev->ctx, "tol_NR", tol_NR_iter);
+        lpm::check(lpm_set_mode(ass.dev->ctx, "stiffness_fd", sm == StiffnessMode::FiniteDifference ? 1 : 0), ass.dev->ctx,
+                   "StiffnessMode");
     }
 
     lpm_ctx *ctx() { return ass.dev->ctx; }

@@ -47,7 +47,7 @@ static double func_phi(const double x, const double L) {
     return std::pow(1 / (1 + std::pow(x / L, p)), q);
 }
 
-// UnitCell: include/unit_cell.h:125-225
+// UnitCell: include/unit_cell.h:7-127
 struct Cell {
     int lattice = 0, dim = 0;
     int nneighbors = 0, nneighbors1 = 0, nneighbors2 = 0;
@@ -61,14 +61,14 @@ static Cell make_cell(int lattice, double r) {
     Cell c;
     c.lattice = lattice;
     c.radius = r;
-    if (lattice == Square2D) {  // unit_cell.h:142-157
+    if (lattice == Square2D) {  // unit_cell.h:44-59
         double d[9] = {1. / 2., -1. / 2., 0., 0., 0., 1. / 2., 0., 1. / 12., -1. / 12.};
         std::memcpy(c.data, d, sizeof d);
         c.dim = 2; c.nneighbors = 8; c.nneighbors1 = 4; c.nneighbors2 = 4;
         c.cutoff[0] = 2.0 * r; c.cutoff[1] = 2.0 * std::sqrt(2.0) * r;
         c.nAFEM = 16; c.nAFEM1 = 12; c.nAFEM2 = 12;
         c.volume = 4 * std::pow(r, 2);
-    } else if (lattice == Hexagon2D) {  // unit_cell.h:159-174
+    } else if (lattice == Hexagon2D) {  // unit_cell.h:61-76
         double d[9] = {std::sqrt(3.0) / 12.0, -std::sqrt(3.0) / 12.0, 0.,
                        -std::sqrt(3.0) / 144.0, std::sqrt(3.0) / 48.0, 0.0,
                        0.0, 0.0, 1.0};
@@ -77,14 +77,14 @@ static Cell make_cell(int lattice, double r) {
         c.nAFEM = 30; c.nAFEM1 = 18; c.nAFEM2 = 18;
         c.cutoff[0] = 2.0 * r; c.cutoff[1] = 2.0 * std::sqrt(3.0) * r;
         c.volume = 2 * std::sqrt(3) * std::pow(r, 2);
-    } else if (lattice == SimpleCubic3D) {  // unit_cell.h:176-191
+    } else if (lattice == SimpleCubic3D) {  // unit_cell.h:78-93
         double d[9] = {1, -1, -1.0, 0.0, 0.0, 1., 0.0, 1 / 18.0, -1 / 18.0};
         std::memcpy(c.data, d, sizeof d);
         c.dim = 3; c.nneighbors = 18; c.nneighbors1 = 6; c.nneighbors2 = 12;
         c.cutoff[0] = 2.0 * r; c.cutoff[1] = 2.0 * std::sqrt(2.0) * r;
         c.nAFEM = 60; c.nAFEM1 = 24; c.nAFEM2 = 54;
         c.volume = std::pow(2 * r, 3);
-    } else if (lattice == FCC3D) {  // unit_cell.h:193-208
+    } else if (lattice == FCC3D) {  // unit_cell.h:95-110
         double d[9] = {0.0, 0.0, std::sqrt(2.0), std::sqrt(2.0) / 4, -std::sqrt(2.0) / 4, -std::sqrt(2.0) / 4,
                        0.0, std::sqrt(2.0) / 24, -std::sqrt(2.0) / 24};
         std::memcpy(c.data, d, sizeof d);
@@ -92,7 +92,7 @@ static Cell make_cell(int lattice, double r) {
         c.cutoff[0] = 2.0 * r; c.cutoff[1] = 2.0 * std::sqrt(2.0) * r;
         c.nAFEM = 60; c.nAFEM1 = 54; c.nAFEM2 = 24;
         c.volume = 4.0 * std::sqrt(2.0) * std::pow(r, 3);
-    } else {  // BCC, unit_cell.h:210-225
+    } else {  // BCC, unit_cell.h:112-127
         double d[9] = {0.0, 0.0, std::sqrt(3.0), 1 / std::sqrt(3.0), -1 / std::sqrt(3.0), 0.,
                        0.0, std::sqrt(3.0) / 14, std::sqrt(3.0) / 14};
         std::memcpy(c.data, d, sizeof d);
@@ -105,7 +105,7 @@ static Cell make_cell(int lattice, double r) {
 }
 
 // el_mapping(r, c). The reference builds it with Eigen::Map, which is
-// column-major, from a row-major-written array (unit_cell.h:147,164,181,198,215):
+// column-major, from a row-major-written array (unit_cell.h:49,66,83,100,117):
 // literal => M(r,c) = data[c*3 + r]; intended (row-major) => data[r*3 + c].
 static double el_map(const Cell &c, bool literal, int r, int col) {
     return literal ? c.data[col * 3 + r] : c.data[r * 3 + col];
@@ -135,6 +135,7 @@ struct Sys {
     int dirichlet_static = 0;// static solver does not eliminate Dirichlet DoFs
     int rollback_pex = 0;    // Pex is not rolled back on a step restart
     int stiffness_fd = 0;    // StiffnessMode::FiniteDifference
+    int fd_exact = 0;        // FD with exact restores and a fresh base force (not the reference)
     double cg_tol = 2.220446049250313e-16;  // Eigen default tolerance (epsilon)
     long cg_maxit = -1;                     // -1 => 2 * ndof (Eigen default)
     int max_NR_iter = 30;
@@ -199,7 +200,7 @@ static void init_particles(Sys &s, int lattice, double radius, int ptype, int n,
     s.sv.assign(NSV * n, 0); s.sv_last.assign(NSV * n, 0);
     s.mat.assign(10 * n, 0);
     // state_var initial values: particle_elastic.h:23-29, particle_elastic_damage.h:177-184,
-    // particle_j2plasticity.h:26-33, particle_fatigue_hcf.h:229-236 (INT_MAX stored as double)
+    // particle_j2plasticity.h:26-33, particle_fatigue_hcf.h:27-28,33-34 (INT_MAX stored as double)
     if (ptype == Elastic || ptype == ElasticDamage) s.nsv = 1;
     else if (ptype == J2Plasticity) s.nsv = 7;
     else { s.nsv = 3; }
@@ -700,10 +701,14 @@ static void reset_particle_state(Sys &s, int i) {
     }
 }
 
-// Stiffness::localStiffnessFD (stiffness.h:103-154) with resumeParticleState (particle.h:183-195)
+// Stiffness::localStiffnessFD (stiffness.h:103-154) with resumeParticleState (particle.h:183-195).
+// Literal: the restore computes (x + h) - h, which can leave x one ulp off, and later blocks see
+// those positions. fd_exact = 1 restores x exactly instead (calc_stiffness then also starts from
+// freshly evaluated forces), the order-independent quotient a parallel evaluation computes.
 static Blk local_stiffness_fd(Sys &s, int i, int j) {
     std::vector<double> Kij;
     double xt[3] = {s.xyz[3 * j], s.xyz[3 * j + 1], s.xyz[3 * j + 2]};
+    const double x_saved[3] = {xt[0], xt[1], xt[2]};
     particle_force(s, i);
     double Pt[3] = {s.Pin[3 * i], s.Pin[3 * i + 1], s.Pin[3 * i + 2]};
     const double h = EPS * s.cell.radius;
@@ -715,7 +720,8 @@ static Blk local_stiffness_fd(Sys &s, int i, int j) {
         for (int c = s.cptr[j]; c < s.cptr[j + 1]; ++c) bonds_force_particle(s, s.cidx[c]);
         particle_force(s, i);
         for (int q = 0; q < s.dim; ++q) Kij.push_back((s.Pin[3 * i + q] - Pt[q]) / EPS / (s.cell.radius));
-        xt[r] -= EPS * s.cell.radius;
+        if (s.fd_exact) xt[r] = x_saved[r];
+        else xt[r] -= EPS * s.cell.radius;
         for (int k = 0; k < 3; ++k) s.xyz[3 * j + k] = xt[k];
         for (int c = s.cptr[j]; c < s.cptr[j + 1]; ++c) {
             bonds_geometry_particle(s, s.cidx[c]);
@@ -753,6 +759,10 @@ static void calc_stiffness(Sys &s) {
         }
     };
     if (s.stiffness_fd) {
+        if (s.fd_exact) {  // base forces at the current positions
+            update_geometry(s);
+            for (int i = 0; i < n; ++i) bonds_force_particle(s, i);
+        }
         for (int i = 0; i < n; ++i) fill(i);  // FD mutates neighbour state: serial, as the reference
     } else {
 #pragma omp parallel for schedule(dynamic, 256)
@@ -1502,6 +1512,7 @@ int ora_set_mode(void *p, const char *what, double v) {
     else if (w == "dirichlet_static") s.dirichlet_static = (int)v;
     else if (w == "rollback_pex") s.rollback_pex = (int)v;
     else if (w == "stiffness_fd") s.stiffness_fd = (int)v;
+    else if (w == "fd_exact") s.fd_exact = (int)v;
     else if (w == "cg_tol") s.cg_tol = v;
     else if (w == "cg_maxit") s.cg_maxit = (long)v;
     else if (w == "max_NR_iter") s.max_NR_iter = (int)v;

@@ -143,8 +143,8 @@ def test_hcf_ensemble_cycles_match_single_specimen_oracle():
         sl = slice(first[s], first[s + 1])
         say(f"   specimen {s} ({len(x)} particles): damage max {Do.max():.4e} diff {np.abs(Dg[sl] - Do).max():.3e}")
         assert Do.max() > 0
-        np.testing.assert_allclose(Dg[sl], Do, rtol=1e-7, atol=1e-15)
-        np.testing.assert_allclose(svg[sl], o.getd("state_var").reshape(-1, 7)[:, :3], rtol=1e-7, atol=1e-12)
+        assert rel(Dg[sl], Do) < 1e-8
+        np.testing.assert_allclose(svg[sl], o.getd("state_var").reshape(-1, 7)[:, :3], rtol=1e-8, atol=1e-12)
 
 
 def test_batched_adjoint_matches_single_specimen_adjoints():

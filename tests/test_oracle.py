@@ -68,7 +68,7 @@ def test_kntv_fcc_matches_fortran_closed_form():
 
 
 def test_literal_mapping_is_the_transpose():
-    # Eigen::Map of a row-major-written array is read column-major (unit_cell.h:164)
+    # Eigen::Map of a row-major-written array is read column-major (unit_cell.h:66)
     xyz, box = hex_corner_plate(6)
     s = O.System(O.HEXAGON2D, 0.2, O.ELASTIC, xyz, box)
     E, mu = 69e3, 0.3
@@ -179,7 +179,7 @@ def test_conns_two_hop_per_layer(lat, box, r, full):
         got = list(ci[cp[i]:cp[i + 1]])
         assert got == sorted(exp)
         assert nlq[i] == sum(1 for j in got if j >= i)
-    # nconn of an interior particle = nneighbors_AFEM + 1 (unit_cell.h:169,187,204)
+    # nconn of an interior particle = nneighbors_AFEM + 1 (unit_cell.h:71,89,106)
     assert (cp[1:] - cp[:-1]).max() == full
 
 
@@ -326,6 +326,45 @@ def test_analytic_block_close_to_finite_difference():
         Kf = s.local_stiffness(i, int(j))
         s.mode(stiffness_fd=0)
         assert np.abs(Ka - Kf).max() < 1e-4 * scale, (j, Ka, Kf)
+
+
+def test_fd_restore_drift_is_bounded():
+    """localStiffnessFD restores a moved particle by (x + h) - h (stiffness.h:138-140): exact
+    unless x + h crosses into the next binade, where the particle is left one ulp off and later
+    blocks see it. The oracle's fd_exact mode (the parallel evaluation's quotient) restores x
+    exactly: equal blocks on a lattice without such coordinates, and blocks within
+    |K| * ulp(x) / h where a coordinate sits within h below a power of two."""
+    xyz = O.generate(O.HEXAGON2D, [0, 7, 0, 6, 0, 1], 0.5)
+
+    def fd(x, exact):
+        s = O.System(O.HEXAGON2D, 0.5, O.ELASTIC, xyz)
+        s.mode(elmap_literal=0)
+        s.set_elastic(0.0, True, 69e3, 0.3, 1.0)
+        s.setd("xyz", x)
+        s.update_geometry()
+        s.update_force_state()
+        s.store()
+        s.mode(stiffness_fd=1, fd_exact=int(exact))
+        s.calc_stiffness()
+        return s.getd("kval"), s.getd("xyz")
+
+    rng = np.random.default_rng(2)
+    x = xyz + 1e-3 * rng.standard_normal(xyz.shape)
+    x[:, 2] = 0
+    h = 1e-6 * 0.5
+    assert not np.any(((x + h) - h) != x)  # no coordinate within h below a power of two
+    ke, xe = fd(x, True)
+    kl, xl = fd(x, False)
+    np.testing.assert_array_equal(kl, ke)
+    np.testing.assert_array_equal(xl, xe)
+    k = int(np.argmin(np.abs(x[:, 0] - 2.0)))
+    x[k, 0] = 2.0 - 1e-7  # (x + h) - h = x - 2^-52
+    ke, xe = fd(x, True)
+    kl, xl = fd(x, False)
+    np.testing.assert_array_equal(xe, x.ravel())
+    assert xl[3 * k] == x[k, 0] - 2.0 ** -52       # the literal restore drifted
+    assert np.any(kl != ke)
+    assert np.abs(kl - ke).max() <= 1e-8 * np.abs(ke).max()
 
 
 def _small_problem(eliminate=True):
