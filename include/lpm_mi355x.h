@@ -283,6 +283,17 @@ int lpm_update_stiffness_disp_bc(lpm_ctx *ctx);
 int lpm_solve_linear_system(lpm_ctx *ctx, int32_t *iters, double *error);
 /* Solver::NewtonIteration (solver.h:53-89); *n_iter = return value of the reference */
 int lpm_newton_iteration(lpm_ctx *ctx, int32_t *n_iter);
+/* The computation of lpm_update_rr (Solver::updateRR with the two norms NewtonIteration takes)
+ * on caller-owned device arrays, allocating nothing. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops).
+ * Pex, Pin: FP64 [3n]; cons: int32 [3n] (disp_constraint); residual: FP64 [dim*n], written;
+ * norms: host double[2] = ||residual||, ||reaction|| (may be NULL: no host synchronisation);
+ * stream: hipStream_t (NULL: the default stream). The per-block partial sums live in a device
+ * array the library holds once per GPU (512 KB): n*dim above LPM_NOALLOC_MAX_DOFS returns
+ * LPM_E_LIMIT. Calls that overlap on one GPU race on that array. Errors: lpm_last_error(NULL). */
+#define LPM_NOALLOC_MAX_DOFS 8388608
+int lpm_update_rr_noalloc(int64_t n, int32_t dim, const double *Pex, const double *Pin, const int32_t *cons,
+                          double *residual, double *norms, void *stream);
 /* Assembly::updateStateVar (assembly.h:161-166) */
 int lpm_update_state_var(lpm_ctx *ctx);
 /* SolverStatic::updateStaticDamage (solver_static.h:31-62) */

@@ -772,6 +772,11 @@ static int check_launch(lpm_ctx *c, const char *what) {
     return LPM_OK;
 }
 
+hipError_t launch_finalize_raw(hipStream_t stream, int kind, int na, const double *partial, int cap, double *scal) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, kind, na, partial, cap, scal);
+    return hipGetLastError();
+}
+
 int launch_finalize(lpm_ctx *c, int kind, int na) {
     if (na > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "reduction exceeds partial buffer");
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, kind, na, (const double *)c->partial,

@@ -14,6 +14,7 @@
 //   matrix     block-sparse with (dim x dim) blocks on the conns pattern, stored SELL-64
 //              (slices of 64 rows, lane-major; see sell_off in reduce.h)
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <vector>
@@ -1118,6 +1119,55 @@ int launch_residual(lpm_ctx *c, double *nr2, double *nR2) {
         if (nr2) *nr2 = c->h_scal[SC_NORM_R];
         if (nR2) *nR2 = c->h_scal[SC_NORM_REAC];
     }
+    return LPM_OK;
+}
+
+// ---------------------------------------------------------------- updateRR without allocation
+// The residual and its two norms (lpm_update_rr's kernels and grid) on caller-owned device
+// arrays. The per-block partial sums and the finalize's scalars live in `scratch`, a file-scope
+// device array that every GPU holds once for the life of the process; its address on each
+// device is looked up on that device's first call. Every slot the finalize reads is written by
+// the same call (one partial pair per block, then the two norms), so nothing needs clearing;
+// calls that overlap on one device race on the array.
+static constexpr int NOALLOC_CAP = LPM_NOALLOC_MAX_DOFS / 256;  // partial pairs (one per block of 256 DoFs)
+static __device__ __align__(256) double scratch[SC_COUNT + 2 * NOALLOC_CAP];
+static constexpr int NOALLOC_DEVICES = 64;
+static std::atomic<double *> g_scratch[NOALLOC_DEVICES];
+
+extern "C" int lpm_update_rr_noalloc(int64_t n, int32_t dim, const double *Pex, const double *Pin, const int32_t *cons,
+                                     double *residual, double *norms, void *stream) {
+    if (n <= 0 || (dim != 2 && dim != 3) || !Pex || !Pin || !cons || !residual)
+        return lpm_fail(nullptr, LPM_E_ARG, "lpm_update_rr_noalloc: bad arguments");
+    const int64_t nd = n * dim;
+    if (nd > LPM_NOALLOC_MAX_DOFS)
+        return lpm_fail(nullptr, LPM_E_LIMIT, "lpm_update_rr_noalloc: " + std::to_string(nd) + " DoFs exceed the limit of " +
+                                                  std::to_string(LPM_NOALLOC_MAX_DOFS));
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_DEVICES)
+        return lpm_fail(nullptr, LPM_E_HIP, "lpm_update_rr_noalloc: no current device");
+    double *scal = g_scratch[dev].load();
+    if (!scal) {
+        const hipError_t e = hipGetSymbolAddress((void **)&scal, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !scal)
+            return lpm_fail(nullptr, LPM_E_HIP, std::string("lpm_update_rr_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch[dev].store(scal);
+    }
+    double *partial = scal + SC_COUNT;
+    hipStream_t st = (hipStream_t)stream;
+    const int g = blocks_for(nd, 256);
+    hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, st, n, dim, cons, Pex, Pin, residual, partial, NOALLOC_CAP);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_finalize_raw(st, FIN_NORMS, g, partial, NOALLOC_CAP, scal);
+    if (e == hipSuccess && norms) {
+        double r[SC_COUNT];
+        e = hipMemcpyAsync(r, scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e == hipSuccess) {
+            norms[0] = std::sqrt(r[SC_NORM_R]);
+            norms[1] = std::sqrt(r[SC_NORM_REAC]);
+        }
+    }
+    if (e != hipSuccess) return lpm_fail(nullptr, LPM_E_HIP, std::string("lpm_update_rr_noalloc: ") + hipGetErrorString(e));
     return LPM_OK;
 }
 

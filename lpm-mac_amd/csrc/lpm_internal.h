@@ -246,6 +246,8 @@ int launch_residual_spec(lpm_ctx *c, std::vector<double> &r2, std::vector<double
 int launch_spec_scale(lpm_ctx *c, double *v);  // v[dof] *= spec_w[specimen]
 // ---- launchers (kernels_cg.hip)
 int launch_finalize(lpm_ctx *c, int kind, int na);
+// k_finalize on caller-given buffers (no context): partial [2 x cap], scal [SC_COUNT]
+hipError_t launch_finalize_raw(hipStream_t stream, int kind, int na, const double *partial, int cap, double *scal);
 int cg_alloc(lpm_ctx *c);
 int cg_solve(lpm_ctx *c, int *iters, double *error, int *success);
 int launch_move(lpm_ctx *c);

@@ -80,6 +80,7 @@ _SIGS = {
     "lpm_update_geometry": (_I, [_P]),
     "lpm_update_force_state": (_I, [_P]),
     "lpm_update_rr": (_I, [_P, _DP, _DP]),
+    "lpm_update_rr_noalloc": (_I, [ctypes.c_int64, _I, _P, _P, _P, _P, _DP, _P]),
     "lpm_calc_stiffness": (_I, [_P]),
     "lpm_update_stiffness_disp_bc": (_I, [_P]),
     "lpm_solve_linear_system": (_I, [_P, _IP, _DP]),

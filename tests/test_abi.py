@@ -106,3 +106,17 @@ def test_dump_number_formatting_matches_printf():
     want = [printf_e4(v) for v in vals]
     bad = [(v, g, w) for v, g, w in zip(vals, got, want) if g != w]
     assert not bad, bad[:10]
+
+
+def test_noalloc_entry_point_refuses_sizes_beyond_its_array():
+    """lpm_update_rr_noalloc checks sizes on the host before any device call: above
+    LPM_NOALLOC_MAX_DOFS it returns LPM_E_LIMIT with a message (no GPU needed)."""
+    import ctypes
+
+    import lpmgpu as G
+
+    L = G.lib()
+    p = ctypes.c_void_p(8)
+    rc = L.lpm_update_rr_noalloc(8388608 // 3 + 1, 3, p, p, p, p, None, None)
+    assert rc == -6 and b"exceed the limit" in L.lpm_last_error(None)
+    assert L.lpm_update_rr_noalloc(10, 4, p, p, p, p, None, None) == -1
