@@ -39,9 +39,15 @@ HBM_PEAK = 8.0e12
 # With int16 column offsets (the default where they fit, as at 64^3), the configuration the
 # tuner kept in both passes: FETCH_SIZE 296578.8 KiB -> 607.3 MB; WRITE_SIZE 4114.8 KiB -> 4.2 MB
 # (calibration 131083.4 KiB fetched for the 256 MiB copy).
+# With the matrix and index streams read by non-temporal loads (the default), the configuration
+# the tuner kept in the WRITE_SIZE pass, k_spmv with unroll 4: FETCH_SIZE 291155.3 KiB -> 596.2 MB
+# (its 28 tuner launches in the FETCH_SIZE pass; every variant fetches 595.6-597.9 MB);
+# WRITE_SIZE 4126.8 KiB -> 4.2 MB (calibration 131082.9 KiB fetched for the copy).
 MEASURED_TRAFFIC = {
     (64, "k_spmv (CG SpMV, full store)"): 317832.7 * 1024 * 262144.0 / 131083.3 + 4121.8 * 1024,
     (64, "k_spmv (CG SpMV, full store, int16 column offsets)"): 296578.8 * 1024 * 262144.0 / 131083.4 + 4114.8 * 1024,
+    (64, "k_spmv (CG SpMV, full store, int16 column offsets, non-temporal matrix loads)"):
+        291155.3 * 1024 * 262144.0 / 131082.9 + 4126.8 * 1024,
 }
 
 
@@ -66,23 +72,18 @@ def spmv_bytes(ctx):
     Full store: every block's 9 values + its column index (4 B, or 2 B when the columns are
     stored as int16 offsets from the row), row pointers, one read of the input vector and one
     write of the output (4 B per DoF; the 16 B per particle padding of the device vectors is
-    not counted).
-    Half store (symmetric matrix): the upper blocks (j >= i) with their column indices, for
-    each lower block a column index and the position of its transpose, row pointers and
-    diagonal slots, and the same vector traffic. A lower block's values are the transposed
-    upper block, read a second time from cache, so they count once."""
+    not counted)."""
     nblk = ctx.size("nconn")
     n = ctx.n
     vec = 8 * ctx.size("ndof")  # input read once + output written once, 4 B per DoF
-    if ctx.stat("spmv_sym_active"):
-        nup, nlo = (nblk + n) // 2, (nblk - n) // 2
-        return 40 * nup + 8 * nlo + 4 * (n + 1) + 4 * n + vec, "k_spmv_sym (CG SpMV, half store)"
     ib = int(ctx.get_mode("spmv_index_bytes"))
     name = "k_spmv (CG SpMV, full store)" if ib == 4 else "k_spmv (CG SpMV, full store, int16 column offsets)"
+    if ib == 2 and int(ctx.get_mode("spmv_nt")):
+        name = name[:-1] + ", non-temporal matrix loads)"
     return (36 + ib) * nblk + 4 * (n + 1) + vec, name
 
 
-def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0, predictor=0, coarse_reuse=0):
+def setup_gpu(n, tol_nr, cg_tol, precond, cg_forcing, coarse_cells=0, predictor=0, coarse_reuse=0):
     import lpmgpu as G
 
     xyz, box, r, L = sc_workload(n)
@@ -90,7 +91,7 @@ def setup_gpu(n, tol_nr, cg_tol, precond, spmv_sym, cg_forcing, coarse_cells=0, 
     ctx = G.Context(G.SC3D, r, G.ELASTIC, xyz, bonds="device")  # bond search on the device
     ctx.set_elastic(0.0, False, 200e3, 0.3, 1.0, elmap_literal=0)
     ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, cg_tol=cg_tol, tol_NR=tol_nr,
-             max_NR_iter=30, cg_maxit=20000, precond=precond, spmv_sym=spmv_sym, cg_forcing=cg_forcing,
+             max_NR_iter=30, cg_maxit=20000, precond=precond, cg_forcing=cg_forcing,
              coarse_cells=coarse_cells, predictor=predictor, coarse_reuse=coarse_reuse)
     z = xyz[:, 2]
     bottom = np.nonzero(z < z.min() + 1e-6)[0]
@@ -113,11 +114,60 @@ def reset_state(ctx, xyz):
     ctx.update_force_state()
 
 
-def cpu_baseline(n, gpu_counts, max_threads):
-    """FP64 oracle on the same 64^3 system: one assembly, one geometry/force/residual pass and a
-    bounded number of Jacobi-CG iterations are timed; a load step is projected with the Newton and
-    CG iteration counts of a GPU run of the same steps with the same algorithm (Jacobi
-    preconditioner, as the oracle and the reference) and tolerances."""
+def host_cpu():
+    """Model name, physical cores and the CPUs this process may run on (its affinity)."""
+    model, cores, cur = None, set(), {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k in ("physical id", "core id"):
+                cur[k] = v
+            elif not line.strip() and cur:
+                cores.add((cur.get("physical id"), cur.get("core id")))
+                cur = {}
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "physical_cores": len(cores) or None, "usable_cpus": usable}
+
+
+def cpu_full_step(n, tol_nr, cg_tol):
+    """One whole load step of the reference algorithm on the host, timed end to end: the FP64
+    oracle from rest, SolverStatic::solveProblemStep with Jacobi CG (Eigen's preconditioner) at
+    the same relative CG tolerance and inexact-Newton forcing as the GPU run, the same Newton test."""
+    import oracle as O
+
+    xyz, box, r, L = sc_workload(n)
+    s = O.System(O.SC3D, r, O.ELASTIC, xyz, box)
+    s.mode(elmap_literal=0, dirichlet_static=1, cg_transpose=0, diag10_literal=0, cg_tol=cg_tol, cg_forcing=1,
+           tol_NR=tol_nr, max_NR_iter=30)
+    s.set_elastic(0.0, False, 200e3, 0.3, 1.0)
+    z = xyz[:, 2]
+    bottom = np.nonzero(z < z.min() + 1e-6)[0]
+    top = np.nonzero(z > z.max() - 1e-6)[0]
+    ls = O.LoadStep()
+    for ax in "xyz":
+        ls.add_disp(bottom, ax, 0.0)
+    ls.add_disp(top, "z", 0.005 * L / 20.0)
+    s.update_geometry()
+    s.update_force_state()
+    t0 = time.perf_counter()
+    ok = s.solve_step_static(ls, -1)
+    t = time.perf_counter() - t0
+    return t, bool(ok), s.info("newton"), int(np.sum(s.geti("cg_log")))
+
+
+def cpu_baseline(n, gpu_counts, max_threads, tol_nr, cg_tol):
+    """The FP64 oracle on the same 64^3 system on the host's cores. The value is one whole load
+    step timed end to end (cpu_full_step). Beside it, the earlier projection: one assembly, one
+    geometry/force/residual pass and a bounded number of Jacobi-CG iterations timed, projected with
+    the Newton and CG counts of the GPU's Jacobi run of the same steps."""
     import oracle as O
 
     xyz, box, r, L = sc_workload(n)
@@ -177,15 +227,24 @@ def cpu_baseline(n, gpu_counts, max_threads):
                           f"({t_force1:.3f} s) on one thread, the CG iterations on {max_threads}")}
     except OSError:
         pass
+    del s
+    t_full, ok, full_newton, full_cg = cpu_full_step(n, tol_nr, cg_tol)
     return {
-        "value": 1.0 / t_step,
+        "value": 1.0 / t_full,
         "unit": "load_steps/s",
         "cores": max_threads,
         "kind": "port",
-        "sample": (f"FP64 oracle, SC {n}^3: 1 assembly ({t_asm:.2f} s), 1 geometry+force+residual pass "
-                   f"({t_force:.3f} s), {n_cg_sample} Jacobi-CG iterations ({t_cg * 1e3:.1f} ms each) timed; "
-                   f"step projected with the per-step counts of the GPU Jacobi run: {asm_per_step:.2f} assemblies, "
-                   f"{newton_per_step:.2f} Newton iterations, {cg_per_step:.0f} CG iterations"),
+        "sample": (f"FP64 oracle, SC {n}^3, the first load step from rest timed end to end ({t_full:.1f} s, "
+                   f"{'converged' if ok else 'NOT converged'}): reference algorithm (Jacobi CG, no predictor) with "
+                   f"the GPU run's CG tolerance {cg_tol:g} and forcing, Newton test {tol_nr:g}; {full_newton} Newton "
+                   f"iterations, {full_cg} CG iterations; OpenMP over particles and matrix rows on {max_threads} threads"),
+        "host": host_cpu(),
+        "projected_from_phases": {
+            "value": 1.0 / t_step, "unit": "load_steps/s",
+            "sample": (f"1 assembly ({t_asm:.2f} s), 1 geometry+force+residual pass ({t_force:.3f} s), "
+                       f"{n_cg_sample} Jacobi-CG iterations ({t_cg * 1e3:.1f} ms each) timed; step projected with the "
+                       f"per-step counts of the GPU Jacobi run: {asm_per_step:.2f} assemblies, {newton_per_step:.2f} Newton "
+                       f"iterations, {cg_per_step:.0f} CG iterations")},
         "reference_faithful_threading": ref,
     }
 
@@ -200,9 +259,7 @@ def main():
     ap.add_argument("--cg-tol", type=float, default=1e-3)
     ap.add_argument("--cg-forcing", type=int, default=1, help="1: inexact-Newton CG tolerance per iteration")
     ap.add_argument("--precond", type=int, default=2,
-                    help="0 Jacobi (as the reference's Eigen CG), 1 block Jacobi, 2 Jacobi + rigid-body coarse correction, "
-                         "3 block Jacobi + rigid-body coarse correction")
-    ap.add_argument("--spmv-sym", type=int, default=0, help="1: half store when K is symmetric, 0: full store")
+                    help="0 Jacobi (as the reference's Eigen CG), 1 block Jacobi, 2 Jacobi + rigid-body coarse correction")
     ap.add_argument("--coarse-cells", type=int, default=0, help="aggregate edge of the two-level preconditioner (0: auto)")
     ap.add_argument("--predictor", type=int, default=1,
                     help="1: start each static step from the previous step's converged increment (Newton runs to the "
@@ -210,9 +267,6 @@ def main():
     ap.add_argument("--coarse-reuse", type=int, default=1,
                     help="1: keep the two-level coarse space across assemblies while it still works (rebuilt when the "
                          "eliminated DoFs change or the CG slows down); 0: rebuild it with every assembly")
-    ap.add_argument("--cg-batch-predict", type=int, default=0,
-                    help="1: a CG solve's first batch covers the previous solve's iteration count (one host round "
-                         "trip for solves of similar length); 0: fixed batches of 16 iterations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--ensemble", type=int, default=0,
@@ -245,19 +299,14 @@ def main():
         return
     if args.ensemble > 0:
         if args.fatigue:
-            # switched off: a run of this mode under rocprofv3 --kernel-trace ended in a host
-            # segmentation fault inside a HIP runtime call made by the CG (cg_t<float, 2, 2>, from
-            # lpm_solve_problem_cyclic_specimens), and its cause has not been found
-            print("bench.py --ensemble N --fatigue is switched off: a profiled run of it crashed with a host "
-                  "segmentation fault inside a HIP runtime call from the CG, cause not yet found", file=sys.stderr)
-            sys.exit(3)
+            fatigue_ensemble_bench(args)
+            return
         if args.steps == 20:
             args.steps = 5
         ensemble_bench(args)
         return
-    ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond, args.spmv_sym, args.cg_forcing,
+    ctx, ls, xyz, t_setup = setup_gpu(args.n, args.tol_nr, args.cg_tol, args.precond, args.cg_forcing,
                                       args.coarse_cells, args.predictor, args.coarse_reuse)
-    ctx.mode(cg_batch_predict=args.cg_batch_predict)
     if args.verbose:
         print(f"setup {t_setup:.2f} s, n={ctx.n} bonds={ctx.size('nbonds')} blocks={ctx.size('nconn')}", flush=True)
     # warmup steps, then back to the initial state so that the timed steps are steps 1..K
@@ -267,6 +316,9 @@ def main():
     reset_state(ctx, xyz)
     ctx.reset_stats()
     ctx.set_timing(args.timing_every)
+    # the timed steps start cold: the first assembly rebuilds the coarse space (a warm-up build
+    # is not carried into the timed region)
+    ctx.mode(coarse_rebuild=1)
     hip_sync(ctx)
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -289,9 +341,8 @@ def main():
     B, kname = spmv_bytes(ctx)
     achieved = B / (spmv_ms * 1e-3) / 1e9
     counts = (ctx.stat("newton_iters") / K, ctx.stat("cg_iters") / K, ctx.stat("assemblies") / K)
-    sym_solves, cg_solves = ctx.stat("sym_solves"), ctx.stat("cg_solves")
+    cg_solves = ctx.stat("cg_solves")
     coarse_builds = ctx.stat("coarse_builds") / K
-    spmv_sym_active = ctx.stat("spmv_sym_active")
     ncoarse = ctx.get_mode("ncoarse")
     # equilibrium check of the last step (size-independent): residual and reaction balance
     nr, nR = ctx.update_rr()
@@ -299,6 +350,7 @@ def main():
         reset_state(ctx, xyz)
         ctx.reset_stats()
         ctx.mode(**modes)
+        ctx.mode(coarse_rebuild=1)
         hip_sync(ctx)
         tr = time.perf_counter()
         for _ in range(K):
@@ -351,18 +403,14 @@ def main():
     # The same K steps again: without the predictor (the reference's Newton start), and with the
     # reference's algorithm (Jacobi, no predictor), whose iteration counts drive the CPU
     # projection; both rates are reported beside the headline.
-    jac = nopred = graph = noreuse = nobp = None
+    jac = nopred = graph = noreuse = None
     jac_counts = counts
     if not args.no_cpu_baseline:
         # the same steps with each CG batch replayed from a captured HIP graph (per-SpMV timing
         # is off in that mode, so the headline and its roofline come from direct launches)
         graph, _ = rerun(cg_graph=1)
         ctx.mode(cg_graph=0)
-        if args.cg_batch_predict:
-            # fixed batches of 16 CG iterations, the default
-            nobp, _ = rerun(cg_batch_predict=0)
-            ctx.mode(cg_batch_predict=1)
-        if args.coarse_reuse and args.precond in (2, 3):
+        if args.coarse_reuse and args.precond == 2:
             # the coarse space rebuilt with every assembly
             noreuse, _ = rerun(coarse_reuse=0)
         if args.predictor:
@@ -388,22 +436,21 @@ def main():
             "particles": ctx.n, "bonds": ctx.size("nbonds"), "bsr_blocks": ctx.size("nconn"),
             "newton_tol": args.tol_nr, "cg_rtol": args.cg_tol, "cg_forcing": args.cg_forcing,
             "predictor": "previous step's converged increment" if args.predictor else "none (the reference's start)",
-            "precond": {0: "jacobi", 1: "block-jacobi", 2: "two-level (jacobi + rigid-body aggregate coarse space)",
-                        3: "two-level (3x3 block jacobi + rigid-body aggregate coarse space)"}[args.precond],
-            "coarse_dofs": int(ncoarse), "coarse_reuse": args.coarse_reuse, "cg_batch_predict": args.cg_batch_predict, "coarse_builds_per_step": coarse_builds,
-            "spmv_store": "half (symmetric)" if spmv_sym_active else "full",
+            "precond": {0: "jacobi", 1: "block-jacobi", 2: "two-level (jacobi + rigid-body aggregate coarse space)"}[args.precond],
+            "coarse_dofs": int(ncoarse), "coarse_reuse": args.coarse_reuse, "coarse_builds_per_step": coarse_builds,
             "spmv_launch": {"slots_unrolled": int(ctx.get_mode("spmv_unr")),
                             "wavefronts_per_workgroup": int(ctx.get_mode("spmv_wpb")),
                             "xcd_remap": int(ctx.get_mode("spmv_remap")), "tuned": int(ctx.get_mode("spmv_tune")),
                             "wavefronts_per_slice": int(ctx.get_mode("spmv_coop")) or 1,
-                            "slices_per_workgroup": int(ctx.get_mode("spmv_spw")) if ctx.get_mode("spmv_coop") else None},
-            "solves_on_half_store": sym_solves, "cg_solves": cg_solves,
+                            "slices_per_workgroup": int(ctx.get_mode("spmv_spw")) if ctx.get_mode("spmv_coop") else None,
+                            "nontemporal_matrix_loads": int(ctx.get_mode("spmv_nt"))},
+            "cg_solves": cg_solves,
             "newton_iters_per_step": counts[0], "cg_iters_per_step": counts[1],
             "final_residual_over_reaction": nr / nR if nR > 0 else None,
             "parallelism": "single GPU",
             "setup_seconds": t_setup,
             "async_dump": dump,
-            "same_steps_with_cg_graph": graph, "same_steps_without_cg_batch_prediction": nobp,
+            "same_steps_with_cg_graph": graph,
             "same_steps_without_coarse_reuse": noreuse,
             "same_steps_without_predictor_or_coarse_reuse": nopred,
             "same_steps_reference_algorithm_jacobi_no_predictor": jac,
@@ -417,7 +464,8 @@ def main():
         },
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.n, jac_counts, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())))
+        out["cpu_baseline"] = cpu_baseline(args.n, jac_counts, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+                                           args.tol_nr, args.cg_tol)
     print(json.dumps(out), flush=True)
 
 
@@ -656,9 +704,10 @@ def fatigue_ensemble_bench(args):
         ctx.close()
         return res
 
+    # direct launches only: a graph-replayed rerun of this mode crashed inside hipGraphLaunch under
+    # rocprofv3's kernel trace (see the design notes), so graph replay stays out of it
     t_batch, n_done, dmax, steps = run_union(0)
-    # the same run with each CG batch replayed from a captured HIP graph
-    t_graph, n_graph, _, _ = run_union(1)
+    print(f"ensemble: {S} specimens, {n_done} cycles in one context: {t_batch:.2f} s", flush=True)
     # the same specimens, one context each, back to back
     t_seq = 0.0
     for s, x in enumerate(parts):
@@ -693,8 +742,7 @@ def fatigue_ensemble_bench(args):
                                f"R=0.1, {cycles} cycles, tau={tau}",
                    "precond": args.precond, "fatigue_steps": steps, "max_damage": dmax,
                    "one_context_per_specimen_specimen_cycles_per_s": S * n_done / t_seq,
-                   "speedup_vs_one_context_per_specimen": t_seq / t_batch,
-                   "with_cg_graph": {"specimen_cycles_per_s": S * n_graph / t_graph}},
+                   "speedup_vs_one_context_per_specimen": t_seq / t_batch},
     }
     print(json.dumps(out), flush=True)
 

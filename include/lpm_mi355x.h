@@ -133,7 +133,7 @@ int lpm_image_particles(const uint8_t *img, int64_t nrows, int64_t ncols, double
 int lpm_destroy(lpm_ctx *ctx);
 const char *lpm_last_error(const lpm_ctx *ctx); /* ctx may be NULL: last error of lpm_create */
 /* sizes: "n","dim","nbonds","nconn","nnl","ndof","knnz","maxconn","maxbond","nl_ptr",
- * "nslots" (SELL-64 slots of the full matrix), "nslots_u","nslots_l" (of the half store) */
+ * "nslots" (SELL-64 slots of the matrix) */
 int64_t lpm_size(const lpm_ctx *ctx, const char *what);
 
 /* Mode switches (each literal reference behaviour has value shown first).
@@ -164,8 +164,7 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *  "precond"          0: Jacobi (Eigen DiagonalPreconditioner) | 1: 3x3 block Jacobi |
  *                        2: Jacobi + additive coarse correction on the rigid-body modes of cubic
  *                        aggregates of particles (two-level; iteration count no longer grows
- *                        with the lattice width) | 3: as 2 with 3x3 block Jacobi in place of
- *                        Jacobi. Changes the CG's rate, not the equilibrium.
+ *                        with the lattice width). Changes the CG's rate, not the equilibrium.
  *  "predictor"        1: a static step starts its free DoFs from the converged positions plus the
  *                        previous converged step's increment (after the step's boundary
  *                        conditions); Newton then runs to the same test. 0 (default): the
@@ -178,22 +177,15 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        (+4) the iterations of the first solve after the last build; rebuilt
  *                        otherwise (stat "coarse_builds" counts builds) | 0 (default): rebuilt with
  *                        every assembly. K itself is reassembled either way.
+ *  "coarse_rebuild"   1: the next assembly rebuilds the coarse space even under coarse_reuse (a
+ *                        cold start); write-only
  *  "cg_graph"         1: capture one batch of CG iterations (cg_batch iterations: vector updates,
  *                        coarse correction, SpMV) into a HIP graph and replay it for every batch
  *                        of every solve; re-captured when sizes, launch configuration or buffers
  *                        change. Per-SpMV timing is off in this mode | 0 (default): launch directly
- *  "cg_batch_predict" 1: without graph replay, a solve's first batch covers the previous solve's
- *                        updates plus two (at most 512), so solves of similar length stop after
- *                        one host round trip; the iterate is the same | 0 (default): batches of
- *                        cg_batch iterations
- *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 and 3 (0: smallest of 8, 9,
+ *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 (0: smallest of 8, 9,
  *                        10, ... giving at most 768 coarse DoFs); get_mode("ncoarse") reads the
  *                        coarse dimension once built
- *  "spmv_sym"         0: SpMV on the full matrix | 1: after each assembly check that the
- *                        stored matrix is symmetric to rounding and, if so, run the SpMV from
- *                        its upper half (each block read from HBM once, the transposed half
- *                        gathered again through the caches; slower on 3D lattices whose
- *                        neighbour planes are further apart than an XCD's L2 holds)
  *  "spmv_tune"        1: on the first solve of each matrix kind time every SpMV launch
  *                        configuration and keep the fastest | 0: use the values below
  *  "spmv_unr" (1,2,4), "spmv_wpb" (wavefronts per workgroup 1,2,4,8), "spmv_remap" (0/1:
@@ -201,7 +193,9 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        64-row slice | 8, 16: that many wavefronts share each slice),
  *                        "spmv_spw" (consecutive slices per cooperative workgroup) -- the SpMV
  *                        launch configuration
- *  "cg_vec_wg"        most workgroups of the CG vector kernels (default 1024; grid-stride beyond)
+ *  "spmv_nt"          1 (default): the SpMV reads the matrix values and column indices with
+ *                        non-temporal loads (read once per SpMV; the input vector keeps the
+ *                        caches) | 0: ordinary loads
  *  "spmv_idx16"       1 (default): the SpMV reads its column indices as int16 offsets j - i when
  *                        every offset of the pattern fits (get_mode("spmv_index_bytes") = 2) |
  *                        0: int32 column ids (4)
@@ -424,9 +418,8 @@ int lpm_specimen_stat(const lpm_ctx *ctx, const char *name, double *out);
  * Counters since the last reset: "newton_iters","cg_iters","cg_solves","assemblies",
  * "spmv_launches" (SpMVs that did work); "spmv_ms" / "spmv_launches_timed" (sum of SpMV kernel
  * time from HIP events and the number of launches it covers, when timing is on);
- * "last_cg_iters","last_cg_error"; "sym_solves" (solves that ran on the half store),
- * "sym_rejects" (assemblies whose matrix failed the symmetry check), "spmv_sym_active",
- * "spmv_tuned_ms" (mean time of the fused CG SpMV in the configuration the tuner kept). */
+ * "last_cg_iters","last_cg_error"; "spmv_tuned_ms" (mean time of the fused CG SpMV in the
+ * configuration the tuner kept). */
 double lpm_stat(const lpm_ctx *ctx, const char *name);
 int lpm_reset_stats(lpm_ctx *ctx);
 /* every > 0: time the CG SpMV of every `every`-th iteration with HIP events on the context

@@ -773,7 +773,7 @@ int lpm_destroy(lpm_ctx *c) {
                     c->cs_sum, c->type, c->cons, c->dis0, c->dis, c->dis_last, c->dLe, c->dLp, c->dLp_last, c->bstrain, c->Kn,
                     c->Tv, c->cs, c->bforce, c->bforce_last, c->bdamage, c->bdamage_last, c->residual, c->disp, c->diag64,
                     c->aval, c->invdiag, c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w, c->scal, c->partial, c->counter, c->iflag, c->sp, c->scol,
-                    c->dslot, c->sp_u, c->sp_l, c->ucol, c->lcol, c->lpos, c->uval, c->scol16,
+                    c->scol16, c->pl_row, c->pl_slot, c->pl_rng, c->pl_b3, c->pl_own,
                     c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part, c->xyz_prev};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -809,8 +809,6 @@ int64_t lpm_size(const lpm_ctx *c, const char *what) {
     if (w == "maxbond") return c->maxbond;
     if (w == "nl_ptr") return c->nlptr ? c->n + 1 : 0;
     if (w == "nslots") return c->nslots;
-    if (w == "nslots_u") return c->nslots_u;
-    if (w == "nslots_l") return c->nslots_l;
     return -1;
 }
 
@@ -827,7 +825,7 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     } else if (w == "cg_tol") c->cg_tol = v;
     else if (w == "cg_maxit") c->cg_maxit = (int64_t)v;
     else if (w == "precond") {
-        if (v != 0 && v != 1 && v != 2 && v != 3) return lpm_fail(c, LPM_E_ARG, "precond must be 0, 1, 2 or 3");
+        if (v != 0 && v != 1 && v != 2) return lpm_fail(c, LPM_E_ARG, "precond must be 0, 1 or 2");
         c->precond = (int)v;
     } else if (w == "coarse_cells") {
         if (c->agg_of) return lpm_fail(c, LPM_E_STATE, "coarse_cells is fixed once the coarse space exists");
@@ -837,12 +835,18 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "tol_NR") c->tol_NR = v;
     else if (w == "verbose") c->verbose = (int)v;
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
-    else if (w == "cg_batch_predict") c->cg_batch_predict = (int)v ? 1 : 0;
-    else if (w == "cg_vec_wg") c->cg_vec_wg = std::max(1, (int)v);
+    else if (w == "asm_plan") c->asm_plan = (int)v ? 1 : 0;
+    else if (w == "spmv_nt") {
+        c->spmv_nt = (int)v ? 1 : 0;
+        c->tune_key = -1;
+    }
     else if (w == "cg_forcing") c->cg_forcing = (int)v ? 1 : 0;
     else if (w == "coarse_graph") c->coarse_graph = (int)v ? 1 : 0;
     else if (w == "cg_graph") c->cg_graph = (int)v ? 1 : 0;
     else if (w == "coarse_reuse") c->coarse_reuse = (int)v ? 1 : 0;
+    else if (w == "coarse_rebuild") {
+        if ((int)v) c->coarse_stale = true;  // the next assembly rebuilds the coarse space
+    }
     else if (w == "predictor") {
         c->predictor = (int)v ? 1 : 0;
         c->pred_valid = false;
@@ -868,9 +872,6 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
         if ((int)v && c->model == LPM_J2PLASTICITY)
             return lpm_fail(c, LPM_E_ARG, "finite-difference stiffness is not provided for J2Plasticity");
         c->stiffness_fd = (int)v ? 1 : 0;
-    } else if (w == "spmv_sym") {
-        c->spmv_sym = (int)v ? 1 : 0;
-        c->sym_version = -1;
     } else return lpm_fail(c, LPM_E_ARG, "unknown mode " + w);
     return LPM_OK;
 }
@@ -892,9 +893,8 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "tol_NR") return c->tol_NR;
     if (w == "verbose") return c->verbose;
     if (w == "cg_batch") return c->cg_batch;
-    if (w == "cg_batch_predict") return c->cg_batch_predict;
-    if (w == "cg_vec_wg") return c->cg_vec_wg;
-    if (w == "spmv_sym") return c->spmv_sym;
+    if (w == "spmv_nt") return c->spmv_nt;
+    if (w == "asm_plan") return c->asm_plan;
     if (w == "spmv_tune") return c->spmv_tune;
     if (w == "cg_forcing") return c->cg_forcing;
     if (w == "coarse_graph") return c->coarse_graph;
@@ -1209,9 +1209,6 @@ double lpm_stat(const lpm_ctx *c, const char *name) {
     if (w == "spmv_launches_timed") return (double)c->st_spmv_timed;
     if (w == "last_cg_iters") return c->last_cg_iters;
     if (w == "last_cg_error") return c->last_cg_err;
-    if (w == "sym_solves") return (double)c->st_sym_solves;
-    if (w == "sym_rejects") return (double)c->st_sym_rejects;
-    if (w == "spmv_sym_active") return c->sym_ok ? 1.0 : 0.0;
     if (w == "spmv_tuned_ms") return c->tune_best_ms;
     return NAN;
 }
@@ -1219,7 +1216,6 @@ double lpm_stat(const lpm_ctx *c, const char *name) {
 int lpm_reset_stats(lpm_ctx *c) {
     if (!c) return LPM_E_ARG;
     c->st_newton = c->st_cg_iters = c->st_cg_solves = c->st_asm = c->st_spmv = c->st_spmv_timed = 0;
-    c->st_sym_solves = c->st_sym_rejects = 0;
     c->st_coarse_builds = 0;
     c->st_spmv_ms = 0;
     return LPM_OK;

@@ -94,6 +94,14 @@ __global__ __launch_bounds__(256) void k_cg_init(int64_t n, const double *__rest
 // ---------------------------------------------------------------- SpMV  w = A u  (+ delta = (w, u))
 // Column of a SELL slot: int32 column ids, or int16 offsets j - i (2 bytes less per block, used
 // when every offset of the pattern fits).
+// Loads of the matrix stream: non-temporal when NT (the values and columns are read once per
+// SpMV; the input vector should keep the caches)
+template <bool NT, typename X>
+__device__ __forceinline__ X ld_stream(const X *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 template <typename IX>
 __device__ __forceinline__ int col_of(IX v, int64_t row) {
     if constexpr (sizeof(IX) == 2) return (int)row + (int)v;
@@ -102,7 +110,7 @@ __device__ __forceinline__ int col_of(IX v, int64_t row) {
 
 // One wavefront per 64-row slice; lane l owns row 64k+l and walks its slots. UNR slots are
 // loaded before they are consumed so that each lane has several independent loads in flight.
-template <typename T, int D, int VP, bool FUSED, int UNR, typename IX>
+template <typename T, int D, int VP, bool FUSED, int UNR, typename IX, bool NT = false>
 __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const int64_t *__restrict__ sp,
                                                const int32_t *__restrict__ cptr, const IX *__restrict__ scol,
                                                const T *__restrict__ aval, const Vec<T, VP> *__restrict__ u,
@@ -128,13 +136,13 @@ __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const i
             Vec<T, VP> pj[UNR];
             T e[UNR][EB];
 #pragma unroll
-            for (int q = 0; q < UNR; ++q) j[q] = (s + q < len) ? col_of(ci[(int64_t)(s + q) * 64], row) : -1;
+            for (int q = 0; q < UNR; ++q) j[q] = (s + q < len) ? col_of(ld_stream<NT>(&ci[(int64_t)(s + q) * 64]), row) : -1;
 #pragma unroll
             for (int q = 0; q < UNR; ++q) {
                 if (j[q] >= 0) {
                     pj[q] = u[j[q]];
 #pragma unroll
-                    for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = av[((int64_t)(s + q) * EB + e2) * 64];
+                    for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = ld_stream<NT>(&av[((int64_t)(s + q) * EB + e2) * 64]);
                 } else {
                     for (int t = 0; t < VP; ++t) pj[q].v[t] = (T)0;
                     for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = (T)0;
@@ -147,10 +155,10 @@ __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const i
         }
         for (; s < W; ++s) {
             if (s < len) {
-                const int j = col_of(ci[(int64_t)s * 64], row);
+                const int j = col_of(ld_stream<NT>(&ci[(int64_t)s * 64]), row);
                 const Vec<T, VP> pj = u[j];
                 for (int r = 0; r < D; ++r)
-                    for (int c = 0; c < D; ++c) acc[r] += av[((int64_t)s * EB + r * D + c) * 64] * pj.v[c];
+                    for (int c = 0; c < D; ++c) acc[r] += ld_stream<NT>(&av[((int64_t)s * EB + r * D + c) * 64]) * pj.v[c];
             }
         }
         if (row < n) {
@@ -174,7 +182,7 @@ __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const i
 // contiguous range of the matrix (a few thousand concurrent streams of one slice per wavefront
 // leave the DRAM pages less well used). The partial row sums meet in LDS and are added in
 // wavefront order, so the result does not depend on scheduling.
-template <typename T, int D, int VP, bool FUSED, int UNR, int NWS, typename IX>
+template <typename T, int D, int VP, bool FUSED, int UNR, int NWS, typename IX, bool NT = false>
 __global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslice, int spw,
                                                          const int64_t *__restrict__ sp, const int32_t *__restrict__ cptr,
                                                          const IX *__restrict__ scol, const T *__restrict__ aval,
@@ -204,7 +212,7 @@ __global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslic
 #pragma unroll
             for (int q = 0; q < UNR; ++q) {
                 const int s = s0 + q * NWS;
-                j[q] = (s < len) ? col_of(ci[(int64_t)s * 64], row) : -1;
+                j[q] = (s < len) ? col_of(ld_stream<NT>(&ci[(int64_t)s * 64]), row) : -1;
             }
 #pragma unroll
             for (int q = 0; q < UNR; ++q) {
@@ -212,7 +220,7 @@ __global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslic
                 if (j[q] >= 0) {
                     pj[q] = u[j[q]];
 #pragma unroll
-                    for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = av[((int64_t)s * EB + e2) * 64];
+                    for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = ld_stream<NT>(&av[((int64_t)s * EB + e2) * 64]);
                 } else {
                     for (int t2 = 0; t2 < VP; ++t2) pj[q].v[t2] = (T)0;
                     for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = (T)0;
@@ -240,236 +248,6 @@ __global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslic
             }
         }
         __syncthreads();  // red is reused by the next slice
-    }
-    if (!FUSED) return;
-    double v[1] = {delta};
-    block_partial<1>(v, partial, cap);
-}
-
-// ---------------------------------------------------------------- symmetric half store
-// When the stored matrix is symmetric (the analytic tangent of an undamaged or uniformly
-// damaged lattice with the diagonal filled symmetrically, Dirichlet rows and columns
-// eliminated), the SpMV reads each off-diagonal block from HBM once: row i keeps the blocks
-// j >= i (upper store), and for j < i it reads block (j, i) of the upper store transposed.
-// Block (j, i) sits in the slice of row j = i - offset, which the same XCD streamed a few MB
-// earlier, so the second read is served from cache in the common case.
-
-// slot of the diagonal block of every row
-__global__ __launch_bounds__(256) void k_dslot(int64_t n, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cidx,
-                                                int32_t *__restrict__ dslot, int32_t *bad) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int lo = cptr[i], hi = cptr[i + 1] - 1, pos = -1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const int v = cidx[mid];
-        if (v == i) { pos = mid; break; }
-        if (v < i) lo = mid + 1;
-        else hi = mid - 1;
-    }
-    if (pos < 0) {
-        atomicOr(bad, 1);
-        dslot[i] = 0;
-    } else {
-        dslot[i] = pos - cptr[i];
-    }
-}
-
-// upper columns; lower (column, position of block (j, i) in the upper store)
-__global__ __launch_bounds__(256) void k_sym_cols(int64_t n, const int32_t *__restrict__ cptr, const int32_t *__restrict__ cidx,
-                                                   const int32_t *__restrict__ dslot, const int64_t *__restrict__ sp_u,
-                                                   const int64_t *__restrict__ sp_l, int32_t *__restrict__ ucol,
-                                                   int32_t *__restrict__ lcol, int32_t *__restrict__ lpos, int32_t *bad) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t k = i >> 6, l = i & 63;
-    const int c0 = cptr[i], nc = cptr[i + 1] - c0, ds = dslot[i];
-    const int Wu = (int)((sp_u[k + 1] - sp_u[k]) >> 6), Wl = (int)((sp_l[k + 1] - sp_l[k]) >> 6);
-    for (int u = 0; u < Wu; ++u) ucol[sp_u[k] + (int64_t)u * 64 + l] = (ds + u < nc) ? cidx[c0 + ds + u] : (int32_t)i;
-    for (int t = 0; t < Wl; ++t) {
-        int32_t col = (int32_t)i, pos = 0;
-        if (t < ds) {
-            const int j = cidx[c0 + t];
-            int lo = cptr[j], hi = cptr[j + 1] - 1, f = -1;
-            while (lo <= hi) {
-                const int mid = (lo + hi) >> 1;
-                const int v = cidx[mid];
-                if (v == i) { f = mid; break; }
-                if (v < i) lo = mid + 1;
-                else hi = mid - 1;
-            }
-            const int uj = f - cptr[j] - dslot[j];  // slot of i among row j's upper blocks
-            if (f < 0 || uj < 1) {
-                atomicOr(bad, 1);
-            } else {
-                col = j;
-                pos = (int32_t)(sp_u[j >> 6] + (int64_t)uj * 64 + (j & 63));
-            }
-        }
-        lcol[sp_l[k] + (int64_t)t * 64 + l] = col;
-        lpos[sp_l[k] + (int64_t)t * 64 + l] = pos;
-    }
-}
-
-// Fill the upper store with (A_ij + A_ji^T)/2 and flag any pair that is not symmetric to
-// rounding: |A_ij - A_ji^T| > 1e-6 max(|A_ij|, |A_ji|) (block max norms) + 1e-10 max|diag A_ii|.
-template <typename T, int D>
-__global__ __launch_bounds__(256) void k_sym_pack(int64_t n, const int32_t *__restrict__ cptr, const int32_t *__restrict__ dslot,
-                                                   const int32_t *__restrict__ tslot, const int64_t *__restrict__ sp,
-                                                   const T *__restrict__ aval, const int64_t *__restrict__ sp_u,
-                                                   const int32_t *__restrict__ ucol, T *__restrict__ uval, int32_t *asym) {
-    constexpr int EB = D * D;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t k = i >> 6, l = i & 63;
-    const int c0 = cptr[i], nc = cptr[i + 1] - c0, ds = dslot[i];
-    const int Wu = (int)((sp_u[k + 1] - sp_u[k]) >> 6);
-    double dmax = 0.0;
-    for (int r = 0; r < D; ++r) dmax = fmax(dmax, fabs((double)aval[sell_off(sp, i, ds, EB, r * D + r)]));
-    bool bad = false;
-    for (int u = 0; u < Wu; ++u) {
-        T *dst = uval + EB * sp_u[k] + (int64_t)u * EB * 64 + l;
-        if (ds + u >= nc) {
-            for (int e = 0; e < EB; ++e) dst[(int64_t)e * 64] = (T)0;
-            continue;
-        }
-        const int c = ds + u;
-        const int64_t j = ucol[sp_u[k] + (int64_t)u * 64 + l];
-        const int ts = tslot[c0 + c];
-        double a[EB], b[EB];
-        double amax = 0.0, bmax = 0.0, dif = 0.0;
-        for (int e = 0; e < EB; ++e) {
-            a[e] = (double)aval[sell_off(sp, i, c, EB, e)];
-            b[e] = (double)aval[sell_off(sp, j, ts, EB, e)];
-            amax = fmax(amax, fabs(a[e]));
-            bmax = fmax(bmax, fabs(b[e]));
-        }
-        for (int r = 0; r < D; ++r)
-            for (int q = 0; q < D; ++q) {
-                const double x = a[r * D + q], y = b[q * D + r];
-                dif = fmax(dif, fabs(x - y));
-                dst[(int64_t)(r * D + q) * 64] = (T)(0.5 * (x + y));
-            }
-        if (dif > 1e-6 * fmax(amax, bmax) + 1e-10 * dmax) bad = true;
-    }
-    if (bad) atomicOr(asym, 1);
-}
-
-// w = A u from the half store (+ delta = (w, u) partials when FUSED)
-template <typename T, int D, int VP, bool FUSED, int UNR>
-__global__ __launch_bounds__(512) void k_spmv_sym(int64_t n, int64_t nslice, const int32_t *__restrict__ cptr,
-                                                   const int32_t *__restrict__ dslot, const int64_t *__restrict__ sp_u,
-                                                   const int32_t *__restrict__ ucol, const T *__restrict__ uval,
-                                                   const int64_t *__restrict__ sp_l, const int32_t *__restrict__ lcol,
-                                                   const int32_t *__restrict__ lpos, const Vec<T, VP> *__restrict__ u,
-                                                   Vec<T, VP> *__restrict__ w, double *partial, int cap, const double *scal,
-                                                   int remap) {
-    if (FUSED && scal[SC_DONE] != 0.0) return;
-    constexpr int EB = D * D;
-    const int lane = threadIdx.x & 63;
-    const int64_t k = xcd_block(remap) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    double delta = 0.0;
-    if (k < nslice) {
-        const int64_t row = k * 64 + lane;
-        const int ds = row < n ? dslot[row] : 0;
-        const int ulen = row < n ? cptr[row + 1] - cptr[row] - ds : 0;
-        const int llen = ds;
-        T acc[D];
-        for (int r = 0; r < D; ++r) acc[r] = (T)0;
-        // upper blocks, row-major as stored
-        {
-            const int64_t base = sp_u[k];
-            const int W = (int)((sp_u[k + 1] - base) >> 6);
-            const int32_t *ci = ucol + base + lane;
-            const T *av = uval + EB * base + lane;
-            int s = 0;
-            for (; s + UNR <= W; s += UNR) {
-                int j[UNR];
-                Vec<T, VP> pj[UNR];
-                T e[UNR][EB];
-#pragma unroll
-                for (int q = 0; q < UNR; ++q) j[q] = (s + q < ulen) ? ci[(int64_t)(s + q) * 64] : -1;
-#pragma unroll
-                for (int q = 0; q < UNR; ++q) {
-                    if (j[q] >= 0) {
-                        pj[q] = u[j[q]];
-#pragma unroll
-                        for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = av[((int64_t)(s + q) * EB + e2) * 64];
-                    } else {
-                        for (int t = 0; t < VP; ++t) pj[q].v[t] = (T)0;
-                        for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = (T)0;
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < UNR; ++q)
-                    for (int r = 0; r < D; ++r)
-                        for (int c = 0; c < D; ++c) acc[r] += e[q][r * D + c] * pj[q].v[c];
-            }
-            for (; s < W; ++s) {
-                if (s < ulen) {
-                    const int j = ci[(int64_t)s * 64];
-                    const Vec<T, VP> pj = u[j];
-                    for (int r = 0; r < D; ++r)
-                        for (int c = 0; c < D; ++c) acc[r] += av[((int64_t)s * EB + r * D + c) * 64] * pj.v[c];
-                }
-            }
-        }
-        // lower blocks: A_ij = (A_ji)^T, read from row j's upper blocks
-        {
-            const int64_t base = sp_l[k];
-            const int W = (int)((sp_l[k + 1] - base) >> 6);
-            const int32_t *ci = lcol + base + lane;
-            const int32_t *pi = lpos + base + lane;
-            int s = 0;
-            for (; s + UNR <= W; s += UNR) {
-                int j[UNR];
-                int64_t q0[UNR];
-                Vec<T, VP> pj[UNR];
-                T e[UNR][EB];
-#pragma unroll
-                for (int q = 0; q < UNR; ++q) {
-                    const bool on = s + q < llen;
-                    j[q] = on ? ci[(int64_t)(s + q) * 64] : -1;
-                    const int64_t pq = on ? (int64_t)pi[(int64_t)(s + q) * 64] : 0;
-                    q0[q] = EB * (pq & ~(int64_t)63) + (pq & 63);
-                }
-#pragma unroll
-                for (int q = 0; q < UNR; ++q) {
-                    if (j[q] >= 0) {
-                        pj[q] = u[j[q]];
-#pragma unroll
-                        for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = uval[q0[q] + (int64_t)e2 * 64];
-                    } else {
-                        for (int t = 0; t < VP; ++t) pj[q].v[t] = (T)0;
-                        for (int e2 = 0; e2 < EB; ++e2) e[q][e2] = (T)0;
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < UNR; ++q)
-                    for (int r = 0; r < D; ++r)
-                        for (int c = 0; c < D; ++c) acc[r] += e[q][c * D + r] * pj[q].v[c];
-            }
-            for (; s < W; ++s) {
-                if (s < llen) {
-                    const int j = ci[(int64_t)s * 64];
-                    const int64_t pq = pi[(int64_t)s * 64];
-                    const int64_t b0 = EB * (pq & ~(int64_t)63) + (pq & 63);
-                    const Vec<T, VP> pj = u[j];
-                    for (int r = 0; r < D; ++r)
-                        for (int c = 0; c < D; ++c) acc[r] += uval[b0 + (int64_t)(c * D + r) * 64] * pj.v[c];
-                }
-            }
-        }
-        if (row < n) {
-            Vec<T, VP> wv;
-            for (int t = 0; t < VP; ++t) wv.v[t] = (T)0;
-            for (int r = 0; r < D; ++r) wv.v[r] = acc[r];
-            w[row] = wv;
-            if (FUSED) {
-                const Vec<T, VP> ui = u[row];
-                for (int r = 0; r < D; ++r) delta += (double)ui.v[r] * (double)acc[r];
-            }
-        }
     }
     if (!FUSED) return;
     double v[1] = {delta};
@@ -828,14 +606,19 @@ int launch_precond_setup(lpm_ctx *c) {
 static bool spmv_idx16(const lpm_ctx *c) { return c->scol16 && c->spmv_idx16; }
 
 static int spmv_grid(const lpm_ctx *c) {
-    if (c->spmv_coop && !c->sym_ok) return blocks_for(c->nslice, c->spmv_spw);
+    if (c->spmv_coop) return blocks_for(c->nslice, c->spmv_spw);
     return blocks_for(c->nslice, c->spmv_wpb);
 }
 
 template <typename T, int D, int VP, bool FUSED, int UNR, int NWS>
 static void spmv_coop_go(lpm_ctx *c, const void *in, void *out, double *part) {
     typedef Vec<T, VP> V;
-    if (spmv_idx16(c))
+    if (spmv_idx16(c) && c->spmv_nt)
+        hipLaunchKernelGGL((k_spmv_coop<T, D, VP, FUSED, UNR, NWS, int16_t, true>), dim3(spmv_grid(c)), dim3(64 * NWS), 0,
+                           c->stream, c->n, c->nslice, c->spmv_spw, c->sp, c->cptr, (const int16_t *)c->scol16,
+                           (const T *)c->aval, (const V *)in, (V *)out, part, c->partial_cap, (const double *)c->scal,
+                           c->spmv_remap);
+    else if (spmv_idx16(c))
         hipLaunchKernelGGL((k_spmv_coop<T, D, VP, FUSED, UNR, NWS, int16_t>), dim3(spmv_grid(c)), dim3(64 * NWS), 0, c->stream,
                            c->n, c->nslice, c->spmv_spw, c->sp, c->cptr, (const int16_t *)c->scol16, (const T *)c->aval,
                            (const V *)in, (V *)out, part, c->partial_cap, (const double *)c->scal, c->spmv_remap);
@@ -849,15 +632,15 @@ template <typename T, int D, int VP, bool FUSED, int UNR>
 static void spmv_go(lpm_ctx *c, const void *in, void *out, double *part) {
     typedef Vec<T, VP> V;
     const int g = spmv_grid(c), tb = 64 * c->spmv_wpb;
-    if (c->spmv_coop && !c->sym_ok) {
+    if (c->spmv_coop) {
         if (c->spmv_coop == 16) spmv_coop_go<T, D, VP, FUSED, (UNR > 1 ? 2 : 1), 16>(c, in, out, part);
         else spmv_coop_go<T, D, VP, FUSED, (UNR > 1 ? 2 : 1), 8>(c, in, out, part);
         return;
     }
-    if (c->sym_ok)
-        hipLaunchKernelGGL((k_spmv_sym<T, D, VP, FUSED, UNR>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice, c->cptr,
-                           c->dslot, c->sp_u, c->ucol, (const T *)c->uval, c->sp_l, c->lcol, c->lpos, (const V *)in, (V *)out,
-                           part, c->partial_cap, (const double *)c->scal, c->spmv_remap);
+    if (spmv_idx16(c) && c->spmv_nt)
+        hipLaunchKernelGGL((k_spmv<T, D, VP, FUSED, UNR, int16_t, true>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice,
+                           c->sp, c->cptr, (const int16_t *)c->scol16, (const T *)c->aval, (const V *)in, (V *)out, part,
+                           c->partial_cap, (const double *)c->scal, c->spmv_remap);
     else if (spmv_idx16(c))
         hipLaunchKernelGGL((k_spmv<T, D, VP, FUSED, UNR, int16_t>), dim3(g), dim3(tb), 0, c->stream, c->n, c->nslice, c->sp,
                            c->cptr, (const int16_t *)c->scol16, (const T *)c->aval, (const V *)in, (V *)out, part,
@@ -882,7 +665,7 @@ static int spmv_launch(lpm_ctx *c, bool fused, void *in, void *out) {
     default:
         fused ? spmv_go<T, D, VP, true, 2>(c, in, out, part) : spmv_go<T, D, VP, false, 2>(c, in, out, part);
     }
-    return check_launch(c, c->sym_ok ? "k_spmv_sym" : (c->spmv_coop ? "k_spmv_coop" : "k_spmv"));
+    return check_launch(c, c->spmv_coop ? "k_spmv_coop" : "k_spmv");
 }
 
 // Launch-configuration tuner: times the fused SpMV the CG runs (v_p -> v_w, delta partials;
@@ -892,18 +675,17 @@ static int spmv_tune_t(lpm_ctx *c) {
     // 4 or 8 wavefronts per workgroup: the fused SpMV leaves one partial per workgroup, which
     // every workgroup of the next vector kernel sums, so fewer workgroups keep that cheap
     // per-wavefront slices: unroll 1, 2, 4 x 4 or 8 wavefronts per workgroup; cooperative
-    // slices (full store only): 8 or 16 wavefronts per slice x unroll 1, 2 x 2 or 4 slices per
-    // workgroup; each with and without the XCD remap
+    // slices: 8 or 16 wavefronts per slice x unroll 1, 2 x 2 or 4 slices per workgroup; each with
+    // and without the XCD remap
     struct Cfg {
         int coop, unr, wpb, spw;
     };
     std::vector<Cfg> cfgs;
     for (int un : {1, 2, 4})
         for (int wp : {4, 8}) cfgs.push_back({0, un, wp, 1});
-    if (!c->sym_ok)
-        for (int nw : {8, 16})
-            for (int un : {1, 2})
-                for (int sw : {2, 4}) cfgs.push_back({nw, un, c->spmv_wpb, sw});
+    for (int nw : {8, 16})
+        for (int un : {1, 2})
+            for (int sw : {2, 4}) cfgs.push_back({nw, un, c->spmv_wpb, sw});
     hipEvent_t a, b;
     LPM_HIP(c, timing_event(&a));
     LPM_HIP(c, timing_event(&b));
@@ -949,98 +731,13 @@ static int spmv_tune_t(lpm_ctx *c) {
 }
 
 static int spmv_tune(lpm_ctx *c) {
-    const int key = (c->sym_ok ? 2 : 0) + c->precision;
+    const int key = c->precision;
     if (!c->spmv_tune || c->tune_key == key) return LPM_OK;
     int rc;
     if (c->precision == 0) rc = c->dim == 3 ? spmv_tune_t<float, 3, 4>(c) : spmv_tune_t<float, 2, 2>(c);
     else rc = c->dim == 3 ? spmv_tune_t<double, 3, 4>(c) : spmv_tune_t<double, 2, 2>(c);
     if (rc) return rc;
     c->tune_key = key;
-    return LPM_OK;
-}
-
-// Index structure of the half store (once per context; the pattern never changes).
-static int sym_setup(lpm_ctx *c) {
-    if (c->dslot) return LPM_OK;
-    const int64_t n = c->n;
-    LPM_HIP(c, hipMalloc(&c->dslot, sizeof(int32_t) * (n > 0 ? n : 1)));
-    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
-    hipLaunchKernelGGL(k_dslot, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->cptr, c->cidx, c->dslot, c->iflag);
-    int rc = check_launch(c, "k_dslot");
-    if (rc) return rc;
-    std::vector<int32_t> ds(n), cp(n + 1);
-    LPM_HIP(c, hipMemcpyAsync(ds.data(), c->dslot, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
-    LPM_HIP(c, hipMemcpyAsync(cp.data(), c->cptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
-    LPM_HIP(c, hipMemcpyAsync(c->h_iflag, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    LPM_HIP(c, hipStreamSynchronize(c->stream));
-    if (c->h_iflag[0]) return lpm_fail(c, LPM_E_TOPO, "a pattern row lacks its diagonal block");
-    std::vector<int64_t> su(c->nslice + 1, 0), sl(c->nslice + 1, 0);
-    for (int64_t k = 0; k < c->nslice; ++k) {
-        int wu = 0, wl = 0;
-        for (int64_t i = 64 * k; i < std::min<int64_t>(n, 64 * k + 64); ++i) {
-            wu = std::max(wu, cp[i + 1] - cp[i] - ds[i]);
-            wl = std::max(wl, ds[i]);
-        }
-        su[k + 1] = su[k] + 64LL * wu;
-        sl[k + 1] = sl[k] + 64LL * wl;
-    }
-    c->nslots_u = su[c->nslice];
-    c->nslots_l = sl[c->nslice];
-    if (c->nslots_u >= 0x7fffffffLL) return lpm_fail(c, LPM_E_LIMIT, "upper store exceeds 32-bit block positions");
-    LPM_HIP(c, hipMalloc(&c->sp_u, sizeof(int64_t) * (c->nslice + 1)));
-    LPM_HIP(c, hipMalloc(&c->sp_l, sizeof(int64_t) * (c->nslice + 1)));
-    LPM_HIP(c, hipMalloc(&c->ucol, sizeof(int32_t) * std::max<int64_t>(c->nslots_u, 1)));
-    LPM_HIP(c, hipMalloc(&c->lcol, sizeof(int32_t) * std::max<int64_t>(c->nslots_l, 1)));
-    LPM_HIP(c, hipMalloc(&c->lpos, sizeof(int32_t) * std::max<int64_t>(c->nslots_l, 1)));
-    LPM_HIP(c, hipMemcpyAsync(c->sp_u, su.data(), sizeof(int64_t) * (c->nslice + 1), hipMemcpyHostToDevice, c->stream));
-    LPM_HIP(c, hipMemcpyAsync(c->sp_l, sl.data(), sizeof(int64_t) * (c->nslice + 1), hipMemcpyHostToDevice, c->stream));
-    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
-    hipLaunchKernelGGL(k_sym_cols, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->cptr, c->cidx, c->dslot, c->sp_u,
-                       c->sp_l, c->ucol, c->lcol, c->lpos, c->iflag);
-    rc = check_launch(c, "k_sym_cols");
-    if (rc) return rc;
-    LPM_HIP(c, hipMemcpyAsync(c->h_iflag, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    LPM_HIP(c, hipStreamSynchronize(c->stream));
-    if (c->h_iflag[0]) return lpm_fail(c, LPM_E_TOPO, "pattern is not structurally symmetric");
-    return LPM_OK;
-}
-
-template <typename T, int D>
-static int sym_pack_t(lpm_ctx *c) {
-    hipLaunchKernelGGL((k_sym_pack<T, D>), dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->dslot,
-                       c->tslot, c->sp, (const T *)c->aval, c->sp_u, c->ucol, (T *)c->uval, c->iflag);
-    return check_launch(c, "k_sym_pack");
-}
-
-// Refresh the half store after the matrix changed; c->sym_ok says whether the SpMV may use it.
-int sym_prepare(lpm_ctx *c) {
-    if (!c->spmv_sym) {
-        c->sym_ok = false;
-        return LPM_OK;
-    }
-    if (c->sym_version == c->k_version && c->uval_prec == c->precision) return LPM_OK;
-    c->sym_ok = false;
-    int rc = sym_setup(c);
-    if (rc) return rc;
-    if (!c->tslot && (rc = launch_tslot(c))) return rc;
-    const size_t es = c->precision == 0 ? sizeof(float) : sizeof(double);
-    if (c->uval && c->uval_prec != c->precision) {
-        hipFree(c->uval);
-        c->uval = nullptr;
-    }
-    if (!c->uval) {
-        LPM_HIP(c, hipMalloc(&c->uval, es * c->dim * c->dim * std::max<int64_t>(c->nslots_u, 1)));
-        c->uval_prec = c->precision;
-    }
-    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
-    if (c->precision == 0) rc = c->dim == 3 ? sym_pack_t<float, 3>(c) : sym_pack_t<float, 2>(c);
-    else rc = c->dim == 3 ? sym_pack_t<double, 3>(c) : sym_pack_t<double, 2>(c);
-    if (rc) return rc;
-    LPM_HIP(c, hipMemcpyAsync(c->h_iflag, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    LPM_HIP(c, hipStreamSynchronize(c->stream));
-    c->sym_ok = c->h_iflag[0] == 0;
-    if (!c->sym_ok) c->st_sym_rejects++;
-    c->sym_version = c->k_version;
     return LPM_OK;
 }
 
@@ -1052,9 +749,8 @@ static int cg_graph_prepare(lpm_ctx *c, int gv, int gs, int batch, F issue) {
     const std::vector<uintptr_t> key = {
         (uintptr_t)c->n, (uintptr_t)c->dim, (uintptr_t)c->precision, (uintptr_t)c->precond, (uintptr_t)gv, (uintptr_t)gs,
         (uintptr_t)batch, (uintptr_t)c->nslice, (uintptr_t)c->spmv_unr, (uintptr_t)c->spmv_wpb, (uintptr_t)c->spmv_coop,
-        (uintptr_t)c->spmv_spw, (uintptr_t)c->spmv_remap, (uintptr_t)c->sym_ok, (uintptr_t)c->partial_cap,
-        (uintptr_t)c->sp, (uintptr_t)c->cptr, (uintptr_t)c->scol, (uintptr_t)c->scol16, (uintptr_t)c->spmv_idx16, (uintptr_t)c->aval, (uintptr_t)c->dslot, (uintptr_t)c->sp_u,
-        (uintptr_t)c->ucol, (uintptr_t)c->uval, (uintptr_t)c->sp_l, (uintptr_t)c->lcol, (uintptr_t)c->lpos,
+        (uintptr_t)c->spmv_spw, (uintptr_t)c->spmv_remap, (uintptr_t)c->spmv_nt, (uintptr_t)c->partial_cap,
+        (uintptr_t)c->sp, (uintptr_t)c->cptr, (uintptr_t)c->scol, (uintptr_t)c->scol16, (uintptr_t)c->spmv_idx16, (uintptr_t)c->aval,
         (uintptr_t)c->v_x, (uintptr_t)c->v_r, (uintptr_t)c->v_u, (uintptr_t)c->v_p, (uintptr_t)c->v_s, (uintptr_t)c->v_w,
         (uintptr_t)c->invdiag, (uintptr_t)c->partial, (uintptr_t)c->scal, (uintptr_t)c->agg_of, (uintptr_t)c->agg_d,
         (uintptr_t)c->agg_ptr, (uintptr_t)c->agg_idx, (uintptr_t)c->dofmask, (uintptr_t)c->cz, (uintptr_t)c->cy,
@@ -1087,8 +783,8 @@ template <typename T, int D, int VP>
 static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     typedef Vec<T, VP> V;
     const int64_t n = c->n;
-    // vector kernels: grid-stride over at most 1024 workgroups (4 per CU; few partials to sum)
-    const int gv = std::max(1, std::min(blocks_for(n, 256), c->cg_vec_wg)), gs = spmv_grid(c);
+    // vector kernels: grid-stride over at most CG_VEC_WG workgroups (4 per CU: few partials to sum)
+    const int gv = std::max(1, std::min(blocks_for(n, 256), CG_VEC_WG)), gs = spmv_grid(c);
     if (gv > c->partial_cap || gs > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "CG grid exceeds partial buffer");
     const int64_t maxit = c->cg_maxit < 0 ? 2 * n * c->dim : c->cg_maxit;
     hipLaunchKernelGGL((k_cg_init<T, D, VP>), dim3(gv), dim3(256), 0, c->stream, n, c->residual, (const T *)c->invdiag,
@@ -1113,13 +809,6 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     // the device flag turns the rest of a batch into no-ops once the solve has stopped
     const int batch = ((c->cg_batch > 0 ? c->cg_batch : 16) + 1) & ~1;
     const bool graph = c->cg_graph != 0;
-    // without graph replay, the first batch can cover the iterations the previous solve needed
-    // (its updates, the iteration that detects convergence and one more, rounded up to even):
-    // solves of similar length then stop in one batch, with one host round trip and few no-op
-    // iterations behind the converging one
-    int first = batch;
-    if (!graph && c->cg_batch_predict && c->cg_last_updates > 0)
-        first = (int)((std::min<int64_t>(c->cg_last_updates + 2, 512) + 1) & ~(int64_t)1);
     // timing: events around the SpMV of every `timing`-th iteration only (an event record costs
     // a few microseconds of idle GPU time, so timing every launch would slow the solve); not in
     // graph mode, whose batches are replayed whole. One pair per timed iteration of a batch.
@@ -1130,7 +819,7 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
         }
     } evs;
     if (c->timing && !graph) {
-        evs.e.resize(2 * (std::max(first, batch) / c->timing + 2));
+        evs.e.resize(2 * (batch / c->timing + 2));
         for (auto &e : evs.e) LPM_HIP(c, timing_event(&e));
     }
     auto timed = [&](int64_t it) { return !graph && c->timing > 0 && it % c->timing == 0; };
@@ -1164,7 +853,7 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
         return check_launch(c, "k_cg_advance");
     };
     if (graph && (rc = cg_graph_prepare(c, gv, gs, batch, [&]() { return issue(0, batch); }))) return rc;
-    for (int nb = first;; nb = batch) {
+    for (const int nb = batch;;) {
         if (graph) LPM_HIP(c, hipGraphLaunch(c->cg_exec, c->stream));
         else if ((rc = issue(it0, nb))) return rc;
         LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
@@ -1193,7 +882,6 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     const double done = c->h_scal[SC_DONE];
     // Eigen reports the updates done before the converging one (its loop breaks before i++)
     const int64_t upd = (int64_t)c->h_scal[SC_ITER];
-    c->cg_last_updates = upd;
     *iters = (int)((done == 1.0 && upd > 0) ? upd - 1 : upd);
     *error = (done == 3.0) ? 0.0 : sqrt(c->h_scal[SC_RR] / rhs2);
     *success = (done == 3.0) || (*error <= c->cg_tol);
@@ -1205,9 +893,7 @@ int cg_solve(lpm_ctx *c, int *iters, double *error, int *success) {
     int rc = launch_precond_setup(c);
     if (rc) return rc;
     if ((rc = coarse_prepare(c))) return rc;
-    if ((rc = sym_prepare(c))) return rc;
     if ((rc = spmv_tune(c))) return rc;
-    if (c->sym_ok) c->st_sym_solves++;
     if (c->precision == 0)
         rc = c->dim == 3 ? cg_t<float, 3, 4>(c, iters, error, success) : cg_t<float, 2, 2>(c, iters, error, success);
     else
@@ -1282,9 +968,8 @@ static int bench_t(lpm_ctx *c, int reps, double *ms) {
 int spmv_bench(lpm_ctx *c, int reps, double *ms) {
     if (!c->k_ready) return lpm_fail(c, LPM_E_STATE, "no assembled matrix");
     if (reps <= 0) return lpm_fail(c, LPM_E_ARG, "reps must be positive");
-    int rc = sym_prepare(c);
+    const int rc = spmv_tune(c);
     if (rc) return rc;
-    if ((rc = spmv_tune(c))) return rc;
     if (c->precision == 0) return c->dim == 3 ? bench_t<float, 3, 4>(c, reps, ms) : bench_t<float, 2, 2>(c, reps, ms);
     return c->dim == 3 ? bench_t<double, 3, 4>(c, reps, ms) : bench_t<double, 2, 2>(c, reps, ms);
 }

@@ -284,31 +284,47 @@ __host__ __device__ inline size_t asm_lds_bytes(int hops, int maxconn) {
            sizeof(int) * 2 * (size_t)hops + sizeof(int) * 4 * (size_t)(maxconn + 1);
 }
 
-template <typename T, int D>
-__global__ __launch_bounds__(256) void k_assemble(
-    int64_t n, int hops, int maxconn, int diag10_literal, int transpose, const int32_t *__restrict__ bptr,
-    const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2, const int32_t *__restrict__ cptr,
-    const int32_t *__restrict__ cidx, const int32_t *__restrict__ tslot, const int64_t *__restrict__ sp,
-    const double *__restrict__ cs, const double *__restrict__ cs_sum, const double *__restrict__ Kn,
-    const double *__restrict__ Tv, const double *__restrict__ bdamage, T *__restrict__ aval, double *__restrict__ diag64) {
-    extern __shared__ double lds[];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    char *base = (char *)lds + (size_t)wid * asm_lds_bytes(hops, maxconn);
-    AsmBond *B = (AsmBond *)base;
-    int *soff = (int *)(B + LPM_MAXB);  // 2-hop segment offsets of each bond of i (+ end)
-    int *slo = soff + LPM_MAXB + 1;     // first same-layer bond of the partner
-    AsmHop *H = (AsmHop *)(slo + LPM_MAXB + 1);
-    int *hslot = (int *)(H + hops);     // block slot each path ends on (-1: none)
-    int *hlist = hslot + hops;          // paths grouped by slot, in path order
-    int *rcol = hlist + hops;           // columns of row i
-    int *lcnt = rcol + (maxconn + 1);   // paths per slot
-    int *lbeg = lcnt + (maxconn + 1);   // start of each slot's list
-    int *lfil = lbeg + (maxconn + 1);   // fill cursor
-    const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid;
-    const bool row = i < n;
-    const int ib0 = row ? bptr[i] : 0, ibm = row ? bsplit[i] : 0, ib1 = row ? bptr[i + 1] : 0;
-    const int nbi = ib1 - ib0, tm = ibm - ib0;
-    const int c0 = row ? cptr[i] : 0, nci = row ? cptr[i + 1] - c0 : 0;
+// LDS of one row's 2-hop bookkeeping, carved from `base` (asm_lds_bytes(hops, maxconn) bytes)
+struct AsmLds {
+    AsmBond *B;   // bonds of i
+    int *soff;    // 2-hop segment offsets of each bond of i (+ end)
+    int *slo;     // first same-layer bond of the partner
+    AsmHop *H;    // the 2-hop paths, in path order
+    int *hslot;   // block slot each path ends on (-1: none)
+    int *hlist;   // paths grouped by slot, in path order
+    int *rcol;    // columns of row i
+    int *lcnt;    // paths per slot
+    int *lbeg;    // start of each slot's list
+    int *lfil;    // fill cursor
+};
+
+__device__ __forceinline__ AsmLds asm_lds(char *base, int hops, int maxconn) {
+    AsmLds s;
+    s.B = (AsmBond *)base;
+    s.soff = (int *)(s.B + LPM_MAXB);
+    s.slo = s.soff + LPM_MAXB + 1;
+    s.H = (AsmHop *)(s.slo + LPM_MAXB + 1);
+    s.hslot = (int *)(s.H + hops);
+    s.hlist = s.hslot + hops;
+    s.rcol = s.hlist + hops;
+    s.lcnt = s.rcol + (maxconn + 1);
+    s.lbeg = s.lcnt + (maxconn + 1);
+    s.lfil = s.lbeg + (maxconn + 1);
+    return s;
+}
+
+// Stages row i's bonds, its 2-hop paths and its columns in LDS and groups the paths by the
+// block they end on, each group in path order. Every thread of the workgroup calls it (it
+// synchronises the workgroup); rows past n stage nothing.
+__device__ void asm_row_lists(const AsmLds &s, int lane, int ib0, int nbi, int tm, int c0, int nci,
+                              const int32_t *__restrict__ bptr, const int32_t *__restrict__ bsplit,
+                              const int32_t *__restrict__ p2, const int32_t *__restrict__ cidx,
+                              const double *__restrict__ cs, const double *__restrict__ Kn,
+                              const double *__restrict__ Tv, const double *__restrict__ bdamage) {
+    AsmBond *B = s.B;
+    int *soff = s.soff, *slo = s.slo, *hslot = s.hslot, *hlist = s.hlist, *rcol = s.rcol;
+    int *lcnt = s.lcnt, *lbeg = s.lbeg, *lfil = s.lfil;
+    AsmHop *H = s.H;
     // bonds of i and the length of each bond's 2-hop list; the row's columns
     if (lane < nbi) {
         const int b = ib0 + lane;
@@ -391,6 +407,27 @@ __global__ __launch_bounds__(256) void k_assemble(
         }
     }
     __syncthreads();
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_assemble(
+    int64_t n, int hops, int maxconn, int diag10_literal, int transpose, const int32_t *__restrict__ bptr,
+    const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2, const int32_t *__restrict__ cptr,
+    const int32_t *__restrict__ cidx, const int32_t *__restrict__ tslot, const int64_t *__restrict__ sp,
+    const double *__restrict__ cs, const double *__restrict__ cs_sum, const double *__restrict__ Kn,
+    const double *__restrict__ Tv, const double *__restrict__ bdamage, T *__restrict__ aval, double *__restrict__ diag64) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const AsmLds s = asm_lds((char *)lds + (size_t)wid * asm_lds_bytes(hops, maxconn), hops, maxconn);
+    const AsmBond *B = s.B;
+    const int *soff = s.soff, *hlist = s.hlist, *rcol = s.rcol, *lcnt = s.lcnt, *lbeg = s.lbeg;
+    const AsmHop *H = s.H;
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid;
+    const bool row = i < n;
+    const int ib0 = row ? bptr[i] : 0, ibm = row ? bsplit[i] : 0, ib1 = row ? bptr[i + 1] : 0;
+    const int nbi = ib1 - ib0, tm = ibm - ib0;
+    const int c0 = row ? cptr[i] : 0, nci = row ? cptr[i + 1] - c0 : 0;
+    asm_row_lists(s, lane, ib0, nbi, tm, c0, nci, bptr, bsplit, p2, cidx, cs, Kn, Tv, bdamage);
     if (!row) return;
     for (int slot = lane; slot < nci; slot += 64) {
         const int j = rcol[slot];
@@ -495,6 +532,308 @@ __global__ __launch_bounds__(256) void k_assemble(
             for (int r = 0; r < D; ++r)
                 for (int c = 0; c < D; ++c) aval[sell_off(sp, j, ts, D * D, r * D + c)] = (T)K[c][r];
         }
+    }
+}
+
+// ---- planned assembly
+// The row lists above depend on the topology only. k_asm_plan records them once per pattern:
+// per row, its blocks ordered direct (a bond i->j in some layer) / 2-hop only / diagonal, and
+// for each non-diagonal block its 2-hop paths in path order as (bond of the partner, bond of i
+// it starts with). k_assemble_plan then computes one row of one block per lane, so the lanes of
+// a loop take the same branch, and sums every entry in the same order as k_assemble.
+//   pl_row[2i]   = direct blocks | 2-hop blocks << 16,  pl_row[2i+1] = paths of the row
+//   pl_slot[c]   = slot | (first bond to j in layer 0, +1) << 16 | (layer 1, +1) << 24
+//   pl_rng[c]    = first path of the block (within the row) | its path count << 16
+//   pl_b3/pl_own[i*hops + e] = path e's bond of the partner (global) / its bond of i (row rank)
+// with c = cptr[i] + the block's rank in that order.
+__host__ __device__ inline size_t asm_plan_build_lds_bytes(int hops, int maxconn) {  // 16-byte multiple
+    return (asm_lds_bytes(hops, maxconn) + sizeof(int) * 3 * (size_t)(maxconn + 1) + 15) & ~(size_t)15;
+}
+
+__global__ __launch_bounds__(256) void k_asm_plan(int64_t n, int hops, int maxconn, const int32_t *__restrict__ bptr,
+                                                   const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2,
+                                                   const int32_t *__restrict__ cptr, const int32_t *__restrict__ cidx,
+                                                   const double *__restrict__ cs, const double *__restrict__ Kn,
+                                                   const double *__restrict__ Tv, const double *__restrict__ bdamage,
+                                                   int32_t *__restrict__ pl_row, int32_t *__restrict__ pl_slot,
+                                                   int32_t *__restrict__ pl_rng, int32_t *__restrict__ pl_b3,
+                                                   uint8_t *__restrict__ pl_own, int32_t *dup) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    char *base = (char *)lds + (size_t)wid * asm_plan_build_lds_bytes(hops, maxconn);
+    const AsmLds s = asm_lds(base, hops, maxconn);
+    int *bjp = (int *)(base + asm_lds_bytes(hops, maxconn));
+    int *rank = bjp + (maxconn + 1);
+    int *ibeg = rank + (maxconn + 1);
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid;
+    const bool row = i < n;
+    const int ib0 = row ? bptr[i] : 0, ibm = row ? bsplit[i] : 0, ib1 = row ? bptr[i + 1] : 0;
+    const int nbi = ib1 - ib0, tm = ibm - ib0;
+    const int c0 = row ? cptr[i] : 0, nci = row ? cptr[i + 1] - c0 : 0;
+    asm_row_lists(s, lane, ib0, nbi, tm, c0, nci, bptr, bsplit, p2, cidx, cs, Kn, Tv, bdamage);
+    // the planned kernel adds the k == j term once per layer: no partner may repeat in a layer
+    if (lane < nbi) {
+        const int lo = lane < tm ? 0 : tm;
+        for (int b = lo; b < lane; ++b)
+            if (s.B[b].p2 == s.B[lane].p2) atomicOr(dup, 1);
+    }
+    // per block: the first bond i->j of each layer (-1 for the diagonal block)
+    for (int q = lane; q < nci; q += 64) {
+        const int j = s.rcol[q];
+        int b0 = -1, b1 = -1;
+        for (int b = 0; b < tm; ++b)
+            if (b0 < 0 && s.B[b].p2 == j) b0 = b;
+        for (int b = tm; b < nbi; ++b)
+            if (b1 < 0 && s.B[b].p2 == j) b1 = b;
+        bjp[q] = (j == i) ? -1 : ((b0 + 1) | ((b1 + 1) << 8));
+    }
+    __syncthreads();
+    if (row && lane == 0) {
+        int k = 0, off = 0, nd = 0, n2 = 0;
+        for (int cls = 0; cls < 3; ++cls)
+            for (int q = 0; q < nci; ++q) {
+                const int v = bjp[q];
+                const int c = v < 0 ? 2 : (v != 0 ? 0 : 1);
+                if (c != cls) continue;
+                rank[q] = k++;
+                ibeg[q] = off;
+                if (c != 2) off += s.lcnt[q];
+                nd += c == 0;
+                n2 += c == 1;
+            }
+        pl_row[2 * i] = nd | (n2 << 16);
+        pl_row[2 * i + 1] = off;
+    }
+    __syncthreads();
+    if (!row) return;
+    for (int q = lane; q < nci; q += 64) {
+        const int v = bjp[q];
+        const int64_t p = c0 + rank[q];
+        const int cnt = v < 0 ? 0 : s.lcnt[q];
+        pl_slot[p] = q | (v < 0 ? 0 : (v << 16));
+        pl_rng[p] = ibeg[q] | (cnt << 16);
+        for (int m = 0; m < cnt; ++m) {
+            const int e = s.hlist[s.lbeg[q] + m];
+            const int t = s.H[e].owner;
+            const int64_t o = i * hops + ibeg[q] + m;
+            pl_b3[o] = s.slo[t] + (e - s.soff[t]);
+            pl_own[o] = (uint8_t)t;
+        }
+    }
+}
+
+// A 2-hop path as the planned assembly stages it: cosines of the partner's bond and its
+// (1 - bdamage) * Tv; the bond of i the path starts with is kept in a byte array beside.
+struct AsmPath {
+    double c[3], wt;
+};
+
+// LDS of one row of the planned assembly: the row's bonds (maxb of them), its paths, their owners
+__host__ __device__ inline size_t asm_plan_lds_bytes(int hops, int maxb) {
+    const size_t h = hops > 0 ? hops : 1;
+    return (sizeof(AsmBond) * (size_t)maxb + sizeof(AsmPath) * h + h + 15) & ~(size_t)15;
+}
+
+constexpr int ASM_NO_PATH = 0x7fffffff;
+
+// Row r of the off-diagonal block of row i in column j (the else-branch of k_assemble, same
+// operations in the same order). DIRECT: some layer has a bond i->j (bjl[l] >= 0). `own` is the
+// owner of path q (ASM_NO_PATH past q1), read one path ahead of its use.
+template <int D, bool DIRECT>
+__device__ __forceinline__ void asm_plan_row(const AsmBond *__restrict__ B, const AsmPath *__restrict__ H,
+                                             const uint8_t *__restrict__ Ho, int tm, int nbi, int j, int r,
+                                             const int (&bjl)[2], int q, int q1, const double *__restrict__ cs_sum,
+                                             double (&K)[D]) {
+    double Kl[2][D];
+    bool flag[2];
+    int own = q < q1 ? Ho[q] : ASM_NO_PATH;
+    for (int l = 0; l < 2; ++l) {
+        for (int c = 0; c < D; ++c) Kl[l][c] = 0.0;
+        const int lo = l == 0 ? 0 : tm, hi = l == 0 ? tm : nbi;
+        bool touched = false;
+        if (DIRECT && bjl[l] >= 0) {  // derivative.h:53-119
+            touched = true;
+            const int bj = bjl[l];
+            const double n1[3] = {B[bj].c[0], B[bj].c[1], B[bj].c[2]};
+            const double wT1 = (1 - B[bj].bd) * B[bj].tv;
+            const double S[3] = {cs_sum[6 * j + 3 * l], cs_sum[6 * j + 3 * l + 1], cs_sum[6 * j + 3 * l + 2]};
+            // the term of the bond i->j itself (k == j), added when the loop reaches b2 = bj: the
+            // layer has no other bond to j (k_asm_plan checks that), and no 2-hop path through j
+            // ends on j
+            double tA[D];
+            {
+                const double w = (1 - B[bj].bd), kn = B[bj].kn, kt = B[bj].kn + B[bj].tv, tv = B[bj].tv;
+                for (int c = 0; c < D; ++c) {
+                    const double pr = (r == c) ? kt * (n1[r] * n1[r]) : kt * n1[r > c ? r : c] * n1[r > c ? c : r];
+                    tA[c] = w * ((-kn * n1[c] + tv * S[c]) * n1[r] - pr);
+                }
+            }
+            for (int b2 = lo; b2 < hi; ++b2) {
+                const double n2r = B[b2].c[r];
+                const double wT2 = (1 - B[b2].bd) * B[b2].tv;
+                const double a = 0.5 * (wT1 + wT2);
+                const double t2 = a * n2r;
+                for (int c = 0; c < D; ++c) Kl[l][c] = (b2 == bj) ? Kl[l][c] + tA[c] : Kl[l][c] - t2 * n1[c];
+                while (own <= b2) {
+                    const double a3 = 0.5 * (H[q].wt + wT2);
+                    for (int c = 0; c < D; ++c) Kl[l][c] -= a3 * n2r * H[q].c[c];
+                    ++q;
+                    own = q < q1 ? Ho[q] : ASM_NO_PATH;
+                }
+            }
+        } else {  // derivative.h:122-139
+            while (own < hi) {
+                touched = true;
+                const int b1 = own;
+                const double a = 0.5 * (H[q].wt + (1 - B[b1].bd) * B[b1].tv);
+                for (int c = 0; c < D; ++c) Kl[l][c] -= a * B[b1].c[r] * H[q].c[c];
+                ++q;
+                own = q < q1 ? Ho[q] : ASM_NO_PATH;
+            }
+        }
+        while (own < hi) {
+            ++q;
+            own = q < q1 ? Ho[q] : ASM_NO_PATH;
+        }
+        flag[l] = touched;
+    }
+    for (int c = 0; c < D; ++c) {
+        K[c] = 0.0;
+        if (flag[0] && flag[1]) K[c] = Kl[0][c] + Kl[1][c];
+        else if (flag[0]) K[c] = Kl[0][c];
+        else if (flag[1]) K[c] = Kl[1][c];
+    }
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void asm_plan_store(T *__restrict__ aval, const int64_t *__restrict__ sp, int transpose,
+                                               int64_t i, int slot, int64_t j, int ts, int r, const double (&K)[D]) {
+    if (!transpose) {
+        for (int c = 0; c < D; ++c) aval[sell_off(sp, i, slot, D * D, r * D + c)] = (T)K[c];
+    } else {
+        for (int c = 0; c < D; ++c) aval[sell_off(sp, j, ts, D * D, c * D + r)] = (T)K[c];
+    }
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_assemble_plan(
+    int64_t n, int hops, int maxb, int diag10_literal, int transpose, const int32_t *__restrict__ bptr,
+    const int32_t *__restrict__ bsplit, const int32_t *__restrict__ p2, const int32_t *__restrict__ cptr,
+    const int32_t *__restrict__ cidx, const int32_t *__restrict__ tslot, const int64_t *__restrict__ sp,
+    const int32_t *__restrict__ pl_row, const int32_t *__restrict__ pl_slot, const int32_t *__restrict__ pl_rng,
+    const int32_t *__restrict__ pl_b3, const uint8_t *__restrict__ pl_own, const double *__restrict__ cs,
+    const double *__restrict__ cs_sum, const double *__restrict__ Kn, const double *__restrict__ Tv,
+    const double *__restrict__ bdamage, T *__restrict__ aval, double *__restrict__ diag64) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t hb = hops > 0 ? hops : 1;
+    AsmBond *B = (AsmBond *)((char *)lds + (size_t)wid * asm_plan_lds_bytes(hops, maxb));
+    AsmPath *H = (AsmPath *)(B + maxb);
+    uint8_t *Ho = (uint8_t *)(H + hb);
+    const int64_t i = xcd_major_block() * (blockDim.x >> 6) + wid;
+    const bool row = i < n;
+    // row-uniform values (one wavefront per row)
+    const int ib0 = __builtin_amdgcn_readfirstlane(row ? bptr[i] : 0);
+    const int ibm = __builtin_amdgcn_readfirstlane(row ? bsplit[i] : 0);
+    const int ib1 = __builtin_amdgcn_readfirstlane(row ? bptr[i + 1] : 0);
+    const int nbi = ib1 - ib0, tm = ibm - ib0;
+    const int c0 = __builtin_amdgcn_readfirstlane(row ? cptr[i] : 0);
+    const int nci = __builtin_amdgcn_readfirstlane(row ? cptr[i + 1] : 0) - c0;
+    const int hdr = __builtin_amdgcn_readfirstlane(row ? pl_row[2 * i] : 0);
+    const int E = __builtin_amdgcn_readfirstlane(row ? pl_row[2 * i + 1] : 0);
+    const int nd = hdr & 0xffff, n2 = hdr >> 16;
+    if (lane < nbi) {
+        const int b = ib0 + lane;
+        AsmBond r;
+        r.c[0] = cs[3 * b];
+        r.c[1] = cs[3 * b + 1];
+        r.c[2] = cs[3 * b + 2];
+        r.tv = Tv[b];
+        r.kn = Kn[b];
+        r.bd = bdamage[b];
+        r.p2 = p2[b];
+        r.pad = 0;
+        B[lane] = r;
+    }
+    // the row's paths, four per lane per round: every load of a round is issued before its stores
+    const int64_t pb = i * hops;
+    for (int e0 = 0; e0 < E; e0 += 256) {
+        int b3[4], ow[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + 64 * u + lane;
+            b3[u] = e < E ? pl_b3[pb + e] : -1;
+            ow[u] = e < E ? pl_own[pb + e] : 0;
+        }
+        AsmPath h[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (b3[u] >= 0) {
+                h[u].c[0] = cs[3 * (int64_t)b3[u]];
+                h[u].c[1] = cs[3 * (int64_t)b3[u] + 1];
+                h[u].c[2] = cs[3 * (int64_t)b3[u] + 2];
+                h[u].wt = (1 - bdamage[b3[u]]) * Tv[b3[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + 64 * u + lane;
+            if (b3[u] >= 0) {
+                H[e] = h[u];
+                Ho[e] = (uint8_t)ow[u];
+            }
+        }
+    }
+    __syncthreads();
+    if (!row) return;
+    // blocks with a bond i->j, then blocks reached by 2-hop paths only: lane = (block, row r)
+    for (int t = lane; t < nd * D; t += 64) {
+        const int k = t / D, r = t - k * D;
+        const int ps = pl_slot[c0 + k], rg = pl_rng[c0 + k];
+        const int slot = ps & 0xffff;
+        const int bjl[2] = {((ps >> 16) & 0xff) - 1, ((ps >> 24) & 0xff) - 1};
+        const int j = cidx[c0 + slot];
+        const int ts = transpose ? tslot[c0 + slot] : 0;
+        double K[D];
+        asm_plan_row<D, true>(B, H, Ho, tm, nbi, j, r, bjl, rg & 0xffff, (rg & 0xffff) + (rg >> 16), cs_sum, K);
+        asm_plan_store<T, D>(aval, sp, transpose, i, slot, j, ts, r, K);
+    }
+    for (int t = lane; t < n2 * D; t += 64) {
+        const int k = nd + t / D, r = t - (t / D) * D;
+        const int ps = pl_slot[c0 + k], rg = pl_rng[c0 + k];
+        const int slot = ps & 0xffff;
+        const int bjl[2] = {-1, -1};
+        const int j = cidx[c0 + slot];
+        const int ts = transpose ? tslot[c0 + slot] : 0;
+        double K[D];
+        asm_plan_row<D, false>(B, H, Ho, tm, nbi, j, r, bjl, rg & 0xffff, (rg & 0xffff) + (rg >> 16), cs_sum, K);
+        asm_plan_store<T, D>(aval, sp, transpose, i, slot, j, ts, r, K);
+    }
+    // the diagonal block: one lane per upper entry (a, b), summed over both layers' bonds
+    constexpr int NU = D == 3 ? 6 : 3;
+    if (nd + n2 < nci && lane < NU) {
+        const int slot = pl_slot[c0 + nd + n2] & 0xffff;
+        const int a = D == 3 ? (lane == 0 || lane == 1 || lane == 3 ? 0 : (lane == 2 || lane == 4 ? 1 : 2))
+                             : (lane == 2 ? 1 : 0);
+        const int b = D == 3 ? (lane == 0 ? 0 : (lane == 1 || lane == 2 ? 1 : 2)) : (lane == 0 ? 0 : 1);
+        double K = 0.0;
+        for (int l = 0; l < 2; ++l) {
+            const int lo = l == 0 ? 0 : tm, hi = l == 0 ? tm : nbi;
+            const double Sb = cs_sum[6 * i + 3 * l + b];
+            for (int q = lo; q < hi; ++q) {
+                const double w = (1 - B[q].bd), kn = B[q].kn, tv = B[q].tv;
+                const double ca = B[q].c[a], cb = B[q].c[b];
+                if (a == b) K += w * ((kn * cb + tv * Sb) * ca + (kn + tv) * (ca * ca));
+                else K += w * ((kn * cb + tv * Sb) * ca + (kn + tv) * cb * ca);
+            }
+        }
+        const double Kba = (a == 0 && b == 1 && diag10_literal) ? 0.0 : K;  // K[b][a]
+        const int ts = transpose ? tslot[c0 + slot] : slot;
+        // K[a][b] goes to (a, b), or to (b, a) of the transposed store; K[b][a] the other way round
+        aval[sell_off(sp, i, ts, D * D, transpose ? b * D + a : a * D + b)] = (T)K;
+        if (a != b) aval[sell_off(sp, i, ts, D * D, transpose ? a * D + b : b * D + a)] = (T)Kba;
+        else diag64[(int64_t)D * i + a] = K;
     }
 }
 
@@ -1202,8 +1541,76 @@ int launch_spec_scale(lpm_ctx *c, double *v) {
     return check_launch(c, "k_spec_scale");
 }
 
+static int read_flag(lpm_ctx *c, int *any);
+
+// Builds the assembly plan once per pattern (k_asm_plan). Returns false, with nothing built,
+// where the plan's 16-bit fields cannot hold the pattern.
+static int asm_plan_build(lpm_ctx *c, bool *ok) {
+    *ok = false;
+    if (c->asm_hops > 0xffff || c->maxconn > 0xffff || c->asm_unplanned) return LPM_OK;
+    if (c->pl_row) {
+        *ok = true;
+        return LPM_OK;
+    }
+    const int64_t hops = std::max(1, c->asm_hops);
+    LPM_HIP(c, hipMalloc(&c->pl_row, sizeof(int32_t) * 2 * std::max<int64_t>(1, c->n)));
+    LPM_HIP(c, hipMalloc(&c->pl_slot, sizeof(int32_t) * std::max<int64_t>(1, c->nconn)));
+    LPM_HIP(c, hipMalloc(&c->pl_rng, sizeof(int32_t) * std::max<int64_t>(1, c->nconn)));
+    LPM_HIP(c, hipMalloc(&c->pl_b3, sizeof(int32_t) * hops * std::max<int64_t>(1, c->n)));
+    LPM_HIP(c, hipMalloc(&c->pl_own, sizeof(uint8_t) * hops * std::max<int64_t>(1, c->n)));
+    const size_t per = asm_plan_build_lds_bytes(c->asm_hops, c->maxconn);
+    if (per > 64 * 1024) return lpm_fail(c, LPM_E_LIMIT, "assembly plan row exceeds 64 KB of LDS");
+    const int W = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per));
+    LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+    if (c->n > 0)
+        hipLaunchKernelGGL(k_asm_plan, dim3(blocks_for(c->n, W)), dim3(64 * W), per * W, c->stream, c->n, c->asm_hops,
+                           c->maxconn, c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->cs, c->Kn, c->Tv, c->bdamage,
+                           c->pl_row, c->pl_slot, c->pl_rng, c->pl_b3, c->pl_own, c->iflag);
+    int rc = check_launch(c, "k_asm_plan");
+    int dup = 0;
+    if (!rc) rc = read_flag(c, &dup);
+    if (rc) return rc;
+    if (dup) {  // a partner repeats within a layer: assemble with the row-list kernel
+        for (void *p : {(void *)c->pl_row, (void *)c->pl_slot, (void *)c->pl_rng, (void *)c->pl_b3, (void *)c->pl_own})
+            hipFree(p);
+        c->pl_row = c->pl_slot = c->pl_rng = c->pl_b3 = nullptr;
+        c->pl_own = nullptr;
+        c->asm_unplanned = true;
+        return LPM_OK;
+    }
+    *ok = true;
+    return LPM_OK;
+}
+
 int launch_assemble(lpm_ctx *c) {
     if (c->cg_transpose && !c->tslot) return lpm_fail(c, LPM_E_STATE, "transpose map missing");
+    bool planned = false;
+    if (c->asm_plan) {
+        const int rc = asm_plan_build(c, &planned);
+        if (rc) return rc;
+    }
+    if (planned) {
+        const int maxb = std::max(1, c->maxbond);
+        const size_t per = asm_plan_lds_bytes(c->asm_hops, maxb);
+        const int W = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per));
+        const int g = blocks_for(c->n, W);
+#define ASM_PLAN_LAUNCH(T, D)                                                                                          \
+    hipLaunchKernelGGL((k_assemble_plan<T, D>), dim3(g), dim3(64 * W), per * W, c->stream, c->n, c->asm_hops, maxb,       \
+                       c->diag10_literal, c->cg_transpose, c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp,   \
+                       c->pl_row, c->pl_slot, c->pl_rng, c->pl_b3, c->pl_own, c->cs, c->cs_sum, c->Kn, c->Tv,             \
+                       c->bdamage, (T *)c->aval, c->diag64)
+        if (c->n > 0) {
+            if (c->precision == 0) {
+                if (c->dim == 3) ASM_PLAN_LAUNCH(float, 3);
+                else ASM_PLAN_LAUNCH(float, 2);
+            } else {
+                if (c->dim == 3) ASM_PLAN_LAUNCH(double, 3);
+                else ASM_PLAN_LAUNCH(double, 2);
+            }
+        }
+#undef ASM_PLAN_LAUNCH
+        return check_launch(c, "k_assemble_plan");
+    }
     // wavefronts per workgroup: as many as fit 64 KB of LDS, at most 4
     const size_t per = asm_lds_bytes(c->asm_hops, c->maxconn);
     const int W = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per));

@@ -10,6 +10,7 @@
 #include "lpm_mi355x.h"
 
 // Compiled limits, checked on the host before any launch.
+#define CG_VEC_WG 1024 // most workgroups of the CG vector kernels (grid-stride beyond)
 #define LPM_MAXB 32    // bonds per particle (hex 12, SC/FCC 18, BCC 14)
 #define LPM_MAXC 512   // 2-hop candidates per particle before de-duplication
 #define LPM_NSV 7      // state variables per particle (J2 uses 7)
@@ -52,14 +53,16 @@ struct lpm_ctx {
     int maxconn = 0, maxbond = 0;
     int gp_bonds = 0;  // most bonds of any group of 32 consecutive particles (LDS of the bond kernels)
     int asm_hops = 0;  // most 2-hop bond records of any particle (LDS of the assembly)
+    // Assembly plan (mode "asm_plan", k_asm_plan): per row, its blocks by kind and each block's
+    // 2-hop paths; built from the pattern on the first planned assembly
+    int asm_plan = 1;
+    bool asm_unplanned = false;       // the pattern cannot be planned (a partner repeats in a layer)
+    int32_t *pl_row = nullptr, *pl_slot = nullptr, *pl_rng = nullptr, *pl_b3 = nullptr;
+    uint8_t *pl_own = nullptr;
     // modes
     int diag10_literal = 1, cg_transpose = 1, dirichlet_static = 0, rollback_pex = 0;
     int precision = 0, precond = 0, verbose = 0, max_NR_iter = 30, cg_batch = 16;
-    // mode "cg_batch_predict": without graph replay, a solve's first batch is as long as the
-    // previous solve's update count plus two (cg_last_updates, -1 before the first solve)
-    int cg_batch_predict = 0;
-    int64_t cg_last_updates = -1;
-    int cg_vec_wg = 1024;  // most workgroups of the CG vector kernels (grid-stride beyond)
+    int spmv_nt = 1;       // 1: the SpMV reads the matrix stream with non-temporal loads (0: plain loads)
     double cg_tol = 2.220446049250313e-16, tol_NR = 1e-6;
     int cg_forcing = 0;  // 1: per-Newton-iteration CG tolerance from the remaining reduction
     int64_t cg_maxit = -1;
@@ -72,18 +75,7 @@ struct lpm_ctx {
     int16_t *scol16 = nullptr;        // the same columns as offsets j - i when all fit in int16 (else null)
     int spmv_idx16 = 1;               // mode: the SpMV reads scol16 when it exists
     int64_t nslice = 0, nslots = 0;
-    // Symmetric half of the CG matrix, used by the SpMV when the stored matrix is symmetric:
-    // upper SELL-64 (blocks j >= i of row i, the diagonal first) holding the values, and a
-    // lower SELL-64 list (blocks j < i) of (column, position of block (j, i) in the upper store)
-    int spmv_sym = 0;                 // mode: 1 use it when the symmetry check passes, 0 never
-    int32_t *dslot = nullptr;         // slot of the diagonal block in each row of the pattern
-    int64_t *sp_u = nullptr, *sp_l = nullptr;
-    int32_t *ucol = nullptr, *lcol = nullptr, *lpos = nullptr;
-    int64_t nslots_u = 0, nslots_l = 0;
-    void *uval = nullptr;
-    int uval_prec = -1;
-    int64_t k_version = 0, sym_version = -1;  // assembly generation / generation packed
-    bool sym_ok = false;
+    int64_t k_version = 0;            // assembly generation
     // SpMV launch configuration: slots unrolled per lane, wavefronts per workgroup, XCD remap;
     // with spmv_tune the fastest of all configurations is measured on the first solve of each
     // matrix kind (store, precision) and kept (the pattern never changes)
@@ -206,7 +198,6 @@ struct lpm_ctx {
     std::vector<Group> groups;
     // statistics
     int64_t st_newton = 0, st_cg_iters = 0, st_cg_solves = 0, st_asm = 0, st_spmv = 0, st_spmv_timed = 0;
-    int64_t st_sym_solves = 0, st_sym_rejects = 0;
     double st_spmv_ms = 0, last_cg_err = 0;
     int last_cg_iters = 0, timing = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -254,11 +245,10 @@ int launch_move(lpm_ctx *c);
 int launch_scale_x(lpm_ctx *c);  // x_i *= spec_w[specimen of i]
 int spmv_bench(lpm_ctx *c, int reps, double *ms);
 int launch_precond_setup(lpm_ctx *c);
-// precond 1 and 3 smooth with the inverse 3x3 (2x2) diagonal blocks, 0 and 2 with the inverse
-// diagonal; 2 and 3 add the coarse correction
-static inline bool block_jacobi(const lpm_ctx *c) { return c->precond == 1 || c->precond == 3; }
-static inline bool two_level(const lpm_ctx *c) { return c->precond == 2 || c->precond == 3; }
-int sym_prepare(lpm_ctx *c);
+// precond 1 smooths with the inverse 3x3 (2x2) diagonal blocks, 0 and 2 with the inverse
+// diagonal; 2 adds the coarse correction
+static inline bool block_jacobi(const lpm_ctx *c) { return c->precond == 1; }
+static inline bool two_level(const lpm_ctx *c) { return c->precond == 2; }
 // ---- two-level preconditioner (kernels_coarse.hip)
 int coarse_prepare(lpm_ctx *c);
 int launch_coarse_apply(lpm_ctx *c);  // cz = P^T r, cy = Ac^-1 cz (skipped once the CG is done)

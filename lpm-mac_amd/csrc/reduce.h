@@ -37,6 +37,14 @@ __device__ __forceinline__ int64_t sell_off(const int64_t *sp, int64_t i, int c,
     return (int64_t)EB * sp[i >> 6] + ((int64_t)c * EB + e) * 64 + (i & 63);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b to XCD b % 8). This maps
+// them so that each XCD takes a contiguous range of logical workgroups, for any grid size:
+// XCD x gets floor(g/8) workgroups, plus one when x < g % 8.
+__device__ __forceinline__ int64_t xcd_major_block() {
+    const int64_t b = blockIdx.x, g = gridDim.x, q = g >> 3, rem = g & 7, x = b & 7;
+    return x * q + (x < rem ? x : rem) + (b >> 3);
+}
+
 // Rigid-body modes of one particle at offset d from its aggregate centre: P = [I_D | R(d)] with
 // u = t + w x d (3D, 6 modes) or u = t + theta e_z x d (2D, 3 modes); row k is zero when bit k
 // of `mask` marks DoF k as eliminated.

@@ -136,6 +136,7 @@ struct Sys {
     int rollback_pex = 0;    // Pex is not rolled back on a step restart
     int stiffness_fd = 0;    // StiffnessMode::FiniteDifference
     int fd_exact = 0;        // FD with exact restores and a fresh base force (not the reference)
+    int cg_forcing = 0;      // inexact Newton: per-iteration CG tolerance (not the reference)
     double cg_tol = 2.220446049250313e-16;  // Eigen default tolerance (epsilon)
     long cg_maxit = -1;                     // -1 => 2 * ndof (Eigen default)
     int max_NR_iter = 30;
@@ -799,17 +800,19 @@ static void update_stiffness_disp_bc(Sys &s) {
 // (row-major) view, so A = K^T when cg_transpose is set (solver.h:195).
 static int cg_solve(Sys &s) {
     const int nd = s.dim * s.N;
-    // A in CSR
-    std::vector<int> ar, ac; std::vector<double> av;
+    // A in CSR: K as stored, or its transpose built here
+    std::vector<int> art, act; std::vector<double> avt;
     if (s.cg_transpose) {
-        ar.assign(nd + 1, 0);
-        for (size_t p = 0; p < s.kcol.size(); ++p) ar[s.kcol[p] + 1]++;
-        for (int R = 0; R < nd; ++R) ar[R + 1] += ar[R];
-        ac.resize(s.kcol.size()); av.resize(s.kcol.size());
-        std::vector<int> fill(ar.begin(), ar.end() - 1);
+        art.assign(nd + 1, 0);
+        for (size_t p = 0; p < s.kcol.size(); ++p) art[s.kcol[p] + 1]++;
+        for (int R = 0; R < nd; ++R) art[R + 1] += art[R];
+        act.resize(s.kcol.size()); avt.resize(s.kcol.size());
+        std::vector<int> fill(art.begin(), art.end() - 1);
         for (int R = 0; R < nd; ++R)
-            for (int p = s.krow[R]; p < s.krow[R + 1]; ++p) { int q = fill[s.kcol[p]]++; ac[q] = R; av[q] = s.kval[p]; }
-    } else { ar = s.krow; ac = s.kcol; av = s.kval; }
+            for (int p = s.krow[R]; p < s.krow[R + 1]; ++p) { int q = fill[s.kcol[p]]++; act[q] = R; avt[q] = s.kval[p]; }
+    }
+    const std::vector<int> &ar = s.cg_transpose ? art : s.krow, &ac = s.cg_transpose ? act : s.kcol;
+    const std::vector<double> &av = s.cg_transpose ? avt : s.kval;
     std::vector<double> invdiag(nd, 1.0);
     for (int R = 0; R < nd; ++R)
         for (int p = ar[R]; p < ar[R + 1]; ++p)
@@ -884,7 +887,11 @@ static int newton_iteration(Sys &s) {
     int ni = 0;
     while (norm_residual > s.tol_NR * tol_multiplier) {
         if (++ni > s.max_NR_iter) { s.last_newton = s.max_NR_iter; return s.max_NR_iter; }
+        const double cg_tol_set = s.cg_tol;
+        if (s.cg_forcing)  // the reduction still needed to pass the Newton test (factor 2 margin), in [cg_tol, 0.1]
+            s.cg_tol = std::max(cg_tol_set, std::min(0.1, 0.5 * s.tol_NR * tol_multiplier / norm_residual));
         solve_linear_system(s);
+        s.cg_tol = cg_tol_set;
         update_geometry(s);
         update_force_state(s);
         update_rr(s);
@@ -1513,6 +1520,7 @@ int ora_set_mode(void *p, const char *what, double v) {
     else if (w == "rollback_pex") s.rollback_pex = (int)v;
     else if (w == "stiffness_fd") s.stiffness_fd = (int)v;
     else if (w == "fd_exact") s.fd_exact = (int)v;
+    else if (w == "cg_forcing") s.cg_forcing = (int)v;
     else if (w == "cg_tol") s.cg_tol = v;
     else if (w == "cg_maxit") s.cg_maxit = (long)v;
     else if (w == "max_NR_iter") s.max_NR_iter = (int)v;

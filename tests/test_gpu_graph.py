@@ -19,7 +19,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-def _sc_run(graph, precond, precisions, batch_predict=1, timing=0):
+def _sc_run(graph, precond, precisions, timing=0):
     xyz, box = sc_cube(10)
     g = G.Context(G.SC3D, 0.5, G.ELASTIC, xyz, bonds="device")
     g.set_timing(timing)
@@ -32,7 +32,6 @@ def _sc_run(graph, precond, precisions, batch_predict=1, timing=0):
     for prec in precisions:
         g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=prec, precond=precond, cg_forcing=1,
                cg_tol=1e-3 if prec == 0 else 1e-10, tol_NR=1e-10, max_NR_iter=40, cg_maxit=20000, cg_graph=graph,
-               cg_batch_predict=batch_predict,
                # a fixed SpMV launch (the tuner's choice depends on timings): cooperative slices
                # with the coarse space, one wavefront per slice without
                spmv_tune=0, spmv_unr=2, spmv_wpb=4, spmv_coop=16 if precond == 2 else 0, spmv_spw=2)
@@ -50,21 +49,6 @@ def test_cg_graph_replays_match_direct_launches(precond):
     say("   (cg iterations, newton iterations, SpMVs that did work) direct", cd, "graph", cg)
     assert cd == cg
     for a, b in zip(direct, graph):
-        np.testing.assert_array_equal(a, b)
-
-
-@pytest.mark.parametrize("precond", [0, 2])
-def test_predicted_first_batch_matches_fixed_batches(precond):
-    """A first CG batch sized from the previous solve (cg_batch_predict, with the sampled SpMV
-    timing on) stops every solve at the same iteration as fixed batches of 16: the same bits,
-    iteration counts and count of SpMVs that did work."""
-    precisions = [0, 0, 0, 1, 0]
-    say("[gpu] predicted first batch vs fixed batches, SC 10^3, precond", precond)
-    fixed, cf = _sc_run(0, precond, precisions, batch_predict=0, timing=16)
-    pred, cp = _sc_run(0, precond, precisions, batch_predict=1, timing=16)
-    say("   (cg iterations, newton iterations, SpMVs that did work) fixed", cf, "predicted", cp)
-    assert cf == cp
-    for a, b in zip(fixed, pred):
         np.testing.assert_array_equal(a, b)
 
 

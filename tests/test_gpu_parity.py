@@ -210,14 +210,17 @@ def test_assembly_bitwise(lat, transpose, diag10):
     K = sp.csr_matrix((v, c, r), shape=(len(r) - 1, len(r) - 1))
     A = (K.T if transpose else K).tocsr()
     A.sort_indices()
-    for prec in (1, 0):
-        p.g.mode(precision=prec)
-        p.g.calc_stiffness()
-        gr, gc, gv = p.g.matrix()
-        np.testing.assert_array_equal(gr, A.indptr)
-        np.testing.assert_array_equal(gc, A.indices)
-        exp = A.data if prec == 1 else A.data.astype(np.float32).astype(np.float64)
-        np.testing.assert_array_equal(gv, exp)
+    for plan in (1, 0):  # the planned kernel and the row-list kernel
+        p.g.mode(asm_plan=plan)
+        for prec in (1, 0):
+            p.g.mode(precision=prec)
+            p.g.calc_stiffness()
+            gr, gc, gv = p.g.matrix()
+            np.testing.assert_array_equal(gr, A.indptr)
+            np.testing.assert_array_equal(gc, A.indices)
+            exp = A.data if prec == 1 else A.data.astype(np.float32).astype(np.float64)
+            np.testing.assert_array_equal(gv, exp)
+    p.g.mode(asm_plan=1)  # its diagonal feeds the Dirichlet value ||diag K|| below
     # Dirichlet elimination (stiffness.h:265-294)
     cons = np.zeros(3 * len(xyz), dtype=np.int32)
     cons[[0, 1, 5, 9, 3 * 7 + 1]] = 1
@@ -237,6 +240,47 @@ def test_assembly_bitwise(lat, transpose, diag10):
     np.testing.assert_array_equal(gc, A.indices)
     d = np.abs(gv - A.data)
     assert d.max() <= 1e-12 * np.abs(A.data).max(), d.max()
+
+
+@pytest.mark.parametrize("lat,r,box", [(O.SQUARE2D, 0.5, [0, 9, 0, 9, 0, 0]), (O.HEXAGON2D, 0.5, [0, 9, 0, 9, 0, 0]),
+                                       (O.SC3D, 0.5, [0, 5, 0, 5, 0, 5]), (O.FCC3D, 0.5, [0, 4, 0, 4, 0, 4]),
+                                       (O.BCC3D, 0.5, [0, 4, 0, 4, 0, 4])])
+def test_planned_assembly_matches_row_list_kernel(lat, r, box):
+    """The planned assembly (blocks grouped by kind, one block row per lane) against the
+    row-list kernel, which the test above pins to the oracle: every lattice, a deformed state
+    with broken and partly damaged bonds, both stores and precisions, bit for bit."""
+    import lpmgpu as G
+
+    say("[gpu] planned assembly", lat)
+    if lat == O.BCC3D:  # cubic cells of edge 4r/sqrt(3) with their body centres
+        a = 4 * r / np.sqrt(3.0)
+        c = np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(-1, 3) * a
+        xyz = np.concatenate([c, c + 0.5 * a])
+    else:
+        xyz = G.generate_lattice(lat, box, r)
+    g = G.Context(lat, r, O.ELASTIC, xyz)
+    g.set_elastic(0.0, lat in (O.SQUARE2D, O.HEXAGON2D), 200e3, 0.3, 1.0, elmap_literal=0)
+    rng = np.random.default_rng(lat)
+    x = xyz + 1e-2 * r * rng.standard_normal(xyz.shape)
+    if lat in (O.SQUARE2D, O.HEXAGON2D):
+        x[:, 2] = xyz[:, 2]
+    bd = np.where(rng.random(g.size("nbonds")) < 0.1, 1.0, 0.0)
+    bd[rng.random(len(bd)) < 0.1] = 0.4
+    g.set("xyz", x)
+    g.set("bdamage", bd)
+    g.update_geometry()
+    for transpose, diag10 in ((0, 0), (1, 1)):
+        g.mode(cg_transpose=transpose, diag10_literal=diag10)
+        for prec in (1, 0):
+            g.mode(precision=prec)
+            out = []
+            for plan in (0, 1):
+                g.mode(asm_plan=plan)
+                g.calc_stiffness()
+                out.append(g.matrix())
+            for a, b in zip(out[0], out[1]):
+                np.testing.assert_array_equal(b, a)
+    g.close()
 
 
 # ----------------------------------------------------------------- CG and one Newton step
@@ -276,58 +320,6 @@ def test_cg_fp64_matches_dense_solution():
     x_exp = xyz.copy()
     x_exp[top, 2] += 0.01
     np.testing.assert_array_equal(p.g.get("xyz").reshape(-1, 3), x_exp + d.reshape(-1, 3))
-
-
-@pytest.mark.parametrize("lattice", ["sc", "hex"])
-@pytest.mark.parametrize("precision", [0, 1])
-def test_half_store_spmv_matches_full(lattice, precision):
-    """The half (upper) store that the SpMV uses for a symmetric matrix gives the CG solution
-    of the full SELL store on a deformed lattice; a matrix that is not symmetric (diagonal
-    K[1][0] left 0, derivative.h:38) is detected and solved from the full store."""
-    say("[gpu] half store", lattice, "precision", precision)
-    if lattice == "sc":
-        p, xyz, lo, hi = _bar(n=6)
-        axes, dim = "xyz", 3
-    else:
-        xyz, box = hex_plate(14, 14)
-        p = Pair(O.HEXAGON2D, 0.5, O.ELASTIC, xyz, box)
-        p.set_elastic(0.0, False, 69e3, 0.3, 1.0)
-        lo, hi = faces(xyz, 1)
-        axes, dim = "xy", 2
-    g = p.g
-    _, ls_g = p.load(disp=[(lo, a, 0.0) for a in axes] + [(hi, axes[-1], 0.01)])
-    x_def = deformed(xyz, 5, dim=dim)
-    tol = 1e-6 if precision == 0 else 1e-13
-    sols = {}
-    for sym in (1, 0):
-        g.set("xyz", x_def)
-        g.set("disp_constraint", np.zeros(3 * len(xyz), dtype=np.int32))
-        g.apply_bc(ls_g)
-        g.mode(cg_transpose=0, diag10_literal=0, cg_tol=tol, precision=precision, cg_maxit=20000, spmv_sym=sym)
-        g.update_geometry()
-        g.update_force_state()
-        g.update_rr()
-        g.calc_stiffness()
-        g.update_stiffness_disp_bc()
-        g.reset_stats()
-        it, err = g.solve_linear_system()
-        assert err <= tol
-        assert g.stat("sym_solves") == sym and g.stat("sym_rejects") == 0
-        sols[sym] = g.get("disp")
-        # the isolated SpMV runs on the same store
-        assert g.spmv_bench(3) > 0
-    assert rel(sols[1], sols[0]) < (2e-5 if precision == 0 else 1e-11)
-    # not symmetric: rejected, full store used, same answer as the oracle's dense solve
-    g.set("xyz", x_def)
-    g.mode(diag10_literal=1, spmv_sym=1)
-    g.update_geometry()
-    g.update_force_state()
-    g.update_rr()
-    g.calc_stiffness()
-    g.update_stiffness_disp_bc()
-    g.reset_stats()
-    g.solve_linear_system()
-    assert g.stat("sym_rejects") == 1 and g.stat("sym_solves") == 0
 
 
 @pytest.mark.parametrize("precision", [0, 1])
