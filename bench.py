@@ -725,6 +725,8 @@ def fatigue_ensemble_bench(args):
         hip_sync(c1)
         t_seq += time.perf_counter() - t0
         c1.close()
+        if (s + 1) % 8 == 0:
+            print(f"one context per specimen: {s + 1}/{S} done, {t_seq:.2f} s", flush=True)
     out = {
         "metric": "specimen fatigue cycles/s, 64 uneven HCF dogbones in one context",
         "value": S * n_done / t_batch,

@@ -193,6 +193,12 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        64-row slice | 8, 16: that many wavefronts share each slice),
  *                        "spmv_spw" (consecutive slices per cooperative workgroup) -- the SpMV
  *                        launch configuration
+ *  "asm_plan"         1 (default): analytic assembly from a per-pattern plan (each row's blocks
+ *                        grouped by kind with their 2-hop paths, built on the first assembly;
+ *                        one block row per lane) | 0: the row-list kernel, which rebuilds each
+ *                        row's path lists in LDS on every assembly. Both give the same blocks bit
+ *                        for bit; patterns with a partner repeated within a layer use the row-list
+ *                        kernel either way
  *  "spmv_nt"          1 (default): the SpMV reads the matrix values and column indices with
  *                        non-temporal loads (read once per SpMV; the input vector keeps the
  *                        caches) | 0: ordinary loads

@@ -268,7 +268,7 @@ __global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslic
 // Returns alpha, beta and the stop code in coef (shared).
 __device__ void cg_scalar_step(const double *prr, const double *pg, const double *pd, int np_vec, int np_spmv,
                                double *scal, int it, double tol, double maxit, double *coef) {
-    __shared__ double red[3][4];
+    __shared__ double red[3][16];
     double t[3] = {0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < np_vec; b += blockDim.x) {
         t[0] += prr[b];
@@ -282,7 +282,10 @@ __device__ void cg_scalar_step(const double *prr, const double *pg, const double
         if (lane == 0) red[k][wid] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && scal[SC_DONE] != 0.0) {  // the solve has stopped: change nothing
+        coef[0] = coef[1] = 0.0;
+        coef[2] = scal[SC_DONE];
+    } else if (threadIdx.x == 0) {
         const int nw = blockDim.x >> 6;
         double tot[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < 3; ++k)
@@ -327,19 +330,29 @@ __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__
                                                  Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
                                                  Vec<T, VP> *__restrict__ s, const Vec<T, VP> *__restrict__ w,
                                                  double *partial, int cap, int np_vec, int np_spmv, double *scal, int k) {
-    if (scal[SC_DONE] != 0.0) return;
-    // iteration k of the batch that starts at scal[SC_IT0]
+    // iteration k of the batch that starts at scal[SC_IT0]; the scalar step reads the stop flag
     const int it = (int)scal[SC_IT0] + k;
     const double tol = scal[SC_TOL], maxit = scal[SC_MAXIT];
+    // the first particle's vectors are requested before the scalar step waits on the partials
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Vec<T, VP> p0, s0, x0, r0, u0, w0;
+    if (i0 < n) {
+        p0 = p[i0];
+        s0 = s[i0];
+        x0 = x[i0];
+        r0 = r[i0];
+        u0 = u[i0];
+        w0 = w[i0];
+    }
     __shared__ double coef[3];  // alpha, beta, done
     const double *pv = partial + (int64_t)(it & 1) * 2 * cap;
     cg_scalar_step(pv, pv + cap, partial + (int64_t)4 * cap, np_vec, np_spmv, scal, it, tol, maxit, coef);
     if (coef[2] != 0.0) return;
     double v[2] = {0.0, 0.0};
     const T alpha = (T)coef[0], beta = (T)coef[1];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        Vec<T, VP> pv = p[i], sv = s[i], xv = x[i], rv = r[i], zv;
-        const Vec<T, VP> uv = u[i], wv = w[i];
+    for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        Vec<T, VP> pv = i == i0 ? p0 : p[i], sv = i == i0 ? s0 : s[i], xv = i == i0 ? x0 : x[i], rv = i == i0 ? r0 : r[i], zv;
+        const Vec<T, VP> uv = i == i0 ? u0 : u[i], wv = i == i0 ? w0 : w[i];
         for (int k = 0; k < D; ++k) {
             pv.v[k] = uv.v[k] + beta * pv.v[k];
             sv.v[k] = wv.v[k] + beta * sv.v[k];
@@ -367,18 +380,28 @@ __global__ __launch_bounds__(256) void k_cg_vec_a(int64_t n, Vec<T, VP> *__restr
                                                    const Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
                                                    Vec<T, VP> *__restrict__ s, const Vec<T, VP> *__restrict__ w,
                                                    double *partial, int cap, int np_vec, int np_spmv, double *scal, int k) {
-    if (scal[SC_DONE] != 0.0) return;
-    const int it = (int)scal[SC_IT0] + k;
+    const int it = (int)scal[SC_IT0] + k;  // the scalar step reads the stop flag
     const double tol = scal[SC_TOL], maxit = scal[SC_MAXIT];
+    // the first particle's vectors are requested before the scalar step waits on the partials
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Vec<T, VP> p0, s0, x0, r0, u0, w0;
+    if (i0 < n) {
+        p0 = p[i0];
+        s0 = s[i0];
+        x0 = x[i0];
+        r0 = r[i0];
+        u0 = u[i0];
+        w0 = w[i0];
+    }
     __shared__ double coef[3];
     cg_scalar_step(partial + (int64_t)(it & 1) * 2 * cap, partial + cap, partial + (int64_t)4 * cap, np_vec, np_spmv, scal,
                    it, tol, maxit, coef);
     if (coef[2] != 0.0) return;
     double v[1] = {0.0};
     const T alpha = (T)coef[0], beta = (T)coef[1];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        Vec<T, VP> pv = p[i], sv = s[i], xv = x[i], rv = r[i];
-        const Vec<T, VP> uv = u[i], wv = w[i];
+    for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        Vec<T, VP> pv = i == i0 ? p0 : p[i], sv = i == i0 ? s0 : s[i], xv = i == i0 ? x0 : x[i], rv = i == i0 ? r0 : r[i];
+        const Vec<T, VP> uv = i == i0 ? u0 : u[i], wv = i == i0 ? w0 : w[i];
         for (int k = 0; k < D; ++k) {
             pv.v[k] = uv.v[k] + beta * pv.v[k];
             sv.v[k] = wv.v[k] + beta * sv.v[k];
@@ -402,16 +425,29 @@ __global__ __launch_bounds__(256) void k_cg_vec_b(int64_t n, const T *__restrict
                                                    const int32_t *__restrict__ agg_of, const double *__restrict__ agg_d,
                                                    const uint8_t *__restrict__ mask, const double *__restrict__ cy,
                                                    double *partial, int cap, const double *scal) {
-    if (scal[SC_DONE] != 0.0) return;
     constexpr int M = D == 3 ? 6 : 3;  // compile-time, so that P stays in registers
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the first particle's inputs are requested together with the stop flag
+    Vec<T, VP> r0;
+    double d0[3] = {0.0, 0.0, 0.0};
+    int a0 = 0;
+    unsigned m0 = 0;
+    if (i0 < n) {
+        r0 = r[i0];
+        a0 = agg_of[i0];
+        m0 = mask[i0];
+        for (int k = 0; k < 3; ++k) d0[k] = agg_d[3 * i0 + k];
+    }
+    if (scal[SC_DONE] != 0.0) return;
     double v[1] = {0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const Vec<T, VP> rv = r[i];
+    for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const Vec<T, VP> rv = i == i0 ? r0 : r[i];
         Vec<T, VP> zv;
         precond_apply<T, D, VP>(invd, block_pre, i, rv, zv);
         double P[3][6];
-        rigid_modes<D>(&agg_d[3 * i], mask[i], P);
-        const double *y = cy + (int64_t)agg_of[i] * M;
+        if (i == i0) rigid_modes<D>(d0, m0, P);
+        else rigid_modes<D>(&agg_d[3 * i], mask[i], P);
+        const double *y = cy + (int64_t)(i == i0 ? a0 : agg_of[i]) * M;
         for (int k = 0; k < D; ++k) {
             double t = 0.0;
             for (int m = 0; m < M; ++m) t += P[k][m] * y[m];
@@ -743,7 +779,7 @@ static int spmv_tune(lpm_ctx *c) {
 
 // Captures one batch of CG iterations into c->cg_exec unless the graph captured before is
 // still valid: the key lists every value the batch's launches take (sizes, grids, the tuned
-// SpMV configuration, the half-store switch and every buffer), so any change re-captures.
+// SpMV configuration and every buffer), so any change re-captures.
 template <typename F>
 static int cg_graph_prepare(lpm_ctx *c, int gv, int gs, int batch, F issue) {
     const std::vector<uintptr_t> key = {
@@ -753,8 +789,8 @@ static int cg_graph_prepare(lpm_ctx *c, int gv, int gs, int batch, F issue) {
         (uintptr_t)c->sp, (uintptr_t)c->cptr, (uintptr_t)c->scol, (uintptr_t)c->scol16, (uintptr_t)c->spmv_idx16, (uintptr_t)c->aval,
         (uintptr_t)c->v_x, (uintptr_t)c->v_r, (uintptr_t)c->v_u, (uintptr_t)c->v_p, (uintptr_t)c->v_s, (uintptr_t)c->v_w,
         (uintptr_t)c->invdiag, (uintptr_t)c->partial, (uintptr_t)c->scal, (uintptr_t)c->agg_of, (uintptr_t)c->agg_d,
-        (uintptr_t)c->agg_ptr, (uintptr_t)c->agg_idx, (uintptr_t)c->dofmask, (uintptr_t)c->cz, (uintptr_t)c->cy,
-        (uintptr_t)c->Ainv, (uintptr_t)c->nagg, (uintptr_t)c->coarse_S, (uintptr_t)c->nmode, (uintptr_t)c->cgrp_cb,
+        (uintptr_t)c->agg_ptr, (uintptr_t)c->agg_idx, (uintptr_t)c->dofmask, (uintptr_t)c->cz, (uintptr_t)c->cy, (uintptr_t)c->agg_dm,
+        (uintptr_t)c->Ainv, (uintptr_t)c->nagg, (uintptr_t)c->nmode, (uintptr_t)c->cgrp_cb,
         (uintptr_t)c->cgrp_off, (uintptr_t)c->ncgrp, (uintptr_t)c->cgrp_nbmax};
     if (c->cg_exec && key == c->cg_key) return LPM_OK;
     if (c->cg_exec) {

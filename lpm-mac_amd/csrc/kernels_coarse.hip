@@ -303,28 +303,54 @@ __global__ __launch_bounds__(256) void k_coarse_undead(const int32_t *__restrict
     if (dead[i] || dead[j]) A[idx] = 0.0;
 }
 
-// Partial restriction: workgroup (g, s) sums (P_i^T r_i)[m] over the s-th chunk of 1024 members
-// of aggregate g into czp[(g S + s) M + m].
+// Restriction cz = P^T r: one workgroup per aggregate g sums (P_i^T r_i)[m] over its members
+// into czf[g M + m]. Each thread takes every RESTRICT_WG-th member, RESTRICT_PER at a time with
+// all their loads issued first; the offsets from the centre are read in member order. (A split of
+// an aggregate over several workgroups needs a cross-workgroup sum; on this GPU the device-scope
+// fence that publishes a partial writes back the XCD's L2, which cost more than it saved.)
+constexpr int RESTRICT_WG = 1024;
+constexpr int RESTRICT_PER = 4;
+
 template <typename T, int D, int VP>
-__global__ __launch_bounds__(1024) void k_coarse_restrict(int S, const int32_t *__restrict__ agg_ptr,
-                                                          const int32_t *__restrict__ agg_idx, const double *__restrict__ agg_d,
-                                                          const uint8_t *__restrict__ mask, const T *__restrict__ r,
-                                                          double *__restrict__ czp, const double *__restrict__ scal) {
-    if (scal[SC_DONE] != 0.0) return;
+__global__ __launch_bounds__(RESTRICT_WG) void k_coarse_restrict(const int32_t *__restrict__ agg_ptr,
+                                                                 const int32_t *__restrict__ agg_idx,
+                                                                 const double *__restrict__ agg_dm,
+                                                                 const uint8_t *__restrict__ mask, const T *__restrict__ r,
+                                                                 double *__restrict__ czf, const double *__restrict__ scal) {
     constexpr int M = D == 3 ? 6 : 3;
-    __shared__ double red[M][16];
-    const int g = blockIdx.x / S, sidx = blockIdx.x % S;
+    constexpr int NW = RESTRICT_WG / 64;
+    __shared__ double red[M][NW];
+    const int g = blockIdx.x;
+    const int beg = agg_ptr[g], end = agg_ptr[g + 1];
     double acc[M];
     for (int a = 0; a < M; ++a) acc[a] = 0.0;
-    const int m0 = agg_ptr[g] + sidx * blockDim.x + threadIdx.x;
-    if (m0 < agg_ptr[g + 1]) {
-        const int64_t i = agg_idx[m0];
-        double P[3][6];
-        rigid_modes<D>(&agg_d[3 * i], mask[i], P);
-        double ri[3] = {0, 0, 0};
-        for (int k = 0; k < D; ++k) ri[k] = (double)r[VP * i + k];
-        for (int a = 0; a < M; ++a)
-            for (int k = 0; k < D; ++k) acc[a] += P[k][a] * ri[k];
+    // the first members' ids are requested together with the stop flag
+    int64_t ii0[RESTRICT_PER];
+#pragma unroll
+    for (int u = 0; u < RESTRICT_PER; ++u) {
+        const int m = beg + (int)threadIdx.x + u * RESTRICT_WG;
+        ii0[u] = m < end ? agg_idx[m] : -1;
+    }
+    if (scal[SC_DONE] != 0.0) return;
+    for (int m0 = beg + (int)threadIdx.x; m0 < end; m0 += RESTRICT_WG * RESTRICT_PER) {
+        int64_t ii[RESTRICT_PER];
+#pragma unroll
+        for (int u = 0; u < RESTRICT_PER; ++u) {
+            const int m = m0 + u * RESTRICT_WG;
+            ii[u] = m0 == beg + (int)threadIdx.x ? ii0[u] : (m < end ? agg_idx[m] : -1);
+        }
+#pragma unroll
+        for (int u = 0; u < RESTRICT_PER; ++u) {
+            const int m = m0 + u * RESTRICT_WG;
+            if (ii[u] >= 0) {
+                double P[3][6];
+                rigid_modes<D>(&agg_dm[3 * (int64_t)m], mask[ii[u]], P);
+                double ri[3] = {0, 0, 0};
+                for (int k = 0; k < D; ++k) ri[k] = (double)r[VP * ii[u] + k];
+                for (int a = 0; a < M; ++a)
+                    for (int k = 0; k < D; ++k) acc[a] += P[k][a] * ri[k];
+            }
+        }
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int a = 0; a < M; ++a) {
@@ -334,36 +360,40 @@ __global__ __launch_bounds__(1024) void k_coarse_restrict(int S, const int32_t *
     }
     __syncthreads();
     if (threadIdx.x < M) {
-        double s = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[threadIdx.x][w];
-        czp[(int64_t)blockIdx.x * M + threadIdx.x] = s;
+        double v = 0.0;
+        for (int w = 0; w < NW; ++w) v += red[threadIdx.x][w];
+        czf[(int64_t)g * M + threadIdx.x] = v;
     }
 }
 
-// cy = Ac^-1 cz for group blockIdx.y: every workgroup first sums the group's restriction
-// partials into cz (LDS, fixed order), then each wavefront forms one row of the group's block,
-// lanes striding the row (Ac^-1 is symmetric, row = column).
-__global__ __launch_bounds__(256) void k_coarse_gemv(int M, int S, const int32_t *__restrict__ cb,
-                                                      const int64_t *__restrict__ off, const double *__restrict__ Ainv_all,
-                                                      const double *__restrict__ czp, double *__restrict__ cy,
-                                                      const double *__restrict__ scal) {
-    if (scal[SC_DONE] != 0.0) return;
+// cy = Ac^-1 cz for group blockIdx.y: each wavefront forms one row of the group's block, lanes
+// striding the row (Ac^-1 is symmetric, row = column), against cz staged in LDS.
+__global__ __launch_bounds__(256) void k_coarse_gemv(const int32_t *__restrict__ cb, const int64_t *__restrict__ off,
+                                                      const double *__restrict__ Ainv_all, const double *__restrict__ czf,
+                                                      double *__restrict__ cy, const double *__restrict__ scal) {
     const int b = blockIdx.y, base = cb[b], nc = cb[b + 1] - base;
     if ((int)(blockIdx.x * (blockDim.x >> 6)) >= nc) return;  // uniform over the workgroup
-    __shared__ double cz[1024];
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-        const int g = (base + c) / M, m = (base + c) % M;
-        double s = 0.0;
-        for (int t = 0; t < S; ++t) s += czp[((int64_t)g * S + t) * M + m];
-        cz[c] = s;
-    }
-    __syncthreads();
+    __shared__ double cz[1024];  // a group's block has at most 1024 DoFs
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // this wavefront's row of Ac^-1, requested together with the stop flag
+    const double *Arow = Ainv_all + off[b] + (int64_t)r * nc;
+    double a[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = lane + 64 * q;
+        a[q] = (r < nc && c < nc) ? Arow[c] : 0.0;
+    }
+    if (scal[SC_DONE] != 0.0) return;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) cz[c] = czf[base + c];
+    __syncthreads();
     if (r >= nc) return;
-    const double *Ainv = Ainv_all + off[b];
     double s = 0.0;
-    for (int c = lane; c < nc; c += 64) s += Ainv[(int64_t)r * nc + c] * cz[c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = lane + 64 * q;
+        if (c < nc) s += a[q] * cz[c];
+    }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) cy[base + r] = s;
 }
@@ -387,7 +417,7 @@ void coarse_free(lpm_ctx *c) {
     if (c->gj_graph) hipGraphExecDestroy(c->gj_graph);
     void *ptrs[] = {c->agg_of, c->agg_ptr, c->agg_idx, c->agg_d, c->ra_ptr, c->ra_id,  c->cq_ptr,
                     c->cq_f,   c->cq_gh,   c->cvbuf,   c->Ac,    c->cz,     c->cy,     c->dofmask,
-                    c->cgrp_cb, c->cgrp_off, c->agg_grp, c->dofmask_prev};
+                    c->cgrp_cb, c->cgrp_off, c->agg_grp, c->dofmask_prev, c->agg_dm};
     for (void *p : ptrs)
         if (p) hipFree(p);
 }
@@ -581,9 +611,6 @@ static int coarse_setup(lpm_ctx *c) {
     LPM_HIP(c, hipMalloc(&c->ra_ptr, sizeof(int32_t) * (n + 1)));
     LPM_HIP(c, hipMalloc(&c->ra_id, sizeof(int32_t) * std::max<size_t>(rid.size(), 1)));
     LPM_HIP(c, hipMalloc(&c->cvbuf, sizeof(double) * M * M * std::max<size_t>(rid.size(), 1)));
-    int maxm = 1;
-    for (int g = 0; g < nagg; ++g) maxm = std::max(maxm, aptr[g + 1] - aptr[g]);
-    c->coarse_S = (maxm + 1023) / 1024;
     // two ping-pong buffers of the packed blocks, the groups' NB x NB block inverses, their row
     // panels, the original diagonal and the dead-row flags
     c->ncgrp = (int)nsp;
@@ -597,10 +624,15 @@ static int coarse_setup(lpm_ctx *c) {
     LPM_HIP(c, hipMemcpyAsync(c->cgrp_cb, cb.data(), sizeof(int32_t) * (nsp + 1), hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->cgrp_off, off.data(), sizeof(int64_t) * (nsp + 1), hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->agg_grp, agg_grp.data(), sizeof(int32_t) * nagg, hipMemcpyHostToDevice, c->stream));
-    LPM_HIP(c, hipMalloc(&c->cz, sizeof(double) * (size_t)nagg * c->coarse_S * M));
+    LPM_HIP(c, hipMalloc(&c->cz, sizeof(double) * std::max(nc, 1)));
     LPM_HIP(c, hipMalloc(&c->cy, sizeof(double) * nc));
     LPM_HIP(c, hipMalloc(&c->dofmask, n));
-    LPM_HIP(c, hipMemsetAsync(c->cz, 0, sizeof(double) * (size_t)nagg * c->coarse_S * M, c->stream));
+    LPM_HIP(c, hipMalloc(&c->agg_dm, sizeof(double) * 3 * std::max<int64_t>(n, 1)));
+    LPM_HIP(c, hipMemsetAsync(c->cz, 0, sizeof(double) * std::max(nc, 1), c->stream));
+    std::vector<double> dm(3 * n);  // offsets from the centre in member order
+    for (int64_t m = 0; m < n; ++m)
+        for (int k = 0; k < 3; ++k) dm[3 * m + k] = d[3 * (int64_t)aidx[m] + k];
+    LPM_HIP(c, hipMemcpyAsync(c->agg_dm, dm.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemsetAsync(c->cy, 0, sizeof(double) * nc, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->cq_gh, qgh.data(), sizeof(int32_t) * qgh.size(), hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->agg_of, agg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
@@ -729,14 +761,14 @@ int coarse_prepare(lpm_ctx *c) {
 
 template <typename T, int D, int VP>
 static int coarse_apply_t(lpm_ctx *c) {
-    hipLaunchKernelGGL((k_coarse_restrict<T, D, VP>), dim3(c->nagg * c->coarse_S), dim3(1024), 0, c->stream, c->coarse_S,
-                       c->agg_ptr, c->agg_idx, c->agg_d, c->dofmask, (const T *)c->v_r, c->cz, (const double *)c->scal);
+    hipLaunchKernelGGL((k_coarse_restrict<T, D, VP>), dim3(c->nagg), dim3(RESTRICT_WG), 0, c->stream, c->agg_ptr,
+                       c->agg_idx, c->agg_dm, c->dofmask, (const T *)c->v_r, c->cz, (const double *)c->scal);
     int rc = ck(c, "k_coarse_restrict");
     if (rc) return rc;
     // 4 wavefronts per workgroup, one row of a group's block per wavefront
     hipLaunchKernelGGL(k_coarse_gemv, dim3(blocks_for(std::max(c->cgrp_nbmax, 1), 4), c->ncgrp), dim3(256), 0, c->stream,
-                       c->nmode, c->coarse_S, (const int32_t *)c->cgrp_cb, (const int64_t *)c->cgrp_off,
-                       (const double *)c->Ainv, (const double *)c->cz, c->cy, (const double *)c->scal);
+                       (const int32_t *)c->cgrp_cb, (const int64_t *)c->cgrp_off, (const double *)c->Ainv,
+                       (const double *)c->cz, c->cy, (const double *)c->scal);
     return ck(c, "k_coarse_gemv");
 }
 

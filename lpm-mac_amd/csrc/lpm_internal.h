@@ -10,7 +10,7 @@
 #include "lpm_mi355x.h"
 
 // Compiled limits, checked on the host before any launch.
-#define CG_VEC_WG 1024 // most workgroups of the CG vector kernels (grid-stride beyond)
+#define CG_VEC_WG 1024  // most workgroups of the CG vector kernels (grid-stride beyond)
 #define LPM_MAXB 32    // bonds per particle (hex 12, SC/FCC 18, BCC 14)
 #define LPM_MAXC 512   // 2-hop candidates per particle before de-duplication
 #define LPM_NSV 7      // state variables per particle (J2 uses 7)
@@ -102,8 +102,8 @@ struct lpm_ctx {
     int64_t *cgrp_off = nullptr;      // block offsets (ncgrp + 1)
     double *Ac = nullptr;             // Gauss-Jordan ping-pong buffers, block inverses, row panels, diagonal, flags
     double *Ainv = nullptr;           // the buffer of Ac that holds Ac^-1 (inside Ac)
-    double *cz = nullptr, *cy = nullptr;  // restriction partials (nagg * coarse_S * nmode) / correction
-    int coarse_S = 1;                 // restriction workgroups per aggregate
+    double *cz = nullptr, *cy = nullptr;  // restriction P^T r / correction Ac^-1 P^T r (ncoarse each)
+    double *agg_dm = nullptr;         // agg_d in member order (agg_idx)
     hipGraphExec_t gj_graph = nullptr;  // the inversion's launches, captured once (coarse_graph = 1)
     // CG batch graph (mode "cg_graph"): one batch of CG iterations captured once and replayed;
     // cg_key lists every value its launches captured (grids, launch configuration, buffers)
