@@ -657,7 +657,7 @@ static int create(const lpm_topology *t, const lpm_cut *cuts, int32_t ncut, lpm_
     c->cutoff[1] = uc[6];
     c->volume = uc[7];
     c->n = n;
-    c->vpad = c->dim == 3 ? 4 : 2;
+    c->vpad = c->dim == 3 ? 3 : 2;
     c->precision = 0;
     c->cg_tol = 1e-6;
     auto fail = [&](int rc) {
@@ -694,8 +694,8 @@ static int create(const lpm_topology *t, const lpm_cut *cuts, int32_t ncut, lpm_
     } else {
         c->nbond = nbond;
         c->maxbond = maxb;
-        for (int64_t p0 = 0; p0 < n; p0 += 32)
-            c->gp_bonds = std::max<int>(c->gp_bonds, bp[std::min<int64_t>(n, p0 + 32)] - bp[p0]);
+        for (int64_t p0 = 0; p0 < n; p0 += LPM_BOND_GP)
+            c->gp_bonds = std::max<int>(c->gp_bonds, bp[std::min<int64_t>(n, p0 + LPM_BOND_GP)] - bp[p0]);
         // 2-hop records of a row: for each bond i->k, the bonds of k in the same layer
         for (int64_t i = 0; i < n; ++i) {
             int hops = 0;

@@ -363,8 +363,8 @@ static int add_x_t(lpm_ctx *c, double *lam) {
 }
 
 static int add_x(lpm_ctx *c, double *lam) {
-    if (c->precision == 0) return c->dim == 3 ? add_x_t<float, 3, 4>(c, lam) : add_x_t<float, 2, 2>(c, lam);
-    return c->dim == 3 ? add_x_t<double, 3, 4>(c, lam) : add_x_t<double, 2, 2>(c, lam);
+    if (c->precision == 0) return c->dim == 3 ? add_x_t<float, 3, 3>(c, lam) : add_x_t<float, 2, 2>(c, lam);
+    return c->dim == 3 ? add_x_t<double, 3, 3>(c, lam) : add_x_t<double, 2, 2>(c, lam);
 }
 
 // Adjoint of the linear objective G = a.Pin + b.x at the current state (a, b: dim*n device

@@ -27,8 +27,9 @@
 
 namespace {
 
+// One particle's DoFs in a CG vector: 3 entries in 3D (12 or 24 bytes, element-aligned), 2 in 2D
 template <typename T, int VP>
-struct alignas(sizeof(T) * VP) Vec {
+struct alignas((VP & (VP - 1)) == 0 ? sizeof(T) * VP : sizeof(T)) Vec {
     T v[VP];
 };
 
@@ -633,8 +634,8 @@ static int precond_t(lpm_ctx *c) {
 }
 
 int launch_precond_setup(lpm_ctx *c) {
-    if (c->precision == 0) return c->dim == 3 ? precond_t<float, 3, 4>(c) : precond_t<float, 2, 2>(c);
-    return c->dim == 3 ? precond_t<double, 3, 4>(c) : precond_t<double, 2, 2>(c);
+    if (c->precision == 0) return c->dim == 3 ? precond_t<float, 3, 3>(c) : precond_t<float, 2, 2>(c);
+    return c->dim == 3 ? precond_t<double, 3, 3>(c) : precond_t<double, 2, 2>(c);
 }
 
 // workgroups of the SpMV launch (= partials the fused SpMV leaves)
@@ -770,8 +771,8 @@ static int spmv_tune(lpm_ctx *c) {
     const int key = c->precision;
     if (!c->spmv_tune || c->tune_key == key) return LPM_OK;
     int rc;
-    if (c->precision == 0) rc = c->dim == 3 ? spmv_tune_t<float, 3, 4>(c) : spmv_tune_t<float, 2, 2>(c);
-    else rc = c->dim == 3 ? spmv_tune_t<double, 3, 4>(c) : spmv_tune_t<double, 2, 2>(c);
+    if (c->precision == 0) rc = c->dim == 3 ? spmv_tune_t<float, 3, 3>(c) : spmv_tune_t<float, 2, 2>(c);
+    else rc = c->dim == 3 ? spmv_tune_t<double, 3, 3>(c) : spmv_tune_t<double, 2, 2>(c);
     if (rc) return rc;
     c->tune_key = key;
     return LPM_OK;
@@ -931,9 +932,9 @@ int cg_solve(lpm_ctx *c, int *iters, double *error, int *success) {
     if ((rc = coarse_prepare(c))) return rc;
     if ((rc = spmv_tune(c))) return rc;
     if (c->precision == 0)
-        rc = c->dim == 3 ? cg_t<float, 3, 4>(c, iters, error, success) : cg_t<float, 2, 2>(c, iters, error, success);
+        rc = c->dim == 3 ? cg_t<float, 3, 3>(c, iters, error, success) : cg_t<float, 2, 2>(c, iters, error, success);
     else
-        rc = c->dim == 3 ? cg_t<double, 3, 4>(c, iters, error, success) : cg_t<double, 2, 2>(c, iters, error, success);
+        rc = c->dim == 3 ? cg_t<double, 3, 3>(c, iters, error, success) : cg_t<double, 2, 2>(c, iters, error, success);
     if (!rc && two_level(c) && c->coarse_reuse) {
         // a kept coarse space is rebuilt before the next solve once the CG needs clearly more
         // iterations than the first solve after the last build
@@ -963,8 +964,8 @@ static int scale_x_t(lpm_ctx *c) {
 }
 
 int launch_scale_x(lpm_ctx *c) {
-    if (c->precision == 0) return c->dim == 3 ? scale_x_t<float, 3, 4>(c) : scale_x_t<float, 2, 2>(c);
-    return c->dim == 3 ? scale_x_t<double, 3, 4>(c) : scale_x_t<double, 2, 2>(c);
+    if (c->precision == 0) return c->dim == 3 ? scale_x_t<float, 3, 3>(c) : scale_x_t<float, 2, 2>(c);
+    return c->dim == 3 ? scale_x_t<double, 3, 3>(c) : scale_x_t<double, 2, 2>(c);
 }
 
 template <typename T, int D, int VP>
@@ -975,8 +976,8 @@ static int move_t(lpm_ctx *c) {
 }
 
 int launch_move(lpm_ctx *c) {
-    if (c->precision == 0) return c->dim == 3 ? move_t<float, 3, 4>(c) : move_t<float, 2, 2>(c);
-    return c->dim == 3 ? move_t<double, 3, 4>(c) : move_t<double, 2, 2>(c);
+    if (c->precision == 0) return c->dim == 3 ? move_t<float, 3, 3>(c) : move_t<float, 2, 2>(c);
+    return c->dim == 3 ? move_t<double, 3, 3>(c) : move_t<double, 2, 2>(c);
 }
 
 template <typename T, int D, int VP>
@@ -1006,8 +1007,8 @@ int spmv_bench(lpm_ctx *c, int reps, double *ms) {
     if (reps <= 0) return lpm_fail(c, LPM_E_ARG, "reps must be positive");
     const int rc = spmv_tune(c);
     if (rc) return rc;
-    if (c->precision == 0) return c->dim == 3 ? bench_t<float, 3, 4>(c, reps, ms) : bench_t<float, 2, 2>(c, reps, ms);
-    return c->dim == 3 ? bench_t<double, 3, 4>(c, reps, ms) : bench_t<double, 2, 2>(c, reps, ms);
+    if (c->precision == 0) return c->dim == 3 ? bench_t<float, 3, 3>(c, reps, ms) : bench_t<float, 2, 2>(c, reps, ms);
+    return c->dim == 3 ? bench_t<double, 3, 3>(c, reps, ms) : bench_t<double, 2, 2>(c, reps, ms);
 }
 
 int copy_bench(lpm_ctx *c, int64_t bytes, int reps, double *ms) {

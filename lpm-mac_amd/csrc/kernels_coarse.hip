@@ -773,6 +773,6 @@ static int coarse_apply_t(lpm_ctx *c) {
 }
 
 int launch_coarse_apply(lpm_ctx *c) {
-    if (c->precision == 0) return c->dim == 3 ? coarse_apply_t<float, 3, 4>(c) : coarse_apply_t<float, 2, 2>(c);
-    return c->dim == 3 ? coarse_apply_t<double, 3, 4>(c) : coarse_apply_t<double, 2, 2>(c);
+    if (c->precision == 0) return c->dim == 3 ? coarse_apply_t<float, 3, 3>(c) : coarse_apply_t<float, 2, 2>(c);
+    return c->dim == 3 ? coarse_apply_t<double, 3, 3>(c) : coarse_apply_t<double, 2, 2>(c);
 }

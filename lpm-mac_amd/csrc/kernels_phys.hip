@@ -37,7 +37,7 @@ __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b 
 // Per-bond work runs lane-per-bond (coalesced loads and stores of the bond arrays); the
 // per-particle sums then run thread-per-particle over the bond values staged in LDS, in bond
 // order, so every sum is formed in the reference's order.
-constexpr int GP = 32;  // particles per workgroup of the bond kernels
+constexpr int GP = LPM_BOND_GP;  // particles per workgroup of the bond kernels
 
 __global__ __launch_bounds__(64) void k_geometry_force(
     int64_t n, int model, int do_force, const int32_t *__restrict__ bptr, const int32_t *__restrict__ bsplit,
@@ -1959,8 +1959,8 @@ int device_topology(lpm_ctx *c, const double *xyz_host, const lpm_cut *cuts_host
     c->nbond = tot;
     c->maxbond = maxb;
     c->gp_bonds = 0;
-    for (int64_t p0 = 0; p0 < n; p0 += 32)
-        c->gp_bonds = std::max<int>(c->gp_bonds, bp[std::min<int64_t>(n, p0 + 32)] - bp[p0]);
+    for (int64_t p0 = 0; p0 < n; p0 += LPM_BOND_GP)
+        c->gp_bonds = std::max<int>(c->gp_bonds, bp[std::min<int64_t>(n, p0 + LPM_BOND_GP)] - bp[p0]);
 #define ALT(ptr, cntv)                                                                                          \
     if (hipMalloc(&c->ptr, sizeof(*c->ptr) * std::max<int64_t>((cntv), 1)) != hipSuccess) {                     \
         cleanup();                                                                                               \

@@ -12,6 +12,8 @@
 // Compiled limits, checked on the host before any launch.
 #define CG_VEC_WG 1024  // most workgroups of the CG vector kernels (grid-stride beyond)
 #define LPM_MAXB 32    // bonds per particle (hex 12, SC/FCC 18, BCC 14)
+#define LPM_BOND_GP 16 // consecutive particles per workgroup of the bond kernels (their LDS is sized by
+                       // the most bonds of any such group, gp_bonds)
 #define LPM_MAXC 512   // 2-hop candidates per particle before de-duplication
 #define LPM_NSV 7      // state variables per particle (J2 uses 7)
 #define LPM_NMAT 10    // model parameters per particle
@@ -176,7 +178,7 @@ struct lpm_ctx {
     int aval_prec = -1;          // precision aval was allocated for
     void *invdiag = nullptr;     // Jacobi (ndof) or block Jacobi (9 per particle)
     void *v_x = nullptr, *v_r = nullptr, *v_u = nullptr, *v_p = nullptr, *v_s = nullptr, *v_w = nullptr;  // padded vectors
-    int vpad = 4;                // entries per particle in the CG vectors (4 in 3D, 2 in 2D)
+    int vpad = 3;                // entries per particle in the CG vectors (3 in 3D, 2 in 2D)
     double *scal = nullptr;      // SC_COUNT device scalars
     double *partial = nullptr;   // per-block partial sums (3 x partial_cap)
     int partial_cap = 0;
