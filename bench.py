@@ -701,12 +701,30 @@ def fatigue_ensemble_bench(args):
         hip_sync(ctx)
         t = time.perf_counter() - t0
         res = (t, n_done, float(ctx.get("damage").max()), ctx.stat("cg_solves"))
+        # batched sensitivity of the summed top-row reactions to every particle's (A, B, d) through
+        # the last fatigue update: re-solve at the current loads with the updated damage, then one
+        # transposed solve for all specimens (no bond or nonlocal neighbour joins two of them)
+        hold = []
+        for top, bottom in groups:
+            hold += [(top, "x", 0.0), (top, "y", 0.0), (top, "z", 0.0), (bottom, "x", 0.0), (bottom, "y", 0.0),
+                     (bottom, "z", 0.0)]
+        assert ctx.solve_step_fatigue(ctx.load_step(disp=hold))
+        w = np.zeros(2 * ctx.n)
+        for top, _ in groups:
+            w[2 * top + 1] = 1.0
+        hip_sync(ctx)
+        ta = time.perf_counter()
+        Gv, dA, dB, dd, nref, rres = ctx.adjoint_fatigue_params(w_pin=w)
+        hip_sync(ctx)
+        adj = {"seconds": time.perf_counter() - ta, "objective": "sum of the 64 top-row reactions", "G": Gv,
+               "refinements": nref, "relative_residual": rres, "particles_with_nonzero_dG_dA": int(np.count_nonzero(dA)),
+               "max_abs_dG_dA": float(np.abs(dA).max())}
         ctx.close()
-        return res
+        return res + (adj,)
 
-    # direct launches only: a graph-replayed rerun of this mode crashed inside hipGraphLaunch under
-    # rocprofv3's kernel trace (see the design notes), so graph replay stays out of it
-    t_batch, n_done, dmax, steps = run_union(0)
+    # direct launches only: a graph-replayed rerun of this mode crashed on the host inside
+    # hipGraphLaunch under rocprofv3's kernel trace, and replay was never faster here
+    t_batch, n_done, dmax, steps, adj = run_union(0)
     print(f"ensemble: {S} specimens, {n_done} cycles in one context: {t_batch:.2f} s", flush=True)
     # the same specimens, one context each, back to back
     t_seq = 0.0
@@ -744,7 +762,8 @@ def fatigue_ensemble_bench(args):
                                f"R=0.1, {cycles} cycles, tau={tau}",
                    "precond": args.precond, "fatigue_steps": steps, "max_damage": dmax,
                    "one_context_per_specimen_specimen_cycles_per_s": S * n_done / t_seq,
-                   "speedup_vs_one_context_per_specimen": t_seq / t_batch},
+                   "speedup_vs_one_context_per_specimen": t_seq / t_batch,
+                   "batched_hcf_parameter_adjoint": adj},
     }
     print(json.dumps(out), flush=True)
 

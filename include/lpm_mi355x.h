@@ -383,6 +383,28 @@ int lpm_adjoint_linear(lpm_ctx *ctx, const double *w_pin, const double *w_x, dou
  * created, and one nonlocal length (symmetric neighbour lists). */
 int lpm_adjoint_fatigue_params(lpm_ctx *ctx, const double *w_pin, const double *w_x, double *dG_dA, double *dG_dB,
                                double *dG_dd, double *G, int32_t *refinements, double *rel_residual);
+/* Record a fatigue load path for lpm_adjoint_fatigue_path (no reference counterpart): while on,
+ * every HCF state-variable update (ParticleFatigueHCF::updateParticleStateVariables,
+ * particle_fatigue_hcf.h:127-145, whether called directly or by the static, fatigue-step and
+ * cycle drivers) keeps a checkpoint of its converged state and the energy trackers s0, s1, s2 of
+ * every particle before and after it, and every fatigue damage update
+ * (SolverFatigue::updateFatigueDamage, solver_fatigue.h:75-126) its cycle step, excluded type
+ * and the damage it left. Device memory per recorded state is about one lpm_checkpoint_save
+ * slot. on = 1 starts a new record (dropping the old one), 0 stops and releases it. */
+int lpm_fatigue_record(lpm_ctx *ctx, int32_t on);
+/* Sensitivity of the linear objective G = w_pin.Pin + w_x.x (as in lpm_adjoint_linear) at the
+ * current equilibrium to every particle's HCF parameters A, B, d (mat rows 0-2; n doubles each,
+ * any may be NULL) through the whole recorded path: each fatigue update's rate law and nonlocal
+ * average (with the weights' dependence on the positions of the state the update was taken in),
+ * the bond damage each update raises (bd = max(bd, D_i, D_j), std::max order), and each recorded
+ * state's energy as the cycle's minimum, maximum or the running maximum s0 carries it to later
+ * updates. One transposed solve at the current state and one at every recorded state whose
+ * energy reaches a later update (*solves counts them; *max_rel_residual is the largest final
+ * relative residual of their refinements). Thresholds are held at their recorded branches:
+ * clamped damage and min/max ties contribute nothing. Restores the current state afterwards.
+ * Needs the HCF model, a recorded update, one nonlocal length. */
+int lpm_adjoint_fatigue_path(lpm_ctx *ctx, const double *w_pin, const double *w_x, double *dG_dA, double *dG_dB,
+                             double *dG_dd, double *G, int32_t *solves, double *max_rel_residual);
 
 /* ------------------------------------------------------------------ load-step checkpoints
  * (no reference counterpart: the reference keeps only the last converged copy,

@@ -780,6 +780,7 @@ int lpm_destroy(lpm_ctx *c) {
     dump_free(c);
     coarse_free(c);
     fd_free(c);
+    fat_record_clear(c);
     for (void *p : c->ckpt)
         if (p) hipFree(p);
     if (c->cg_exec) hipGraphExecDestroy(c->cg_exec);
@@ -1332,6 +1333,46 @@ int lpm_adjoint_fatigue_params(lpm_ctx *c, const double *w_pin, const double *w_
     return LPM_OK;
 }
 
+int lpm_fatigue_record(lpm_ctx *c, int32_t on) {
+    if (!c) return LPM_E_ARG;
+    fat_record_clear(c);
+    c->fat_rec = on != 0;
+    return LPM_OK;
+}
+
+int lpm_adjoint_fatigue_path(lpm_ctx *c, const double *w_pin, const double *w_x, double *dG_dA, double *dG_dB,
+                             double *dG_dd, double *G, int32_t *solves, double *max_rel_residual) {
+    if (!c) return LPM_E_ARG;
+    if (!w_pin && !w_x) return lpm_fail(c, LPM_E_ARG, "no objective weights");
+    if (c->nonlocal_L && c->n > 0) {
+        std::vector<double> L(c->n);
+        LPM_HIP(c, hipMemcpyAsync(L.data(), c->nonlocal_L, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+        LPM_HIP(c, hipStreamSynchronize(c->stream));
+        for (double v : L)
+            if (v != L[0]) return lpm_fail(c, LPM_E_STATE, "fatigue-path sensitivity needs one nonlocal length");
+    }
+    const int64_t nd = c->n * c->dim;
+    double *d = nullptr;
+    LPM_HIP(c, hipMalloc(&d, sizeof(double) * 2 * std::max<int64_t>(nd, 1)));
+    int rc = LPM_OK;
+    if (w_pin && hipMemcpyAsync(d, w_pin, sizeof(double) * nd, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "adjoint weights upload");
+    if (!rc && w_x && hipMemcpyAsync(d + nd, w_x, sizeof(double) * nd, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = lpm_fail(c, LPM_E_HIP, "adjoint weights upload");
+    int ns = 0;
+    double rr = 0;
+    const bool rec = c->fat_rec;
+    c->fat_rec = false;  // the reverse pass restores states; nothing it does is part of the path
+    if (!rc) rc = adjoint_fatigue_path(c, w_pin ? d : nullptr, w_x ? d + nd : nullptr, dG_dA, dG_dB, dG_dd, G, &ns, &rr);
+    c->fat_rec = rec;
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
+    hipFree(d);
+    if (rc) return rc;
+    if (solves) *solves = ns;
+    if (max_rel_residual) *max_rel_residual = rr;
+    return LPM_OK;
+}
+
 // The arrays a checkpoint holds, in buffer order: {pointer, doubles} (cons, int32, is stored
 // after them, padded to whole doubles).
 static std::vector<std::pair<double *, int64_t>> ckpt_fields(lpm_ctx *c) {
@@ -1342,16 +1383,13 @@ static std::vector<std::pair<double *, int64_t>> ckpt_fields(lpm_ctx *c) {
             {c->bforce_last, nb},  {c->bdamage, nb},     {c->bdamage_last, nb}};
 }
 
-static int checkpoint_copy(lpm_ctx *c, int32_t slot, bool save) {
-    if (slot < 0 || slot >= LPM_MAX_CHECKPOINTS) return lpm_fail(c, LPM_E_ARG, "checkpoint slot out of range");
+static int ckpt_copy_buf(lpm_ctx *c, void **pbuf, bool save) {
     const auto f = ckpt_fields(c);
     int64_t tot = 0;
     for (auto &x : f) tot += x.second;
     const int64_t ncons = (3 * c->n + 1) / 2;  // int32 flags in doubles
-    if ((int64_t)c->ckpt.size() <= slot) c->ckpt.resize(slot + 1, nullptr);
-    if (!save && !c->ckpt[slot]) return lpm_fail(c, LPM_E_STATE, "checkpoint slot " + std::to_string(slot) + " is empty");
-    if (!c->ckpt[slot]) LPM_HIP(c, hipMalloc(&c->ckpt[slot], sizeof(double) * (tot + ncons)));
-    double *buf = (double *)c->ckpt[slot];
+    if (!*pbuf) LPM_HIP(c, hipMalloc(pbuf, sizeof(double) * (tot + ncons)));
+    double *buf = (double *)*pbuf;
     int64_t off = 0;
     for (auto &x : f) {
         if (x.second > 0) {
@@ -1370,6 +1408,17 @@ static int checkpoint_copy(lpm_ctx *c, int32_t slot, bool save) {
     }
     LPM_HIP(c, hipStreamSynchronize(c->stream));
     return LPM_OK;
+}
+
+extern "C++" int ckpt_save_buf(lpm_ctx *c, void **buf) { return ckpt_copy_buf(c, buf, true); }
+
+extern "C++" int ckpt_restore_buf(lpm_ctx *c, void *buf) { return ckpt_copy_buf(c, &buf, false); }
+
+static int checkpoint_copy(lpm_ctx *c, int32_t slot, bool save) {
+    if (slot < 0 || slot >= LPM_MAX_CHECKPOINTS) return lpm_fail(c, LPM_E_ARG, "checkpoint slot out of range");
+    if ((int64_t)c->ckpt.size() <= slot) c->ckpt.resize(slot + 1, nullptr);
+    if (!save && !c->ckpt[slot]) return lpm_fail(c, LPM_E_STATE, "checkpoint slot " + std::to_string(slot) + " is empty");
+    return ckpt_copy_buf(c, &c->ckpt[slot], save);
 }
 
 int lpm_checkpoint_save(lpm_ctx *c, int32_t slot) {

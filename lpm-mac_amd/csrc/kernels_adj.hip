@@ -305,6 +305,247 @@ __global__ __launch_bounds__(256) void k_fat_params(int64_t n, int undamaged, do
     out[2 * n + j] = gd;
 }
 
+
+// ---------------------------------------------------------------- adjoint over a fatigue load path
+// Recorded path (lpm_fatigue_record): every HCF state-variable update at a converged state and
+// every fatigue damage update, in order. Going backwards, the adjoints held are those of the
+// current epoch's bond damage (mu_bd), of the particle damage (mu_D) and of the three energy
+// trackers s0 = max over the history of s2, s1 / s2 = min / max of the cycle's energies
+// (particle_fatigue_hcf.h:127-145): mu_s[3i + 0..2].
+
+// s0, s1, s2 of every particle (3 per particle)
+__global__ __launch_bounds__(256) void k_sv_snap(int64_t n, const double *__restrict__ sv, double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = sv[LPM_NSV * i + k];
+}
+
+// Back through one state-variable update: s2' = max(E, s2), s1' = min(E, s1), s0' = max(s2', s0).
+// A tracker's adjoint goes to E where the update changed it, else to its value before; the
+// adjoint of E is written to muE.
+__global__ __launch_bounds__(256) void k_fat_track_rev(int64_t n, const double *__restrict__ sb, const double *__restrict__ sa,
+                                                        double *__restrict__ mu_s, double *__restrict__ muE) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s0b = sb[3 * i], s1b = sb[3 * i + 1], s2b = sb[3 * i + 2];
+    const double s0a = sa[3 * i], s1a = sa[3 * i + 1], s2a = sa[3 * i + 2];
+    double m0 = mu_s[3 * i], m1 = mu_s[3 * i + 1], m2 = mu_s[3 * i + 2];
+    if (s0a != s0b) {  // s0' = s2'
+        m2 += m0;
+        m0 = 0.0;
+    }
+    double mE = 0.0;
+    if (s2a != s2b) {
+        mE += m2;
+        m2 = 0.0;
+    }
+    if (s1a != s1b) {
+        mE += m1;
+        m1 = 0.0;
+    }
+    mu_s[3 * i] = m0;
+    mu_s[3 * i + 1] = m1;
+    mu_s[3 * i + 2] = m2;
+    muE[i] = mE;
+}
+
+// Objective sum_i muE_i E_i with the HCF strain energy (particle_fatigue_hcf.h:82-125)
+//   E_i = sum_l [ sum_{b in l} Kn_b dLe_b^2 / 2 + TdLe_total(i,l) dLe_total(i,l) / 2 ] / V_m,i:
+// first the per-bond coefficient c_b = muE_i dE_i/d dLe_b ...
+__global__ __launch_bounds__(256) void k_energy_coef(int64_t n, double volume, int nneighbors, const int32_t *__restrict__ bptr,
+                                                      const int32_t *__restrict__ bsplit, const double *__restrict__ Kn,
+                                                      const double *__restrict__ Tv, const double *__restrict__ dLe,
+                                                      const double *__restrict__ dLe_total,
+                                                      const double *__restrict__ TdLe_total, const double *__restrict__ muE,
+                                                      double *__restrict__ coef) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b0 = bptr[i], bm = bsplit[i], b1 = bptr[i + 1];
+    const double V = volume * (b1 - b0) / nneighbors;
+    const double m = muE[i];
+    for (int b = b0; b < b1; ++b) {
+        const int l = b < bm ? 0 : 1;
+        coef[b] = m == 0.0 ? 0.0 : m * (Kn[b] * dLe[b] + 0.5 * (Tv[b] * dLe_total[2 * i + l] + TdLe_total[2 * i + l])) / V;
+    }
+}
+
+// ... then gx_p = sum over p's bonds b of cs_b (c_b + c_opp(b)): dLe_b and dLe_opp(b) both
+// move with the bond length, whose gradient at p is cs_b (zero for a broken bond). Adds to gx.
+__global__ __launch_bounds__(256) void k_energy_gx(int64_t n, int dim, const int32_t *__restrict__ bptr,
+                                                    const int32_t *__restrict__ opp, const double *__restrict__ cs,
+                                                    const double *__restrict__ coef, double *__restrict__ gx) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int b = bptr[p]; b < bptr[p + 1]; ++b) {
+        const double w = coef[b] + coef[opp[b]];
+        for (int k = 0; k < 3; ++k) g[k] += w * cs[3 * b + k];
+    }
+    for (int k = 0; k < dim; ++k) gx[(int64_t)dim * p + k] += g[k];
+}
+
+// Back through bd' = max(bd, max(D'_i, D'_j)) at an update: the adjoint of a bond whose damage
+// the update raised goes to the particle whose D' won std::max (the bond's owner on a tie), the
+// others stay with the bond damage before the update. Gathered per particle over its bonds and
+// their reverses; the routed bonds are cleared by k_fat_route_clear afterwards.
+__global__ __launch_bounds__(256) void k_fat_route(int64_t n, const int32_t *__restrict__ bptr, const int32_t *__restrict__ p2,
+                                                    const int32_t *__restrict__ opp, const double *__restrict__ Dn,
+                                                    const double *__restrict__ bd_prev, const double *__restrict__ bd_next,
+                                                    const double *__restrict__ mu_bd, double *__restrict__ mu_D) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double Di = Dn[i];
+    double s = 0.0;
+    for (int b = bptr[i]; b < bptr[i + 1]; ++b) {
+        const double Dj = Dn[p2[b]];
+        const int o = opp[b];
+        if (bd_next[b] != bd_prev[b] && Di >= Dj) s += mu_bd[b];  // bond i->j: max(D_i, D_j), D_i on a tie
+        if (bd_next[o] != bd_prev[o] && Di > Dj) s += mu_bd[o];   // bond j->i: max(D_j, D_i)
+    }
+    mu_D[i] += s;
+}
+
+__global__ __launch_bounds__(256) void k_fat_route_clear(int64_t nb, const double *__restrict__ bd_prev,
+                                                          const double *__restrict__ bd_next, double *__restrict__ mu_bd) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb && bd_next[b] != bd_prev[b]) mu_bd[b] = 0.0;
+}
+
+// Back through D' = D + dt NL(Ddot) (clamped to 1 at the threshold): the adjoint of the nonlocal
+// rate mu_nl = dt mu_D' where D' was neither clamped nor excluded; mu_D passes to D unclamped.
+__global__ __launch_bounds__(256) void k_fat_update_rev(int64_t n, int undamaged, double dt, const int32_t *__restrict__ type,
+                                                         const double *__restrict__ Dn, double *__restrict__ mu_D,
+                                                         double *__restrict__ mu_nl) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (type[i] == undamaged) {
+        mu_nl[i] = 0.0;
+        return;
+    }
+    const bool clamped = Dn[i] == 1.0;
+    mu_nl[i] = clamped ? 0.0 : dt * mu_D[i];
+    if (clamped) mu_D[i] = 0.0;
+}
+
+// HCF rate of particle j from its trackers after the cycle (s = s0, s1, s2)
+__device__ __forceinline__ double hcf_rate(const double *m, const double *s) {
+    const double r = s[2] - s[1];
+    return r > 0 ? m[0] * pow(s[0], m[1]) * pow(r, m[2]) : 0.0;
+}
+
+// The update's nonlocal normaliser W_i and rate NL_i = sum_j phi_ij V_j Ddot_j / W_i
+// (SolverFatigue::updateFatigueDamage, solver_fatigue.h:75-114) at the update's positions.
+__global__ __launch_bounds__(256) void k_fat_nl(int64_t n, int undamaged, double volume, int nneighbors,
+                                                 const int32_t *__restrict__ type, const int32_t *__restrict__ nlptr,
+                                                 const int32_t *__restrict__ nlidx, const int32_t *__restrict__ bptr,
+                                                 const double *__restrict__ xyz, const double *__restrict__ L,
+                                                 const double *__restrict__ mat, const double *__restrict__ sa,
+                                                 double *__restrict__ W, double *__restrict__ NL) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double A = 0.0, dn = 0.0;
+    if (type[i] != undamaged) {
+        for (int q = nlptr[i]; q < nlptr[i + 1]; ++q) {
+            const int64_t j = nlidx[q];
+            if (type[j] == undamaged) continue;
+            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1], dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+            const double u = 1 / (1 + pow(sqrt(dx * dx + dy * dy + dz * dz) / L[i], 8.0));
+            const double V = volume * (bptr[j + 1] - bptr[j]) / nneighbors;
+            dn += hcf_rate(&mat[LPM_NMAT * j], &sa[3 * j]) * u * u * V;
+            A += u * u * V;
+        }
+    }
+    W[i] = A;
+    NL[i] = A > 0.0 ? dn / A : 0.0;
+}
+
+// Back through the rate law and the nonlocal weights' values: per particle j,
+// S_j = sum_{i: j in NL(i)} mu_nl_i phi_ij V_j / W_i (gathered over NL(j), the lists being
+// symmetric); then dG/dA_j += S_j s0^B r^d, dG/dB_j += S_j Ddot_j ln s0, dG/dd_j += S_j Ddot_j ln r,
+// and the trackers' adjoints (the cycle's s1, s2 restart from these; s0 accumulates):
+// mu_s0 += S_j B Ddot_j / s0, mu_s2 = +S_j d Ddot_j / r, mu_s1 = -S_j d Ddot_j / r.
+__global__ __launch_bounds__(256) void k_fat_rate_rev(int64_t n, int undamaged, double volume, int nneighbors,
+                                                       const int32_t *__restrict__ type, const int32_t *__restrict__ nlptr,
+                                                       const int32_t *__restrict__ nlidx, const int32_t *__restrict__ bptr,
+                                                       const double *__restrict__ xyz, const double *__restrict__ L,
+                                                       const double *__restrict__ W, const double *__restrict__ mu_nl,
+                                                       const double *__restrict__ mat, const double *__restrict__ sa,
+                                                       double *__restrict__ gth, double *__restrict__ mu_s) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double m1 = 0.0, m2 = 0.0, m0 = mu_s[3 * j];
+    if (type[j] != undamaged) {
+        const double Vj = volume * (bptr[j + 1] - bptr[j]) / nneighbors;
+        double S = 0.0;
+        for (int q = nlptr[j]; q < nlptr[j + 1]; ++q) {
+            const int64_t i = nlidx[q];
+            if (type[i] == undamaged || W[i] == 0.0 || mu_nl[i] == 0.0) continue;
+            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1], dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+            const double u = 1 / (1 + pow(sqrt(dx * dx + dy * dy + dz * dz) / L[i], 8.0));
+            S += mu_nl[i] * u * u * Vj / W[i];
+        }
+        const double *m = &mat[LPM_NMAT * j];
+        const double s0 = sa[3 * j], r = sa[3 * j + 2] - sa[3 * j + 1];
+        if (r > 0 && S != 0.0) {
+            const double base = pow(s0, m[1]) * pow(r, m[2]), rate = m[0] * base;
+            gth[j] += S * base;
+            if (s0 > 0) {
+                gth[n + j] += S * rate * log(s0);
+                m0 += S * m[1] * rate / s0;
+            }
+            gth[2 * n + j] += S * rate * log(r);
+            m2 = S * m[2] * rate / r;
+            m1 = -m2;
+        }
+    }
+    mu_s[3 * j] = m0;
+    mu_s[3 * j + 1] = m1;
+    mu_s[3 * j + 2] = m2;
+}
+
+// Back through the nonlocal weights' positions: the gradient of sum_i mu_nl_i NL_i with respect
+// to the update's positions, phi(r) = (1 + (r/L)^8)^-2, dphi/dr = -16 u^3 (r/L)^7 / L:
+//   gx_p = sum_{j in NL(p)} [c(p,j) + c(j,p)] (x_p - x_j) / r_pj,
+//   c(i,j) = mu_nl_i V_j (Ddot_j - NL_i) / W_i dphi/dr(r_ij, L_i). Adds to gx.
+__global__ __launch_bounds__(256) void k_fat_weights_gx(int64_t n, int dim, int undamaged, double volume, int nneighbors,
+                                                         const int32_t *__restrict__ type, const int32_t *__restrict__ nlptr,
+                                                         const int32_t *__restrict__ nlidx, const int32_t *__restrict__ bptr,
+                                                         const double *__restrict__ xyz, const double *__restrict__ L,
+                                                         const double *__restrict__ W, const double *__restrict__ NL,
+                                                         const double *__restrict__ mu_nl, const double *__restrict__ mat,
+                                                         const double *__restrict__ sa, double *__restrict__ gx) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || type[p] == undamaged) return;
+    const double Vp = volume * (bptr[p + 1] - bptr[p]) / nneighbors;
+    const double Rp = hcf_rate(&mat[LPM_NMAT * p], &sa[3 * p]);
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int q = nlptr[p]; q < nlptr[p + 1]; ++q) {
+        const int64_t j = nlidx[q];
+        if (j == p || type[j] == undamaged) continue;
+        const double e[3] = {xyz[3 * p] - xyz[3 * j], xyz[3 * p + 1] - xyz[3 * j + 1], xyz[3 * p + 2] - xyz[3 * j + 2]};
+        const double r = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        if (r == 0.0) continue;
+        double c = 0.0;
+        if (W[p] > 0.0 && mu_nl[p] != 0.0) {
+            const double t = r / L[p], u = 1 / (1 + pow(t, 8.0));
+            const double dphi = -16.0 * u * u * u * pow(t, 7.0) / L[p];
+            const double Vj = volume * (bptr[j + 1] - bptr[j]) / nneighbors;
+            c += mu_nl[p] * Vj * (hcf_rate(&mat[LPM_NMAT * j], &sa[3 * j]) - NL[p]) / W[p] * dphi;
+        }
+        if (W[j] > 0.0 && mu_nl[j] != 0.0) {
+            const double t = r / L[j], u = 1 / (1 + pow(t, 8.0));
+            const double dphi = -16.0 * u * u * u * pow(t, 7.0) / L[j];
+            c += mu_nl[j] * Vp * (Rp - NL[j]) / W[j] * dphi;
+        }
+        for (int k = 0; k < 3; ++k) g[k] += c * e[k] / r;
+    }
+    for (int k = 0; k < dim; ++k) gx[(int64_t)dim * p + k] += g[k];
+}
+
+__global__ __launch_bounds__(256) void k_add(int64_t m, const double *__restrict__ a, double *__restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) b[i] += a[i];
+}
 }  // namespace
 
 static int check(lpm_ctx *c, const char *what) {
@@ -527,4 +768,173 @@ int adjoint_fatigue_params(lpm_ctx *c, const double *a_dev, const double *b_dev,
     if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
     hipFree(buf);
     return rc;
+}
+
+// ---------------------------------------------------------------- fatigue path recording
+void fat_record_clear(lpm_ctx *c) {
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &e : c->fat_events) {
+        if (e.ckpt) hipFree(e.ckpt);
+        if (e.data) hipFree(e.data);
+    }
+    c->fat_events.clear();
+}
+
+// A state-variable update at a converged state: before it (after = false) the state and the
+// trackers, after it the trackers again.
+int fat_record_state(lpm_ctx *c, bool after) {
+    const int64_t n = c->n;
+    const int g = blocks_for(n, 256);
+    if (!after) {
+        FatEvent e;
+        e.kind = FatEvent::STATE;
+        c->fat_events.push_back(e);
+        FatEvent &b = c->fat_events.back();
+        LPM_HIP(c, hipMalloc(&b.data, sizeof(double) * 6 * std::max<int64_t>(n, 1)));
+        int rc = ckpt_save_buf(c, &b.ckpt);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_sv_snap, dim3(g), dim3(256), 0, c->stream, n, (const double *)c->sv, b.data);
+        return check(c, "k_sv_snap");
+    }
+    if (c->fat_events.empty() || c->fat_events.back().kind != FatEvent::STATE)
+        return lpm_fail(c, LPM_E_STATE, "fatigue path record out of order");
+    hipLaunchKernelGGL(k_sv_snap, dim3(g), dim3(256), 0, c->stream, n, (const double *)c->sv, c->fat_events.back().data + 3 * n);
+    return check(c, "k_sv_snap");
+}
+
+// A fatigue damage update: its step, excluded type and the damage it left.
+int fat_record_update(lpm_ctx *c, double dt, int undamaged) {
+    FatEvent e;
+    e.kind = FatEvent::UPDATE;
+    e.dt = dt;
+    e.undamaged = undamaged;
+    c->fat_events.push_back(e);
+    FatEvent &b = c->fat_events.back();
+    LPM_HIP(c, hipMalloc(&b.data, sizeof(double) * std::max<int64_t>(c->n, 1)));
+    LPM_HIP(c, hipMemcpyAsync(b.data, c->damage, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    return LPM_OK;
+}
+
+// dG/dA, dG/dB, dG/dd per particle (host arrays, n each, any may be null) for G = a.Pin + b.x at
+// the current equilibrium, through every recorded state and fatigue update: one transposed solve
+// at the current state, then, going back over the record, one per recorded state whose energy
+// reaches a later update (through the trackers), each with the objective sum_i muE_i E_i plus,
+// at the state an update was taken in, the nonlocal weights' dependence on its positions.
+int adjoint_fatigue_path(lpm_ctx *c, const double *a_dev, const double *b_dev, double *gA, double *gB, double *gd,
+                         double *G, int *solves, double *max_rr) {
+    if (c->model != 3) return lpm_fail(c, LPM_E_STATE, "fatigue-path sensitivity needs the HCF model");
+    if (!c->nlptr) return lpm_fail(c, LPM_E_STATE, "no nonlocal neighbour lists");
+    auto &ev = c->fat_events;
+    bool any_update = false;
+    for (size_t k = 0; k < ev.size(); ++k)
+        if (ev[k].kind == FatEvent::UPDATE) {
+            any_update = true;
+            if (k == 0 || ev[k - 1].kind != FatEvent::STATE)
+                return lpm_fail(c, LPM_E_STATE, "a recorded fatigue update must follow a recorded state update");
+        }
+    if (!any_update) return lpm_fail(c, LPM_E_STATE, "no recorded fatigue damage update");
+    const int64_t n = c->n, nb = std::max<int64_t>(c->nbond, 1), nd = n * c->dim;
+    const int gn = blocks_for(n, 256), gb = blocks_for(nb, 256), gdx = blocks_for(nd, 256);
+    // mu_bd, bd_next, coef (nb each); mu_D, muE, mu_nl, W, NL (n); mu_s, gth (3n); gx, gxw (nd)
+    double *buf = nullptr;
+    const size_t tot = 3 * (size_t)nb + 5 * (size_t)n + 6 * (size_t)n + 2 * (size_t)nd;
+    LPM_HIP(c, hipMalloc(&buf, sizeof(double) * std::max<size_t>(tot, 1)));
+    double *mu_bd = buf, *bd_next = mu_bd + nb, *coef = bd_next + nb, *mu_D = coef + nb, *muE = mu_D + n;
+    double *mu_nl = muE + n, *W = mu_nl + n, *NL = W + n, *mu_s = NL + n, *gth = mu_s + 3 * n, *gx = gth + 3 * n;
+    double *gxw = gx + nd;
+    void *final_state = nullptr;
+    auto done = [&](int code) {
+        if (final_state) {
+            if (!code) code = ckpt_restore_buf(c, final_state);
+            hipFree(final_state);
+        }
+        if (hipStreamSynchronize(c->stream) != hipSuccess && !code) code = lpm_fail(c, LPM_E_HIP, "stream synchronisation");
+        hipFree(buf);
+        return code;
+    };
+    if (hipMemsetAsync(buf, 0, sizeof(double) * tot, c->stream) != hipSuccess) return done(lpm_fail(c, LPM_E_HIP, "clear"));
+    int rc = ckpt_save_buf(c, &final_state);
+    if (rc) return done(rc);
+    int nsolve = 0, it = 0;
+    double rr = 0.0, rmax = 0.0;
+    // the objective at the current state
+    rc = adjoint_linear(c, a_dev, b_dev, nullptr, nullptr, nullptr, G, &it, &rr, mu_bd);
+    if (rc) return done(rc);
+    ++nsolve;
+    rmax = rr;
+    if (hipMemcpyAsync(bd_next, c->bdamage, sizeof(double) * c->nbond, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        return done(lpm_fail(c, LPM_E_HIP, "copy"));
+    bool pending_w = false;    // gxw holds the weights' term for the next state back
+    int64_t restored = -1;     // event whose state the context holds
+    for (int64_t k = (int64_t)ev.size() - 1; k >= 0; --k) {
+        const FatEvent &e = ev[k];
+        if (e.kind == FatEvent::UPDATE) {
+            // the state the update was taken in: its bond damage, positions and trackers
+            const FatEvent &s = ev[k - 1];
+            if ((rc = ckpt_restore_buf(c, s.ckpt))) return done(rc);
+            restored = k - 1;
+            const double *sa = s.data + 3 * n;
+            hipLaunchKernelGGL(k_fat_route, dim3(gn), dim3(256), 0, c->stream, n, c->bptr, c->p2, c->opp, e.data,
+                               (const double *)c->bdamage, (const double *)bd_next, (const double *)mu_bd, mu_D);
+            hipLaunchKernelGGL(k_fat_route_clear, dim3(gb), dim3(256), 0, c->stream, c->nbond, (const double *)c->bdamage,
+                               (const double *)bd_next, mu_bd);
+            hipLaunchKernelGGL(k_fat_update_rev, dim3(gn), dim3(256), 0, c->stream, n, e.undamaged, e.dt, c->type, e.data, mu_D,
+                               mu_nl);
+            hipLaunchKernelGGL(k_fat_nl, dim3(gn), dim3(256), 0, c->stream, n, e.undamaged, c->volume, c->nneighbors, c->type,
+                               c->nlptr, c->nlidx, c->bptr, (const double *)c->xyz, (const double *)c->nonlocal_L,
+                               (const double *)c->mat, sa, W, NL);
+            hipLaunchKernelGGL(k_fat_rate_rev, dim3(gn), dim3(256), 0, c->stream, n, e.undamaged, c->volume, c->nneighbors,
+                               c->type, c->nlptr, c->nlidx, c->bptr, (const double *)c->xyz, (const double *)c->nonlocal_L,
+                               (const double *)W, (const double *)mu_nl, (const double *)c->mat, sa, gth, mu_s);
+            if (hipMemsetAsync(gxw, 0, sizeof(double) * nd, c->stream) != hipSuccess) return done(lpm_fail(c, LPM_E_HIP, "clear"));
+            hipLaunchKernelGGL(k_fat_weights_gx, dim3(gn), dim3(256), 0, c->stream, n, c->dim, e.undamaged, c->volume,
+                               c->nneighbors, c->type, c->nlptr, c->nlidx, c->bptr, (const double *)c->xyz,
+                               (const double *)c->nonlocal_L, (const double *)W, (const double *)NL, (const double *)mu_nl,
+                               (const double *)c->mat, sa, gxw);
+            if ((rc = check(c, "fatigue update adjoint"))) return done(rc);
+            pending_w = true;
+            continue;
+        }
+        // a state-variable update: the trackers back to the state's energy
+        if (restored != k && (rc = ckpt_restore_buf(c, e.ckpt))) return done(rc);
+        restored = k;
+        hipLaunchKernelGGL(k_fat_track_rev, dim3(gn), dim3(256), 0, c->stream, n, (const double *)e.data,
+                           (const double *)(e.data + 3 * n), mu_s, muE);
+        if (hipMemsetAsync(gx, 0, sizeof(double) * nd, c->stream) != hipSuccess) return done(lpm_fail(c, LPM_E_HIP, "clear"));
+        hipLaunchKernelGGL(k_energy_coef, dim3(gn), dim3(256), 0, c->stream, n, c->volume, c->nneighbors, c->bptr, c->bsplit,
+                           (const double *)c->Kn, (const double *)c->Tv, (const double *)c->dLe, (const double *)c->dLe_total,
+                           (const double *)c->TdLe_total, (const double *)muE, coef);
+        hipLaunchKernelGGL(k_energy_gx, dim3(gn), dim3(256), 0, c->stream, n, c->dim, c->bptr, c->opp, (const double *)c->cs,
+                           (const double *)coef, gx);
+        if (pending_w) {
+            hipLaunchKernelGGL(k_add, dim3(gdx), dim3(256), 0, c->stream, nd, (const double *)gxw, gx);
+            pending_w = false;
+        }
+        if ((rc = check(c, "state adjoint objective"))) return done(rc);
+        // skip the solve when the objective has no weight here
+        double g2 = 0.0;
+        {
+            std::vector<double> h(nd);
+            LPM_HIP(c, hipMemcpyAsync(h.data(), gx, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream));
+            LPM_HIP(c, hipStreamSynchronize(c->stream));
+            for (double v : h) g2 += v * v;
+        }
+        if (g2 > 0.0) {
+            double *dbd = coef;  // coef is free again
+            if ((rc = adjoint_linear(c, nullptr, gx, nullptr, nullptr, nullptr, nullptr, &it, &rr, dbd))) return done(rc);
+            ++nsolve;
+            rmax = std::max(rmax, rr);
+            hipLaunchKernelGGL(k_add, dim3(gb), dim3(256), 0, c->stream, c->nbond, (const double *)dbd, mu_bd);
+            if ((rc = check(c, "k_add"))) return done(rc);
+        }
+        if (hipMemcpyAsync(bd_next, c->bdamage, sizeof(double) * c->nbond, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            return done(lpm_fail(c, LPM_E_HIP, "copy"));
+    }
+    double *outs[3] = {gA, gB, gd};
+    for (int k = 0; k < 3 && !rc; ++k)
+        if (outs[k] && hipMemcpyAsync(outs[k], gth + k * n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = lpm_fail(c, LPM_E_HIP, "fatigue-path sensitivity copy");
+    if (solves) *solves = nsolve;
+    if (max_rr) *max_rr = rmax;
+    return done(rc);
 }

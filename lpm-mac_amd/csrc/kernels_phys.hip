@@ -1833,10 +1833,16 @@ int launch_assemble_fd(lpm_ctx *c) {
 
 int launch_state_var(lpm_ctx *c) {
     if (c->model == 0) return LPM_OK;  // ParticleElastic has no state update (particle.h:62)
+    const bool rec = c->fat_rec && c->model == HCF;
+    if (rec) {
+        const int rc = fat_record_state(c, false);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_state_var, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->model, c->volume,
                        c->nneighbors, c->bptr, c->bsplit, c->Kn, c->dLe, c->dLe_total, c->TdLe_total, c->stress, c->cs,
                        c->dis, c->mat, c->damage, c->sv_last, c->sv, c->dLp, c->Ddot_local);
-    return check_launch(c, "k_state_var");
+    const int rc = check_launch(c, "k_state_var");
+    return (rc || !rec) ? rc : fat_record_state(c, true);
 }
 
 int launch_nonlocal_ddot(lpm_ctx *c, int undamaged) {
@@ -1884,6 +1890,7 @@ int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any) {
                        dNdt, c->type, c->mat, c->Ddot_nonlocal, c->damage_last, c->damage, c->sv, c->iflag);
     rc = check_launch(c, "k_fatigue_damage");
     if (rc) return rc;
+    if (c->fat_rec && c->model == HCF && (rc = fat_record_update(c, dNdt, undamaged))) return rc;
     return read_flag(c, any);
 }
 

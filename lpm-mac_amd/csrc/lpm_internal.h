@@ -12,7 +12,7 @@
 // Compiled limits, checked on the host before any launch.
 #define CG_VEC_WG 1024  // most workgroups of the CG vector kernels (grid-stride beyond)
 #define LPM_MAXB 32    // bonds per particle (hex 12, SC/FCC 18, BCC 14)
-#define LPM_BOND_GP 16 // consecutive particles per workgroup of the bond kernels (their LDS is sized by
+#define LPM_BOND_GP 8  // consecutive particles per workgroup of the bond kernels (their LDS is sized by
                        // the most bonds of any such group, gp_bonds)
 #define LPM_MAXC 512   // 2-hop candidates per particle before de-duplication
 #define LPM_NSV 7      // state variables per particle (J2 uses 7)
@@ -43,6 +43,19 @@ struct Group {
     int32_t *ids = nullptr;
     int64_t n = 0;
     bool dup = false;  // duplicate ids: applied by one thread in list order
+};
+
+// One event of a recorded fatigue load path (lpm_fatigue_record): a state-variable update at a
+// converged state (a checkpoint of the state, and s0, s1, s2 of every particle before and after
+// the update: 6 per particle) or a fatigue damage update (its step, excluded type and the damage
+// it left: 1 per particle)
+struct FatEvent {
+    enum { STATE = 1, UPDATE = 2 };
+    int kind = 0;
+    double dt = 0.0;
+    int undamaged = -1;
+    void *ckpt = nullptr;
+    double *data = nullptr;
 };
 
 struct lpm_ctx {
@@ -148,6 +161,9 @@ struct lpm_ctx {
     double fat_dt = 0.0;
     int fat_undamaged = -1;
     bool fat_valid = false;
+    // fatigue load path record (lpm_fatigue_record)
+    bool fat_rec = false;
+    std::vector<FatEvent> fat_events;
     // Specimen ensemble: independent particle systems (contiguous id ranges, no bond between
     // them) solved in the same launches, each with its own Newton test.
     int nspec = 0;                    // 0: one system
@@ -269,6 +285,14 @@ int adjoint_linear(lpm_ctx *c, const double *a_dev, const double *b_dev, double 
                    int *refinements, double *rel_residual, double *gBd_dev = nullptr);
 int adjoint_fatigue_params(lpm_ctx *c, const double *a_dev, const double *b_dev, double *gA, double *gB, double *gd,
                            double *G, int *refinements, double *rel_residual);
+int adjoint_fatigue_path(lpm_ctx *c, const double *a_dev, const double *b_dev, double *gA, double *gB, double *gd,
+                         double *G, int *solves, double *max_rr);
+int fat_record_state(lpm_ctx *c, bool after);
+int fat_record_update(lpm_ctx *c, double dt, int undamaged);
+void fat_record_clear(lpm_ctx *c);
+// checkpoints into a buffer of the context's own (allocated by save when *buf is null)
+int ckpt_save_buf(lpm_ctx *c, void **buf);
+int ckpt_restore_buf(lpm_ctx *c, void *buf);
 
 // helpers
 static inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -110,6 +110,8 @@ _SIGS = {
     "lpm_adjoint_reaction": (_I, [_P, ctypes.c_int32, ctypes.c_int32, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
     "lpm_adjoint_linear": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
     "lpm_adjoint_fatigue_params": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
+    "lpm_fatigue_record": (_I, [_P, ctypes.c_int32]),
+    "lpm_adjoint_fatigue_path": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32), _DP]),
     "lpm_checkpoint_save": (_I, [_P, ctypes.c_int32]),
     "lpm_checkpoint_restore": (_I, [_P, ctypes.c_int32]),
     "lpm_checkpoint_release": (_I, [_P, ctypes.c_int32]),
@@ -552,6 +554,26 @@ class Context:
                                                    gd.ctypes.data_as(_DP), ctypes.byref(G), ctypes.byref(it),
                                                    ctypes.byref(rr)))
         return G.value, ga, gb, gd, it.value, rr.value
+
+    def fatigue_record(self, on=True):
+        """Start (dropping any earlier record) or stop recording the fatigue load path."""
+        self._chk(lib().lpm_fatigue_record(self.h, 1 if on else 0))
+
+    def adjoint_fatigue_path(self, w_pin=None, w_x=None):
+        """dG/dA, dG/dB, dG/dd per particle (HCF law) through the whole recorded fatigue path, for
+        G = w_pin . Pin + w_x . x at the current equilibrium. Returns (G, dA, dB, dd, transposed
+        solves, largest relative residual)."""
+        nd = self.n * self.size("dim")
+        w = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (w_pin, w_x)]
+        for a in w:
+            assert a is None or a.size == nd
+        ga, gb, gd = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        G, rr, ns = _D(), _D(), ctypes.c_int32()
+        ptr = [None if a is None else a.ctypes.data_as(_DP) for a in w]
+        self._chk(lib().lpm_adjoint_fatigue_path(self.h, ptr[0], ptr[1], ga.ctypes.data_as(_DP), gb.ctypes.data_as(_DP),
+                                                 gd.ctypes.data_as(_DP), ctypes.byref(G), ctypes.byref(ns),
+                                                 ctypes.byref(rr)))
+        return G.value, ga, gb, gd, ns.value, rr.value
 
     def checkpoint_save(self, slot):
         self._chk(lib().lpm_checkpoint_save(self.h, int(slot)))
