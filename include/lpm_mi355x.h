@@ -199,6 +199,11 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        row's path lists in LDS on every assembly. Both give the same blocks bit
  *                        for bit; patterns with a partner repeated within a layer use the row-list
  *                        kernel either way
+ *  "nl_cells"         1 (default): lpm_search_nonlocal bins the particles (of each specimen) on a
+ *                        grid of cells no narrower than the largest cutoff and tests the 27 (9
+ *                        in 2D) surrounding cells, then sorts each list by id | 0: every pair is
+ *                        tested. Both give the same lists; the grid falls back to every pair
+ *                        when a list exceeds 4096 entries
  *  "spmv_nt"          1 (default): the SpMV reads the matrix values and column indices with
  *                        non-temporal loads (read once per SpMV; the input vector keeps the
  *                        caches) | 0: ordinary loads

@@ -837,6 +837,7 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "verbose") c->verbose = (int)v;
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
     else if (w == "asm_plan") c->asm_plan = (int)v ? 1 : 0;
+    else if (w == "nl_cells") c->nl_cells = (int)v ? 1 : 0;
     else if (w == "spmv_nt") {
         c->spmv_nt = (int)v ? 1 : 0;
         c->tune_key = -1;
@@ -896,6 +897,7 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "cg_batch") return c->cg_batch;
     if (w == "spmv_nt") return c->spmv_nt;
     if (w == "asm_plan") return c->asm_plan;
+    if (w == "nl_cells") return c->nl_cells;
     if (w == "spmv_tune") return c->spmv_tune;
     if (w == "cg_forcing") return c->cg_forcing;
     if (w == "coarse_graph") return c->coarse_graph;

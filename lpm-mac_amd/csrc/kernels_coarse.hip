@@ -324,27 +324,31 @@ __global__ __launch_bounds__(RESTRICT_WG) void k_coarse_restrict(const int32_t *
     const int beg = agg_ptr[g], end = agg_ptr[g + 1];
     double acc[M];
     for (int a = 0; a < M; ++a) acc[a] = 0.0;
-    // the first members' ids are requested together with the stop flag
+    // the first members' ids and offsets are requested together with the stop flag
     int64_t ii0[RESTRICT_PER];
+    double dm0[RESTRICT_PER][3];
 #pragma unroll
     for (int u = 0; u < RESTRICT_PER; ++u) {
         const int m = beg + (int)threadIdx.x + u * RESTRICT_WG;
         ii0[u] = m < end ? agg_idx[m] : -1;
+        for (int k = 0; k < 3; ++k) dm0[u][k] = m < end ? agg_dm[3 * (int64_t)m + k] : 0.0;
     }
     if (scal[SC_DONE] != 0.0) return;
     for (int m0 = beg + (int)threadIdx.x; m0 < end; m0 += RESTRICT_WG * RESTRICT_PER) {
+        const bool first = m0 == beg + (int)threadIdx.x;
         int64_t ii[RESTRICT_PER];
 #pragma unroll
         for (int u = 0; u < RESTRICT_PER; ++u) {
             const int m = m0 + u * RESTRICT_WG;
-            ii[u] = m0 == beg + (int)threadIdx.x ? ii0[u] : (m < end ? agg_idx[m] : -1);
+            ii[u] = first ? ii0[u] : (m < end ? agg_idx[m] : -1);
         }
 #pragma unroll
         for (int u = 0; u < RESTRICT_PER; ++u) {
             const int m = m0 + u * RESTRICT_WG;
             if (ii[u] >= 0) {
-                double P[3][6];
-                rigid_modes<D>(&agg_dm[3 * (int64_t)m], mask[ii[u]], P);
+                double P[3][6], d[3];
+                for (int k = 0; k < 3; ++k) d[k] = first ? dm0[u][k] : agg_dm[3 * (int64_t)m + k];
+                rigid_modes<D>(d, mask[ii[u]], P);
                 double ri[3] = {0, 0, 0};
                 for (int k = 0; k < D; ++k) ri[k] = (double)r[VP * ii[u] + k];
                 for (int a = 0; a < M; ++a)
@@ -384,8 +388,19 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(const int32_t *__restrict__
         const int c = lane + 64 * q;
         a[q] = (r < nc && c < nc) ? Arow[c] : 0.0;
     }
+    // cz is requested with the stop flag too (a group has at most 4 x 256 DoFs)
+    double zq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = (int)threadIdx.x + q * (int)blockDim.x;
+        zq[q] = c < nc ? czf[base + c] : 0.0;
+    }
     if (scal[SC_DONE] != 0.0) return;
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) cz[c] = czf[base + c];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = (int)threadIdx.x + q * (int)blockDim.x;
+        if (c < nc) cz[c] = zq[q];
+    }
     __syncthreads();
     if (r >= nc) return;
     double s = 0.0;

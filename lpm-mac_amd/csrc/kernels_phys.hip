@@ -1233,6 +1233,88 @@ __global__ __launch_bounds__(256) void k_nonlocal_search(int64_t n, double ratio
     if (act && !nlidx) counts[i] = cnt;
 }
 
+// The same lists from a grid of cells no narrower than the largest cutoff, one layer of cells per
+// specimen (cell = (specimen * nc2 + z) * nc1 + y) * nc0 + x): each particle tests the members of
+// the 27 surrounding cells of its own layer with the expression above, then k_nl_sort orders
+// every list by id, the order of the all-pairs loop.
+__global__ __launch_bounds__(256) void k_nl_bin(int64_t n, const double *__restrict__ xyz, double lo0, double lo1,
+                                                 double lo2, double h, int64_t nc0, int64_t nc1, int64_t nc2,
+                                                 const int32_t *__restrict__ spec_of, int32_t *__restrict__ pcell,
+                                                 int32_t *__restrict__ count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double lo[3] = {lo0, lo1, lo2};
+    const int64_t nc[3] = {nc0, nc1, nc2};
+    int64_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        q[k] = (int64_t)floor((xyz[3 * i + k] - lo[k]) / h);
+        q[k] = q[k] < 0 ? 0 : (q[k] > nc[k] - 1 ? nc[k] - 1 : q[k]);
+    }
+    const int64_t layer = spec_of ? spec_of[i] : 0;
+    const int64_t cell = ((layer * nc2 + q[2]) * nc1 + q[1]) * nc0 + q[0];
+    pcell[i] = (int32_t)cell;
+    atomicAdd(&count[cell], 1);
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_nl_cells(int64_t n, double ratio, const double *__restrict__ xyz,
+                                                   const double *__restrict__ L, const int32_t *__restrict__ pcell,
+                                                   int64_t nc0, int64_t nc1, int64_t nc2, const int32_t *__restrict__ head,
+                                                   const int32_t *__restrict__ items, int32_t *__restrict__ counts,
+                                                   const int32_t *__restrict__ nlptr, int32_t *__restrict__ nlidx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2], cut = ratio * L[i];
+    const int64_t ncs = nc0 * nc1 * nc2, cell = pcell[i];
+    const int64_t layer = cell / ncs, loc = cell - layer * ncs;
+    const int64_t cx = loc % nc0, cy = (loc / nc0) % nc1, cz = loc / (nc0 * nc1);
+    int cnt = 0;
+    int64_t o = FILL ? nlptr[i] : 0;
+    for (int64_t ez = -1; ez <= 1; ++ez)
+        for (int64_t ey = -1; ey <= 1; ++ey)
+            for (int64_t ex = -1; ex <= 1; ++ex) {
+                const int64_t x = cx + ex, y = cy + ey, z = cz + ez;
+                if (x < 0 || y < 0 || z < 0 || x >= nc0 || y >= nc1 || z >= nc2) continue;
+                const int64_t cc = layer * ncs + (z * nc1 + y) * nc0 + x;
+                for (int t = head[cc]; t < head[cc + 1]; ++t) {
+                    const int64_t j = items[t];
+                    const double dx = x0 - xyz[3 * j], dy = y0 - xyz[3 * j + 1], dz = z0 - xyz[3 * j + 2];
+                    if (sqrt(dx * dx + dy * dy + dz * dz) < cut) {
+                        if (FILL) nlidx[o++] = (int32_t)j;
+                        ++cnt;
+                    }
+                }
+            }
+    if (!FILL) counts[i] = cnt;
+}
+
+// Sorts list i (workgroup i, 64 threads) ascending: a bitonic network in LDS over the list
+// padded with INT32_MAX to P, the power of two the launch sizes the LDS for (P >= every length).
+__global__ __launch_bounds__(64) void k_nl_sort(const int32_t *__restrict__ nlptr, int32_t *__restrict__ nlidx) {
+    extern __shared__ int32_t srt[];
+    const int64_t i = blockIdx.x;
+    const int o = nlptr[i], k = nlptr[i + 1] - o;
+    if (k <= 1) return;  // uniform over the workgroup
+    int P = 2;
+    while (P < k) P <<= 1;
+    for (int e = threadIdx.x; e < P; e += 64) srt[e] = e < k ? nlidx[o + e] : INT32_MAX;
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < (P >> 1); t += 64) {
+                const int a = 2 * t - (t & (stride - 1)), b = a + stride;  // pair t of this stage
+                const bool up = (a & size) == 0;
+                const int32_t u = srt[a], v = srt[b];
+                if ((u > v) == up) {
+                    srt[a] = v;
+                    srt[b] = u;
+                }
+            }
+            __syncthreads();
+        }
+    for (int e = threadIdx.x; e < k; e += 64) nlidx[o + e] = srt[e];
+}
+
 
 // ---------------------------------------------------------------- bond search on the device
 // Assembly::createBonds (assembly.h:212-234) tests every pair; here particles are binned on a
@@ -2076,7 +2158,125 @@ int launch_tslot(lpm_ctx *c) {
     return LPM_OK;
 }
 
+// Cell-grid nonlocal search (mode nl_cells). Sets *done when it built the lists; leaves them
+// to the all-pairs search (*done false, nothing changed) when a coordinate or the largest cutoff
+// is not finite, the grid would need 2^31 cells, or a list exceeds the sort's NL_SORT_MAX.
+constexpr int NL_SORT_MAX = 4096;
+
+static int nonlocal_search_cells(lpm_ctx *c, double ratio, bool *done) {
+    *done = false;
+    const int64_t n = c->n;
+    std::vector<double> hx(3 * n), hL(n);
+    LPM_HIP(c, hipMemcpyAsync(hx.data(), c->xyz, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(hL.data(), c->nonlocal_L, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, cutmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            const double v = hx[3 * i + k];
+            if (!std::isfinite(v)) return LPM_OK;
+            lo[k] = std::min(lo[k], v);
+            hi[k] = std::max(hi[k], v);
+        }
+        const double cut = ratio * hL[i];  // a NaN cutoff accepts no partner and does not widen the grid
+        if (cut > cutmax) cutmax = cut;
+    }
+    if (!std::isfinite(cutmax)) return LPM_OK;
+    // a cell edge above every cutoff: partners lie in adjacent cells; at most max(4 n, 65536)
+    // cells (coarser cells only add candidates)
+    double h = cutmax > 0.0 ? cutmax * (1.0 + 1e-6) : 1.0;
+    const int64_t layers = c->nspec > 0 ? c->nspec : 1, cap = std::max<int64_t>(4 * n, 65536);
+    int64_t nc[3], ncell = 0;
+    for (;;) {
+        double cells = (double)layers;
+        for (int k = 0; k < 3; ++k) {
+            nc[k] = std::max<int64_t>(1, (int64_t)std::floor((hi[k] - lo[k]) / h) + 1);
+            cells *= (double)nc[k];
+        }
+        if (cells <= (double)cap) {
+            ncell = layers * nc[0] * nc[1] * nc[2];
+            break;
+        }
+        h *= 1.5;
+    }
+    if (ncell >= 0x7fffffffLL) return LPM_OK;
+    int32_t *pcell = nullptr, *count = nullptr, *head = nullptr, *items = nullptr, *counts = nullptr;
+    auto cleanup = [&]() {
+        void *ps[] = {pcell, count, head, items, counts};
+        for (void *q : ps)
+            if (q) hipFree(q);
+    };
+    auto al = [&](int32_t **q, int64_t m) { return hipMalloc(q, sizeof(int32_t) * std::max<int64_t>(m, 1)) == hipSuccess; };
+    if (!al(&pcell, n) || !al(&count, ncell) || !al(&head, ncell + 1) || !al(&items, n) || !al(&counts, n)) {
+        cleanup();
+        return lpm_fail(c, LPM_E_NOMEM, "nonlocal search scratch");
+    }
+    int rc = LPM_OK;
+    auto hip_ok = [&](hipError_t e, const char *what) {
+        if (e == hipSuccess) return true;
+        rc = lpm_fail(c, LPM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+        return false;
+    };
+    std::vector<int32_t> h_count(ncell), h_head(ncell + 1, 0), h_cnt(n);
+    std::vector<int32_t> p(n + 1, 0);
+    int maxlen = 0;
+    if (!hip_ok(hipMemsetAsync(count, 0, sizeof(int32_t) * ncell, c->stream), "nonlocal search")) { cleanup(); return rc; }
+    hipLaunchKernelGGL(k_nl_bin, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const double *)c->xyz, lo[0], lo[1],
+                       lo[2], h, nc[0], nc[1], nc[2], (const int32_t *)(c->nspec > 0 ? c->spec_of : nullptr), pcell, count);
+    if ((rc = check_launch(c, "k_nl_bin"))) { cleanup(); return rc; }
+    if (!hip_ok(hipMemcpyAsync(h_count.data(), count, sizeof(int32_t) * ncell, hipMemcpyDeviceToHost, c->stream), "nonlocal search") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "nonlocal search")) { cleanup(); return rc; }
+    for (int64_t q = 0; q < ncell; ++q) h_head[q + 1] = h_head[q] + h_count[q];
+    if (!hip_ok(hipMemcpyAsync(head, h_head.data(), sizeof(int32_t) * (ncell + 1), hipMemcpyHostToDevice, c->stream), "nonlocal search") ||
+        !hip_ok(hipMemsetAsync(count, 0, sizeof(int32_t) * ncell, c->stream), "nonlocal search")) { cleanup(); return rc; }
+    hipLaunchKernelGGL(k_cell_fill, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const int32_t *)pcell,
+                       (const int32_t *)head, count, items);
+    if ((rc = check_launch(c, "k_cell_fill"))) { cleanup(); return rc; }
+    hipLaunchKernelGGL((k_nl_cells<false>), dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, ratio, (const double *)c->xyz,
+                       (const double *)c->nonlocal_L, (const int32_t *)pcell, nc[0], nc[1], nc[2], (const int32_t *)head,
+                       (const int32_t *)items, counts, (const int32_t *)nullptr, (int32_t *)nullptr);
+    if ((rc = check_launch(c, "k_nl_cells(count)"))) { cleanup(); return rc; }
+    if (!hip_ok(hipMemcpyAsync(h_cnt.data(), counts, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream), "nonlocal search") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "nonlocal search")) { cleanup(); return rc; }
+    for (int64_t i = 0; i < n; ++i) {
+        if ((int64_t)p[i] + h_cnt[i] > 0x7fffffff) { cleanup(); return lpm_fail(c, LPM_E_LIMIT, "nonlocal list too long"); }
+        p[i + 1] = p[i] + h_cnt[i];
+        maxlen = std::max(maxlen, h_cnt[i]);
+    }
+    if (maxlen > NL_SORT_MAX) {
+        cleanup();
+        return LPM_OK;
+    }
+    if (c->nlptr) hipFree(c->nlptr);
+    if (c->nlidx) hipFree(c->nlidx);
+    c->nlptr = c->nlidx = nullptr;
+    c->nnl = p[n];
+    if (!hip_ok(hipMalloc(&c->nlptr, sizeof(int32_t) * (n + 1)), "nonlocal lists") ||
+        !hip_ok(hipMalloc(&c->nlidx, sizeof(int32_t) * (c->nnl > 0 ? c->nnl : 1)), "nonlocal lists") ||
+        !hip_ok(hipMemcpyAsync(c->nlptr, p.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, c->stream), "nonlocal lists")) {
+        cleanup();
+        return rc;
+    }
+    hipLaunchKernelGGL((k_nl_cells<true>), dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, ratio, (const double *)c->xyz,
+                       (const double *)c->nonlocal_L, (const int32_t *)pcell, nc[0], nc[1], nc[2], (const int32_t *)head,
+                       (const int32_t *)items, counts, (const int32_t *)c->nlptr, c->nlidx);
+    if ((rc = check_launch(c, "k_nl_cells(fill)"))) { cleanup(); return rc; }
+    int P = 2;
+    while (P < maxlen) P <<= 1;
+    hipLaunchKernelGGL(k_nl_sort, dim3(n), dim3(64), sizeof(int32_t) * P, c->stream, (const int32_t *)c->nlptr, c->nlidx);
+    if ((rc = check_launch(c, "k_nl_sort"))) { cleanup(); return rc; }
+    hip_ok(hipStreamSynchronize(c->stream), "nonlocal search");
+    cleanup();
+    if (!rc) *done = true;
+    return rc;
+}
+
 int launch_nonlocal_search(lpm_ctx *c, double ratio) {
+    if (c->nl_cells && c->n > 0) {
+        bool done = false;
+        const int rc = nonlocal_search_cells(c, ratio, &done);
+        if (rc || done) return rc;
+    }
     int32_t *counts = nullptr;
     LPM_HIP(c, hipMalloc(&counts, sizeof(int32_t) * (c->n + 1)));
     hipLaunchKernelGGL(k_nonlocal_search, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, ratio, c->xyz,

@@ -66,7 +66,7 @@ struct lpm_ctx {
     double radius = 0, volume = 0, cutoff[2] = {0, 0};
     int64_t n = 0, nbond = 0, nconn = 0, nnl = 0;
     int maxconn = 0, maxbond = 0;
-    int gp_bonds = 0;  // most bonds of any group of 32 consecutive particles (LDS of the bond kernels)
+    int gp_bonds = 0;  // most bonds of any group of LPM_BOND_GP consecutive particles (LDS of the bond kernels)
     int asm_hops = 0;  // most 2-hop bond records of any particle (LDS of the assembly)
     // Assembly plan (mode "asm_plan", k_asm_plan): per row, its blocks by kind and each block's
     // 2-hop paths; built from the pattern on the first planned assembly
@@ -78,6 +78,7 @@ struct lpm_ctx {
     int diag10_literal = 1, cg_transpose = 1, dirichlet_static = 0, rollback_pex = 0;
     int precision = 0, precond = 0, verbose = 0, max_NR_iter = 30, cg_batch = 16;
     int spmv_nt = 1;       // 1: the SpMV reads the matrix stream with non-temporal loads (0: plain loads)
+    int nl_cells = 1;      // 1: the nonlocal search bins particles on a grid (0: every pair is tested)
     double cg_tol = 2.220446049250313e-16, tol_NR = 1e-6;
     int cg_forcing = 0;  // 1: per-Newton-iteration CG tolerance from the remaining reduction
     int64_t cg_maxit = -1;
