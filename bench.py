@@ -262,6 +262,8 @@ def main():
     ap.add_argument("--precond", type=int, default=2,
                     help="0 Jacobi (as the reference's Eigen CG), 1 block Jacobi, 2 Jacobi + rigid-body coarse correction")
     ap.add_argument("--coarse-cells", type=int, default=0, help="aggregate edge of the two-level preconditioner (0: auto)")
+    ap.add_argument("--coarse-budget", type=int, default=0,
+                    help="coarse DoFs per specimen of the automatic aggregate size (0: the library default)")
     ap.add_argument("--predictor", type=int, default=1,
                     help="1: start each static step from the previous step's converged increment (Newton runs to the "
                          "same test); 0: the reference's start")
@@ -496,6 +498,8 @@ def ensemble_bench(args):
         ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond,
                  cg_tol=args.cg_tol, cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=30, cg_maxit=50000,
                  predictor=args.predictor)
+        if args.coarse_budget > 0:
+            ctx.mode(coarse_budget=args.coarse_budget)
         return ctx
 
     def bcs(xyzs, offs, pulls_):
@@ -670,6 +674,8 @@ def fatigue_ensemble_bench(args):
         ctx.search_nonlocal(1.5)
         ctx.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond,
                  cg_tol=args.cg_tol, cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=30, cg_maxit=50000)
+        if args.coarse_budget > 0:
+            ctx.mode(coarse_budget=args.coarse_budget)
         ctx.update_geometry()
         ctx.update_force_state()
 

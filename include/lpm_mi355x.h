@@ -184,8 +184,10 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        of every solve; re-captured when sizes, launch configuration or buffers
  *                        change. Per-SpMV timing is off in this mode | 0 (default): launch directly
  *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 (0: smallest of 8, 9,
- *                        10, ... giving at most 768 coarse DoFs); get_mode("ncoarse") reads the
- *                        coarse dimension once built
+ *                        10, ... giving at most "coarse_budget" coarse DoFs per specimen);
+ *                        get_mode("ncoarse") reads the coarse dimension once built
+ *  "coarse_budget"    coarse DoFs per specimen the automatic size stays within (1..1024,
+ *                        default 768); like coarse_cells, fixed once the coarse space exists
  *  "spmv_tune"        1: on the first solve of each matrix kind time every SpMV launch
  *                        configuration and keep the fastest | 0: use the values below
  *  "spmv_unr" (1,2,4), "spmv_wpb" (wavefronts per workgroup 1,2,4,8), "spmv_remap" (0/1:

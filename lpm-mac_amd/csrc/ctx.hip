@@ -831,6 +831,10 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     } else if (w == "coarse_cells") {
         if (c->agg_of) return lpm_fail(c, LPM_E_STATE, "coarse_cells is fixed once the coarse space exists");
         c->coarse_cells = std::max(0, (int)v);
+    } else if (w == "coarse_budget") {
+        if (c->agg_of) return lpm_fail(c, LPM_E_STATE, "coarse_budget is fixed once the coarse space exists");
+        if (v < 1 || v > 1024) return lpm_fail(c, LPM_E_ARG, "coarse_budget must lie in 1..1024");
+        c->coarse_budget = (int)v;
     }
     else if (w == "max_NR_iter") c->max_NR_iter = (int)v;
     else if (w == "tol_NR") c->tol_NR = v;
@@ -905,6 +909,7 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "coarse_reuse") return c->coarse_reuse;
     if (w == "predictor") return c->predictor;
     if (w == "coarse_cells") return c->coarse_cells;
+    if (w == "coarse_budget") return c->coarse_budget;
     if (w == "ncoarse") return c->ncoarse;
     if (w == "spmv_unr") return c->spmv_unr;
     if (w == "spmv_wpb") return c->spmv_wpb;

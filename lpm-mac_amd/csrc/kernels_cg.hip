@@ -804,7 +804,7 @@ static int cg_graph_prepare(lpm_ctx *c, int gv, int gs, int batch, F issue) {
         (uintptr_t)c->sp, (uintptr_t)c->cptr, (uintptr_t)c->scol, (uintptr_t)c->scol16, (uintptr_t)c->spmv_idx16, (uintptr_t)c->aval,
         (uintptr_t)c->v_x, (uintptr_t)c->v_r, (uintptr_t)c->v_u, (uintptr_t)c->v_p, (uintptr_t)c->v_s, (uintptr_t)c->v_w,
         (uintptr_t)c->invdiag, (uintptr_t)c->partial, (uintptr_t)c->scal, (uintptr_t)c->agg_of, (uintptr_t)c->agg_d,
-        (uintptr_t)c->agg_ptr, (uintptr_t)c->agg_idx, (uintptr_t)c->dofmask, (uintptr_t)c->cz, (uintptr_t)c->cy, (uintptr_t)c->agg_dm,
+        (uintptr_t)c->agg_ptr, (uintptr_t)c->agg_idx, (uintptr_t)c->dofmask, (uintptr_t)c->cz, (uintptr_t)c->cy, (uintptr_t)c->agg_dm, (uintptr_t)c->agg_max,
         (uintptr_t)c->Ainv, (uintptr_t)c->nagg, (uintptr_t)c->nmode, (uintptr_t)c->cgrp_cb,
         (uintptr_t)c->cgrp_off, (uintptr_t)c->ncgrp, (uintptr_t)c->cgrp_nbmax};
     if (c->cg_exec && key == c->cg_key) return LPM_OK;

@@ -303,22 +303,23 @@ __global__ __launch_bounds__(256) void k_coarse_undead(const int32_t *__restrict
     if (dead[i] || dead[j]) A[idx] = 0.0;
 }
 
-// Restriction cz = P^T r: one workgroup per aggregate g sums (P_i^T r_i)[m] over its members
-// into czf[g M + m]. Each thread takes every RESTRICT_WG-th member, RESTRICT_PER at a time with
-// all their loads issued first; the offsets from the centre are read in member order. (A split of
-// an aggregate over several workgroups needs a cross-workgroup sum; on this GPU the device-scope
-// fence that publishes a partial writes back the XCD's L2, which cost more than it saved.)
-constexpr int RESTRICT_WG = 1024;
+// Restriction cz = P^T r: one workgroup of RW threads per aggregate g sums (P_i^T r_i)[m] over
+// its members into czf[g M + m]. Each thread takes every RW-th member, RESTRICT_PER at a time with
+// all their loads issued first; the offsets from the centre are read in member order. RW is the
+// smallest of 64, 256 and 1024 that covers the largest aggregate in one round (else 1024), so
+// that small aggregates do not launch idle wavefronts. (A split of an aggregate over several
+// workgroups needs a cross-workgroup sum; on this GPU the device-scope fence that publishes a
+// partial writes back the XCD's L2, which cost more than it saved.)
 constexpr int RESTRICT_PER = 4;
 
-template <typename T, int D, int VP>
-__global__ __launch_bounds__(RESTRICT_WG) void k_coarse_restrict(const int32_t *__restrict__ agg_ptr,
+template <typename T, int D, int VP, int RW>
+__global__ __launch_bounds__(RW) void k_coarse_restrict(const int32_t *__restrict__ agg_ptr,
                                                                  const int32_t *__restrict__ agg_idx,
                                                                  const double *__restrict__ agg_dm,
                                                                  const uint8_t *__restrict__ mask, const T *__restrict__ r,
                                                                  double *__restrict__ czf, const double *__restrict__ scal) {
     constexpr int M = D == 3 ? 6 : 3;
-    constexpr int NW = RESTRICT_WG / 64;
+    constexpr int NW = RW / 64;
     __shared__ double red[M][NW];
     const int g = blockIdx.x;
     const int beg = agg_ptr[g], end = agg_ptr[g + 1];
@@ -329,22 +330,22 @@ __global__ __launch_bounds__(RESTRICT_WG) void k_coarse_restrict(const int32_t *
     double dm0[RESTRICT_PER][3];
 #pragma unroll
     for (int u = 0; u < RESTRICT_PER; ++u) {
-        const int m = beg + (int)threadIdx.x + u * RESTRICT_WG;
+        const int m = beg + (int)threadIdx.x + u * RW;
         ii0[u] = m < end ? agg_idx[m] : -1;
         for (int k = 0; k < 3; ++k) dm0[u][k] = m < end ? agg_dm[3 * (int64_t)m + k] : 0.0;
     }
     if (scal[SC_DONE] != 0.0) return;
-    for (int m0 = beg + (int)threadIdx.x; m0 < end; m0 += RESTRICT_WG * RESTRICT_PER) {
+    for (int m0 = beg + (int)threadIdx.x; m0 < end; m0 += RW * RESTRICT_PER) {
         const bool first = m0 == beg + (int)threadIdx.x;
         int64_t ii[RESTRICT_PER];
 #pragma unroll
         for (int u = 0; u < RESTRICT_PER; ++u) {
-            const int m = m0 + u * RESTRICT_WG;
+            const int m = m0 + u * RW;
             ii[u] = first ? ii0[u] : (m < end ? agg_idx[m] : -1);
         }
 #pragma unroll
         for (int u = 0; u < RESTRICT_PER; ++u) {
-            const int m = m0 + u * RESTRICT_WG;
+            const int m = m0 + u * RW;
             if (ii[u] >= 0) {
                 double P[3][6], d[3];
                 for (int k = 0; k < 3; ++k) d[k] = first ? dm0[u][k] : agg_dm[3 * (int64_t)m + k];
@@ -517,7 +518,7 @@ static int coarse_setup(lpm_ctx *c) {
                 if (!used[kk]) used[kk] = 1, ++ncell;
             }
             // splitting only adds aggregates: a size whose plain cells exceed the budget is skipped
-            if (c->coarse_cells <= 0 && ncell * M > 768 && ncells > 1) continue;
+            if (c->coarse_cells <= 0 && ncell * M > c->coarse_budget && ncells > 1) continue;
             for (int64_t i = first[s]; i < first[s + 1]; ++i) uf[i] = (int32_t)i;
             for (int64_t i = first[s]; i < first[s + 1]; ++i)
                 for (int b = hbp[i]; b < hbp[i + 1]; ++b) {
@@ -546,7 +547,7 @@ static int coarse_setup(lpm_ctx *c) {
                 }
             }
             if (ncells == 1) break;
-            if (c->coarse_cells > 0 || (int64_t)ns * M <= 768) break;
+            if (c->coarse_cells > 0 || (int64_t)ns * M <= c->coarse_budget) break;
         }
         if ((int64_t)ns * M > 1024)
             return lpm_fail(c, LPM_E_LIMIT, "coarse block above 1024 DoFs: raise coarse_cells");
@@ -615,6 +616,8 @@ static int coarse_setup(lpm_ctx *c) {
         LPM_HIP(c, hipStreamSynchronize(c->stream));
     }
     c->nagg = nagg;
+    c->agg_max = 0;
+    for (int g = 0; g < nagg; ++g) c->agg_max = std::max(c->agg_max, aptr[g + 1] - aptr[g]);
     c->nmode = M;
     c->ncoarse = nc;
     c->nra = (int64_t)rid.size();
@@ -776,8 +779,13 @@ int coarse_prepare(lpm_ctx *c) {
 
 template <typename T, int D, int VP>
 static int coarse_apply_t(lpm_ctx *c) {
-    hipLaunchKernelGGL((k_coarse_restrict<T, D, VP>), dim3(c->nagg), dim3(RESTRICT_WG), 0, c->stream, c->agg_ptr,
-                       c->agg_idx, c->agg_dm, c->dofmask, (const T *)c->v_r, c->cz, (const double *)c->scal);
+#define RESTRICT_LAUNCH(RW)                                                                                            \
+    hipLaunchKernelGGL((k_coarse_restrict<T, D, VP, RW>), dim3(c->nagg), dim3(RW), 0, c->stream, c->agg_ptr, c->agg_idx, \
+                       c->agg_dm, c->dofmask, (const T *)c->v_r, c->cz, (const double *)c->scal)
+    if (c->agg_max <= 64 * RESTRICT_PER) RESTRICT_LAUNCH(64);
+    else if (c->agg_max <= 256 * RESTRICT_PER) RESTRICT_LAUNCH(256);
+    else RESTRICT_LAUNCH(1024);
+#undef RESTRICT_LAUNCH
     int rc = ck(c, "k_coarse_restrict");
     if (rc) return rc;
     // 4 wavefronts per workgroup, one row of a group's block per wavefront

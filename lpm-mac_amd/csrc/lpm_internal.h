@@ -101,7 +101,9 @@ struct lpm_ctx {
     // Two-level preconditioner (precond == 2): Jacobi plus an additive coarse correction on
     // rigid-body modes of cubic aggregates of particles, M^-1 = D^-1 + P Ac^-1 P^T, Ac = P^T K P.
     int coarse_cells = 0;             // aggregate edge in lattice spacings (0: automatic)
+    int coarse_budget = 768;          // automatic size: the smallest cells whose coarse DoFs per group fit this
     int nagg = 0, nmode = 0, ncoarse = 0;
+    int agg_max = 0;                  // members of the largest aggregate (sizes the restriction's workgroups)
     int32_t *agg_of = nullptr, *agg_ptr = nullptr, *agg_idx = nullptr;  // particle -> aggregate; members
     double *agg_d = nullptr;          // x0_i - centre of its aggregate (3 per particle)
     int32_t *ra_ptr = nullptr, *ra_id = nullptr;  // distinct aggregates of the columns of each row
