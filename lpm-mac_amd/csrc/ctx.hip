@@ -77,6 +77,63 @@ __global__ void k_force_bc(int64_t m, const int32_t *__restrict__ ids, int seria
     }
 }
 
+// A load step's BCs in one launch: one workgroup applies them in order (force BCs first, as
+// above), each with the per-BC kernels' arithmetic, with a barrier between consecutive BCs so that
+// a particle named by several sees them in sequence. The descriptors travel as the kernel argument.
+struct BcItem {
+    const int32_t *ids;
+    int64_t m;
+    double v[3];                     // fx, fy, fz or (step at v[0])
+    int32_t kind, mode, axis, serial;  // kind 0 force, 1 displacement
+};
+constexpr int BC_BATCH = 64;        // 64 x 56 B of descriptors: within the 4 KB kernel argument limit
+constexpr int64_t BC_BATCH_MAX_IDS = 65536;  // larger groups keep their own grid-wide launch
+struct BcBatch {
+    BcItem it[BC_BATCH];
+    int32_t count;
+};
+
+__global__ __launch_bounds__(1024) void k_bc_batch(BcBatch b, double *__restrict__ xyz, int32_t *__restrict__ cons,
+                                                    double *__restrict__ Pex) {
+    for (int k = 0; k < b.count; ++k) {
+        const BcItem &e = b.it[k];
+        const int64_t lo = e.serial ? 0 : threadIdx.x, hi = e.serial ? (threadIdx.x == 0 ? e.m : 0) : e.m;
+        const int64_t st = e.serial ? 1 : blockDim.x;
+        if (e.kind == 0) {
+            const int num = (int)e.m;
+            for (int64_t q = lo; q < hi; q += st) {
+                const int64_t i = e.ids[q];
+                if (e.mode == LPM_RELATIVE) {
+                    Pex[3 * i] += e.v[0] / num;
+                    Pex[3 * i + 1] += e.v[1] / num;
+                    Pex[3 * i + 2] += e.v[2] / num;
+                } else {
+                    Pex[3 * i] = e.v[0] / num;
+                    Pex[3 * i + 1] = e.v[1] / num;
+                    Pex[3 * i + 2] = e.v[2] / num;
+                }
+            }
+        } else {
+            for (int64_t q = lo; q < hi; q += st) {
+                const int64_t i = e.ids[q];
+                double d[3] = {0, 0, 0};
+                d[e.axis] = e.v[0];
+                if (e.mode == LPM_RELATIVE) {
+                    xyz[3 * i] = xyz[3 * i] + d[0];
+                    xyz[3 * i + 1] = xyz[3 * i + 1] + d[1];
+                    xyz[3 * i + 2] = xyz[3 * i + 2] + d[2];
+                } else {
+                    xyz[3 * i] = d[0];
+                    xyz[3 * i + 1] = d[1];
+                    xyz[3 * i + 2] = d[2];
+                }
+                cons[3 * i + e.axis] = 1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 struct Field {
     void *p;
     size_t es;
@@ -164,11 +221,29 @@ int apply_bc(lpm_ctx *c, const lpm_load_step *s) {
     if (!s) return lpm_fail(c, LPM_E_ARG, "null load step");
     if ((s->n_force > 0 && !s->force) || (s->n_disp > 0 && !s->disp) || s->n_force < 0 || s->n_disp < 0)
         return lpm_fail(c, LPM_E_ARG, "inconsistent load step");
+    BcBatch batch;
+    batch.count = 0;
+    auto flush = [&]() -> int {
+        if (batch.count == 0) return LPM_OK;
+        hipLaunchKernelGGL(k_bc_batch, dim3(1), dim3(1024), 0, c->stream, batch, c->xyz, c->cons, c->Pex);
+        batch.count = 0;
+        LPM_HIP(c, hipGetLastError());
+        return LPM_OK;
+    };
+    auto add = [&](const BcItem &e) -> int {
+        batch.it[batch.count++] = e;
+        return batch.count == BC_BATCH ? flush() : LPM_OK;
+    };
     for (int k = 0; k < s->n_force; ++k) {  // updateForceBC first (solver_static.h:103)
         const lpm_force_bc &f = s->force[k];
         RC(check_group(c, f.group));
         const Group &g = c->groups[f.group];
         if (g.n == 0) continue;
+        if (g.n <= BC_BATCH_MAX_IDS) {
+            RC(add({g.ids, g.n, {f.fx, f.fy, f.fz}, 0, f.mode, 0, g.dup ? 1 : 0}));
+            continue;
+        }
+        RC(flush());
         const int blocks = g.dup ? 1 : blocks_for(g.n, 256);
         hipLaunchKernelGGL(k_force_bc, dim3(blocks), dim3(256), 0, c->stream, g.n, g.ids, g.dup ? 1 : 0, f.mode, f.fx, f.fy,
                            f.fz, c->Pex);
@@ -181,12 +256,17 @@ int apply_bc(lpm_ctx *c, const lpm_load_step *s) {
         if (axis < 0) continue;  // the reference ignores other flags
         const Group &g = c->groups[d.group];
         if (g.n == 0) continue;
+        if (g.n <= BC_BATCH_MAX_IDS) {
+            RC(add({g.ids, g.n, {d.step, 0.0, 0.0}, 1, d.mode, axis, g.dup ? 1 : 0}));
+            continue;
+        }
+        RC(flush());
         const int blocks = g.dup ? 1 : blocks_for(g.n, 256);
         hipLaunchKernelGGL(k_disp_bc, dim3(blocks), dim3(256), 0, c->stream, g.n, g.ids, g.dup ? 1 : 0, d.mode, axis, d.step,
                            c->xyz, c->cons);
         LPM_HIP(c, hipGetLastError());
     }
-    return LPM_OK;
+    return flush();
 }
 
 }  // namespace

@@ -23,6 +23,34 @@ def say(*a):
 
 
 # ------------------------------------------------------------------ Absolute boundary conditions
+def test_load_step_bcs_apply_in_order():
+    """Solver::updateForceBC then updateDisplacementBC (solver.h:130-190), one BC after the other:
+    overlapping groups, Absolute after Relative and back on the same particles, a group with a
+    repeated id (applied once per occurrence), and more BCs than one launch of the batched BC kernel
+    holds (64), against the oracle bit for bit in positions, Pex and constraint flags."""
+    say("[gpu] load-step BCs in order")
+    xyz, box = hex_plate(12, 12)
+    p = Pair(O.HEXAGON2D, 0.5, O.ELASTIC, xyz, box)
+    p.set_elastic(0.0, True, 69e3, 0.3, 1.0)
+    n = len(xyz)
+    a, b = np.arange(0, 40), np.arange(25, 70)
+    dup = np.array([3, 3, 4, 90, 3])
+    disp = [(a, "x", 1e-3), (b, "x", 2.5e-3, O.ABSOLUTE), (a, "y", -4e-4), (dup, "y", 1e-4), (b, "z", 0.0),
+            (dup, "x", 7e-4, O.ABSOLUTE), (a, "x", 3e-4)]
+    force = [(a, 1.0, 2.0, 0.0), (b, -3.0, 0.5, 0.0, O.ABSOLUTE), (dup, 0.25, 0.0, 0.0), (a, 0.0, 1.0, 0.0)]
+    rng = np.random.default_rng(11)
+    for k in range(70):  # one BC per particle group of 3, past the 64 of one batch
+        ids = rng.choice(n, 3, replace=False)
+        disp.append((ids, "xyz"[k % 2], float(rng.uniform(-1e-3, 1e-3)), O.ABSOLUTE if k % 5 == 0 else O.RELATIVE))
+    ls_o, ls_g = p.load(disp=disp, force=force)
+    for _ in range(2):
+        p.o.apply_bc(ls_o)
+        p.g.apply_bc(ls_g)
+        np.testing.assert_array_equal(p.g.get("xyz"), p.o.getd("xyz"))
+        np.testing.assert_array_equal(p.g.get("Pex"), p.o.getd("Pex"))
+        np.testing.assert_array_equal(p.g.get("disp_constraint"), p.o.geti("cons"))
+
+
 def test_absolute_bcs_match_oracle():
     """Solver::updateDisplacementBC / updateForceBC in LoadMode::Absolute (solver.h:140-161,
     176-183): a displacement BC moves each particle of the group *to* (step, 0, 0) (or (0, step, 0),
