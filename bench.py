@@ -447,6 +447,8 @@ def main():
                             "wavefronts_per_slice": int(ctx.get_mode("spmv_coop")) or 1,
                             "slices_per_workgroup": int(ctx.get_mode("spmv_spw")) if ctx.get_mode("spmv_coop") else None,
                             "nontemporal_matrix_loads": int(ctx.get_mode("spmv_nt"))},
+            "assembly_launch": {"rows_per_workgroup": int(ctx.get_mode("asm_wpb")), "tuned": int(ctx.get_mode("asm_tune")),
+                                "tuned_ms": ctx.stat("asm_tuned_ms")},
             "cg_solves": cg_solves,
             "newton_iters_per_step": counts[0], "cg_iters_per_step": counts[1],
             "final_residual_over_reaction": nr / nR if nR > 0 else None,

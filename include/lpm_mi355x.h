@@ -201,6 +201,11 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        row's path lists in LDS on every assembly. Both give the same blocks bit
  *                        for bit; patterns with a partner repeated within a layer use the row-list
  *                        kernel either way
+ *  "asm_tune"         1 (default): on the first planned assembly of each precision time 1, 2 and 4
+ *                        rows (wavefronts) per workgroup, as the LDS allows, and keep the fastest
+ *                        (every configuration writes the same blocks) | 0: use "asm_wpb"
+ *  "asm_wpb"          rows per workgroup of the planned assembly (1, 2, 4; 0 until tuned: the
+ *                        most the LDS allows)
  *  "nl_cells"         1 (default): lpm_search_nonlocal bins the particles (of each specimen) on a
  *                        grid of cells no narrower than the largest cutoff and tests the 27 (9
  *                        in 2D) surrounding cells, then sorts each list by id | 0: every pair is
@@ -454,7 +459,7 @@ int lpm_specimen_stat(const lpm_ctx *ctx, const char *name, double *out);
  * "spmv_launches" (SpMVs that did work); "spmv_ms" / "spmv_launches_timed" (sum of SpMV kernel
  * time from HIP events and the number of launches it covers, when timing is on);
  * "last_cg_iters","last_cg_error"; "spmv_tuned_ms" (mean time of the fused CG SpMV in the
- * configuration the tuner kept). */
+ * configuration the tuner kept); "asm_tuned_ms" (the same for the planned assembly). */
 double lpm_stat(const lpm_ctx *ctx, const char *name);
 int lpm_reset_stats(lpm_ctx *ctx);
 /* every > 0: time the CG SpMV of every `every`-th iteration with HIP events on the context

@@ -921,6 +921,13 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "verbose") c->verbose = (int)v;
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
     else if (w == "asm_plan") c->asm_plan = (int)v ? 1 : 0;
+    else if (w == "asm_tune") {
+        c->asm_tune = (int)v ? 1 : 0;
+        c->asm_tune_key = -1;
+    } else if (w == "asm_wpb") {
+        if (v != 1 && v != 2 && v != 4) return lpm_fail(c, LPM_E_ARG, "asm_wpb must be 1, 2 or 4");
+        c->asm_wpb = (int)v;
+    }
     else if (w == "nl_cells") c->nl_cells = (int)v ? 1 : 0;
     else if (w == "spmv_nt") {
         c->spmv_nt = (int)v ? 1 : 0;
@@ -981,6 +988,8 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "cg_batch") return c->cg_batch;
     if (w == "spmv_nt") return c->spmv_nt;
     if (w == "asm_plan") return c->asm_plan;
+    if (w == "asm_tune") return c->asm_tune;
+    if (w == "asm_wpb") return c->asm_wpb;
     if (w == "nl_cells") return c->nl_cells;
     if (w == "spmv_tune") return c->spmv_tune;
     if (w == "cg_forcing") return c->cg_forcing;
@@ -1298,6 +1307,7 @@ double lpm_stat(const lpm_ctx *c, const char *name) {
     if (w == "last_cg_iters") return c->last_cg_iters;
     if (w == "last_cg_error") return c->last_cg_err;
     if (w == "spmv_tuned_ms") return c->tune_best_ms;
+    if (w == "asm_tuned_ms") return c->asm_tuned_ms;
     return NAN;
 }
 

@@ -1674,24 +1674,62 @@ int launch_assemble(lpm_ctx *c) {
     if (planned) {
         const int maxb = std::max(1, c->maxbond);
         const size_t per = asm_plan_lds_bytes(c->asm_hops, maxb);
-        const int W = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per));
-        const int g = blocks_for(c->n, W);
+        // rows (wavefronts) per workgroup: at most 4 and at most 64 KB of LDS
+        const int wmax = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per));
+        auto launch = [&](int W) -> int {
+            const int g = blocks_for(c->n, W);
 #define ASM_PLAN_LAUNCH(T, D)                                                                                          \
     hipLaunchKernelGGL((k_assemble_plan<T, D>), dim3(g), dim3(64 * W), per * W, c->stream, c->n, c->asm_hops, maxb,       \
                        c->diag10_literal, c->cg_transpose, c->bptr, c->bsplit, c->p2, c->cptr, c->cidx, c->tslot, c->sp,   \
                        c->pl_row, c->pl_slot, c->pl_rng, c->pl_b3, c->pl_own, c->cs, c->cs_sum, c->Kn, c->Tv,             \
                        c->bdamage, (T *)c->aval, c->diag64)
-        if (c->n > 0) {
-            if (c->precision == 0) {
-                if (c->dim == 3) ASM_PLAN_LAUNCH(float, 3);
-                else ASM_PLAN_LAUNCH(float, 2);
-            } else {
-                if (c->dim == 3) ASM_PLAN_LAUNCH(double, 3);
-                else ASM_PLAN_LAUNCH(double, 2);
+            if (c->n > 0) {
+                if (c->precision == 0) {
+                    if (c->dim == 3) ASM_PLAN_LAUNCH(float, 3);
+                    else ASM_PLAN_LAUNCH(float, 2);
+                } else {
+                    if (c->dim == 3) ASM_PLAN_LAUNCH(double, 3);
+                    else ASM_PLAN_LAUNCH(double, 2);
+                }
             }
-        }
 #undef ASM_PLAN_LAUNCH
-        return check_launch(c, "k_assemble_plan");
+            return check_launch(c, "k_assemble_plan");
+        };
+        // Launch tuner: every row is computed by one wavefront alone, so each configuration
+        // writes the same blocks; the candidates are timed on this assembly (the last launch
+        // leaves its result) and the fastest is kept for the context.
+        if (c->asm_tune && c->asm_tune_key != c->precision && c->n > 0) {
+            hipEvent_t a, b;
+            LPM_HIP(c, hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+            if (hipEventCreateWithFlags(&b, hipEventDisableSystemFence) != hipSuccess) {
+                hipEventDestroy(a);
+                return lpm_fail(c, LPM_E_HIP, "assembly tuner events");
+            }
+            const int reps = 2;
+            double best = 1e300;
+            int bw = wmax, rc = LPM_OK;
+            for (int W = 1; W <= wmax && !rc; W *= 2) {
+                rc = launch(W);  // warm
+                if (!rc && hipEventRecord(a, c->stream) != hipSuccess) rc = lpm_fail(c, LPM_E_HIP, "assembly tuner");
+                for (int k = 0; k < reps && !rc; ++k) rc = launch(W);
+                if (!rc && (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess))
+                    rc = lpm_fail(c, LPM_E_HIP, "assembly tuner");
+                float ms = 0;
+                if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = lpm_fail(c, LPM_E_HIP, "assembly tuner");
+                if (!rc && ms < best) {
+                    best = ms;
+                    bw = W;
+                }
+            }
+            hipEventDestroy(a);
+            hipEventDestroy(b);
+            if (rc) return rc;
+            c->asm_wpb = bw;
+            c->asm_tuned_ms = best / reps;
+            c->asm_tune_key = c->precision;
+            return launch(bw);
+        }
+        return launch(c->asm_wpb > 0 ? std::min(c->asm_wpb, wmax) : wmax);
     }
     // wavefronts per workgroup: as many as fit 64 KB of LDS, at most 4
     const size_t per = asm_lds_bytes(c->asm_hops, c->maxconn);

@@ -72,6 +72,10 @@ struct lpm_ctx {
     // 2-hop paths; built from the pattern on the first planned assembly
     int asm_plan = 1;
     bool asm_unplanned = false;       // the pattern cannot be planned (a partner repeats in a layer)
+    // planned-assembly launch: wavefronts (rows) per workgroup, 1, 2 or 4 as the LDS allows; with
+    // asm_tune the fastest is measured on the first planned assembly of each precision and kept
+    int asm_tune = 1, asm_wpb = 0, asm_tune_key = -1;
+    double asm_tuned_ms = 0;          // mean time of the planned assembly in the kept configuration
     int32_t *pl_row = nullptr, *pl_slot = nullptr, *pl_rng = nullptr, *pl_b3 = nullptr;
     uint8_t *pl_own = nullptr;
     // modes

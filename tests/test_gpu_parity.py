@@ -278,8 +278,16 @@ def test_planned_assembly_matches_row_list_kernel(lat, r, box):
                 g.mode(asm_plan=plan)
                 g.calc_stiffness()
                 out.append(g.matrix())
-            for a, b in zip(out[0], out[1]):
-                np.testing.assert_array_equal(b, a)
+            # every launch shape of the planned kernel (the tuner's candidates) writes the same blocks
+            g.mode(asm_tune=0)
+            for w in (1, 2, 4):
+                g.mode(asm_wpb=w)
+                g.calc_stiffness()
+                out.append(g.matrix())
+            g.mode(asm_tune=1)
+            for o in out[1:]:
+                for a, b in zip(out[0], o):
+                    np.testing.assert_array_equal(b, a)
     g.close()
 
 
