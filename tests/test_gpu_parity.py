@@ -442,6 +442,49 @@ def test_two_level_preconditioner(lattice, precision):
     assert iters[2] < iters[0]
 
 
+_BAR13 = {}
+
+
+def _bar13_oracle():
+    """The oracle's equilibrium of the SC 13^3 bar step below (computed once for the module)."""
+    if not _BAR13:
+        p, xyz, lo, hi = _bar(n=13)
+        ls_o, _ = p.load(disp=[(lo, a, 0.0) for a in "xyz"] + [(hi, "z", 0.02)])
+        p.o.mode(dirichlet_static=1, cg_tol=1e-13, tol_NR=1e-11, max_NR_iter=50)
+        p.o.update_geometry()
+        p.o.update_force_state()
+        assert p.o.solve_step_static(ls_o, -1)
+        _BAR13["u"] = p.o.getd("xyz") - xyz.ravel()
+    return _BAR13["u"]
+
+
+@pytest.mark.parametrize("cells", [3, 7, 12])
+def test_two_level_restriction_workgroup_sizes(cells):
+    """The restriction's workgroup (64, 256 or 1024 threads, the fewest that cover the largest
+    aggregate in one round of 4 members per thread) on SC 13^3 with aggregates of 3, 7 and 12
+    spacings (up to 64, 512 and 2197 members): the equilibrium matches the oracle to 1e-8 in
+    FP32 and FP64, and the coarse correction cuts the CG iterations against Jacobi."""
+    say("[gpu] two-level restriction, aggregate edge", cells)
+    u_o = _bar13_oracle()
+    for precision in (0, 1):
+        iters = {}
+        for pc in (0, 2):
+            p, xyz, lo, hi = _bar(n=13)
+            _, ls_g = p.load(disp=[(lo, a, 0.0) for a in "xyz"] + [(hi, "z", 0.02)])
+            p.g.mode(dirichlet_static=1, cg_tol=1e-6 if precision == 0 else 1e-12, tol_NR=1e-11, max_NR_iter=50,
+                     precision=precision, cg_transpose=0, diag10_literal=0, cg_maxit=20000, precond=pc,
+                     coarse_cells=cells)
+            p.g.update_geometry()
+            p.g.update_force_state()
+            assert p.g.solve_step_static(ls_g, -1)
+            u_g = p.g.get("xyz") - xyz.ravel()
+            iters[pc] = p.g.stat("cg_iters")
+            say(f"   precision {precision} precond {pc} ncoarse {p.g.get_mode('ncoarse')} "
+                f"disp rel err {rel(u_g, u_o):.2e} cg {iters[pc]}")
+            assert rel(u_g, u_o) < 1e-8
+        assert iters[2] < iters[0]
+
+
 @pytest.mark.parametrize("cells", [1, 2])
 def test_two_level_dependent_coarse_modes(cells):
     """Aggregates of one or two lattice spacings on a hex plate whose top row has its y DoF
