@@ -271,25 +271,24 @@ __device__ void cg_scalar_step(const double *prr, const double *pg, const double
                                double *scal, int it, double tol, double maxit, double *coef) {
     __shared__ double red[3][16];
     double t[3] = {0.0, 0.0, 0.0};
-    // the first SCALAR_PER partials of each thread are all requested before the first add (a
-    // loop that adds as it loads waits out one memory latency per partial)
-    constexpr int SCALAR_PER = 4;
-    double q[3][SCALAR_PER];
+    // Each thread's partials are requested SCALAR_PER at a time before they are added (a loop
+    // that adds as it loads waits out one memory latency per partial); the additions keep the
+    // ascending order of b, so every workgroup forms the same sums.
+    constexpr int SCALAR_PER = 8;
+    const int bd = blockDim.x, npmax = np_vec > np_spmv ? np_vec : np_spmv;
+    for (int b0 = threadIdx.x; b0 < npmax; b0 += SCALAR_PER * bd) {
+        double q[3][SCALAR_PER];
 #pragma unroll
-    for (int u = 0; u < SCALAR_PER; ++u) {
-        const int b = threadIdx.x + u * blockDim.x;
-        q[0][u] = b < np_vec ? prr[b] : 0.0;
-        q[1][u] = b < np_vec ? pg[b] : 0.0;
-        q[2][u] = b < np_spmv ? pd[b] : 0.0;
-    }
+        for (int u = 0; u < SCALAR_PER; ++u) {
+            const int b = b0 + u * bd;
+            q[0][u] = b < np_vec ? prr[b] : 0.0;
+            q[1][u] = b < np_vec ? pg[b] : 0.0;
+            q[2][u] = b < np_spmv ? pd[b] : 0.0;
+        }
 #pragma unroll
-    for (int u = 0; u < SCALAR_PER; ++u)
-        for (int k = 0; k < 3; ++k) t[k] += q[k][u];
-    for (int b = threadIdx.x + SCALAR_PER * blockDim.x; b < np_vec; b += blockDim.x) {
-        t[0] += prr[b];
-        t[1] += pg[b];
+        for (int u = 0; u < SCALAR_PER; ++u)
+            for (int k = 0; k < 3; ++k) t[k] += q[k][u];
     }
-    for (int b = threadIdx.x + SCALAR_PER * blockDim.x; b < np_spmv; b += blockDim.x) t[2] += pd[b];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k) {
         double v = t[k];
