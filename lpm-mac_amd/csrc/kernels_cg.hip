@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256) void k_cg_vec_b(int64_t n, const T *__restrict
         r0 = r[i0];
         a0 = agg_of[i0];
         m0 = mask[i0];
-        for (int k = 0; k < 3; ++k) d0[k] = agg_d[3 * i0 + k];
+        for (int k = 0; k < D; ++k) d0[k] = agg_d[D * i0 + k];
     }
     if (scal[SC_DONE] != 0.0) return;
     double v[1] = {0.0};
@@ -460,7 +460,7 @@ __global__ __launch_bounds__(256) void k_cg_vec_b(int64_t n, const T *__restrict
         precond_apply<T, D, VP>(invd, block_pre, i, rv, zv);
         double P[3][6];
         if (i == i0) rigid_modes<D>(d0, m0, P);
-        else rigid_modes<D>(&agg_d[3 * i], mask[i], P);
+        else rigid_modes<D>(&agg_d[D * i], mask[i], P);
         const double *y = cy + (int64_t)(i == i0 ? a0 : agg_of[i]) * M;
         for (int k = 0; k < D; ++k) {
             double t = 0.0;

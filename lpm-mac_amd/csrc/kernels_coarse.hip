@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_coarse_rows(int64_t n, const int32_t *_
     const int nc = cptr[i + 1] - cptr[i];
     const int32_t *ci = scol + sp[i >> 6] + (i & 63);
     double Pi[3][6];
-    rigid_modes<D>(&agg_d[3 * i], mask[i], Pi);
+    rigid_modes<D>(&agg_d[D * i], mask[i], Pi);
     for (int f = ra_ptr[i]; f < ra_ptr[i + 1]; ++f) {
         const int h = ra_id[f];
         double W[D][M];
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_coarse_rows(int64_t n, const int32_t *_
             const int64_t j = ci[(int64_t)c * 64];
             if (agg_of[j] != h) continue;
             double Pj[3][6];
-            rigid_modes<D>(&agg_d[3 * j], mask[j], Pj);
+            rigid_modes<D>(&agg_d[D * j], mask[j], Pj);
             double Kb[D][D];
             for (int r = 0; r < D; ++r)
                 for (int q = 0; q < D; ++q) Kb[r][q] = (double)aval[sell_off(sp, i, c, EB, r * D + q)];
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(RW) void k_coarse_restrict(const int32_t *__restric
     for (int u = 0; u < RESTRICT_PER; ++u) {
         const int m = beg + (int)threadIdx.x + u * RW;
         ii0[u] = m < end ? agg_idx[m] : -1;
-        for (int k = 0; k < 3; ++k) dm0[u][k] = m < end ? agg_dm[3 * (int64_t)m + k] : 0.0;
+        for (int k = 0; k < D; ++k) dm0[u][k] = m < end ? agg_dm[D * (int64_t)m + k] : 0.0;
     }
     if (scal[SC_DONE] != 0.0) return;
     for (int m0 = beg + (int)threadIdx.x; m0 < end; m0 += RW * RESTRICT_PER) {
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(RW) void k_coarse_restrict(const int32_t *__restric
             const int m = m0 + u * RW;
             if (ii[u] >= 0) {
                 double P[3][6], d[3];
-                for (int k = 0; k < 3; ++k) d[k] = first ? dm0[u][k] : agg_dm[3 * (int64_t)m + k];
+                for (int k = 0; k < D; ++k) d[k] = first ? dm0[u][k] : agg_dm[D * (int64_t)m + k];
                 rigid_modes<D>(d, mask[ii[u]], P);
                 double ri[3] = {0, 0, 0};
                 for (int k = 0; k < D; ++k) ri[k] = (double)r[VP * ii[u] + k];
@@ -625,7 +625,7 @@ static int coarse_setup(lpm_ctx *c) {
     LPM_HIP(c, hipMalloc(&c->agg_of, sizeof(int32_t) * n));
     LPM_HIP(c, hipMalloc(&c->agg_ptr, sizeof(int32_t) * (nagg + 1)));
     LPM_HIP(c, hipMalloc(&c->agg_idx, sizeof(int32_t) * n));
-    LPM_HIP(c, hipMalloc(&c->agg_d, sizeof(double) * 3 * n));
+    LPM_HIP(c, hipMalloc(&c->agg_d, sizeof(double) * D * std::max<int64_t>(n, 1)));
     LPM_HIP(c, hipMalloc(&c->ra_ptr, sizeof(int32_t) * (n + 1)));
     LPM_HIP(c, hipMalloc(&c->ra_id, sizeof(int32_t) * std::max<size_t>(rid.size(), 1)));
     LPM_HIP(c, hipMalloc(&c->cvbuf, sizeof(double) * M * M * std::max<size_t>(rid.size(), 1)));
@@ -645,18 +645,22 @@ static int coarse_setup(lpm_ctx *c) {
     LPM_HIP(c, hipMalloc(&c->cz, sizeof(double) * std::max(nc, 1)));
     LPM_HIP(c, hipMalloc(&c->cy, sizeof(double) * nc));
     LPM_HIP(c, hipMalloc(&c->dofmask, n));
-    LPM_HIP(c, hipMalloc(&c->agg_dm, sizeof(double) * 3 * std::max<int64_t>(n, 1)));
+    LPM_HIP(c, hipMalloc(&c->agg_dm, sizeof(double) * D * std::max<int64_t>(n, 1)));
     LPM_HIP(c, hipMemsetAsync(c->cz, 0, sizeof(double) * std::max(nc, 1), c->stream));
-    std::vector<double> dm(3 * n);  // offsets from the centre in member order
+    // the offsets on the device keep D entries per particle (the modes read no z offset in 2D), in
+    // particle order (agg_d) and in member order (agg_dm)
+    std::vector<double> dD(D * n), dm(D * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < D; ++k) dD[D * i + k] = d[3 * i + k];
     for (int64_t m = 0; m < n; ++m)
-        for (int k = 0; k < 3; ++k) dm[3 * m + k] = d[3 * (int64_t)aidx[m] + k];
-    LPM_HIP(c, hipMemcpyAsync(c->agg_dm, dm.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+        for (int k = 0; k < D; ++k) dm[D * m + k] = d[3 * (int64_t)aidx[m] + k];
+    LPM_HIP(c, hipMemcpyAsync(c->agg_dm, dm.data(), sizeof(double) * D * n, hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemsetAsync(c->cy, 0, sizeof(double) * nc, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->cq_gh, qgh.data(), sizeof(int32_t) * qgh.size(), hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->agg_of, agg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->agg_ptr, aptr.data(), sizeof(int32_t) * (nagg + 1), hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->agg_idx, aidx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
-    LPM_HIP(c, hipMemcpyAsync(c->agg_d, d.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(c, hipMemcpyAsync(c->agg_d, dD.data(), sizeof(double) * D * n, hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->ra_ptr, rptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipMemcpyAsync(c->ra_id, rid.data(), sizeof(int32_t) * rid.size(), hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipStreamSynchronize(c->stream));

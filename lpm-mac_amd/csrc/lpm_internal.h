@@ -109,7 +109,7 @@ struct lpm_ctx {
     int nagg = 0, nmode = 0, ncoarse = 0;
     int agg_max = 0;                  // members of the largest aggregate (sizes the restriction's workgroups)
     int32_t *agg_of = nullptr, *agg_ptr = nullptr, *agg_idx = nullptr;  // particle -> aggregate; members
-    double *agg_d = nullptr;          // x0_i - centre of its aggregate (3 per particle)
+    double *agg_d = nullptr;          // x0_i - centre of its aggregate (dim entries per particle)
     int32_t *ra_ptr = nullptr, *ra_id = nullptr;  // distinct aggregates of the columns of each row
     int64_t nra = 0;
     int32_t *cq_ptr = nullptr, *cq_f = nullptr, *cq_gh = nullptr;  // coarse blocks: contributing row entries
