@@ -271,6 +271,21 @@ __device__ void cg_scalar_step(const double *prr, const double *pg, const double
                                double *scal, int it, double tol, double maxit, double *coef) {
     __shared__ double red[3][16];
     double t[3] = {0.0, 0.0, 0.0};
+    // the scalars this step reads are requested with the partials: the stop flag and the previous
+    // iteration's gamma and alpha (written by an earlier launch; this launch's workgroup 0 writes
+    // only the current parity slots, and a stop it records now is the decision every workgroup
+    // reaches from the same sums) and the threshold (written at iteration 0, which computes its own)
+    const int cur = it & 1, prev = cur ^ 1;
+    // both parity slots are read, so that the loads do not wait on it
+    double done0 = 0.0, thr0 = 0.0, gam0 = 1.0, gam1 = 1.0, alp0 = 1.0, alp1 = 1.0;
+    if (threadIdx.x == 0) {
+        done0 = scal[SC_DONE];
+        thr0 = scal[SC_THR];
+        gam0 = scal[SC_GAMMA_P];
+        gam1 = scal[SC_GAMMA_P + 1];
+        alp0 = scal[SC_ALPHA_P];
+        alp1 = scal[SC_ALPHA_P + 1];
+    }
     // Each thread's partials are requested SCALAR_PER at a time before they are added (a loop
     // that adds as it loads waits out one memory latency per partial); the additions keep the
     // ascending order of b, so every workgroup forms the same sums.
@@ -296,28 +311,27 @@ __device__ void cg_scalar_step(const double *prr, const double *pg, const double
         if (lane == 0) red[k][wid] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0 && scal[SC_DONE] != 0.0) {  // the solve has stopped: change nothing
+    if (threadIdx.x == 0 && done0 != 0.0) {  // the solve has stopped: change nothing
         coef[0] = coef[1] = 0.0;
-        coef[2] = scal[SC_DONE];
+        coef[2] = done0;
     } else if (threadIdx.x == 0) {
         const int nw = blockDim.x >> 6;
         double tot[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < 3; ++k)
             for (int q = 0; q < nw; ++q) tot[k] += red[k][q];
         const double rr = tot[0], gamma = tot[1], delta = tot[2];
-        const int cur = it & 1, prev = cur ^ 1;
         double alpha = 0.0, beta = 0.0, done = 0.0, thr;
         if (it == 0) {
             thr = fmax(tol * tol * rr, DBL_MIN);
             done = (rr == 0.0) ? 3.0 : ((rr < thr) ? 1.0 : (maxit <= 0 ? 2.0 : 0.0));
             alpha = gamma / delta;
         } else {
-            thr = scal[SC_THR];
+            thr = thr0;
             if (rr < thr) done = 1.0;
             else if ((double)it >= maxit) done = 2.0;
             else {
-                beta = gamma / scal[SC_GAMMA_P + prev];
-                alpha = gamma / (delta - beta * gamma / scal[SC_ALPHA_P + prev]);
+                beta = gamma / (prev ? gam1 : gam0);
+                alpha = gamma / (delta - beta * gamma / (prev ? alp1 : alp0));
             }
         }
         coef[0] = alpha;
@@ -344,7 +358,9 @@ __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__
                                                  Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
                                                  Vec<T, VP> *__restrict__ s, const Vec<T, VP> *__restrict__ w,
                                                  double *partial, int cap, int np_vec, int np_spmv, double *scal, int k) {
-    // iteration k of the batch that starts at scal[SC_IT0]; the scalar step reads the stop flag
+    // iteration k of the batch that starts at scal[SC_IT0] (an even iteration: batches have an
+    // even number of iterations, so the parity of the partial slots is k's); the scalar step
+    // reads the stop flag
     const int it = (int)scal[SC_IT0] + k;
     const double tol = scal[SC_TOL], maxit = scal[SC_MAXIT];
     // the first particle's vectors are requested before the scalar step waits on the partials
@@ -359,7 +375,7 @@ __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__
         w0 = w[i0];
     }
     __shared__ double coef[3];  // alpha, beta, done
-    const double *pv = partial + (int64_t)(it & 1) * 2 * cap;
+    const double *pv = partial + (int64_t)(k & 1) * 2 * cap;
     cg_scalar_step(pv, pv + cap, partial + (int64_t)4 * cap, np_vec, np_spmv, scal, it, tol, maxit, coef);
     if (coef[2] != 0.0) return;
     double v[2] = {0.0, 0.0};
@@ -384,7 +400,7 @@ __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__
             v[1] += (double)rv.v[k] * (double)zv.v[k];
         }
     }
-    block_partial<2>(v, partial + (int64_t)((it & 1) ^ 1) * 2 * cap, cap);
+    block_partial<2>(v, partial + (int64_t)((k & 1) ^ 1) * 2 * cap, cap);
 }
 
 // Two-level variant, first half: the same scalar step (gamma partials at the fixed offset cap),
@@ -394,7 +410,7 @@ __global__ __launch_bounds__(256) void k_cg_vec_a(int64_t n, Vec<T, VP> *__restr
                                                    const Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ p,
                                                    Vec<T, VP> *__restrict__ s, const Vec<T, VP> *__restrict__ w,
                                                    double *partial, int cap, int np_vec, int np_spmv, double *scal, int k) {
-    const int it = (int)scal[SC_IT0] + k;  // the scalar step reads the stop flag
+    const int it = (int)scal[SC_IT0] + k;  // SC_IT0 is even: the parity of the partial slots is k's
     const double tol = scal[SC_TOL], maxit = scal[SC_MAXIT];
     // the first particle's vectors are requested before the scalar step waits on the partials
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -408,7 +424,7 @@ __global__ __launch_bounds__(256) void k_cg_vec_a(int64_t n, Vec<T, VP> *__restr
         w0 = w[i0];
     }
     __shared__ double coef[3];
-    cg_scalar_step(partial + (int64_t)(it & 1) * 2 * cap, partial + cap, partial + (int64_t)4 * cap, np_vec, np_spmv, scal,
+    cg_scalar_step(partial + (int64_t)(k & 1) * 2 * cap, partial + cap, partial + (int64_t)4 * cap, np_vec, np_spmv, scal,
                    it, tol, maxit, coef);
     if (coef[2] != 0.0) return;
     double v[1] = {0.0};
@@ -428,7 +444,7 @@ __global__ __launch_bounds__(256) void k_cg_vec_a(int64_t n, Vec<T, VP> *__restr
         r[i] = rv;
         for (int k = 0; k < D; ++k) v[0] += (double)rv.v[k] * (double)rv.v[k];
     }
-    block_partial<1>(v, partial + (int64_t)((it & 1) ^ 1) * 2 * cap, cap);
+    block_partial<1>(v, partial + (int64_t)((k & 1) ^ 1) * 2 * cap, cap);
 }
 
 // Two-level variant, second half: u = D^-1 r + P Ac^-1 P^T r (the coarse correction cy is
