@@ -595,8 +595,8 @@ __global__ __launch_bounds__(256) void k_move(int64_t n, const Vec<T, VP> *__res
 }
 
 // ---------------------------------------------------------------- copy of known size
-// Streams `n16` 16-byte words from a to b: the reference kernel whose bytes (n16*16 read,
-// n16*16 written) calibrate hardware traffic counters and give the practical HBM ceiling.
+// Streams `n16` 16-byte words from a to b: a kernel of known bytes (n16*16 read, n16*16 written)
+// that calibrates hardware traffic counters and gives the practical HBM ceiling.
 __global__ __launch_bounds__(256) void k_copy_calib(int64_t n16, const float4 *__restrict__ a, float4 *__restrict__ b) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
         b[i] = a[i];
