@@ -262,6 +262,8 @@ def main():
     ap.add_argument("--precond", type=int, default=2,
                     help="0 Jacobi (as the reference's Eigen CG), 1 block Jacobi, 2 Jacobi + rigid-body coarse correction")
     ap.add_argument("--coarse-cells", type=int, default=0, help="aggregate edge of the two-level preconditioner (0: auto)")
+    ap.add_argument("--mode", action="append", default=[], metavar="KEY=VALUE",
+                    help="headline: an extra lpm_set_mode before the warm-up (repeatable)")
     ap.add_argument("--coarse-budget", type=int, default=0,
                     help="coarse DoFs per specimen of the automatic aggregate size (0: the library default)")
     ap.add_argument("--predictor", type=int, default=1,
@@ -313,6 +315,9 @@ def main():
     if args.verbose:
         print(f"setup {t_setup:.2f} s, n={ctx.n} bonds={ctx.size('nbonds')} blocks={ctx.size('nconn')}", flush=True)
     # warmup steps, then back to the initial state so that the timed steps are steps 1..K
+    for kv in args.mode:
+        key, val = kv.split("=")
+        ctx.mode(**{key: float(val)})
     for _ in range(args.warmup):
         ok = ctx.solve_step_static(ls, -1)
         assert ok
