@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_cg_init(int64_t n, const double *__rest
         scal[SC_MAXIT] = maxit;
     }
     double v[2] = {0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = xcd_major_block() * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         Vec<T, VP> rv, zv, zero;
         for (int k = 0; k < VP; ++k) { rv.v[k] = (T)0; zero.v[k] = (T)0; }
         for (int k = 0; k < D; ++k) rv.v[k] = (T)b64[D * i + k];
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void k_cg_vec(int64_t n, const T *__restrict__
     const int it = (int)scal[SC_IT0] + k;
     const double tol = scal[SC_TOL], maxit = scal[SC_MAXIT];
     // the first particle's vectors are requested before the scalar step waits on the partials
-    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i0 = xcd_major_block() * blockDim.x + threadIdx.x;  // XCD x: a contiguous particle range
     Vec<T, VP> p0, s0, x0, r0, u0, w0;
     if (i0 < n) {
         p0 = p[i0];
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(256) void k_cg_vec_a(int64_t n, Vec<T, VP> *__restr
     const int it = (int)scal[SC_IT0] + k;  // SC_IT0 is even: the parity of the partial slots is k's
     const double tol = scal[SC_TOL], maxit = scal[SC_MAXIT];
     // the first particle's vectors are requested before the scalar step waits on the partials
-    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i0 = xcd_major_block() * blockDim.x + threadIdx.x;  // XCD x: a contiguous particle range
     Vec<T, VP> p0, s0, x0, r0, u0, w0;
     if (i0 < n) {
         p0 = p[i0];
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void k_cg_vec_b(int64_t n, const T *__restrict
                                                    const uint8_t *__restrict__ mask, const double *__restrict__ cy,
                                                    double *partial, int cap, const double *scal) {
     constexpr int M = D == 3 ? 6 : 3;  // compile-time, so that P stays in registers
-    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i0 = xcd_major_block() * blockDim.x + threadIdx.x;  // XCD x: a contiguous particle range
     // the first particle's inputs are requested together with the stop flag
     Vec<T, VP> r0;
     double d0[3] = {0.0, 0.0, 0.0};
