@@ -117,16 +117,22 @@ __global__ __launch_bounds__(512) void k_spmv(int64_t n, int64_t nslice, const i
                                                const T *__restrict__ aval, const Vec<T, VP> *__restrict__ u,
                                                Vec<T, VP> *__restrict__ w, double *partial, int cap, const double *scal,
                                                int remap) {
-    if (FUSED && scal[SC_DONE] != 0.0) return;
     constexpr int EB = D * D;
     const int lane = threadIdx.x & 63;
     const int64_t k = xcd_block(remap) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // the slice's row length and slot range are requested together with the stop flag
+    const int64_t row = k * 64 + lane;
+    int len = 0;
+    int64_t base = 0, send = 0;
+    if (k < nslice) {
+        len = row < n ? cptr[row + 1] - cptr[row] : 0;
+        base = sp[k];
+        send = sp[k + 1];
+    }
+    if (FUSED && scal[SC_DONE] != 0.0) return;
     double delta = 0.0;
     if (k < nslice) {
-        const int64_t row = k * 64 + lane;
-        const int len = row < n ? cptr[row + 1] - cptr[row] : 0;
-        const int64_t base = sp[k];
-        const int W = (int)((sp[k + 1] - base) >> 6);
+        const int W = (int)((send - base) >> 6);
         const IX *ci = scol + base + lane;
         const T *av = aval + EB * base + lane;
         T acc[D];
@@ -189,19 +195,28 @@ __global__ __launch_bounds__(64 * NWS) void k_spmv_coop(int64_t n, int64_t nslic
                                                          const IX *__restrict__ scol, const T *__restrict__ aval,
                                                          const Vec<T, VP> *__restrict__ u, Vec<T, VP> *__restrict__ w,
                                                          double *partial, int cap, const double *scal, int remap) {
-    if (FUSED && scal[SC_DONE] != 0.0) return;
     constexpr int EB = D * D;
     __shared__ T red[NWS][D][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t grp = xcd_block(remap);
+    // the first slice's row length and slot range are requested together with the stop flag
+    const int64_t kf = grp * spw, rowf = kf * 64 + lane;
+    int lenf = 0;
+    int64_t basef = 0, endf = 0;
+    if (kf < nslice) {
+        lenf = rowf < n ? cptr[rowf + 1] - cptr[rowf] : 0;
+        basef = sp[kf];
+        endf = sp[kf + 1];
+    }
+    if (FUSED && scal[SC_DONE] != 0.0) return;
     double delta = 0.0;
     for (int t = 0; t < spw; ++t) {
         const int64_t k = grp * spw + t;
         if (k >= nslice) break;  // uniform over the workgroup
         const int64_t row = k * 64 + lane;
-        const int len = row < n ? cptr[row + 1] - cptr[row] : 0;
-        const int64_t base = sp[k];
-        const int W = (int)((sp[k + 1] - base) >> 6);
+        const int len = t == 0 ? lenf : (row < n ? cptr[row + 1] - cptr[row] : 0);
+        const int64_t base = t == 0 ? basef : sp[k];
+        const int W = (int)(((t == 0 ? endf : sp[k + 1]) - base) >> 6);
         const IX *ci = scol + base + lane;
         const T *av = aval + EB * base + lane;
         T acc[D];
