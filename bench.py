@@ -266,9 +266,10 @@ def main():
                     help="headline: an extra lpm_set_mode before the warm-up (repeatable)")
     ap.add_argument("--coarse-budget", type=int, default=0,
                     help="coarse DoFs per specimen of the automatic aggregate size (0: the library default)")
-    ap.add_argument("--predictor", type=int, default=1,
-                    help="1: start each static step from the previous step's converged increment (Newton runs to the "
-                         "same test); 0: the reference's start")
+    ap.add_argument("--predictor", type=int, default=2,
+                    help="2: start each static step from the quadratic through the last three converged states, 1: "
+                         "from the previous step's converged increment (Newton runs to the same test either way); 0: "
+                         "the reference's start. The cracked plate with bond breaking (--config 4) uses at most 1")
     ap.add_argument("--coarse-reuse", type=int, default=1,
                     help="1: keep the two-level coarse space across assemblies while it still works (rebuilt when the "
                          "eliminated DoFs change or the CG slows down); 0: rebuild it with every assembly")
@@ -411,13 +412,17 @@ def main():
     # The same K steps again: without the predictor (the reference's Newton start), and with the
     # reference's algorithm (Jacobi, no predictor), whose iteration counts drive the CPU
     # projection; both rates are reported beside the headline.
-    jac = nopred = graph = noreuse = None
+    jac = nopred = graph = noreuse = linpred = None
     jac_counts = counts
     if not args.no_cpu_baseline:
         # the same steps with each CG batch replayed from a captured HIP graph (per-SpMV timing
         # is off in that mode, so the headline and its roofline come from direct launches)
         graph, _ = rerun(cg_graph=1)
         ctx.mode(cg_graph=0)
+        if args.predictor == 2:
+            # the linear predictor (the previous step's increment) on the same steps
+            linpred, _ = rerun(predictor=1)
+            ctx.mode(predictor=args.predictor)
         if args.coarse_reuse and args.precond == 2:
             # the coarse space rebuilt with every assembly
             noreuse, _ = rerun(coarse_reuse=0)
@@ -443,7 +448,8 @@ def main():
             "workload": f"SC {args.n}^3 elastic uniaxial tension, {ctx.n} particles, 0.5% strain in 20 steps",
             "particles": ctx.n, "bonds": ctx.size("nbonds"), "bsr_blocks": ctx.size("nconn"),
             "newton_tol": args.tol_nr, "cg_rtol": args.cg_tol, "cg_forcing": args.cg_forcing,
-            "predictor": "previous step's converged increment" if args.predictor else "none (the reference's start)",
+            "predictor": {0: "none (the reference's start)", 1: "previous step's converged increment",
+                          2: "quadratic through the last three converged states"}[args.predictor],
             "precond": {0: "jacobi", 1: "block-jacobi", 2: "two-level (jacobi + rigid-body aggregate coarse space)"}[args.precond],
             "coarse_dofs": int(ncoarse), "coarse_reuse": args.coarse_reuse, "coarse_builds_per_step": coarse_builds,
             "spmv_launch": {"slots_unrolled": int(ctx.get_mode("spmv_unr")),
@@ -461,6 +467,7 @@ def main():
             "setup_seconds": t_setup,
             "async_dump": dump,
             "same_steps_with_cg_graph": graph,
+            "same_steps_with_the_linear_predictor": linpred,
             "same_steps_without_coarse_reuse": noreuse,
             "same_steps_without_predictor_or_coarse_reuse": nopred,
             "same_steps_reference_algorithm_jacobi_no_predictor": jac,
@@ -872,8 +879,11 @@ def config4_bench(args):
     bs2 = (bp2[:-1] + np.bincount(p1[lay == 0], minlength=n)).astype(np.int32)
     g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=(bp2, bs2, p2.astype(np.int32)))
     g.set_elastic(0.0, True, 69e3, 0.3, 1e-3, elmap_literal=0)
+    # bonds break along the way, so the load path is not smooth: the quadratic predictor measured
+    # 142 load steps/s (2.0 Newton per step) against 151 (1.7) for the linear one here
+    pred4 = min(args.predictor, 1)
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
-           cg_forcing=args.cg_forcing, tol_NR=1e-6, max_NR_iter=50, cg_maxit=50000, predictor=args.predictor,
+           cg_forcing=args.cg_forcing, tol_NR=1e-6, max_NR_iter=50, cg_maxit=50000, predictor=pred4,
            cg_graph=args.cg_graph, coarse_cells=args.coarse_cells, coarse_reuse=args.coarse_reuse)
     y = xyz[:, 1]
     lo, hi = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
@@ -904,7 +914,7 @@ def config4_bench(args):
                 {"workload": f"hex 400 x 400, {n} particles, edge crack 0.25 W, critical stretch 1e-3, "
                              f"opening {inc} per edge per step, {steps} steps",
                  "bonds": g.size("nbonds"), "broken_bonds": int((bd > 0.5).sum()), "newton_iters_per_step": newton,
-                 "cg_iters_per_step": cg, "newton_tol": 1e-6, "predictor": args.predictor, "precond": args.precond,
+                 "cg_iters_per_step": cg, "newton_tol": 1e-6, "predictor": pred4, "precond": args.precond,
                  "cg_graph": args.cg_graph, "coarse_dofs": int(g.get_mode("ncoarse")), "coarse_reuse": args.coarse_reuse,
                  "adjoint": {"seconds_polish_and_adjoint": t_adj, "refinements": nref, "relative_residual": rres,
                              "top_reaction": F}})

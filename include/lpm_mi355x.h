@@ -167,9 +167,11 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        with the lattice width). Changes the CG's rate, not the equilibrium.
  *  "predictor"        1: a static step starts its free DoFs from the converged positions plus the
  *                        previous converged step's increment (after the step's boundary
- *                        conditions); Newton then runs to the same test. 0 (default): the
- *                        reference's start, the converged positions. Reset by a halving restart
- *                        and by uploading positions.
+ *                        conditions); Newton then runs to the same test. 2: from the quadratic
+ *                        through the last three converged states (the increment plus its change),
+ *                        linear while only two exist. 0 (default): the reference's start, the
+ *                        converged positions. Reset by a halving restart and by uploading
+ *                        positions.
  *  "coarse_graph"     1: replay the coarse inversion's launches from a HIP graph captured once |
  *                        0 (default): launch them directly
  *  "coarse_reuse"     1: keep the two-level coarse space (Ac and its inverse) across assemblies

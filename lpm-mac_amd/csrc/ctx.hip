@@ -47,14 +47,22 @@ __global__ void k_disp_bc(int64_t m, const int32_t *__restrict__ ids, int serial
 
 // Solver::updateForceBC (solver.h:168-190) for one ForceBC
 // Load-step predictor: every DoF that the step does not prescribe starts from its converged
-// position plus the previous step's converged increment (xyz_last - xyz_prev); xyz_prev then
-// takes the converged position the increment of the next step will be measured from.
+// position plus an extrapolated increment: order 1 the previous step's converged increment
+// (xyz_last - xyz_prev), order 2 that plus its change from the step before (the quadratic through
+// the last three converged states); order 0 moves nothing. xyz_prev (and xyz_prev2 when given)
+// then take the converged positions the next step extrapolates from.
 __global__ void k_predict(int64_t n3, const int32_t *__restrict__ cons, const double *__restrict__ xyz_last,
-                          double *__restrict__ xyz_prev, double *__restrict__ xyz, int apply) {
+                          double *__restrict__ xyz_prev, double *__restrict__ xyz_prev2, double *__restrict__ xyz,
+                          int order) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n3) return;
-    const double last = xyz_last[q];
-    if (apply && cons[q] != 1) xyz[q] = xyz[q] + (last - xyz_prev[q]);
+    const double last = xyz_last[q], p1 = xyz_prev[q];
+    if (order > 0 && cons[q] != 1) {
+        double inc = last - p1;
+        if (order == 2) inc = inc + (inc - (p1 - xyz_prev2[q]));
+        xyz[q] = xyz[q] + inc;
+    }
+    if (xyz_prev2) xyz_prev2[q] = p1;
     xyz_prev[q] = last;
 }
 
@@ -471,7 +479,7 @@ int store_state(lpm_ctx *c) {
 
 int reset_state(lpm_ctx *c, bool reset_xyz) {
     const size_t n = c->n, nb = c->nbond, D = sizeof(double);
-    c->pred_valid = false;  // the step is re-run (halved): no increment to extrapolate
+    c->pred_hist = 0;  // the step is re-run (halved): no increment to extrapolate
     if (reset_xyz) RC(copy_d2d(c, c->xyz, c->xyz_last, 3 * n * D));
     RC(copy_d2d(c, c->damage, c->damage_last, n * D));
     RC(copy_d2d(c, c->sv, c->sv_last, LPM_NSV * n * D));
@@ -492,9 +500,14 @@ int predict(lpm_ctx *c) {
     const int64_t n3 = 3 * c->n;
     if (!c->xyz_prev) {
         LPM_HIP(c, hipMalloc(&c->xyz_prev, sizeof(double) * n3));
-        c->pred_valid = false;
+        c->pred_hist = 0;
     }
-    if (c->pred_valid) {
+    if (c->predictor == 2 && !c->xyz_prev2) {
+        LPM_HIP(c, hipMalloc(&c->xyz_prev2, sizeof(double) * n3));
+        c->pred_hist = std::min(c->pred_hist, 1);
+    }
+    const int order = std::min(c->pred_hist, c->predictor);
+    if (order > 0) {
         // Newton's test keeps the reference's multiplier: the norms at the reference's start
         RC(geometry_force_state(c));
         if (c->nspec > 0) {
@@ -509,9 +522,10 @@ int predict(lpm_ctx *c) {
         }
     }
     hipLaunchKernelGGL(k_predict, dim3(blocks_for(n3, 256)), dim3(256), 0, c->stream, n3, (const int32_t *)c->cons,
-                       (const double *)c->xyz_last, c->xyz_prev, c->xyz, c->pred_valid ? 1 : 0);
+                       (const double *)c->xyz_last, c->xyz_prev, c->predictor == 2 ? c->xyz_prev2 : (double *)nullptr,
+                       c->xyz, order);
     LPM_HIP(c, hipGetLastError());
-    c->pred_valid = true;
+    c->pred_hist = std::min(c->pred_hist + 1, c->predictor == 2 ? 2 : 1);
     return LPM_OK;
 }
 
@@ -854,7 +868,7 @@ int lpm_destroy(lpm_ctx *c) {
                     c->Tv, c->cs, c->bforce, c->bforce_last, c->bdamage, c->bdamage_last, c->residual, c->disp, c->diag64,
                     c->aval, c->invdiag, c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w, c->scal, c->partial, c->counter, c->iflag, c->sp, c->scol,
                     c->scol16, c->pl_row, c->pl_slot, c->pl_rng, c->pl_b3, c->pl_own,
-                    c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part, c->xyz_prev};
+                    c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part, c->xyz_prev, c->xyz_prev2};
     for (void *p : ptrs)
         if (p) hipFree(p);
     dump_free(c);
@@ -941,8 +955,9 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
         if ((int)v) c->coarse_stale = true;  // the next assembly rebuilds the coarse space
     }
     else if (w == "predictor") {
-        c->predictor = (int)v ? 1 : 0;
-        c->pred_valid = false;
+        if (v != 0 && v != 1 && v != 2) return lpm_fail(c, LPM_E_ARG, "predictor must be 0, 1 or 2");
+        c->predictor = (int)v;
+        c->pred_hist = 0;
     }
     else if (w == "spmv_tune") {
         c->spmv_tune = (int)v ? 1 : 0;
@@ -1022,7 +1037,7 @@ int lpm_upload(lpm_ctx *c, const char *name, const void *host, int64_t count) {
     if (count == 0) return LPM_OK;
     LPM_HIP(c, hipMemcpyAsync(f.p, host, f.es * count, hipMemcpyHostToDevice, c->stream));
     LPM_HIP(c, hipStreamSynchronize(c->stream));
-    if (w == "xyz" || w == "xyz_last") c->pred_valid = false;  // positions set by hand: nothing to extrapolate
+    if (w == "xyz" || w == "xyz_last") c->pred_hist = 0;  // positions set by hand: nothing to extrapolate
     return LPM_OK;
 }
 
@@ -1498,7 +1513,7 @@ static int ckpt_copy_buf(lpm_ctx *c, void **pbuf, bool save) {
     if (save) LPM_HIP(c, hipMemcpyAsync(buf + off, c->cons, sizeof(int32_t) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
     else LPM_HIP(c, hipMemcpyAsync(c->cons, buf + off, sizeof(int32_t) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
     if (!save) {
-        c->pred_valid = false;  // the increment before the restored state is not known
+        c->pred_hist = 0;  // the increments before the restored state are not known
         c->k_ready = false;
         c->k_version++;         // constraint flags changed: the coarse space is rebuilt with K
         RC(geometry_force_state(c));

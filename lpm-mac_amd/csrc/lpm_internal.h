@@ -134,10 +134,11 @@ struct lpm_ctx {
     std::vector<uintptr_t> cg_key;
     int coarse_graph = 0;             // 1: replay the inversion from a HIP graph, 0: launch directly
     // Load-step predictor (mode "predictor"): free DoFs start each step from the converged
-    // position plus the previous step's converged increment
+    // position extrapolated from the previous converged steps: 1 linearly (plus the previous
+    // increment), 2 quadratically (through the last three converged states) once they exist
     int predictor = 0;
-    bool pred_valid = false;          // xyz_prev holds the converged positions of the step before
-    double *xyz_prev = nullptr;
+    int pred_hist = 0;                // converged states held before the last: 1 xyz_prev, 2 also xyz_prev2
+    double *xyz_prev = nullptr, *xyz_prev2 = nullptr;
     // Newton tolerance multipliers max(||r0||, ||R0||) evaluated at the reference's start (the
     // converged positions with the step's boundary conditions, before the prediction), one per
     // specimen (one for a single system); used by the next Newton call only, then cleared

@@ -209,7 +209,7 @@ def test_batched_adjoint_matches_single_specimen_adjoints():
 
 
 def test_ensemble_predictor_reaches_the_same_equilibrium():
-    """Load-step predictor on a specimen ensemble with small strain increments (1e-4 to 3e-4 of
+    """Load-step predictor (linear and quadratic) on a specimen ensemble with small strain increments (1e-4 to 3e-4 of
     each specimen's height per step, as in the ensemble benchmark): every specimen converges to
     Newton 1e-10, whose multiplier is taken at the reference's start, and ends within 1e-8 of the
     run without the predictor."""
@@ -222,7 +222,7 @@ def test_ensemble_predictor_reaches_the_same_equilibrium():
         cases.append((x, e * (x[:, 1].max() - x[:, 1].min())))
     xyz, bonds, first = _union(parts)
     out = []
-    for pred in (0, 1):
+    for pred in (0, 1, 2):
         g = G.Context(G.HEXAGON2D, 0.5, G.ELASTIC, xyz, bonds=bonds)
         g.set_specimens(first)
         g.set_elastic(0.0, True, 110e3, 0.34, 1.0, elmap_literal=0)
@@ -242,6 +242,7 @@ def test_ensemble_predictor_reaches_the_same_equilibrium():
         say("   predictor", pred, "Newton per specimen (last step)", g.specimen_stat("newton_iters"))
         out.append(g.get("xyz").reshape(-1, 3) - xyz)
     for s in range(len(cases)):
-        err = rel(out[1][first[s]:first[s + 1]], out[0][first[s]:first[s + 1]])
-        say(f"   specimen {s}: displacement rel diff {err:.2e}")
-        assert err < 1e-8
+        for p in (1, 2):
+            err = rel(out[p][first[s]:first[s + 1]], out[0][first[s]:first[s + 1]])
+            say(f"   specimen {s}, predictor {p}: displacement rel diff {err:.2e}")
+            assert err < 1e-8

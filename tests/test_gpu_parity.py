@@ -377,11 +377,11 @@ def _headline_oracle_trajectory():
     return _HEADLINE_ORACLE["traj"]
 
 
-@pytest.mark.parametrize("forcing,precond,predictor", [(0, 0, 0), (1, 0, 0), (1, 2, 0), (1, 2, 1), (1, 0, 1)])
+@pytest.mark.parametrize("forcing,precond,predictor", [(0, 0, 0), (1, 0, 0), (1, 2, 0), (1, 2, 1), (1, 0, 1), (1, 2, 2)])
 def test_headline_settings_equilibrium_parity(forcing, precond, predictor):
     """The headline benchmark's solver settings (FP32 matrix/SpMV/CG with rtol 1e-3, optional
-    inexact-Newton forcing, Jacobi or two-level preconditioner, optional load-step predictor,
-    Newton tol 1e-10) on a 12^3 SC bar in uniaxial tension, 5 load steps: displacements and
+    inexact-Newton forcing, Jacobi or two-level preconditioner, optional linear or quadratic
+    load-step predictor, Newton tol 1e-10) on a 12^3 SC bar in uniaxial tension, 5 load steps: displacements and
     reactions within 1e-8 of the FP64 oracle (reference start, no predictor) solved tightly."""
     say("[gpu] headline settings forcing", forcing, "precond", precond, "predictor", predictor)
     traj = _headline_oracle_trajectory()
