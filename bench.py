@@ -105,6 +105,13 @@ def setup_gpu(n, tol_nr, cg_tol, precond, cg_forcing, coarse_cells=0, predictor=
     return ctx, ls, xyz, time.time() - t0
 
 
+def apply_modes(ctx, args):
+    """The --mode KEY=VALUE settings, applied after a workload's own modes."""
+    for kv in args.mode:
+        key, val = kv.split("=")
+        ctx.mode(**{key: float(val)})
+
+
 def reset_state(ctx, xyz):
     n = ctx.n
     ctx.set("xyz", xyz)
@@ -263,7 +270,7 @@ def main():
                     help="0 Jacobi (as the reference's Eigen CG), 1 block Jacobi, 2 Jacobi + rigid-body coarse correction")
     ap.add_argument("--coarse-cells", type=int, default=0, help="aggregate edge of the two-level preconditioner (0: auto)")
     ap.add_argument("--mode", action="append", default=[], metavar="KEY=VALUE",
-                    help="headline: an extra lpm_set_mode before the warm-up (repeatable)")
+                    help="an extra lpm_set_mode after each workload's own modes (repeatable)")
     ap.add_argument("--coarse-budget", type=int, default=0,
                     help="coarse DoFs per specimen of the automatic aggregate size (0: the library default)")
     ap.add_argument("--predictor", type=int, default=2,
@@ -316,9 +323,7 @@ def main():
     if args.verbose:
         print(f"setup {t_setup:.2f} s, n={ctx.n} bonds={ctx.size('nbonds')} blocks={ctx.size('nconn')}", flush=True)
     # warmup steps, then back to the initial state so that the timed steps are steps 1..K
-    for kv in args.mode:
-        key, val = kv.split("=")
-        ctx.mode(**{key: float(val)})
+    apply_modes(ctx, args)
     for _ in range(args.warmup):
         ok = ctx.solve_step_static(ls, -1)
         assert ok
@@ -514,6 +519,7 @@ def ensemble_bench(args):
                  predictor=args.predictor)
         if args.coarse_budget > 0:
             ctx.mode(coarse_budget=args.coarse_budget)
+        apply_modes(ctx, args)
         return ctx
 
     def bcs(xyzs, offs, pulls_):
@@ -690,6 +696,7 @@ def fatigue_ensemble_bench(args):
                  cg_tol=args.cg_tol, cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=30, cg_maxit=50000)
         if args.coarse_budget > 0:
             ctx.mode(coarse_budget=args.coarse_budget)
+        apply_modes(ctx, args)
         ctx.update_geometry()
         ctx.update_force_state()
 
@@ -827,6 +834,7 @@ def config3_bench(args):
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
            cg_forcing=args.cg_forcing, tol_NR=args.tol_nr, max_NR_iter=50, cg_maxit=20000, predictor=args.predictor,
            cg_graph=args.cg_graph, coarse_reuse=args.coarse_reuse)
+    apply_modes(g, args)
     steps = 50
     ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", 0.0), (lo, "z", 0.0), (hi, "z", 0.02 * L / steps)])
     g.update_geometry()
@@ -885,6 +893,7 @@ def config4_bench(args):
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=args.precond, cg_tol=args.cg_tol,
            cg_forcing=args.cg_forcing, tol_NR=1e-6, max_NR_iter=50, cg_maxit=50000, predictor=pred4,
            cg_graph=args.cg_graph, coarse_cells=args.coarse_cells, coarse_reuse=args.coarse_reuse)
+    apply_modes(g, args)
     y = xyz[:, 1]
     lo, hi = np.nonzero(y < y.min() + 1e-6)[0], np.nonzero(y > y.max() - 1e-6)[0]
     inc, steps = 0.0006, 40

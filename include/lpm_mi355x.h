@@ -185,6 +185,15 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        coarse correction, SpMV) into a HIP graph and replay it for every batch
  *                        of every solve; re-captured when sizes, launch configuration or buffers
  *                        change. Per-SpMV timing is off in this mode | 0 (default): launch directly
+ *  "cg_batch"         CG iterations issued before the host reads the device's stop flag (rounded
+ *                        up to even; default 16); iterations issued past the stop are no-ops, so
+ *                        the batch sizes never change results
+ *  "cg_batch_auto"    1 (default): a Newton solve starts with one batch covering the iterations
+ *                        that the solve at the same Newton index of the previous Newton loop took,
+ *                        plus 2, then continues in batches of cg_batch_next | 0: batches of
+ *                        cg_batch throughout. Not used under cg_graph
+ *  "cg_batch_next"    batch size after such a sized first batch (rounded up to even; default 6;
+ *                        0: cg_batch)
  *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 (0: smallest of 8, 9,
  *                        10, ... giving at most "coarse_budget" coarse DoFs per specimen);
  *                        get_mode("ncoarse") reads the coarse dimension once built

@@ -344,7 +344,7 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
         if (!std::isfinite(nr[s]) || !std::isfinite(tolm[s])) failed[s] = 1, active[s] = 0;  // see newton_iteration
     }
     c->tolm_ref.clear();
-    for (;;) {
+    for (int newton_round = 0;; ++newton_round) {
         int nact = 0;
         double eta = 0.1;
         for (int s = 0; s < S; ++s) {
@@ -371,7 +371,9 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
         const double cg_tol_set = c->cg_tol;
         const double tau = c->cg_forcing ? std::max(cg_tol_set, eta) : cg_tol_set;
         c->cg_tol = tau / std::sqrt((double)nact);
+        c->cg_slot = std::min(newton_round, 3);
         const int rc_cg = cg_solve(c, &it, &err, &ok);
+        c->cg_slot = -1;
         c->cg_tol = cg_tol_set;
         RC(rc_cg);
         c->st_cg_iters += it;
@@ -447,7 +449,9 @@ int newton_iteration(lpm_ctx *c, int *n_iter) {
             const double eta = 0.5 * c->tol_NR * tol_multiplier / norm_residual;
             c->cg_tol = std::max(cg_tol_set, std::min(0.1, eta));
         }
+        c->cg_slot = std::min(ni - 1, 3);
         const int rc_solve = solve_linear_system(c, nullptr, nullptr);
+        c->cg_slot = -1;
         c->cg_tol = cg_tol_set;
         RC(rc_solve);
         RC(geometry_force_state(c));
@@ -934,6 +938,11 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "tol_NR") c->tol_NR = v;
     else if (w == "verbose") c->verbose = (int)v;
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
+    else if (w == "cg_batch_next") c->cg_batch_next = std::max(0, (int)v);
+    else if (w == "cg_batch_auto") {
+        c->cg_batch_auto = (int)v ? 1 : 0;
+        for (int &h : c->cg_hist) h = -1;
+    }
     else if (w == "asm_plan") c->asm_plan = (int)v ? 1 : 0;
     else if (w == "asm_tune") {
         c->asm_tune = (int)v ? 1 : 0;
@@ -1001,6 +1010,8 @@ double lpm_get_mode(const lpm_ctx *c, const char *name) {
     if (w == "tol_NR") return c->tol_NR;
     if (w == "verbose") return c->verbose;
     if (w == "cg_batch") return c->cg_batch;
+    if (w == "cg_batch_next") return c->cg_batch_next;
+    if (w == "cg_batch_auto") return c->cg_batch_auto;
     if (w == "spmv_nt") return c->spmv_nt;
     if (w == "asm_plan") return c->asm_plan;
     if (w == "asm_tune") return c->asm_tune;

@@ -886,10 +886,19 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     if (coarse && (rc = coarse_u())) return rc;
     rc = spmv_launch<T, D, VP>(c, true, c->v_u, c->v_w);
     if (rc) return rc;
-    // batches of `batch` iterations (an even number, so that a batch starts on parity 0);
-    // the device flag turns the rest of a batch into no-ops once the solve has stopped
-    const int batch = ((c->cg_batch > 0 ? c->cg_batch : 16) + 1) & ~1;
+    // Batches of iterations between reads of the device's stop flag (even sizes, so that a batch
+    // starts on parity 0); the flag turns the rest of a batch into no-ops once the solve has
+    // stopped, so the sizes change launches only, never results. Batches of cg_batch; under
+    // cg_batch_auto a Newton solve whose index has a recorded count starts with a batch covering
+    // that count + 2, followed by batches of cg_batch_next (the few iterations it may lack). Graph
+    // mode replays one captured batch size throughout.
     const bool graph = c->cg_graph != 0;
+    const int batch_set = ((c->cg_batch > 0 ? c->cg_batch : 16) + 1) & ~1;
+    const int slot = c->cg_slot;
+    const bool predicted = !graph && c->cg_batch_auto && slot >= 0 && slot < 4 && c->cg_hist[slot] >= 0;
+    const int batch = predicted ? (std::min(std::max(c->cg_hist[slot] + 2, 4), 512) + 1) & ~1 : batch_set;
+    const int batch_next = predicted && c->cg_batch_next > 0 ? (c->cg_batch_next + 1) & ~1 : batch_set;
+    const int batch_max = std::max(batch, batch_next);
     // timing: events around the SpMV of every `timing`-th iteration only (an event record costs
     // a few microseconds of idle GPU time, so timing every launch would slow the solve); not in
     // graph mode, whose batches are replayed whole. One pair per timed iteration of a batch.
@@ -900,7 +909,7 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
         }
     } evs;
     if (c->timing && !graph) {
-        evs.e.resize(2 * (batch / c->timing + 2));
+        evs.e.resize(2 * (batch_max / c->timing + 2));
         for (auto &e : evs.e) LPM_HIP(c, timing_event(&e));
     }
     auto timed = [&](int64_t it) { return !graph && c->timing > 0 && it % c->timing == 0; };
@@ -934,7 +943,7 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
         return check_launch(c, "k_cg_advance");
     };
     if (graph && (rc = cg_graph_prepare(c, gv, gs, batch, [&]() { return issue(0, batch); }))) return rc;
-    for (const int nb = batch;;) {
+    for (int nb = batch;; nb = batch_next) {
         if (graph) LPM_HIP(c, hipGraphLaunch(c->cg_exec, c->stream));
         else if ((rc = issue(it0, nb))) return rc;
         LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
@@ -956,13 +965,14 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
         }
         if (done) break;
         it0 += nb;
-        if (it0 > maxit + 2 * batch) break;  // the device cap stops earlier; this only guards the loop
+        if (it0 > maxit + 2 * batch_max) break;  // the device cap stops earlier; this only guards the loop
     }
     c->st_spmv += 1;  // the prologue SpMV
     const double rhs2 = c->h_scal[SC_RHS2];
     const double done = c->h_scal[SC_DONE];
     // Eigen reports the updates done before the converging one (its loop breaks before i++)
     const int64_t upd = (int64_t)c->h_scal[SC_ITER];
+    if (slot >= 0 && slot < 4) c->cg_hist[slot] = (int)std::min<int64_t>(upd, 1 << 20);
     *iters = (int)((done == 1.0 && upd > 0) ? upd - 1 : upd);
     *error = (done == 3.0) ? 0.0 : sqrt(c->h_scal[SC_RR] / rhs2);
     *success = (done == 3.0) || (*error <= c->cg_tol);

@@ -81,6 +81,12 @@ struct lpm_ctx {
     // modes
     int diag10_literal = 1, cg_transpose = 1, dirichlet_static = 0, rollback_pex = 0;
     int precision = 0, precond = 0, verbose = 0, max_NR_iter = 30, cg_batch = 16;
+    // cg_batch_auto: the first batch of a Newton solve covers the iterations the solve at the same
+    // Newton index took last time (+2), and later batches hold cg_batch_next (0: cg_batch);
+    // cg_slot is that index (0..3, -1 outside a Newton loop) and cg_hist the iterations recorded
+    // per index (-1: none yet)
+    int cg_batch_auto = 1, cg_batch_next = 6, cg_slot = -1;
+    int cg_hist[4] = {-1, -1, -1, -1};
     int spmv_nt = 1;       // 1: the SpMV reads the matrix stream with non-temporal loads (0: plain loads)
     int nl_cells = 1;      // 1: the nonlocal search bins particles on a grid (0: every pair is tested)
     double cg_tol = 2.220446049250313e-16, tol_NR = 1e-6;
