@@ -872,7 +872,8 @@ int lpm_destroy(lpm_ctx *c) {
                     c->Tv, c->cs, c->bforce, c->bforce_last, c->bdamage, c->bdamage_last, c->residual, c->disp, c->diag64,
                     c->aval, c->invdiag, c->v_x, c->v_r, c->v_u, c->v_p, c->v_s, c->v_w, c->scal, c->partial, c->counter, c->iflag, c->sp, c->scol,
                     c->scol16, c->pl_row, c->pl_slot, c->pl_rng, c->pl_b3, c->pl_own,
-                    c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part, c->xyz_prev, c->xyz_prev2};
+                    c->spec_of, c->spec_w, c->spec_blk, c->spec_first_d, c->spec_part, c->xyz_prev, c->xyz_prev2,
+                    c->elim};
     for (void *p : ptrs)
         if (p) hipFree(p);
     dump_free(c);

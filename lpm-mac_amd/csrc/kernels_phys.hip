@@ -837,41 +837,65 @@ __global__ __launch_bounds__(256) void k_assemble_plan(
     }
 }
 
-// squared norm of diag(K), Stiffness::updateStiffnessDispBC (stiffness.h:269-270)
-__global__ __launch_bounds__(256) void k_diag_norm(int64_t nd, const double *__restrict__ diag64, double *partial,
-                                                    int cap) {
+// Squared norm of diag(K), Stiffness::updateStiffnessDispBC (stiffness.h:269-270). Beside it, the
+// DoFs the elimination zeroes, one byte per particle: bit k of elim[p] is set when DoF k of p is
+// constrained or p is fully damaged (the reference's test, stiffness.h:277); the row marks that
+// k_elim_rows sets next are cleared.
+__global__ __launch_bounds__(256) void k_diag_norm(int64_t nd, int D, const double *__restrict__ diag64,
+                                                    const int32_t *__restrict__ cons,
+                                                    const double *__restrict__ damage_visual, uint8_t *__restrict__ elim,
+                                                    uint8_t *__restrict__ touched, double *partial, int cap) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {d < nd ? diag64[d] * diag64[d] : 0.0};
+    if (d < nd && d % D == 0) {
+        const int64_t p = d / D;
+        const bool dead = fabs(damage_visual[p] - 1.0) < LPM_EPS;
+        int m = 0;
+        for (int k = 0; k < D; ++k)
+            if (dead || cons[3 * p + k] == 1) m |= 1 << k;
+        elim[p] = (uint8_t)m;
+        touched[p] = 0;
+    }
     block_partial<1>(v, partial, cap);
 }
 
-__device__ __forceinline__ bool eliminated(int64_t p, int k, const int32_t *cons, const double *damage_visual) {
-    return cons[3 * p + k] == 1 || fabs(damage_visual[p] - 1.0) < LPM_EPS;
+// The rows the elimination writes: a row of an eliminated DoF, and every row with a block in the
+// column of one. The pattern is symmetric (j in row i exactly when i in row j), so these are the
+// columns of the rows of particles with an eliminated DoF (the diagonal block included).
+__global__ __launch_bounds__(256) void k_elim_rows(int64_t n, const int32_t *__restrict__ cptr,
+                                                    const int32_t *__restrict__ cidx, const uint8_t *__restrict__ elim,
+                                                    uint8_t *__restrict__ touched) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || elim[p] == 0) return;
+    for (int s = cptr[p]; s < cptr[p + 1]; ++s) touched[cidx[s]] = 1;
 }
 
 // Stiffness::updateStiffnessDispBC (stiffness.h:272-293): zero the row and the column of every
-// constrained (or fully damaged) DoF and put ||diag K||_2 on its diagonal.
+// eliminated DoF (elim, above) and put ||diag K||_2 on its diagonal. One wavefront per row, a
+// lane per block; rows that k_elim_rows left unmarked return at once, and a block whose row and
+// column particles have no eliminated DoF is left alone.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_dirichlet(int64_t n, const int32_t *__restrict__ cptr,
                                                     const int32_t *__restrict__ cidx, const int64_t *__restrict__ sp,
-                                                    const int32_t *__restrict__ cons,
-                                                    const double *__restrict__ damage_visual, const double *scal,
-                                                    T *__restrict__ aval) {
+                                                    const uint8_t *__restrict__ elim, const uint8_t *__restrict__ touched,
+                                                    const double *scal, T *__restrict__ aval) {
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (i >= n) return;
+    if (i >= n || touched[i] == 0) return;
     const int c0 = cptr[i], nci = cptr[i + 1] - c0;
-    const T nd = (T)sqrt(scal[SC_DIAGNORM]);
-    bool ei[3];
-    for (int r = 0; r < D; ++r) ei[r] = eliminated(i, r, cons, damage_visual);
+    const int mi = elim[i];
     for (int slot = lane; slot < nci; slot += 64) {
         const int64_t j = cidx[c0 + slot];
+        const int mj = elim[j];
+        if ((mi | mj) == 0) continue;
+        const T nd = (T)sqrt(scal[SC_DIAGNORM]);
         for (int q = 0; q < D; ++q) {
-            const bool ej = eliminated(j, q, cons, damage_visual);
+            const bool ej = (mj >> q) & 1;
             for (int r = 0; r < D; ++r) {
-                if (ei[r] || ej) {
+                const bool ei = (mi >> r) & 1;
+                if (ei || ej) {
                     T v = (T)0;
-                    if (ei[r] && j == i && r == q) v = nd;
+                    if (ei && j == i && r == q) v = nd;
                     aval[sell_off(sp, i, slot, D * D, r * D + q)] = v;
                 }
             }
@@ -1756,28 +1780,35 @@ int launch_dirichlet(lpm_ctx *c) {
     const int64_t nd = c->n * c->dim;
     const int g = blocks_for(nd, 256);
     if (g > c->partial_cap) return lpm_fail(c, LPM_E_LIMIT, "diag grid exceeds partial buffer");
-    hipLaunchKernelGGL(k_diag_norm, dim3(g), dim3(256), 0, c->stream, nd, c->diag64, c->partial, c->partial_cap);
+    // elim: n bytes of DoF masks, then n bytes of row marks
+    if (!c->elim) LPM_HIP(c, hipMalloc(&c->elim, sizeof(uint8_t) * 2 * std::max<int64_t>(1, c->n)));
+    uint8_t *touched = c->elim + std::max<int64_t>(1, c->n);
+    hipLaunchKernelGGL(k_diag_norm, dim3(g), dim3(256), 0, c->stream, nd, c->dim, c->diag64, c->cons, c->damage_visual,
+                       c->elim, touched, c->partial, c->partial_cap);
     int rc = check_launch(c, "k_diag_norm");
     if (rc) return rc;
+    hipLaunchKernelGGL(k_elim_rows, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->elim,
+                       touched);
+    if ((rc = check_launch(c, "k_elim_rows"))) return rc;
     rc = launch_finalize(c, FIN_DIAG, g);
     if (rc) return rc;
     const int g2 = blocks_for(c->n, 4);
     if (c->precision == 0) {
         float *a = (float *)c->aval;
         if (c->dim == 3)
-            hipLaunchKernelGGL((k_dirichlet<float, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->cons,
-                               c->damage_visual, c->scal, a);
+            hipLaunchKernelGGL((k_dirichlet<float, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->elim,
+                               touched, c->scal, a);
         else
-            hipLaunchKernelGGL((k_dirichlet<float, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->cons,
-                               c->damage_visual, c->scal, a);
+            hipLaunchKernelGGL((k_dirichlet<float, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->elim,
+                               touched, c->scal, a);
     } else {
         double *a = (double *)c->aval;
         if (c->dim == 3)
-            hipLaunchKernelGGL((k_dirichlet<double, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->cons,
-                               c->damage_visual, c->scal, a);
+            hipLaunchKernelGGL((k_dirichlet<double, 3>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->elim,
+                               touched, c->scal, a);
         else
-            hipLaunchKernelGGL((k_dirichlet<double, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->cons,
-                               c->damage_visual, c->scal, a);
+            hipLaunchKernelGGL((k_dirichlet<double, 2>), dim3(g2), dim3(256), 0, c->stream, c->n, c->cptr, c->cidx, c->sp, c->elim,
+                               touched, c->scal, a);
     }
     return check_launch(c, "k_dirichlet");
 }

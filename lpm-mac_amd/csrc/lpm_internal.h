@@ -223,6 +223,7 @@ struct lpm_ctx {
     // positions and base forces of the current assembly; built on the first FD assembly
     int stiffness_fd = 0;
     int fd_ncolor = 0;
+    uint8_t *elim = nullptr;  // per particle, bit k: DoF k is eliminated by the Dirichlet pass; then n row marks
     int32_t *fd_color = nullptr, *fd_ids = nullptr;  // class of each particle / members by class
     std::vector<int64_t> fd_ptr;                     // class offsets into fd_ids (host)
     double *fd_x0 = nullptr, *fd_P0 = nullptr;
