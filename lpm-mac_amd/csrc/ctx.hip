@@ -423,8 +423,12 @@ int newton_iteration(lpm_ctx *c, int *n_iter) {
     RC(launch_residual(c, &r2, &R2));
     double norm_residual = std::sqrt(r2);
     // the reference's multiplier: at its own start, or at the reference's start before a prediction
-    const double tol_multiplier = c->tolm_ref.size() == 1 ? c->tolm_ref[0] : std::max(norm_residual, std::sqrt(R2));
+    // (copied to h_scal[SC_COUNT..] in stream order before the wait above)
+    const double tol_multiplier =
+        c->tolm_pending ? std::max(std::sqrt(c->h_scal[SC_COUNT]), std::sqrt(c->h_scal[SC_COUNT + 1]))
+                        : (c->tolm_ref.size() == 1 ? c->tolm_ref[0] : std::max(norm_residual, std::sqrt(R2)));
     c->tolm_ref.clear();
+    c->tolm_pending = false;
     if (c->verbose)
         printf("|  Norm of residual is %.5e, norm of reaction is %.5e, tolerance criterion is based on %s force\n",
                norm_residual, std::sqrt(R2), norm_residual > std::sqrt(R2) ? "residual" : "reaction");
@@ -500,6 +504,7 @@ int reset_state(lpm_ctx *c, bool reset_xyz) {
 // step's increment; Newton then corrects from there to the same test
 int predict(lpm_ctx *c) {
     c->tolm_ref.clear();
+    c->tolm_pending = false;
     if (!c->predictor) return LPM_OK;
     const int64_t n3 = 3 * c->n;
     if (!c->xyz_prev) {
@@ -520,9 +525,13 @@ int predict(lpm_ctx *c) {
             c->tolm_ref.resize(c->nspec);
             for (int s = 0; s < c->nspec; ++s) c->tolm_ref[s] = std::max(std::sqrt(r2[s]), std::sqrt(R2[s]));
         } else {
-            double r2 = 0, R2 = 0;
-            RC(launch_residual(c, &r2, &R2));
-            c->tolm_ref.assign(1, std::max(std::sqrt(r2), std::sqrt(R2)));
+            // the two norms follow to the host behind the prediction, the assembly and the first
+            // force pass; newton_iteration reads them after its own first wait
+            static_assert(SC_NORM_REAC == SC_NORM_R + 1, "the reference-start norms are copied as a pair");
+            RC(launch_residual(c, nullptr, nullptr));
+            LPM_HIP(c, hipMemcpyAsync(c->h_scal + SC_COUNT, c->scal + SC_NORM_R, sizeof(double) * 2,
+                                      hipMemcpyDeviceToHost, c->stream));
+            c->tolm_pending = true;
         }
     }
     hipLaunchKernelGGL(k_predict, dim3(blocks_for(n3, 256)), dim3(256), 0, c->stream, n3, (const int32_t *)c->cons,
@@ -834,7 +843,7 @@ static int create(const lpm_topology *t, const lpm_cut *cuts, int32_t ncut, lpm_
     c->partial_cap = std::max({blocks_for(n * c->dim, 256), blocks_for(n, 4), blocks_for(n, 256), 1});
     // CG: (||r||^2, gamma) partials in two parity slots + the SpMV's delta partials
     if ((rc = dalloc(c, &c->partial, 5 * (int64_t)c->partial_cap))) return fail(rc);
-    if (hipHostMalloc((void **)&c->h_scal, sizeof(double) * SC_COUNT) != hipSuccess ||
+    if (hipHostMalloc((void **)&c->h_scal, sizeof(double) * (SC_COUNT + 2)) != hipSuccess ||
         hipHostMalloc((void **)&c->h_iflag, sizeof(int32_t) * 8) != hipSuccess) {
         c->err = "hipHostMalloc failed";
         return fail(LPM_E_NOMEM);

@@ -149,6 +149,7 @@ struct lpm_ctx {
     // converged positions with the step's boundary conditions, before the prediction), one per
     // specimen (one for a single system); used by the next Newton call only, then cleared
     std::vector<double> tolm_ref;
+    bool tolm_pending = false;  // one specimen: the reference-start norms are on their way to h_scal[SC_COUNT..+1]
     uint8_t *dofmask = nullptr;       // per particle: bit k set when DoF k is eliminated
     int64_t coarse_version = -1;
     // mode "coarse_reuse": keep the coarse space and Ac^-1 across assemblies while the eliminated
@@ -214,7 +215,7 @@ struct lpm_ctx {
     int partial_cap = 0;
     unsigned int *counter = nullptr;  // last-block counters (4)
     int32_t *iflag = nullptr;    // device int flags (4)
-    double *h_scal = nullptr;    // pinned host mirror of scal
+    double *h_scal = nullptr;    // pinned host mirror of scal (SC_COUNT + 2: the reference-start norm pair)
     int32_t *h_iflag = nullptr;
     bool k_ready = false;
     // Finite-difference stiffness (mode "stiffness_fd", StiffnessMode::FiniteDifference): a
