@@ -214,6 +214,47 @@ int copy_d2d(lpm_ctx *c, void *dst, const void *src, size_t bytes) {
     return LPM_OK;
 }
 
+namespace {
+// Several device-to-device copies of double arrays in one launch (the state checkpoint's seven
+// arrays): 16-byte pieces in a grid-stride loop per array, an odd last element by thread 0.
+// Every array comes from hipMalloc, so 16-byte alignment holds.
+constexpr int COPY_JOBS = 8;
+struct CopyJobs {
+    const double *src[COPY_JOBS];
+    double *dst[COPY_JOBS];
+    int64_t n[COPY_JOBS];
+    int count;
+};
+
+__global__ __launch_bounds__(256) void k_copy_many(CopyJobs jobs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int j = 0; j < jobs.count; ++j) {
+        const double2 *s = (const double2 *)jobs.src[j];
+        double2 *d = (double2 *)jobs.dst[j];
+        const int64_t n2 = jobs.n[j] / 2;
+        for (int64_t i = t; i < n2; i += stride) d[i] = s[i];
+        if (t == 0 && (jobs.n[j] & 1)) jobs.dst[j][jobs.n[j] - 1] = jobs.src[j][jobs.n[j] - 1];
+    }
+}
+
+int copy_many(lpm_ctx *c, const CopyJobs &jobs) {
+    int64_t most = 0;
+    for (int j = 0; j < jobs.count; ++j) most = std::max(most, jobs.n[j] / 2);
+    if (most == 0) {
+        for (int j = 0; j < jobs.count; ++j)
+            if (jobs.n[j] > 0) {
+                const int r = copy_d2d(c, jobs.dst[j], jobs.src[j], sizeof(double) * jobs.n[j]);
+                if (r) return r;
+            }
+        return LPM_OK;
+    }
+    const int g = (int)std::min<int64_t>(2048, (most + 255) / 256);
+    hipLaunchKernelGGL(k_copy_many, dim3(g), dim3(256), 0, c->stream, jobs);
+    LPM_HIP(c, hipGetLastError());
+    return LPM_OK;
+}
+}  // namespace
+
 #define RC(x)                \
     do {                     \
         int rc_ = (x);       \
@@ -474,15 +515,13 @@ int newton_iteration(lpm_ctx *c, int *n_iter) {
 }
 
 int store_state(lpm_ctx *c) {
-    const size_t n = c->n, nb = c->nbond, D = sizeof(double);
-    RC(copy_d2d(c, c->xyz_last, c->xyz, 3 * n * D));
-    RC(copy_d2d(c, c->damage_last, c->damage, n * D));
-    RC(copy_d2d(c, c->sv_last, c->sv, LPM_NSV * n * D));
-    RC(copy_d2d(c, c->dis_last, c->dis, nb * D));
-    RC(copy_d2d(c, c->bdamage_last, c->bdamage, nb * D));
-    RC(copy_d2d(c, c->bforce_last, c->bforce, nb * D));
-    RC(copy_d2d(c, c->dLp_last, c->dLp, nb * D));
-    return LPM_OK;
+    const int64_t n = c->n, nb = c->nbond;
+    const CopyJobs jobs = {{c->xyz, c->damage, c->sv, c->dis, c->bdamage, c->bforce, c->dLp},
+                           {c->xyz_last, c->damage_last, c->sv_last, c->dis_last, c->bdamage_last, c->bforce_last,
+                            c->dLp_last},
+                           {3 * n, n, LPM_NSV * n, nb, nb, nb, nb},
+                           7};
+    return copy_many(c, jobs);
 }
 
 int reset_state(lpm_ctx *c, bool reset_xyz) {
@@ -553,9 +592,11 @@ int solve_step_static(lpm_ctx *c, const lpm_load_step *s, int undamaged, int *co
         RC(newton_iteration(c, &n_newton));
         if (n_newton >= c->max_NR_iter) break;
         RC(launch_state_var(c));
-        RC(launch_static_damage(c, undamaged, &any));
-        new_damaged = any != 0;
+        // the damage flag comes back behind the state checkpoint, which is stored either way
+        RC(launch_static_damage(c, undamaged, nullptr));
         RC(store_state(c));
+        RC(flag_wait(c, &any));
+        new_damaged = any != 0;
         if (new_damaged) RC(geometry_force_state(c));
     } while (new_damaged);
     *converged = n_newton < c->max_NR_iter;

@@ -2019,7 +2019,16 @@ int launch_static_damage(lpm_ctx *c, int undamaged, int *any) {
                        c->iflag);
     rc = check_launch(c, "k_static_damage");
     if (rc) return rc;
-    return read_flag(c, any);
+    if (any) return read_flag(c, any);
+    // the caller reads the flag later (flag_wait), after queueing more work behind it
+    LPM_HIP(c, hipMemcpyAsync(c->h_iflag, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    return LPM_OK;
+}
+
+int flag_wait(lpm_ctx *c, int *any) {
+    LPM_HIP(c, hipStreamSynchronize(c->stream));
+    *any = c->h_iflag[0] != 0;
+    return LPM_OK;
 }
 
 int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any) {

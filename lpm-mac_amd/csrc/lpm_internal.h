@@ -258,7 +258,8 @@ void fd_free(lpm_ctx *c);
 int launch_dirichlet(lpm_ctx *c);
 int launch_state_var(lpm_ctx *c);
 int launch_nonlocal_ddot(lpm_ctx *c, int undamaged);
-int launch_static_damage(lpm_ctx *c, int undamaged, int *any);
+int launch_static_damage(lpm_ctx *c, int undamaged, int *any);  // any == nullptr: flag copied, read by flag_wait
+int flag_wait(lpm_ctx *c, int *any);  // waits for the stream, then reads the flag launch_static_damage copied
 int launch_fatigue_damage(lpm_ctx *c, double dNdt, int undamaged, int *any);
 int launch_pattern(lpm_ctx *c);
 // joins a dump write still in flight and frees the writer's buffers (lpm_destroy)
