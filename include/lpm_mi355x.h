@@ -190,10 +190,11 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *                        the batch sizes never change results
  *  "cg_batch_auto"    1 (default): a Newton solve starts with one batch covering the iterations
  *                        that the solve at the same Newton index of the previous Newton loop took,
- *                        plus 2, then continues in batches of cg_batch_next | 0: batches of
- *                        cg_batch throughout. Not used under cg_graph
- *  "cg_batch_next"    batch size after such a sized first batch (rounded up to even; default 6;
- *                        0: cg_batch)
+ *                        plus 2, then continues with two batches of cg_batch_next and batches of
+ *                        cg_batch after those | 0: batches of cg_batch throughout. Not used under
+ *                        cg_graph. Every lpm_set_mode call forgets the recorded counts
+ *  "cg_batch_next"    size of the two batches after such a sized first batch (rounded up to
+ *                        even; default 6; 0: cg_batch)
  *  "coarse_cells"     aggregate edge in lattice spacings for precond 2 (0: smallest of 8, 9,
  *                        10, ... giving at most "coarse_budget" coarse DoFs per specimen);
  *                        get_mode("ncoarse") reads the coarse dimension once built

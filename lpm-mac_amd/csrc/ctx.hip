@@ -965,6 +965,9 @@ int64_t lpm_size(const lpm_ctx *c, const char *what) {
 int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     if (!c || !name) return LPM_E_ARG;
     const std::string w(name);
+    // a changed setting (precision, tolerances, preconditioner ...) may change every solve's
+    // length: the recorded counts that size the first CG batches start over
+    for (int &h : c->cg_hist) h = -1;
     if (w == "diag10_literal") c->diag10_literal = (int)v;
     else if (w == "cg_transpose") c->cg_transpose = (int)v;
     else if (w == "dirichlet_static") c->dirichlet_static = (int)v;
@@ -990,10 +993,7 @@ int lpm_set_mode(lpm_ctx *c, const char *name, double v) {
     else if (w == "verbose") c->verbose = (int)v;
     else if (w == "cg_batch") c->cg_batch = std::max(1, (int)v);
     else if (w == "cg_batch_next") c->cg_batch_next = std::max(0, (int)v);
-    else if (w == "cg_batch_auto") {
-        c->cg_batch_auto = (int)v ? 1 : 0;
-        for (int &h : c->cg_hist) h = -1;
-    }
+    else if (w == "cg_batch_auto") c->cg_batch_auto = (int)v ? 1 : 0;
     else if (w == "asm_plan") c->asm_plan = (int)v ? 1 : 0;
     else if (w == "asm_tune") {
         c->asm_tune = (int)v ? 1 : 0;

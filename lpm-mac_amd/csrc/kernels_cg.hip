@@ -890,15 +890,15 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
     // starts on parity 0); the flag turns the rest of a batch into no-ops once the solve has
     // stopped, so the sizes change launches only, never results. Batches of cg_batch; under
     // cg_batch_auto a Newton solve whose index has a recorded count starts with a batch covering
-    // that count + 2, followed by batches of cg_batch_next (the few iterations it may lack). Graph
-    // mode replays one captured batch size throughout.
+    // that count + 2, followed by two batches of cg_batch_next (the few iterations it may lack)
+    // and batches of cg_batch after those. Graph mode replays one captured batch size throughout.
     const bool graph = c->cg_graph != 0;
     const int batch_set = ((c->cg_batch > 0 ? c->cg_batch : 16) + 1) & ~1;
     const int slot = c->cg_slot;
     const bool predicted = !graph && c->cg_batch_auto && slot >= 0 && slot < 4 && c->cg_hist[slot] >= 0;
     const int batch = predicted ? (std::min(std::max(c->cg_hist[slot] + 2, 4), 512) + 1) & ~1 : batch_set;
     const int batch_next = predicted && c->cg_batch_next > 0 ? (c->cg_batch_next + 1) & ~1 : batch_set;
-    const int batch_max = std::max(batch, batch_next);
+    const int batch_max = std::max({batch, batch_next, batch_set});
     // timing: events around the SpMV of every `timing`-th iteration only (an event record costs
     // a few microseconds of idle GPU time, so timing every launch would slow the solve); not in
     // graph mode, whose batches are replayed whole. One pair per timed iteration of a batch.
@@ -943,7 +943,7 @@ static int cg_t(lpm_ctx *c, int *iters, double *error, int *success) {
         return check_launch(c, "k_cg_advance");
     };
     if (graph && (rc = cg_graph_prepare(c, gv, gs, batch, [&]() { return issue(0, batch); }))) return rc;
-    for (int nb = batch;; nb = batch_next) {
+    for (int nb = batch, nbatch = 1;; nb = nbatch++ < 3 ? batch_next : batch_set) {
         if (graph) LPM_HIP(c, hipGraphLaunch(c->cg_exec, c->stream));
         else if ((rc = issue(it0, nb))) return rc;
         LPM_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
