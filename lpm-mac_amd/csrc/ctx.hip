@@ -458,6 +458,8 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
 }
 
 int newton_iteration(lpm_ctx *c, int *n_iter) {
+    // the Jacobi inverse of K runs while the host waits for the first residual
+    if (c->k_ready) RC(launch_precond_setup(c));
     if (c->nspec > 0) return newton_iteration_spec(c, n_iter);
     double r2 = 0, R2 = 0;
     RC(geometry_force_state(c));

@@ -677,9 +677,19 @@ static int precond_t(lpm_ctx *c) {
     return check_launch(c, "k_precond");
 }
 
+// The Jacobi part of the preconditioner depends on K alone: it is formed once per K (assembly
+// generation, precision and preconditioner kind), which lets a Newton loop queue it before its
+// first wait for the residual.
 int launch_precond_setup(lpm_ctx *c) {
-    if (c->precision == 0) return c->dim == 3 ? precond_t<float, 3, 3>(c) : precond_t<float, 2, 2>(c);
-    return c->dim == 3 ? precond_t<double, 3, 3>(c) : precond_t<double, 2, 2>(c);
+    const int key = 8 * c->precision + c->precond;
+    if (c->precond_version == c->k_version && c->precond_key == key) return LPM_OK;
+    int rc;
+    if (c->precision == 0) rc = c->dim == 3 ? precond_t<float, 3, 3>(c) : precond_t<float, 2, 2>(c);
+    else rc = c->dim == 3 ? precond_t<double, 3, 3>(c) : precond_t<double, 2, 2>(c);
+    if (rc) return rc;
+    c->precond_version = c->k_version;
+    c->precond_key = key;
+    return LPM_OK;
 }
 
 // workgroups of the SpMV launch (= partials the fused SpMV leaves)

@@ -102,6 +102,8 @@ struct lpm_ctx {
     int spmv_idx16 = 1;               // mode: the SpMV reads scol16 when it exists
     int64_t nslice = 0, nslots = 0;
     int64_t k_version = 0;            // assembly generation
+    int64_t precond_version = -1;     // K generation the Jacobi inverse was formed from
+    int precond_key = -1;             // and its precision / preconditioner kind
     // SpMV launch configuration: slots unrolled per lane, wavefronts per workgroup, XCD remap;
     // with spmv_tune the fastest of all configurations is measured on the first solve of each
     // matrix kind (store, precision) and kept (the pattern never changes)
