@@ -363,7 +363,6 @@ int solve_linear_system(lpm_ctx *c, int *iters, double *error) {
     return LPM_OK;
 }
 
-// Solver::NewtonIteration (solver.h:53-89)
 // Solver::NewtonIteration (solver.h:53-89) for every specimen of an ensemble at once: each
 // specimen keeps its own tolerance multiplier, residual norm and iteration count. One CG solves
 // all specimens still iterating; each one's right-hand side is scaled to unit norm, so the CG
@@ -373,6 +372,7 @@ static int newton_iteration_spec(lpm_ctx *c, int *n_iter) {
     const int S = c->nspec;
     std::vector<double> r2, R2;
     RC(geometry_force_state(c));
+    RC(coarse_check(c));  // its flag arrives with the residual
     RC(launch_residual_spec(c, r2, R2));
     std::vector<double> tolm(S), nr(S), w(S);
     std::vector<char> active(S), failed(S, 0);
@@ -463,6 +463,7 @@ int newton_iteration(lpm_ctx *c, int *n_iter) {
     if (c->nspec > 0) return newton_iteration_spec(c, n_iter);
     double r2 = 0, R2 = 0;
     RC(geometry_force_state(c));
+    RC(coarse_check(c));  // its flag arrives with the residual
     RC(launch_residual(c, &r2, &R2));
     double norm_residual = std::sqrt(r2);
     // the reference's multiplier: at its own start, or at the reference's start before a prediction

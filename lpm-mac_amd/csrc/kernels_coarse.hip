@@ -674,6 +674,31 @@ static int coarse_rows_t(lpm_ctx *c) {
     return ck(c, "k_coarse_rows");
 }
 
+// The eliminated-DoF mask of the current K, and under coarse reuse its comparison with the mask
+// of the last build; the comparison's flag goes to h_iflag[1] without a wait. A Newton loop
+// queues this behind its first force pass (which leaves damage_visual current), so the flag
+// arrives with the first residual and coarse_prepare need not wait for it on its own.
+int coarse_check(lpm_ctx *c) {
+    if (!two_level(c) || !c->k_ready) return LPM_OK;
+    int rc = coarse_setup(c);
+    if (rc) return rc;
+    if (c->coarse_version == c->k_version || c->coarse_check_version == c->k_version) return LPM_OK;
+    const int64_t n = c->n;
+    hipLaunchKernelGGL(k_dofmask, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->dim, c->cons, c->damage_visual,
+                       c->dofmask);
+    if ((rc = ck(c, "k_dofmask"))) return rc;
+    c->coarse_check_version = c->k_version;
+    c->coarse_check_diff = c->coarse_reuse && c->coarse_built && !c->coarse_stale;
+    if (c->coarse_check_diff) {
+        LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
+        hipLaunchKernelGGL(k_mask_diff, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const uint8_t *)c->dofmask,
+                           (const uint8_t *)c->dofmask_prev, c->iflag);
+        if ((rc = ck(c, "k_mask_diff"))) return rc;
+        LPM_HIP(c, hipMemcpyAsync(c->h_iflag + 1, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    }
+    return LPM_OK;
+}
+
 // Rebuild and invert Ac after the matrix changed.
 int coarse_prepare(lpm_ctx *c) {
     if (!two_level(c)) return LPM_OK;
@@ -681,19 +706,13 @@ int coarse_prepare(lpm_ctx *c) {
     if (rc) return rc;
     if (c->coarse_version == c->k_version) return LPM_OK;
     const int64_t n = c->n;
-    hipLaunchKernelGGL(k_dofmask, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->dim, c->cons, c->damage_visual,
-                       c->dofmask);
-    if ((rc = ck(c, "k_dofmask"))) return rc;
-    if (c->coarse_reuse && c->coarse_built && !c->coarse_stale) {
+    // the mask of this K (coarse_check, queued earlier or now); the stale flag may have been set
+    // by a solve since, and a stale space is rebuilt whatever the masks say
+    if ((rc = coarse_check(c))) return rc;
+    if (c->coarse_check_diff && c->coarse_reuse && c->coarse_built && !c->coarse_stale) {
         // keep the coarse space of the last build if the eliminated DoFs are the same
-        LPM_HIP(c, hipMemsetAsync(c->iflag, 0, sizeof(int32_t), c->stream));
-        hipLaunchKernelGGL(k_mask_diff, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, (const uint8_t *)c->dofmask,
-                           (const uint8_t *)c->dofmask_prev, c->iflag);
-        if ((rc = ck(c, "k_mask_diff"))) return rc;
-        int32_t changed = 0;
-        LPM_HIP(c, hipMemcpyAsync(&changed, c->iflag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        LPM_HIP(c, hipStreamSynchronize(c->stream));
-        if (!changed) {
+        LPM_HIP(c, hipStreamSynchronize(c->stream));  // returns at once when the flag has arrived
+        if (c->h_iflag[1] == 0) {
             c->coarse_version = c->k_version;
             return LPM_OK;
         }

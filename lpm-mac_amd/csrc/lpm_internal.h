@@ -103,6 +103,8 @@ struct lpm_ctx {
     int64_t nslice = 0, nslots = 0;
     int64_t k_version = 0;            // assembly generation
     int64_t precond_version = -1;     // K generation the Jacobi inverse was formed from
+    int64_t coarse_check_version = -1;  // K generation whose eliminated-DoF mask coarse_check formed
+    bool coarse_check_diff = false;     // and whether it queued the comparison with the last build's
     int precond_key = -1;             // and its precision / preconditioner kind
     // SpMV launch configuration: slots unrolled per lane, wavefronts per workgroup, XCD remap;
     // with spmv_tune the fastest of all configurations is measured on the first solve of each
@@ -282,6 +284,7 @@ int launch_move(lpm_ctx *c);
 int launch_scale_x(lpm_ctx *c);  // x_i *= spec_w[specimen of i]
 int spmv_bench(lpm_ctx *c, int reps, double *ms);
 int launch_precond_setup(lpm_ctx *c);
+int coarse_check(lpm_ctx *c);  // queues the coarse-reuse mask comparison (kernels_coarse.hip)
 // precond 1 smooths with the inverse 3x3 (2x2) diagonal blocks, 0 and 2 with the inverse
 // diagonal; 2 adds the coarse correction
 static inline bool block_jacobi(const lpm_ctx *c) { return c->precond == 1; }
