@@ -457,6 +457,8 @@ def main():
                           2: "quadratic through the last three converged states"}[args.predictor],
             "precond": {0: "jacobi", 1: "block-jacobi", 2: "two-level (jacobi + rigid-body aggregate coarse space)"}[args.precond],
             "coarse_dofs": int(ncoarse), "coarse_reuse": args.coarse_reuse, "coarse_builds_per_step": coarse_builds,
+            "cg_batches": {"sized_first_batch": int(ctx.get_mode("cg_batch_auto")),
+                           "next_batches": int(ctx.get_mode("cg_batch_next")), "batch": int(ctx.get_mode("cg_batch"))},
             "spmv_launch": {"slots_unrolled": int(ctx.get_mode("spmv_unr")),
                             "wavefronts_per_workgroup": int(ctx.get_mode("spmv_wpb")),
                             "xcd_remap": int(ctx.get_mode("spmv_remap")), "tuned": int(ctx.get_mode("spmv_tune")),
