@@ -1,7 +1,8 @@
 """CG batch schedules (modes cg_batch, cg_batch_auto, cg_batch_next) change only how many
 iterations are queued between reads of the device's stop flag: iterations queued past the stop
 are no-ops. Every schedule must therefore give the same bits, the same iteration counts and the
-same Newton counts, on the single-specimen Newton loop and on the specimen-ensemble one."""
+same Newton counts, on the single-specimen Newton loop and on the specimen-ensemble one. The
+SpMV launch shape is fixed (spmv_tune=0), since the tuner's pick sets the order of its sums."""
 import sys
 
 import numpy as np
@@ -30,7 +31,8 @@ def _sc_run(schedule, steps=6):
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=2, cg_tol=1e-3, cg_forcing=1,
            tol_NR=1e-10, max_NR_iter=40, cg_maxit=20000, predictor=2, coarse_reuse=1)
     b, auto, nxt = schedule
-    g.mode(cg_batch=b, cg_batch_auto=auto, cg_batch_next=nxt)
+    # one fixed SpMV launch shape: the tuner's pick sets the SpMV's summation order
+    g.mode(cg_batch=b, cg_batch_auto=auto, cg_batch_next=nxt, spmv_tune=0)
     lo, hi = faces(xyz, 2)
     ls = g.load_step(disp=[(lo, "x", 0.0), (lo, "y", 0.0), (lo, "z", 0.0), (hi, "z", 0.004)])
     g.update_geometry()
@@ -67,7 +69,8 @@ def _ens_run(schedule, steps=4):
     g.mode(dirichlet_static=1, cg_transpose=0, diag10_literal=0, precision=0, precond=2, cg_tol=1e-3, cg_forcing=1,
            tol_NR=1e-10, max_NR_iter=30, cg_maxit=50000, predictor=2)
     b, auto, nxt = schedule
-    g.mode(cg_batch=b, cg_batch_auto=auto, cg_batch_next=nxt)
+    # one fixed SpMV launch shape: the tuner's pick sets the SpMV's summation order
+    g.mode(cg_batch=b, cg_batch_auto=auto, cg_batch_next=nxt, spmv_tune=0)
     disp = []
     for s, (x, pull) in enumerate(cases):
         ids = np.arange(first[s], first[s + 1])
