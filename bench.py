@@ -105,6 +105,15 @@ def setup_gpu(n, tol_nr, cg_tol, precond, cg_forcing, coarse_cells=0, predictor=
     return ctx, ls, xyz, time.time() - t0
 
 
+def spmv_launch_info(ctx):
+    """The CG SpMV launch shape in use (the tuner's pick for the current matrix kind)."""
+    coop = int(ctx.get_mode("spmv_coop"))
+    return {"slots_unrolled": int(ctx.get_mode("spmv_unr")), "wavefronts_per_workgroup": int(ctx.get_mode("spmv_wpb")),
+            "xcd_remap": int(ctx.get_mode("spmv_remap")), "tuned": int(ctx.get_mode("spmv_tune")),
+            "wavefronts_per_slice": coop or 1, "slices_per_workgroup": int(ctx.get_mode("spmv_spw")) if coop else None,
+            "nontemporal_matrix_loads": int(ctx.get_mode("spmv_nt"))}
+
+
 def apply_modes(ctx, args):
     """The --mode KEY=VALUE settings, applied after a workload's own modes."""
     for kv in args.mode:
@@ -459,12 +468,7 @@ def main():
             "coarse_dofs": int(ncoarse), "coarse_reuse": args.coarse_reuse, "coarse_builds_per_step": coarse_builds,
             "cg_batches": {"sized_first_batch": int(ctx.get_mode("cg_batch_auto")),
                            "next_batches": int(ctx.get_mode("cg_batch_next")), "batch": int(ctx.get_mode("cg_batch"))},
-            "spmv_launch": {"slots_unrolled": int(ctx.get_mode("spmv_unr")),
-                            "wavefronts_per_workgroup": int(ctx.get_mode("spmv_wpb")),
-                            "xcd_remap": int(ctx.get_mode("spmv_remap")), "tuned": int(ctx.get_mode("spmv_tune")),
-                            "wavefronts_per_slice": int(ctx.get_mode("spmv_coop")) or 1,
-                            "slices_per_workgroup": int(ctx.get_mode("spmv_spw")) if ctx.get_mode("spmv_coop") else None,
-                            "nontemporal_matrix_loads": int(ctx.get_mode("spmv_nt"))},
+            "spmv_launch": spmv_launch_info(ctx),
             "assembly_launch": {"rows_per_workgroup": int(ctx.get_mode("asm_wpb")), "tuned": int(ctx.get_mode("asm_tune")),
                                 "tuned_ms": ctx.stat("asm_tuned_ms")},
             "cg_solves": cg_solves,
@@ -856,6 +860,7 @@ def config3_bench(args):
                  "bonds": g.size("nbonds"), "newton_iters_per_step": g.stat("newton_iters") / steps,
                  "cg_iters_per_step": g.stat("cg_iters") / steps, "newton_tol": args.tol_nr,
                  "predictor": args.predictor, "precond": args.precond, "cg_graph": args.cg_graph,
+                 "spmv_launch": spmv_launch_info(g),
                  "coarse_reuse": args.coarse_reuse, "coarse_builds_per_step": g.stat("coarse_builds") / steps,
                  "setup_seconds": t_setup,
                  "yielded_fraction": float(np.mean(alpha > 0))})
@@ -911,6 +916,7 @@ def config4_bench(args):
     t = time.perf_counter() - t0
     bd = g.get("bdamage")
     newton, cg = g.stat("newton_iters") / steps, g.stat("cg_iters") / steps
+    spmv32 = spmv_launch_info(g)
     t0 = time.perf_counter()
     g.set_elastic(0.0, True, 69e3, 0.3, 1e9, elmap_literal=0)
     g.set("bdamage", bd)
@@ -927,6 +933,7 @@ def config4_bench(args):
                  "bonds": g.size("nbonds"), "broken_bonds": int((bd > 0.5).sum()), "newton_iters_per_step": newton,
                  "cg_iters_per_step": cg, "newton_tol": 1e-6, "predictor": pred4, "precond": args.precond,
                  "cg_graph": args.cg_graph, "coarse_dofs": int(g.get_mode("ncoarse")), "coarse_reuse": args.coarse_reuse,
+                 "spmv_launch": spmv32,
                  "adjoint": {"seconds_polish_and_adjoint": t_adj, "refinements": nref, "relative_residual": rres,
                              "top_reaction": F}})
     print(json.dumps(out), flush=True)
