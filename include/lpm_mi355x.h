@@ -201,7 +201,9 @@ int64_t lpm_size(const lpm_ctx *ctx, const char *what);
  *  "coarse_budget"    coarse DoFs per specimen the automatic size stays within (1..1024,
  *                        default 768); like coarse_cells, fixed once the coarse space exists
  *  "spmv_tune"        1: on the first solve of each matrix kind time every SpMV launch
- *                        configuration and keep the fastest | 0: use the values below
+ *                        configuration (two passes, the faster timing of each) and keep the
+ *                        fastest one-wavefront-per-slice shape unless a cooperative shape is more
+ *                        than 3% faster | 0: use the values below
  *  "spmv_unr" (1,2,4), "spmv_wpb" (wavefronts per workgroup 1,2,4,8), "spmv_remap" (0/1:
  *                        XCD-contiguous workgroup order), "spmv_coop" (0: one wavefront per
  *                        64-row slice | 8, 16: that many wavefronts share each slice),
